@@ -1,0 +1,90 @@
+/*
+ * parRSB.h -- drop-in public interface of the MI355X recursive spectral
+ * bisection partitioner.
+ *
+ * Mirrors the reference interface src/parRSB.h:21-48:
+ *   - parrsb_options: the same fields in the same order (parRSB.h:21-41),
+ *     plus one appended field `nparts` (see below);
+ *   - parrsb_default_options (parRSB.h:43, values from parrsb.c:26-44);
+ *   - parrsb_part_mesh(part, vtx, xyz, tag, nel, nv, options, comm)
+ *     (parRSB.h:45-48, behaviour of parrsb.c:853-906).
+ *
+ * Differences a caller can observe:
+ *   - `nparts` (appended, so callers that fill fields by name keep working;
+ *     callers compiled against the reference header must be recompiled since
+ *     the struct grew by one int).  0 means "one part per rank of comm", the
+ *     reference behaviour; P > 0 makes one rank produce P parts on one GPU.
+ *     The environment variable PARRSB_NPARTS overrides it, like every other
+ *     PARRSB_* variable (parrsb.c:48-77).
+ *   - All element data of the call must live on one rank (the one that owns
+ *     the GPU).  A communicator with more than one rank is rejected with
+ *     return code 3.
+ *   - A device (HIP) failure returns 2 instead of aborting.
+ */
+#ifndef PARRSB_AMD_PARRSB_H
+#define PARRSB_AMD_PARRSB_H
+
+#if defined(PARRSB_NO_MPI)
+/* Callers without MPI: MPICH's ABI (MPI_Comm is an int handle). */
+typedef int MPI_Comm;
+#ifndef MPI_COMM_WORLD
+#define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#define MPI_COMM_SELF ((MPI_Comm)0x44000001)
+#endif
+#else
+#include <mpi.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  /* General options (parRSB.h:23-30) */
+  int partitioner; /* 0 - RSB, 1 - RCB, 2 - RIB (default 0) */
+  int tagged;      /* tagged partitioning: 0 - no, 1 - yes (default 0) */
+  int levels;      /* 1 or 2 (default 2); clamped to <= 2 on return */
+  int find_disconnected_comps; /* count components of each half (default 1) */
+  int repair;        /* parsed, unused, as in the reference (default 0) */
+  int verbose_level; /* default 1 */
+  int profile_level; /* default 0 */
+  /* RSB options (parRSB.h:32-37) */
+  int rsb_algo;       /* 0 - Lanczos (1 - MG is not provided) */
+  int rsb_pre;        /* 0 - none, 1 - RCB, 2 - RIB (default 1) */
+  int rsb_max_iter;   /* Lanczos iterations per pass (default 50) */
+  int rsb_max_passes; /* Lanczos restarts (default 50) */
+  double rsb_tol;     /* CG residual reduction factor (default 1e-5) */
+  int rsb_dump_stats; /* parsed, unused (default 0) */
+  /* RSB MG options (parRSB.h:39-40), accepted and ignored */
+  int rsb_mg_grammian;
+  int rsb_mg_factor;
+  /* Appended: number of parts; 0 => size of comm (reference behaviour). */
+  int nparts;
+} parrsb_options;
+
+extern parrsb_options parrsb_default_options;
+
+/*
+ * Partition nel elements with nv vertices each (nv = 8: hexahedra in 3D,
+ * nv = 4: quadrilaterals in 2D) into P parts.
+ *   part[nel]            out: part id in [0, P) for each input element
+ *   vtx[nel*nv]          global vertex ids (non-zero), element-major
+ *   xyz[nel*nv*ndim]     per-vertex coordinates (ndim = 3 if nv == 8 else 2),
+ *                        or NULL (then options->rsb_pre is set to 0)
+ *   tag[nel]             only used when options->tagged == 1
+ *   options              read and MUTATED: PARRSB_* environment overrides,
+ *                        levels clamp, rsb_pre = 0 when xyz == NULL
+ * Returns 0 on success, 1 if tagged == 1 and tag == NULL (parrsb.c:873-877),
+ * 2 on a device failure, 3 on unsupported input (see header comment),
+ * 4 if both vtx and xyz are NULL (the reference aborts, parrsb.c:244-249).
+ */
+int parrsb_part_mesh(int *part, const long long *const vtx,
+                     const double *const xyz, const int *const tag,
+                     const int nel, const int nv, parrsb_options *const options,
+                     MPI_Comm comm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,832 @@
+/*
+ * CPU oracle (test infrastructure only; see oracle.h).  FP64 throughout,
+ * compiled without floating-point contraction like the reference's
+ * `-std=c99` build (Makefile:2), so that sums and products round exactly as
+ * written.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+void oracle_free(void *p) { free(p); }
+
+static void *xmalloc(size_t n) {
+  void *p = malloc(n ? n : 1);
+  if (!p) {
+    fprintf(stderr, "oracle: out of memory (%zu bytes)\n", n);
+    abort();
+  }
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stable merge sort of (key, id) records by key; equal keys keep their
+ * relative order.  -0.0 and +0.0 compare equal, as C's `<` has them. */
+typedef struct {
+  double key;
+  int id;
+} kv_t;
+
+static void merge_sort_kv(kv_t *a, kv_t *tmp, int n) {
+  if (n < 2) return;
+  for (int w = 1; w < n; w *= 2) {
+    for (int lo = 0; lo < n; lo += 2 * w) {
+      int mid = lo + w < n ? lo + w : n;
+      int hi = lo + 2 * w < n ? lo + 2 * w : n;
+      int i = lo, j = mid, k = lo;
+      while (i < mid && j < hi) {
+        if (a[j].key < a[i].key) tmp[k++] = a[j++];
+        else tmp[k++] = a[i++];
+      }
+      while (i < mid) tmp[k++] = a[i++];
+      while (j < hi) tmp[k++] = a[j++];
+    }
+    memcpy(a, tmp, sizeof(kv_t) * (size_t)n);
+  }
+}
+
+/* Stable sort of order[0..n) by key(order[i]) where keys are given per
+ * position (pkey[i] belongs to order[i]).  pkey is permuted alongside. */
+static void stable_sort_by_pos_key(int *order, double *pkey, int n) {
+  kv_t *a = xmalloc(sizeof(kv_t) * (size_t)n), *t = xmalloc(sizeof(kv_t) * (size_t)n);
+  for (int i = 0; i < n; i++) a[i].key = pkey[i], a[i].id = i;
+  merge_sort_kv(a, t, n);
+  int *o2 = xmalloc(sizeof(int) * (size_t)n);
+  double *k2 = xmalloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) o2[i] = order[a[i].id], k2[i] = pkey[a[i].id];
+  memcpy(order, o2, sizeof(int) * (size_t)n);
+  memcpy(pkey, k2, sizeof(double) * (size_t)n);
+  free(a), free(t), free(o2), free(k2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Centroids, parrsb.c:139-145: start from 0.0, add the corners in order,
+ * divide by nv. */
+void oracle_centroids(const double *xyz, int nel, int nv, double *coord) {
+  int ndim = (nv == 8) ? 3 : 2;
+  for (int e = 0; e < nel; e++) {
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < nv; v++)
+      for (int d = 0; d < ndim; d++) c[d] += xyz[(size_t)e * ndim * nv + v * ndim + d];
+    for (int d = 0; d < ndim; d++) c[d] /= nv;
+    coord[3 * (size_t)e + 0] = c[0];
+    coord[3 * (size_t)e + 1] = c[1];
+    coord[3 * (size_t)e + 2] = c[2];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Virtual-rank arithmetic. */
+int64_t oracle_rank_count(int64_t n, int p, int r) {
+  int64_t q = n / p, rem = n - q * p;
+  return (r < p - rem) ? q : q + 1; /* sort.c:58-66 */
+}
+
+int64_t oracle_lower_count(int64_t n, int p) {
+  int64_t q = n / p, rem = n - q * p;
+  int64_t hl = (p + 1) / 2; /* ranks [0, ceil(p/2)) form bin 0, rsb.c:241 */
+  int64_t extra = hl - (p - rem);
+  return q * hl + (extra > 0 ? extra : 0);
+}
+
+int64_t oracle_balance_expected(int64_t n, int p, int bin) {
+  int64_t q = n / p, rem = n - q * p; /* rsb.c:138-140 */
+  int64_t np_half = bin == 0 ? (p + 1) / 2 : p / 2;
+  return q * np_half + rem / 2 + (rem % 2) * (1 - bin);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RCB, rcb.c:7-137.  For p > 1 the segment is sorted along the axis of
+ * largest extent and split at the boundary of the lower ceil(p/2) virtual
+ * ranks; at p == 1 the local recursion splits at floor(n/2).  Ties in the
+ * extent go to the lower axis (strict `>`). */
+static void rcb_rec(const double *coord, int ndim, int *order, double *key, int n,
+                    int p) {
+  if (n <= 1) return;
+  double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = 0; i < n; i++) {
+    const double *c = coord + 3 * (size_t)order[i];
+    for (int d = 0; d < ndim; d++) {
+      if (c[d] < mn[d]) mn[d] = c[d];
+      if (c[d] > mx[d]) mx[d] = c[d];
+    }
+  }
+  double len[3];
+  for (int d = 0; d < ndim; d++) len[d] = mx[d] - mn[d];
+  int axis = 0;
+  if (len[1] > len[0]) axis = 1;
+  if (ndim == 3 && len[2] > len[axis]) axis = 2;
+
+  for (int i = 0; i < n; i++) key[i] = coord[3 * (size_t)order[i] + axis];
+  stable_sort_by_pos_key(order, key, n);
+
+  if (p > 1) {
+    int nl = (int)oracle_lower_count(n, p);
+    rcb_rec(coord, ndim, order, key, nl, (p + 1) / 2);
+    rcb_rec(coord, ndim, order + nl, key + nl, n - nl, p / 2);
+  } else {
+    int mid = n / 2;
+    rcb_rec(coord, ndim, order, key, mid, 1);
+    rcb_rec(coord, ndim, order + mid, key + mid, n - mid, 1);
+  }
+}
+
+void oracle_rcb_order(const double *coord, int ndim, int *order, int n, int p) {
+  double *key = xmalloc(sizeof(double) * (size_t)n);
+  rcb_rec(coord, ndim, order, key, n, p);
+  free(key);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Slot grouping: slots (e, j) of a sub-mesh grouped by vertex id.  Vertex
+ * id 0 is "no connection" in gather-scatter: such a slot forms its own
+ * group. */
+typedef struct {
+  long long vid;
+  int slot;
+} vs_t;
+
+static int cmp_vs(const void *a, const void *b) {
+  const vs_t *x = a, *y = b;
+  if (x->vid != y->vid) return x->vid < y->vid ? -1 : 1;
+  return x->slot - y->slot;
+}
+
+/* group[slot] = group id; returns number of groups; order of slots within
+ * a group is ascending slot index. gstart/gslots describe groups (CSR). */
+static int group_slots(const long long *vtx, int nv, const int *elems, int n, int *group,
+                       int **gstart_out, int **gslots_out) {
+  int ns = n * nv;
+  vs_t *a = xmalloc(sizeof(vs_t) * (size_t)ns);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < nv; j++) {
+      a[i * nv + j].vid = vtx[(size_t)elems[i] * nv + j];
+      a[i * nv + j].slot = i * nv + j;
+    }
+  qsort(a, (size_t)ns, sizeof(vs_t), cmp_vs);
+  int *gstart = xmalloc(sizeof(int) * (size_t)(ns + 1)), *gslots = xmalloc(sizeof(int) * (size_t)ns);
+  int ng = 0;
+  for (int s = 0; s < ns; s++) {
+    if (s == 0 || a[s].vid != a[s - 1].vid || a[s].vid == 0) gstart[ng++] = s;
+    group[a[s].slot] = ng - 1;
+    gslots[s] = a[s].slot;
+  }
+  gstart[ng] = ns;
+  free(a);
+  if (gstart_out) *gstart_out = gstart; else free(gstart);
+  if (gslots_out) *gslots_out = gslots; else free(gslots);
+  return ng;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gather-scatter Laplacian of a sub-mesh, laplacian.c:159-205:
+ *   mult(v)  = number of slots of the sub-mesh with vertex id v
+ *   diag_e   = sum over the slots of e of mult(v)
+ *   (Lu)_e   = diag_e u_e - sum over the slots of e of (sum of u over the
+ *              slots sharing that vertex id, own slot included). */
+typedef struct {
+  int n, nv, ng;
+  int *group;   /* per slot */
+  int *gstart, *gslots;
+  double *diag; /* per element */
+  double *gsum; /* per group scratch */
+} gs_lap;
+
+static void gs_lap_init(gs_lap *L, const long long *vtx, int nv, const int *elems, int n) {
+  L->n = n, L->nv = nv;
+  L->group = xmalloc(sizeof(int) * (size_t)n * nv);
+  L->ng = group_slots(vtx, nv, elems, n, L->group, &L->gstart, &L->gslots);
+  L->diag = xmalloc(sizeof(double) * (size_t)n);
+  L->gsum = xmalloc(sizeof(double) * (size_t)L->ng);
+  for (int i = 0; i < n; i++) {
+    double d = 0.0;
+    for (int j = 0; j < nv; j++) {
+      int g = L->group[i * nv + j];
+      d += (double)(L->gstart[g + 1] - L->gstart[g]);
+    }
+    L->diag[i] = d;
+  }
+}
+
+static void gs_lap_apply(const gs_lap *L, const double *u, double *v) {
+  int nv = L->nv;
+  for (int g = 0; g < L->ng; g++) {
+    double s = 0.0;
+    for (int k = L->gstart[g]; k < L->gstart[g + 1]; k++) s += u[L->gslots[k] / nv];
+    L->gsum[g] = s;
+  }
+  for (int i = 0; i < L->n; i++) {
+    double r = L->diag[i] * u[i];
+    for (int j = 0; j < nv; j++) r -= L->gsum[L->group[i * nv + j]];
+    v[i] = r;
+  }
+}
+
+static void gs_lap_free(gs_lap *L) {
+  free(L->group), free(L->gstart), free(L->gslots), free(L->diag), free(L->gsum);
+}
+
+void oracle_laplacian_apply(const long long *vtx, int nv, const int *elems, int n,
+                            const double *u, double *v) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  gs_lap_apply(&L, u, v);
+  gs_lap_free(&L);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dual graph of the whole mesh (mat.c:17-61, 203-249 semantics): for each
+ * element, the elements sharing at least one vertex id, with weight = the
+ * number of shared ids. */
+static int cmp_int(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+int64_t oracle_dual_graph(const long long *vtx, int nel, int nv, int64_t **rowptr_out,
+                          int **col_out, int **wgt_out) {
+  int *elems = xmalloc(sizeof(int) * (size_t)nel);
+  for (int i = 0; i < nel; i++) elems[i] = i;
+  int *group = xmalloc(sizeof(int) * (size_t)nel * nv), *gstart, *gslots;
+  group_slots(vtx, nv, elems, nel, group, &gstart, &gslots);
+  int64_t *rowptr = xmalloc(sizeof(int64_t) * ((size_t)nel + 1));
+  /* Two passes: count distinct neighbours, then fill. */
+  int cap = 64;
+  int *cand = xmalloc(sizeof(int) * (size_t)cap);
+  int64_t nnz = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int *col = NULL, *wgt = NULL;
+    if (pass == 1) {
+      col = xmalloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+      wgt = xmalloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+    }
+    int64_t pos = 0;
+    for (int e = 0; e < nel; e++) {
+      int nc = 0;
+      for (int j = 0; j < nv; j++) {
+        int g = group[(size_t)e * nv + j];
+        for (int k = gstart[g]; k < gstart[g + 1]; k++) {
+          int e2 = gslots[k] / nv;
+          if (e2 == e) continue;
+          if (nc == cap) cap *= 2, cand = realloc(cand, sizeof(int) * (size_t)cap);
+          cand[nc++] = e2;
+        }
+      }
+      qsort(cand, (size_t)nc, sizeof(int), cmp_int);
+      if (pass == 0) rowptr[e] = pos;
+      for (int k = 0; k < nc;) {
+        int k2 = k;
+        while (k2 < nc && cand[k2] == cand[k]) k2++;
+        if (pass == 1) col[pos] = cand[k], wgt[pos] = k2 - k;
+        pos++;
+        k = k2;
+      }
+    }
+    if (pass == 0) {
+      nnz = pos;
+      rowptr[nel] = nnz;
+    } else {
+      *col_out = col;
+      *wgt_out = wgt;
+    }
+  }
+  free(cand), free(group), free(gstart), free(gslots), free(elems);
+  *rowptr_out = rowptr;
+  return nnz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Implicit QL with shifts (tridiagonal eigen-solve), the steps of
+ * fiedler.c:328-432: deflation test |e_m| / (|d_m| + |d_m+1|) < 1e-12, at
+ * most 30 sweeps per eigenvalue, eigenvectors accumulated by plane
+ * rotations on the columns, then transposed to rows and normalised. */
+static double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+int oracle_tqli(const double *diag, const double *upper, int n, double *z, double *eval) {
+  if (n == 0) return 0;
+  double *d = xmalloc(sizeof(double) * 2 * (size_t)n), *e = d + n;
+  for (int i = 0; i < n; i++) d[i] = diag[i];
+  for (int i = 0; i < n - 1; i++) e[i] = upper[i];
+  e[n - 1] = 0.0;
+  for (int i = 0; i < n * n; i++) z[i] = 0.0;
+  for (int i = 0; i < n; i++) z[i * n + i] = 1.0;
+
+  for (int l = 0; l < n; l++) {
+    int sweeps = 0, m;
+    do {
+      for (m = l; m < n - 1; m++) {
+        double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) / dd < 1e-12) break;
+      }
+      if (m == l) break;
+      if (sweeps++ == 30) {
+        for (int i = 0; i < n; i++) eval[i] = d[i];
+        free(d);
+        return 1;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = sqrt(g * g + 1.0);
+      g = d[m] - d[l] + e[l] / (g + sign_of(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      int i;
+      for (i = m - 1; i >= l; i--) {
+        double f = s * e[i], b = c * e[i];
+        if (fabs(f) >= fabs(g)) {
+          c = g / f;
+          r = sqrt(c * c + 1.0);
+          e[i + 1] = f * r;
+          s = 1.0 / r;
+          c = c * s;
+        } else {
+          s = f / g;
+          r = sqrt(s * s + 1.0);
+          e[i + 1] = g * r;
+          c = 1.0 / r;
+          s = s * c;
+        }
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        for (int k = 0; k < n; k++) {
+          double zk1 = z[k * n + i + 1], zk = z[k * n + i];
+          z[k * n + i + 1] = s * zk + c * zk1;
+          z[k * n + i] = c * zk - s * zk1;
+        }
+      }
+      /* The loop always ends with i == l - 1, so the reference's early
+       * `continue` (r < tol && i >= l) can never fire. */
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    } while (1);
+  }
+  /* columns -> rows */
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++) {
+      double t = z[i * n + j];
+      z[i * n + j] = z[j * n + i];
+      z[j * n + i] = t;
+    }
+  for (int k = 0; k < n; k++) {
+    double nrm = 0.0;
+    for (int i = 0; i < n; i++) nrm += z[k * n + i] * z[k * n + i];
+    if (nrm > 0.0) nrm = sqrt(fabs(nrm));
+    double scale = 1.0 / nrm;
+    for (int i = 0; i < n; i++) z[k * n + i] *= scale;
+  }
+  for (int i = 0; i < n; i++) eval[i] = d[i];
+  free(d);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CG-Lanczos, fiedler.c:13-30 (dot, ortho) and 434-561. */
+static double dotp(const double *y, const double *x, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+static void ortho(double *q, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += q[i];
+  s /= n;
+  for (int i = 0; i < n; i++) q[i] -= s;
+}
+
+static int lanczos_pass(double *diag, double *upper, double *rr, int n, int niter,
+                        double tol, const double *f, const gs_lap *L, double *rn3) {
+  double *r = xmalloc(sizeof(double) * 3 * (size_t)n), *p = r + n, *w = p + n;
+  for (int i = 0; i < n; i++) r[i] = f[i], p[i] = 0.0;
+  ortho(r, n);
+  double rtz1 = 1.0, pap = 0.0, alpha, beta, rtz2, pap_old;
+  double rtr = dotp(r, r, n);
+  double rnorm = sqrt(rtr), rtol = rnorm * tol;
+  rn3[0] = rnorm, rn3[1] = rtol;
+  double rni = 1.0 / rnorm;
+  for (int i = 0; i < n; i++) rr[i] = r[i] * rni;
+  int iter;
+  for (iter = 0; iter < niter; iter++) {
+    rtz2 = rtz1, rtz1 = rtr;
+    beta = rtz1 / rtz2;
+    if (iter == 0) beta = 0.0;
+    for (int i = 0; i < n; i++) p[i] = beta * p[i] + r[i];
+    ortho(p, n);
+    gs_lap_apply(L, p, w);
+    pap_old = pap, pap = dotp(w, p, n);
+    alpha = rtz1 / pap;
+    for (int i = 0; i < n; i++) r[i] = r[i] - alpha * w[i];
+    rtr = dotp(r, r, n);
+    rnorm = sqrt(rtr), rni = 1.0 / rnorm;
+    for (int i = 0; i < n; i++) rr[(size_t)(iter + 1) * n + i] = r[i] * rni;
+    if (iter == 0) {
+      diag[iter] = pap / rtz1;
+    } else {
+      diag[iter] = (beta * beta * pap_old + pap) / rtz1;
+      upper[iter - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
+    }
+    if (rnorm < rtol) {
+      iter++;
+      break;
+    }
+  }
+  rn3[2] = rnorm;
+  free(r);
+  return iter;
+}
+
+static int lanczos_run(double *fied, const gs_lap *L, int n, double *initv, int miter,
+                       int mpass, double tol, double *lambda_out, double *rn3) {
+  if (n < miter) miter = n;
+  double *alpha = xmalloc(sizeof(double) * (size_t)(2 * miter)), *beta = alpha + miter;
+  double *rr = xmalloc(sizeof(double) * (size_t)(miter + 1) * n);
+  double *ev = xmalloc(sizeof(double) * (size_t)miter * miter);
+  double *el = xmalloc(sizeof(double) * (size_t)miter);
+  int iter = miter, ipass;
+  double lam = 0.0;
+  for (ipass = 0; iter == miter && ipass < mpass; ipass++) {
+    iter = lanczos_pass(alpha, beta, rr, n, miter, tol, initv, L, rn3);
+    oracle_tqli(alpha, beta, iter, ev, el);
+    double emin = fabs(el[0]);
+    int imin = 0;
+    for (int i = 1; i < iter; i++)
+      if (fabs(el[i]) < emin) emin = fabs(el[i]), imin = i;
+    lam = el[imin];
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      for (int j = 0; j < iter; j++) s += rr[(size_t)j * n + i] * ev[imin * iter + j];
+      fied[i] = s;
+    }
+    ortho(fied, n);
+    for (int i = 0; i < n; i++) initv[i] = fied[i];
+  }
+  free(alpha), free(rr), free(ev), free(el);
+  if (lambda_out) *lambda_out = lam;
+  return (ipass - 1) * miter + iter;
+}
+
+static int fiedler_common(const long long *vtx, int nv, const int *elems, int n,
+                          int max_iter, int max_passes, double tol, double *initv,
+                          double *fiedler, double *lambda_out, double *rnorm_out) {
+  double rn3[3] = {0, 0, 0};
+  ortho(initv, n);
+  double rtr = dotp(initv, initv, n), rni = 1.0 / sqrt(rtr);
+  for (int i = 0; i < n; i++) initv[i] *= rni;
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  int it = lanczos_run(fiedler, &L, n, initv, max_iter, max_passes, tol, lambda_out, rn3);
+  gs_lap_free(&L);
+  double nrm = 0.0;
+  for (int i = 0; i < n; i++) nrm += fiedler[i] * fiedler[i];
+  double ni = 1.0 / sqrt(nrm);
+  for (int i = 0; i < n; i++) fiedler[i] *= ni;
+  if (rnorm_out) rnorm_out[0] = rn3[0], rnorm_out[1] = rn3[1], rnorm_out[2] = rn3[2];
+  return it;
+}
+
+int oracle_fiedler(const long long *vtx, int nv, const int *elems, int n, int max_iter,
+                   int max_passes, double tol, double *fiedler, double *lambda_out,
+                   double *rnorm_out) {
+  /* Start vector, fiedler.c:572-578: global position + 1, plus 1000 n on
+   * the first half. */
+  double *initv = xmalloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    initv[i] = i + 1.0;
+    if (i < n / 2) initv[i] += (double)(1000 * (long long)n);
+  }
+  int it = fiedler_common(vtx, nv, elems, n, max_iter, max_passes, tol, initv, fiedler,
+                          lambda_out, rnorm_out);
+  free(initv);
+  return it;
+}
+
+int oracle_fiedler_from(const long long *vtx, int nv, const int *elems, int n,
+                        int max_iter, int max_passes, double tol, const double *init,
+                        double *fiedler, double *lambda_out, double *rnorm_out) {
+  double *initv = xmalloc(sizeof(double) * (size_t)n);
+  memcpy(initv, init, sizeof(double) * (size_t)n);
+  int it = fiedler_common(vtx, nv, elems, n, max_iter, max_passes, tol, initv, fiedler,
+                          lambda_out, rnorm_out);
+  free(initv);
+  return it;
+}
+
+int oracle_lanczos_pass(const long long *vtx, int nv, const int *elems, int n, int niter,
+                        double tol, const double *f, double *diag, double *upper,
+                        double *basis) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  double rn3[3];
+  int k = lanczos_pass(diag, upper, basis, n, niter, tol, f, &L, rn3);
+  gs_lap_free(&L);
+  return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Components by union-find over shared vertex ids. */
+static int uf_find(int *par, int x) {
+  while (par[x] != x) {
+    par[x] = par[par[x]];
+    x = par[x];
+  }
+  return x;
+}
+
+static void uf_union(int *par, int a, int b) {
+  a = uf_find(par, a), b = uf_find(par, b);
+  if (a == b) return;
+  if (a < b) par[b] = a;
+  else par[a] = b;
+}
+
+int oracle_components(const long long *vtx, int nv, const int *elems, int n) {
+  if (n == 0) return 0;
+  int *group = xmalloc(sizeof(int) * (size_t)n * nv), *gstart, *gslots;
+  int ng = group_slots(vtx, nv, elems, n, group, &gstart, &gslots);
+  int *par = xmalloc(sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; i++) par[i] = i;
+  for (int g = 0; g < ng; g++)
+    for (int k = gstart[g] + 1; k < gstart[g + 1]; k++)
+      uf_union(par, gslots[gstart[g]] / nv, gslots[k] / nv);
+  int nc = 0;
+  for (int i = 0; i < n; i++) nc += (uf_find(par, i) == i);
+  free(group), free(gstart), free(gslots), free(par);
+  return nc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interface flags for a bisected segment: flag[i] = 1 if element elems[i]
+ * of side `side` shares a vertex id with an element of the other side
+ * (rsb.c:145-165: gather-scatter add over the parent's ranks). */
+static void interface_flags(const long long *vtx, int nv, const int *elems, int n, int nl,
+                            int sender_is_lower, int *flag) {
+  int *group = xmalloc(sizeof(int) * (size_t)n * nv), *gstart, *gslots;
+  int ng = group_slots(vtx, nv, elems, n, group, &gstart, &gslots);
+  /* receivers touching each group */
+  int *recv = xmalloc(sizeof(int) * (size_t)ng);
+  for (int g = 0; g < ng; g++) {
+    int c = 0;
+    for (int k = gstart[g]; k < gstart[g + 1]; k++) {
+      int i = gslots[k] / nv;
+      int lower = i < nl;
+      if (lower != sender_is_lower) c++;
+    }
+    recv[g] = c;
+  }
+  for (int i = 0; i < n; i++) {
+    flag[i] = 0;
+    for (int j = 0; j < nv; j++)
+      if (recv[group[i * nv + j]] > 0) {
+        flag[i] = 1;
+        break;
+      }
+  }
+  free(group), free(gstart), free(gslots), free(recv);
+}
+
+/* Distinct other segments sharing a vertex id with each segment
+ * (statistics.c:17-113); returns the maximum over segments. */
+typedef struct {
+  long long vid;
+  int seg;
+} vseg_t;
+
+static int cmp_vseg(const void *a, const void *b) {
+  const vseg_t *x = a, *y = b;
+  if (x->vid != y->vid) return x->vid < y->vid ? -1 : 1;
+  return (x->seg > y->seg) - (x->seg < y->seg);
+}
+
+static int cmp_ll(const void *a, const void *b) {
+  long long x = *(const long long *)a, y = *(const long long *)b;
+  return (x > y) - (x < y);
+}
+
+static int max_neighbors(const long long *vtx, int nv, const int *segof, int nel, int nseg) {
+  size_t ns = (size_t)nel * nv;
+  vseg_t *a = xmalloc(sizeof(vseg_t) * ns);
+  for (size_t s = 0; s < ns; s++) a[s].vid = vtx[s], a[s].seg = segof[s / nv];
+  qsort(a, ns, sizeof(vseg_t), cmp_vseg);
+  size_t cap = 1024, np = 0;
+  long long *pairs = xmalloc(sizeof(long long) * cap);
+  for (size_t s = 0; s < ns;) {
+    size_t t = s;
+    while (t < ns && a[t].vid == a[s].vid && a[s].vid != 0) t++;
+    if (t == s) t = s + 1;
+    for (size_t i = s; i < t; i++)
+      for (size_t j = s; j < t; j++)
+        if (a[i].seg != a[j].seg) {
+          if (np == cap) cap *= 2, pairs = realloc(pairs, sizeof(long long) * cap);
+          pairs[np++] = (long long)a[i].seg * nseg + a[j].seg;
+        }
+    s = t;
+  }
+  qsort(pairs, np, sizeof(long long), cmp_ll);
+  int *cnt = calloc((size_t)nseg, sizeof(int));
+  for (size_t i = 0; i < np; i++)
+    if (i == 0 || pairs[i] != pairs[i - 1]) cnt[pairs[i] / nseg]++;
+  int mx = 0;
+  for (int s = 0; s < nseg; s++) mx = cnt[s] > mx ? cnt[s] : mx;
+  free(a), free(pairs), free(cnt);
+  return mx;
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int start, n, p, r0;
+} seg_t;
+
+int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel, int nv,
+                     const oracle_opts *o, oracle_report *rep) {
+  int P = o->nparts;
+  if (P < 1 || nel < 0) return 3;
+  int ndim = (nv == 8) ? 3 : 2;
+  double *coord = xmalloc(sizeof(double) * 3 * (size_t)(nel ? nel : 1));
+  int rsb_pre = o->rsb_pre;
+  if (xyz) oracle_centroids(xyz, nel, nv, coord);
+  else rsb_pre = 0;
+  int *order = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
+  for (int i = 0; i < nel; i++) order[i] = i;
+  seg_t *segs = xmalloc(sizeof(seg_t) * (size_t)P), *nsegs = xmalloc(sizeof(seg_t) * (size_t)P);
+  int nseg = 1;
+  segs[0].start = 0, segs[0].n = nel, segs[0].p = P, segs[0].r0 = 0;
+  int ncuts = 0;
+  while ((1 << ncuts) < P) ncuts++;
+  if (rep) memset(rep, 0, sizeof(*rep)), rep->nlevels = ncuts;
+  double *fv = xmalloc(sizeof(double) * (size_t)(nel ? nel : 1));
+  int *flag = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
+  int *tmp = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
+  int *segof = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
+
+  for (int cut = 0; cut < ncuts; cut++) {
+    int nn = 0;
+    int itmin = 1 << 30, itmax = 0, cmin = 1 << 30, cmax = 0, nbal = 0, ncut = 0;
+    double lmin = DBL_MAX, lmax = -DBL_MAX;
+    for (int s = 0; s < nseg; s++) {
+      seg_t sg = segs[s];
+      if (sg.p == 1) {
+        nsegs[nn++] = sg;
+        continue;
+      }
+      ncut++;
+      int *el = order + sg.start, n = sg.n, p = sg.p;
+      /* pre-partition (rsb.c:282-291) */
+      if (rsb_pre == 1) {
+        oracle_rcb_order(coord, ndim, el, n, p);
+      } else {
+        for (int i = 0; i < n; i++) fv[i] = (double)el[i];
+        stable_sort_by_pos_key(el, fv, n);
+      }
+      /* Fiedler (fiedler.c:563-621) */
+      int it = 0;
+      double lam = 0.0;
+      if (n >= 2) {
+        it = oracle_fiedler(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol, fv, &lam,
+                            NULL);
+      } else {
+        for (int i = 0; i < n; i++) fv[i] = 0.0;
+      }
+      itmin = it < itmin ? it : itmin, itmax = it > itmax ? it : itmax;
+      lmin = lam < lmin ? lam : lmin, lmax = lam > lmax ? lam : lmax;
+      /* sort by Fiedler value (rsb.c:308) */
+      stable_sort_by_pos_key(el, fv, n);
+      int nl = (int)oracle_lower_count(n, p);
+      /* components of each half before balancing (rsb.c:320-327) */
+      if (o->find_disconnected_comps) {
+        int c0 = oracle_components(vtx, nv, el, nl);
+        int c1 = oracle_components(vtx, nv, el + nl, n - nl);
+        int c = c0 < c1 ? c0 : c1, C = c0 > c1 ? c0 : c1;
+        cmin = c < cmin ? c : cmin, cmax = C > cmax ? C : cmax;
+      }
+      /* balance (rsb.c:118-229) */
+      int64_t expl = oracle_balance_expected(n, p, 0), expu = oracle_balance_expected(n, p, 1);
+      int sender_lower = -1;
+      int64_t scnt = 0;
+      if (nl > expl) sender_lower = 1, scnt = nl - expl;
+      else if (n - nl > expu) sender_lower = 0, scnt = (n - nl) - expu;
+      if (scnt > 0) {
+        interface_flags(vtx, nv, el, n, nl, sender_lower, flag);
+        int lo = sender_lower ? 0 : nl, hi = sender_lower ? nl : n;
+        int nif = 0;
+        for (int i = lo; i < hi; i++) nif += flag[i];
+        if (nif >= scnt) {
+          /* Candidates ordered by mul*fiedler, ties by position (rsb.c:168-187):
+           * the upper side offers its smallest values, the lower side its
+           * largest. */
+          int *cand = xmalloc(sizeof(int) * (size_t)nif);
+          double *ck = xmalloc(sizeof(double) * (size_t)nif);
+          int k = 0;
+          for (int i = lo; i < hi; i++)
+            if (flag[i]) cand[k] = i, ck[k] = sender_lower ? -fv[i] : fv[i], k++;
+          stable_sort_by_pos_key(cand, ck, nif);
+          /* moving: mark membership (1 = lower) */
+          for (int i = 0; i < n; i++) tmp[i] = i < nl;
+          for (int j = 0; j < scnt; j++) tmp[cand[j]] = !sender_lower;
+          /* stable partition in the current (sorted) order */
+          int *o2 = xmalloc(sizeof(int) * (size_t)n);
+          int a = 0;
+          for (int i = 0; i < n; i++)
+            if (tmp[i]) o2[a++] = el[i];
+          int nl2 = a;
+          for (int i = 0; i < n; i++)
+            if (!tmp[i]) o2[a++] = el[i];
+          memcpy(el, o2, sizeof(int) * (size_t)n);
+          free(o2), free(cand), free(ck);
+          nl = nl2;
+          nbal++;
+        }
+      }
+      nsegs[nn].start = sg.start, nsegs[nn].n = nl, nsegs[nn].p = (p + 1) / 2;
+      nsegs[nn].r0 = sg.r0, nn++;
+      nsegs[nn].start = sg.start + nl, nsegs[nn].n = n - nl, nsegs[nn].p = p / 2;
+      nsegs[nn].r0 = sg.r0 + (p + 1) / 2, nn++;
+    }
+    seg_t *t = segs;
+    segs = nsegs, nsegs = t, nseg = nn;
+    if (rep && cut < ORACLE_MAX_LEVELS) {
+      rep->niter_min[cut] = ncut ? itmin : 0, rep->niter_max[cut] = itmax;
+      rep->ncomp_min[cut] = o->find_disconnected_comps ? cmin : 0;
+      rep->ncomp_max[cut] = cmax;
+      rep->nsegs[cut] = ncut;
+      rep->nbalanced[cut] = nbal;
+      rep->lambda_min[cut] = ncut ? lmin : 0, rep->lambda_max[cut] = ncut ? lmax : 0;
+      for (int s = 0; s < nseg; s++)
+        for (int i = 0; i < segs[s].n; i++) segof[order[segs[s].start + i]] = s;
+      rep->nbrs_max[cut] = max_neighbors(vtx, nv, segof, nel, nseg);
+    }
+  }
+  for (int s = 0; s < nseg; s++)
+    for (int i = 0; i < segs[s].n; i++) part[order[segs[s].start + i]] = segs[s].r0;
+  free(coord), free(order), free(segs), free(nsegs), free(fv), free(flag), free(tmp),
+      free(segof);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  long long vid;
+  int part;
+} vp_t;
+
+static int cmp_vp(const void *a, const void *b) {
+  const vp_t *x = a, *y = b;
+  if (x->vid != y->vid) return x->vid < y->vid ? -1 : 1;
+  return (x->part > y->part) - (x->part < y->part);
+}
+
+void oracle_quality(const int *part, const long long *vtx, int nel, int nv, int nparts,
+                    int64_t *out) {
+  int64_t *rowptr;
+  int *col, *wgt;
+  oracle_dual_graph(vtx, nel, nv, &rowptr, &col, &wgt);
+  int64_t wcut = 0, ucut = 0;
+  int *par = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
+  for (int i = 0; i < nel; i++) par[i] = i;
+  for (int i = 0; i < nel; i++)
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) {
+      int j = col[k];
+      if (part[i] != part[j]) {
+        if (j > i) wcut += wgt[k], ucut++;
+      } else {
+        uf_union(par, i, j);
+      }
+    }
+  int64_t *sz = calloc((size_t)nparts, sizeof(int64_t));
+  int64_t *nc = calloc((size_t)nparts, sizeof(int64_t));
+  for (int i = 0; i < nel; i++) {
+    sz[part[i]]++;
+    if (uf_find(par, i) == i) nc[part[i]]++;
+  }
+  int64_t mx = 0, mn = INT64_MAX, tot = 0, cmx = 0, empty = 0;
+  for (int q = 0; q < nparts; q++) {
+    mx = sz[q] > mx ? sz[q] : mx;
+    mn = sz[q] < mn ? sz[q] : mn;
+    tot += nc[q];
+    cmx = nc[q] > cmx ? nc[q] : cmx;
+    empty += sz[q] == 0;
+  }
+  size_t ns = (size_t)nel * nv;
+  vp_t *a = xmalloc(sizeof(vp_t) * (ns ? ns : 1));
+  for (size_t s = 0; s < ns; s++) a[s].vid = vtx[s], a[s].part = part[s / nv];
+  qsort(a, ns, sizeof(vp_t), cmp_vp);
+  int64_t vol = 0;
+  for (size_t s = 0; s < ns;) {
+    size_t t = s;
+    int64_t k = 0;
+    while (t < ns && a[t].vid == a[s].vid) {
+      if (t == s || a[t].part != a[t - 1].part) k++;
+      t++;
+    }
+    if (a[s].vid != 0) vol += k * (k - 1);
+    s = t;
+  }
+  out[0] = wcut, out[1] = ucut, out[2] = mx, out[3] = mn, out[4] = tot, out[5] = cmx;
+  out[6] = vol, out[7] = empty;
+  free(rowptr), free(col), free(wgt), free(par), free(sz), free(nc), free(a);
+}

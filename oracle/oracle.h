@@ -1,0 +1,135 @@
+/*
+ * CPU oracle: a plain-C, FP64 restatement of the reference's recursive
+ * spectral bisection path, used ONLY as the checker by tests/, by the smoke
+ * check and by the CPU-baseline leg of bench.py.  The product never links,
+ * loads or calls anything here.
+ *
+ * The reference assigns one part per MPI rank (rsb.c:244-260).  The oracle
+ * emulates P "virtual ranks" inside one process: a segment is a contiguous
+ * range of an element order plus the range of virtual ranks it is spread
+ * over.  Every distributed step of the reference is restated on that
+ * representation:
+ *   - element distribution after a global sort       sort.c:48-70, 106-117
+ *   - RCB order with virtual-rank split points        rcb.c:7-137
+ *   - Lanczos start vector from the global position   fiedler.c:563-585
+ *   - CG-Lanczos, TQLI, Ritz restarts                 fiedler.c:13-30, 323-561
+ *   - shared-vertex Laplacian (gather-scatter form)   laplacian.c:159-205
+ *   - component counts of each half                   components.c:160-328
+ *   - exact-count balance of the two halves           rsb.c:118-229
+ *   - neighbour-part count per part                   statistics.c:17-113
+ * Ties in every sort are broken by the current order (stable sorts).  The
+ * reference breaks ties between ranks by crystal-router arrival order, which
+ * no fixture pins; the stable rule is the convention both the oracle and the
+ * device path follow.
+ */
+#ifndef PARRSB_ORACLE_H
+#define PARRSB_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int find_disconnected_comps;
+  int rsb_pre; /* 0 none (sort by global id), 1 RCB */
+  int max_iter, max_passes;
+  double tol;
+  int nparts;
+} oracle_opts;
+
+/* Per-cut (level) statistics, as the reference's metric stack records them
+ * (metrics.c:40-48): min/max over the segments that were cut at that level. */
+#define ORACLE_MAX_LEVELS 32
+typedef struct {
+  int nlevels;
+  int niter_min[ORACLE_MAX_LEVELS], niter_max[ORACLE_MAX_LEVELS];
+  int ncomp_min[ORACLE_MAX_LEVELS], ncomp_max[ORACLE_MAX_LEVELS];
+  int nbrs_max[ORACLE_MAX_LEVELS];
+  int nsegs[ORACLE_MAX_LEVELS];
+  int nbalanced[ORACLE_MAX_LEVELS]; /* segments where interface elements moved */
+  double lambda_min[ORACLE_MAX_LEVELS], lambda_max[ORACLE_MAX_LEVELS];
+} oracle_report;
+
+/* Centroids: FP64 mean of the nv corners, summed in corner order
+ * (parrsb.c:139-145).  coord is nel x 3; coord[e*3+2] = 0 in 2D. */
+void oracle_centroids(const double *xyz, int nel, int nv, double *coord);
+
+/* Number of elements on virtual rank r when n elements are spread over p
+ * ranks after a global sort: the first p - (n mod p) ranks get floor(n/p),
+ * the others one more (sort.c:48-70). */
+int64_t oracle_rank_count(int64_t n, int p, int r);
+/* Elements on the lower ceil(p/2) ranks. */
+int64_t oracle_lower_count(int64_t n, int p);
+/* Expected size of the lower (bin 0) / upper (bin 1) half after balancing
+ * (rsb.c:138-140). */
+int64_t oracle_balance_expected(int64_t n, int p, int bin);
+
+/* RCB order of a segment (rcb.c:116-137): `order` holds n element indices in
+ * the current order and is permuted in place into the RCB order for p
+ * virtual ranks.  coord is indexed by element (x3). */
+void oracle_rcb_order(const double *coord, int ndim, int *order, int n, int p);
+
+/* Weighted dual graph of the whole mesh: w_ij = number of shared vertex ids
+ * (mat.c:17-61, 203-249).  CSR with columns sorted ascending, no diagonal.
+ * Arrays are malloc'ed; free with oracle_free. */
+int64_t oracle_dual_graph(const long long *vtx, int nel, int nv, int64_t **rowptr,
+                          int **col, int **wgt);
+void oracle_free(void *p);
+
+/* Apply the shared-vertex Laplacian of the sub-mesh `elems[0..n)` (the
+ * gather-scatter form of laplacian.c:159-205) to u, writing v. */
+void oracle_laplacian_apply(const long long *vtx, int nv, const int *elems, int n,
+                            const double *u, double *v);
+
+/* Implicit QL on a symmetric tridiagonal (fiedler.c:328-432).  evec is n x n,
+ * row k = eigenvector k (normalised), eval[n].  Returns 0, or 1 after 30
+ * sweeps without deflation (then evec is left as the reference leaves it). */
+int oracle_tqli(const double *diag, const double *upper, int n, double *evec,
+                double *eval);
+
+/* Fiedler vector of the sub-mesh elems[0..n) (given in pre-partition order)
+ * by the reference's CG-Lanczos with restarts (fiedler.c:513-621).
+ * fiedler[n] is normalised.  Returns the iteration count the reference
+ * records (RSB_FIEDLER_CALC_NITER).  lambda_out (may be NULL) receives the
+ * Ritz value of the last pass; rnorm_out (may be NULL) receives
+ * {initial, target, final} residual norms of the last pass. */
+int oracle_fiedler(const long long *vtx, int nv, const int *elems, int n,
+                   int max_iter, int max_passes, double tol, double *fiedler,
+                   double *lambda_out, double *rnorm_out);
+
+/* Same Lanczos, from a caller-supplied start vector (length n, need not be
+ * normalised or orthogonal to 1). */
+int oracle_fiedler_from(const long long *vtx, int nv, const int *elems, int n,
+                        int max_iter, int max_passes, double tol,
+                        const double *initv, double *fiedler, double *lambda_out,
+                        double *rnorm_out);
+
+/* One CG-Lanczos pass (fiedler.c:434-511) from start vector f (length n):
+ * writes diag[k], upper[k-1] and the normalised residual basis
+ * basis[(k+1) x n]; returns k, the number of iterations taken. */
+int oracle_lanczos_pass(const long long *vtx, int nv, const int *elems, int n, int niter,
+                        double tol, const double *f, double *diag, double *upper,
+                        double *basis);
+
+/* Connected components of the sub-mesh elems[0..n) under shared-vertex
+ * adjacency (components.c:160-328 returns the same count). */
+int oracle_components(const long long *vtx, int nv, const int *elems, int n);
+
+/* Full partitioner: part[nel] in [0, nparts).  Returns 0 on success. */
+int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel,
+                     int nv, const oracle_opts *opts, oracle_report *rep);
+
+/* Partition quality.  out[0] weighted edge cut (sum over undirected edges of
+ * shared-vertex weight), out[1] unweighted cut edges, out[2] max part size,
+ * out[3] min part size, out[4] total components over parts, out[5] max
+ * components in one part, out[6] comm volume sum_v k_v (k_v - 1), out[7]
+ * number of empty parts. */
+void oracle_quality(const int *part, const long long *vtx, int nel, int nv,
+                    int nparts, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,220 @@
+"""ctypes bindings of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, by the smoke check and by the CPU-baseline leg of bench.py
+-- never by the product.  Builds oracle/liboracle.so with `make` on first use
+if it is missing."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+class Opts(C.Structure):
+    _fields_ = [("find_disconnected_comps", C.c_int), ("rsb_pre", C.c_int),
+                ("max_iter", C.c_int), ("max_passes", C.c_int), ("tol", C.c_double),
+                ("nparts", C.c_int)]
+
+
+ML = 32
+
+
+class Report(C.Structure):
+    _fields_ = [("nlevels", C.c_int),
+                ("niter_min", C.c_int * ML), ("niter_max", C.c_int * ML),
+                ("ncomp_min", C.c_int * ML), ("ncomp_max", C.c_int * ML),
+                ("nbrs_max", C.c_int * ML), ("nsegs", C.c_int * ML),
+                ("nbalanced", C.c_int * ML),
+                ("lambda_min", C.c_double * ML), ("lambda_max", C.c_double * ML)]
+
+    def as_dict(self):
+        n = self.nlevels
+        return {f: list(getattr(self, f))[:n] for f, _ in self._fields_ if f != "nlevels"}
+
+
+P = C.c_void_p
+
+
+def _declare(L):
+    L.oracle_centroids.argtypes = [P, C.c_int, C.c_int, P]
+    L.oracle_rank_count.argtypes = [C.c_int64, C.c_int, C.c_int]
+    L.oracle_rank_count.restype = C.c_int64
+    L.oracle_lower_count.argtypes = [C.c_int64, C.c_int]
+    L.oracle_lower_count.restype = C.c_int64
+    L.oracle_balance_expected.argtypes = [C.c_int64, C.c_int, C.c_int]
+    L.oracle_balance_expected.restype = C.c_int64
+    L.oracle_rcb_order.argtypes = [P, C.c_int, P, C.c_int, C.c_int]
+    L.oracle_dual_graph.argtypes = [P, C.c_int, C.c_int, C.POINTER(P), C.POINTER(P),
+                                    C.POINTER(P)]
+    L.oracle_dual_graph.restype = C.c_int64
+    L.oracle_free.argtypes = [P]
+    L.oracle_laplacian_apply.argtypes = [P, C.c_int, P, C.c_int, P, P]
+    L.oracle_tqli.argtypes = [P, P, C.c_int, P, P]
+    L.oracle_tqli.restype = C.c_int
+    L.oracle_fiedler.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double, P,
+                                 P, P]
+    L.oracle_fiedler.restype = C.c_int
+    L.oracle_fiedler_from.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, P, P, P, P]
+    L.oracle_fiedler_from.restype = C.c_int
+    L.oracle_lanczos_pass.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_double, P, P,
+                                      P, P]
+    L.oracle_lanczos_pass.restype = C.c_int
+    L.oracle_components.argtypes = [P, C.c_int, P, C.c_int]
+    L.oracle_components.restype = C.c_int
+    L.oracle_part_mesh.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(Opts),
+                                   C.POINTER(Report)]
+    L.oracle_part_mesh.restype = C.c_int
+    L.oracle_quality.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def centroids(xyz, nv):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nel = xyz.shape[0]
+    out = np.zeros((nel, 3), dtype=np.float64)
+    lib().oracle_centroids(_p(xyz), nel, nv, _p(out))
+    return out
+
+
+def rank_count(n, p, r):
+    return lib().oracle_rank_count(n, p, r)
+
+
+def lower_count(n, p):
+    return lib().oracle_lower_count(n, p)
+
+
+def balance_expected(n, p, b):
+    return lib().oracle_balance_expected(n, p, b)
+
+
+def rcb_order(coord, ndim, order, p):
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    order = np.array(order, dtype=np.int32, copy=True)
+    lib().oracle_rcb_order(_p(coord), ndim, _p(order), order.size, p)
+    return order
+
+
+def dual_graph(vtx, nv):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    nel = vtx.size // nv
+    rp, cp, wp = P(), P(), P()
+    nnz = lib().oracle_dual_graph(_p(vtx), nel, nv, C.byref(rp), C.byref(cp), C.byref(wp))
+    rowptr = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_int64)), (nel + 1,)).copy()
+    col = np.ctypeslib.as_array(C.cast(cp, C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy()
+    w = np.ctypeslib.as_array(C.cast(wp, C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy()
+    for x in (rp, cp, wp):
+        lib().oracle_free(x)
+    return rowptr, col, w
+
+
+def laplacian_apply(vtx, nv, elems, u):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    v = np.zeros_like(u)
+    lib().oracle_laplacian_apply(_p(vtx), nv, _p(elems), elems.size, _p(u), _p(v))
+    return v
+
+
+def tqli(diag, upper):
+    d = np.ascontiguousarray(diag, dtype=np.float64)
+    e = np.ascontiguousarray(upper, dtype=np.float64)
+    n = d.size
+    if e.size == 0:
+        e = np.zeros(1)
+    z = np.zeros((n, n), dtype=np.float64)
+    ev = np.zeros(n, dtype=np.float64)
+    rc = lib().oracle_tqli(_p(d), _p(e), n, _p(z), _p(ev))
+    return rc, ev, z
+
+
+def fiedler(vtx, nv, elems, max_iter=50, max_passes=50, tol=1e-5, init=None):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    n = elems.size
+    f = np.zeros(n, dtype=np.float64)
+    lam = C.c_double()
+    rn = np.zeros(3, dtype=np.float64)
+    if init is None:
+        it = lib().oracle_fiedler(_p(vtx), nv, _p(elems), n, max_iter, max_passes, tol,
+                                  _p(f), C.byref(lam), _p(rn))
+    else:
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        it = lib().oracle_fiedler_from(_p(vtx), nv, _p(elems), n, max_iter, max_passes,
+                                       tol, _p(init), _p(f), C.byref(lam), _p(rn))
+    return f, it, lam.value, rn
+
+
+def lanczos_pass(vtx, nv, elems, f, niter, tol=1e-5):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = elems.size
+    d = np.zeros(niter)
+    e = np.zeros(max(niter, 1))
+    basis = np.zeros((niter + 1, n))
+    k = lib().oracle_lanczos_pass(_p(vtx), nv, _p(elems), n, niter, tol, _p(f), _p(d),
+                                  _p(e), _p(basis))
+    return k, d[:k], e[:max(k - 1, 0)], basis[:k + 1]
+
+
+def components(vtx, nv, elems):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    return lib().oracle_components(_p(vtx), nv, _p(elems), elems.size)
+
+
+def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
+              max_passes=50, tol=1e-5):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    nel = vtx.size // nv
+    xp = None
+    if xyz is not None:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        xp = _p(xyz)
+    part = np.full(nel, -1, dtype=np.int32)
+    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts)
+    rep = Report()
+    rc = lib().oracle_part_mesh(_p(part), _p(vtx), xp, nel, nv, C.byref(o), C.byref(rep))
+    if rc != 0:
+        raise RuntimeError(f"oracle_part_mesh returned {rc}")
+    return part, rep.as_dict()
+
+
+QUALITY_KEYS = ("edge_cut", "cut_edges", "max_part", "min_part", "components",
+                "max_components", "comm_volume", "empty_parts")
+
+
+def quality(part, vtx, nv, nparts):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    out = np.zeros(8, dtype=np.int64)
+    lib().oracle_quality(_p(part), _p(vtx), part.size, nv, nparts, _p(out))
+    return dict(zip(QUALITY_KEYS, (int(x) for x in out)))
