@@ -1,0 +1,80 @@
+/*
+ * parrsb_amd.h -- entry points of the MI355X build beyond the drop-in
+ * parrsb_part_mesh (parRSB.h).  None of them has a counterpart in the
+ * reference; they serve callers whose mesh already lives in GPU memory,
+ * benchmarks and diagnostics.
+ */
+#ifndef PARRSB_AMD_H
+#define PARRSB_AMD_H
+
+#include <stdint.h>
+
+#include "parRSB.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PARRSB_AMD_MAX_LEVELS 32
+
+/* Report of the last partitioning call (per bisection level = "cut", as the
+ * reference's metric stack records them, metrics.c:40-48 and 68-106). */
+typedef struct {
+  int nlevels, nparts, nel, precision;
+  double seconds;             /* wall time from entry to return */
+  double phase_ms[8];         /* setup, pre-partition, Laplacian, Lanczos,
+                                 TQLI+Ritz, sort, components, balance */
+  double lanczos_bytes;       /* algorithmic bytes of all CG-Lanczos iterations */
+  double lanczos_ms;          /* time of the iteration loops */
+  double lanczos_iters;       /* iterations launched (3 kernels each) */
+  double spmv_bytes;          /* algorithmic bytes of the SpMV kernel */
+  double spmv_ms;             /* time inside SpMV launches (profiling on) */
+  double spmv_launches;
+  int64_t graph_nnz, graph_wsum; /* dual graph: off-diagonals, sum of weights */
+  int nsegs[PARRSB_AMD_MAX_LEVELS];      /* segments bisected at the level */
+  int niter_min[PARRSB_AMD_MAX_LEVELS];  /* RSB_FIEDLER_CALC_NITER */
+  int niter_max[PARRSB_AMD_MAX_LEVELS];
+  int npass_max[PARRSB_AMD_MAX_LEVELS];
+  int ncomp_min[PARRSB_AMD_MAX_LEVELS];  /* RSB_COMPONENTS_NCOMP */
+  int ncomp_max[PARRSB_AMD_MAX_LEVELS];
+  int nbalanced[PARRSB_AMD_MAX_LEVELS];  /* segments where balancing moved elements */
+  int nconverged_fail[PARRSB_AMD_MAX_LEVELS]; /* segments at max_iter*max_passes */
+  double lambda_min[PARRSB_AMD_MAX_LEVELS], lambda_max[PARRSB_AMD_MAX_LEVELS];
+  double resid_max[PARRSB_AMD_MAX_LEVELS]; /* max ||Lf - lambda f|| / lambda */
+} parrsb_amd_report;
+
+/* Same contract as parrsb_part_mesh, with every array in GPU memory of the
+ * selected device (PARRSB_AMD_DEVICE, default 0): d_part[nel] out,
+ * d_vtx[nel*nv], d_xyz[nel*nv*ndim] or NULL.  No MPI: one caller, one GPU;
+ * the number of parts is options->nparts (>= 1). */
+int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *d_xyz,
+                             int nel, int nv, parrsb_options *options);
+
+/* Copy the report of the last call made in this process. */
+int parrsb_amd_last_report(parrsb_amd_report *rep);
+
+/* Message of the last failing call ("" if none). */
+const char *parrsb_amd_last_error(void);
+
+/* Lanczos vector precision for later calls: 64 (reference, default) or 32
+ * (FP32 vectors, compensated dot products, FP64 scalars and eigen-solve).
+ * Overridden by PARRSB_AMD_PRECISION. */
+int parrsb_amd_set_precision(int bits);
+
+/* 1 = time every SpMV launch with events (roofline measurements). */
+int parrsb_amd_set_profiling(int on);
+
+/* Quality of a partition (host arrays): out[0] weighted edge cut, out[1] cut
+ * edges, out[2] max part size, out[3] min part size, out[4] components over
+ * all parts, out[5] max components of a part, out[6] comm volume
+ * sum_v k_v (k_v - 1), out[7] empty parts.  Computed on the GPU. */
+int parrsb_amd_partition_quality(const int *part, const long long *vtx, int nel, int nv,
+                                 int nparts, int64_t *out);
+/* Same, device arrays. */
+int parrsb_amd_partition_quality_dev(const int *d_part, const long long *d_vtx, int nel,
+                                     int nv, int nparts, int64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+/*
+ * rsbgpu.h -- the thin C ABI between the C host driver and the HIP device
+ * layer.  Plain pointers and sizes only; the device state lives behind the
+ * opaque `rsbgpu` handle.  Every function returns 0 on success and a
+ * non-zero code on failure (the message is in rsbgpu_error()).
+ *
+ * Data model (one handle = one partitioning call on one GPU):
+ *   elements      e in [0, nel), centroids and vertex ids on the device
+ *   order         the current element order (a permutation of [0, nel));
+ *                 segment s is the contiguous range
+ *                 [cstart[s], cstart[s] + n[s]) of it, spread over the
+ *                 virtual ranks [r0[s], r0[s] + p[s])
+ *   rows          the Lanczos vectors of a level: segment s owns the padded
+ *                 row range [pstart[s], pstart[s] + ceil64(n[s]))
+ */
+#ifndef PARRSB_AMD_RSBGPU_H
+#define PARRSB_AMD_RSBGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rsbgpu rsbgpu;
+
+/* Per-level results the host reads back (arrays of length nseg). */
+typedef struct {
+  int *niter;       /* Lanczos iterations (RSB_FIEDLER_CALC_NITER) */
+  int *npass;       /* Lanczos passes */
+  double *lambda;   /* Ritz value of the last pass */
+  double *rnorm;    /* 3 per segment: initial, target, final CG residual */
+  double *resid;    /* relative eigen-residual ||Lf - lambda f|| / lambda */
+} rsbgpu_fiedler_out;
+
+int rsbgpu_open(rsbgpu **g, int device);
+void rsbgpu_close(rsbgpu *g);
+const char *rsbgpu_error(const rsbgpu *g);
+/* Milliseconds spent in each phase so far (events on the device stream):
+ * t[0] setup (copies, centroids, ranks, graph), t[1] pre-partition,
+ * t[2] Laplacian assembly, t[3] Lanczos, t[4] TQLI + Ritz, t[5] sort,
+ * t[6] components, t[7] balance + split. */
+int rsbgpu_phase_times(rsbgpu *g, double *t8);
+/* Lanczos traffic accumulated over the handle's life:
+ *   v6[0] algorithmic bytes of the CG-Lanczos iterations, v6[1] ms of the
+ *   iteration loops, v6[2] iterations launched;
+ *   v6[3] algorithmic bytes of the SpMV kernel alone, v6[4] ms inside SpMV
+ *   launches (only with profiling on), v6[5] SpMV launches timed. */
+int rsbgpu_lanczos_traffic(rsbgpu *g, double *v6);
+/* 1 = bracket every SpMV launch with events (for the roofline figure). */
+int rsbgpu_set_profiling(rsbgpu *g, int spmv_events);
+/* Parity hook: tridiagonal eigen-solve of one (d, e) on the device; y gets
+ * the coefficients of the eigenvector of smallest |eigenvalue|. */
+int rsbgpu_test_tqli(const double *d, const double *e, int n, double *y, double *lam);
+
+/* Load the mesh.  If on_device != 0 the pointers are device pointers that
+ * stay valid for the lifetime of the handle; else they are copied.  Computes
+ * centroids (if xyz != NULL), per-axis coordinate ranks and the weighted
+ * dual graph.  Sets the order to the identity. */
+int rsbgpu_set_mesh(rsbgpu *g, const long long *vtx, const double *xyz, int nel, int nv,
+                    int on_device);
+/* Graph statistics: number of directed off-diagonal entries and sum of
+ * weights. */
+int rsbgpu_graph_stats(rsbgpu *g, int64_t *nnz, int64_t *wsum);
+
+/* Install the segment table of the current level (host arrays). */
+int rsbgpu_set_segments(rsbgpu *g, int nseg, const int *cstart, const int *n, const int *p,
+                        const int *r0);
+
+/* Pre-partition every segment with p > 1: 1 = RCB (rcb.c), 0 = sort by
+ * global id (rsb.c:284-287), 2 = RIB (rib.c, deterministic axis). */
+int rsbgpu_prepartition(rsbgpu *g, int rsb_pre);
+
+/* Assemble the shared-vertex Laplacian of every segment with p > 1 (cut
+ * edges dropped) in padded row space. */
+int rsbgpu_level_laplacian(rsbgpu *g);
+
+/* Fiedler vectors of every segment with p > 1.  precision: 64 or 32.
+ * init_mode: 0 = the reference's start vector from the pre-partition
+ * position; 1 = a start vector set by rsbgpu_set_start (per element).
+ * The result is held on the device per compact position. */
+int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_passes, double tol,
+                   int init_mode, rsbgpu_fiedler_out *out);
+int rsbgpu_set_start(rsbgpu *g, const float *d_start_per_element);
+
+/* Stable sort of every segment with p > 1 by its Fiedler values. */
+int rsbgpu_fiedler_sort(rsbgpu *g);
+
+/* Bisect every segment with p > 1 after the Fiedler sort.
+ *   nlower[s]   in: size of the lower half after the sort;
+ *               out: size after balancing
+ *   sender[s]   -1 none, 0 lower half sends, 1 upper half sends
+ *   scount[s]   number of interface elements to move
+ *   find_comps  count components of both halves before balancing
+ *   ncomp[2s], ncomp[2s+1]  out: components of lower / upper half
+ *   moved[s]    out: 1 if elements were moved */
+int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const int *scount,
+                  int find_comps, int *ncomp, int *moved);
+
+/* part[e] = r0 of the segment holding e.  part_is_device selects whether
+ * `part` is a device pointer. */
+int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device);
+
+/* Copy the current order / Fiedler values (per compact position) to the
+ * host; used by the parity tests. */
+int rsbgpu_get_order(rsbgpu *g, int *order_host);
+int rsbgpu_get_fiedler(rsbgpu *g, double *fiedler_host);
+/* Copy the assembled level Laplacian in padded-row CSR form (host arrays,
+ * row-major, columns ascending): rowptr[mpad+1], col[], wgt[].  Returns the
+ * padded row count and nnz through the pointers when the arrays are NULL. */
+int rsbgpu_get_level_csr(rsbgpu *g, int64_t *mpad, int64_t *nnz, int64_t *rowptr, int *col,
+                         int *wgt, int *row_elem);
+int rsbgpu_get_graph(rsbgpu *g, int64_t *rowptr, int *col, int *wgt);
+int rsbgpu_get_centroids(rsbgpu *g, double *coord);
+
+/* Quality of a partition held on the device (part[] device pointer):
+ * out[0] weighted edge cut, out[1] cut edges, out[2] max part, out[3] min
+ * part, out[4] components over all parts, out[5] max components in a part,
+ * out[6] comm volume, out[7] empty parts. */
+int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t *out);
+
+/* Small helpers for callers holding host data: copy n ints to a new device
+ * buffer (freed with rsbgpu_free). */
+int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out);
+void rsbgpu_free(void *dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,341 @@
+"""parrsb_amd -- recursive spectral bisection on one MI355X.
+
+Python view of the C ABI in include/ (parRSB.h, parrsb_amd.h, rsbgpu.h),
+loaded from the in-tree libparrsb_amd.so.  There is no fallback: without the
+built library, or without a GPU, every call raises.
+
+    part = parrsb_amd.part_mesh(vtx, xyz, nv, nparts)     # parrsb_part_mesh
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libparrsb_amd.so")
+
+_lib = None
+
+
+class ParrsbError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"parrsb returned {rc}: {msg}")
+        self.rc = rc
+
+
+class Options(C.Structure):
+    """parrsb_options (parRSB.h) with the appended `nparts` field."""
+    _fields_ = [("partitioner", C.c_int), ("tagged", C.c_int), ("levels", C.c_int),
+                ("find_disconnected_comps", C.c_int), ("repair", C.c_int),
+                ("verbose_level", C.c_int), ("profile_level", C.c_int),
+                ("rsb_algo", C.c_int), ("rsb_pre", C.c_int), ("rsb_max_iter", C.c_int),
+                ("rsb_max_passes", C.c_int), ("rsb_tol", C.c_double),
+                ("rsb_dump_stats", C.c_int), ("rsb_mg_grammian", C.c_int),
+                ("rsb_mg_factor", C.c_int), ("nparts", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+ML = 32
+
+
+class Report(C.Structure):
+    _fields_ = [("nlevels", C.c_int), ("nparts", C.c_int), ("nel", C.c_int),
+                ("precision", C.c_int), ("seconds", C.c_double),
+                ("phase_ms", C.c_double * 8), ("lanczos_bytes", C.c_double),
+                ("lanczos_ms", C.c_double), ("lanczos_iters", C.c_double),
+                ("spmv_bytes", C.c_double), ("spmv_ms", C.c_double),
+                ("spmv_launches", C.c_double), ("graph_nnz", C.c_int64),
+                ("graph_wsum", C.c_int64), ("nsegs", C.c_int * ML),
+                ("niter_min", C.c_int * ML), ("niter_max", C.c_int * ML),
+                ("npass_max", C.c_int * ML), ("ncomp_min", C.c_int * ML),
+                ("ncomp_max", C.c_int * ML), ("nbalanced", C.c_int * ML),
+                ("nconverged_fail", C.c_int * ML), ("lambda_min", C.c_double * ML),
+                ("lambda_max", C.c_double * ML), ("resid_max", C.c_double * ML)]
+
+    def as_dict(self):
+        d = {}
+        n = max(0, min(self.nlevels, ML))
+        for f, t in self._fields_:
+            v = getattr(self, f)
+            if f == "phase_ms":
+                d[f] = dict(zip(("setup", "prepartition", "laplacian", "lanczos", "tqli_ritz",
+                                 "sort", "components", "balance"), list(v)))
+            elif hasattr(v, "__len__"):
+                d[f] = list(v)[:n]
+            else:
+                d[f] = v
+        return d
+
+
+class FiedlerOut(C.Structure):
+    _fields_ = [("niter", C.c_void_p), ("npass", C.c_void_p), ("lam", C.c_void_p),
+                ("rnorm", C.c_void_p), ("resid", C.c_void_p)]
+
+
+P = C.c_void_p
+I = C.c_int
+
+# name: (restype, argtypes) -- every function declared in include/*.h
+SIGNATURES = {
+    # parRSB.h
+    "parrsb_part_mesh": (I, [P, P, P, P, I, I, C.POINTER(Options), I]),
+    # parrsb_amd.h
+    "parrsb_amd_part_mesh_dev": (I, [P, P, P, I, I, C.POINTER(Options)]),
+    "parrsb_amd_last_report": (I, [C.POINTER(Report)]),
+    "parrsb_amd_last_error": (C.c_char_p, []),
+    "parrsb_amd_set_precision": (I, [I]),
+    "parrsb_amd_set_profiling": (I, [I]),
+    "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
+    "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
+    # rsbgpu.h
+    "rsbgpu_open": (I, [C.POINTER(P), I]),
+    "rsbgpu_close": (None, [P]),
+    "rsbgpu_error": (C.c_char_p, [P]),
+    "rsbgpu_phase_times": (I, [P, P]),
+    "rsbgpu_lanczos_traffic": (I, [P, P]),
+    "rsbgpu_set_profiling": (I, [P, I]),
+    "rsbgpu_test_tqli": (I, [P, P, I, P, P]),
+    "rsbgpu_set_mesh": (I, [P, P, P, I, I, I]),
+    "rsbgpu_graph_stats": (I, [P, P, P]),
+    "rsbgpu_set_segments": (I, [P, I, P, P, P, P]),
+    "rsbgpu_prepartition": (I, [P, I]),
+    "rsbgpu_level_laplacian": (I, [P]),
+    "rsbgpu_fiedler": (I, [P, I, I, I, C.c_double, I, C.POINTER(FiedlerOut)]),
+    "rsbgpu_set_start": (I, [P, P]),
+    "rsbgpu_fiedler_sort": (I, [P]),
+    "rsbgpu_bisect": (I, [P, P, P, P, I, P, P]),
+    "rsbgpu_write_part": (I, [P, P, I]),
+    "rsbgpu_get_order": (I, [P, P]),
+    "rsbgpu_get_fiedler": (I, [P, P]),
+    "rsbgpu_get_level_csr": (I, [P, P, P, P, P, P, P]),
+    "rsbgpu_get_graph": (I, [P, P, P, P]),
+    "rsbgpu_get_centroids": (I, [P, P]),
+    "rsbgpu_quality": (I, [P, P, I, P]),
+    "rsbgpu_upload_ints": (I, [P, P, I, C.POINTER(P)]),
+    "rsbgpu_free": (None, [P]),
+}
+
+
+def lib():
+    """The loaded C library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run parrsb_amd/build.py "
+                              "(or __graft_entry__.build()) first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_options(**kw) -> Options:
+    o = Options.in_dll(lib(), "parrsb_default_options")
+    opts = Options()
+    C.memmove(C.byref(opts), C.byref(o), C.sizeof(Options))
+    for k, v in kw.items():
+        setattr(opts, k, v)
+    return opts
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def last_report() -> dict:
+    r = Report()
+    lib().parrsb_amd_last_report(C.byref(r))
+    return r.as_dict()
+
+
+def last_error() -> str:
+    return (lib().parrsb_amd_last_error() or b"").decode()
+
+
+def set_precision(bits: int):
+    rc = lib().parrsb_amd_set_precision(bits)
+    if rc:
+        raise ParrsbError(rc, "precision must be 32 or 64")
+
+
+def set_profiling(on: bool):
+    lib().parrsb_amd_set_profiling(1 if on else 0)
+
+
+def part_mesh(vtx, xyz, nv, nparts, tag=None, comm=0, options=None, **kw):
+    """parrsb_part_mesh on host arrays; returns (part, options_after_call)."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64) if vtx is not None else None
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64) if xyz is not None else None
+    nel = (vtx.size if vtx is not None else xyz.size // (3 if nv == 8 else 2)) // nv
+    opts = options if options is not None else default_options(verbose_level=0)
+    opts.nparts = nparts
+    for k, v in kw.items():
+        setattr(opts, k, v)
+    part = np.full(nel, -1, dtype=np.int32)
+    tg = np.ascontiguousarray(tag, dtype=np.int32) if tag is not None else None
+    rc = lib().parrsb_part_mesh(_p(part), _p(vtx), _p(xyz), _p(tg), nel, nv, C.byref(opts),
+                                comm)
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return part, opts
+
+
+def part_mesh_dev(d_part, d_vtx, d_xyz, nel, nv, nparts, options=None, **kw):
+    """Device-pointer variant (pointers as ints, e.g. torch tensor data_ptr())."""
+    opts = options if options is not None else default_options(verbose_level=0)
+    opts.nparts = nparts
+    for k, v in kw.items():
+        setattr(opts, k, v)
+    rc = lib().parrsb_amd_part_mesh_dev(d_part, d_vtx, d_xyz, nel, nv, C.byref(opts))
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return opts
+
+
+QUALITY_KEYS = ("edge_cut", "cut_edges", "max_part", "min_part", "components",
+                "max_components", "comm_volume", "empty_parts")
+
+
+def quality(part, vtx, nv, nparts) -> dict:
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    out = np.zeros(8, dtype=np.int64)
+    rc = lib().parrsb_amd_partition_quality(_p(part), _p(vtx), part.size, nv, nparts, _p(out))
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return dict(zip(QUALITY_KEYS, (int(x) for x in out)))
+
+
+def quality_dev(d_part, d_vtx, nel, nv, nparts) -> dict:
+    out = np.zeros(8, dtype=np.int64)
+    rc = lib().parrsb_amd_partition_quality_dev(d_part, d_vtx, nel, nv, nparts, _p(out))
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return dict(zip(QUALITY_KEYS, (int(x) for x in out)))
+
+
+class Device:
+    """Step-by-step access to the device layer (rsbgpu.h), for parity tests."""
+
+    def __init__(self, device: int = 0):
+        self.h = P()
+        rc = lib().rsbgpu_open(C.byref(self.h), device)
+        if rc:
+            raise ParrsbError(rc, "rsbgpu_open failed (no GPU?)")
+        self.nel = 0
+
+    def close(self):
+        if self.h:
+            lib().rsbgpu_close(self.h)
+            self.h = P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc:
+            raise ParrsbError(rc, (lib().rsbgpu_error(self.h) or b"").decode())
+
+    def set_mesh(self, vtx, xyz, nv):
+        self._vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+        self._xyz = np.ascontiguousarray(xyz, dtype=np.float64) if xyz is not None else None
+        self.nel = self._vtx.size // nv
+        self.nv = nv
+        self._chk(lib().rsbgpu_set_mesh(self.h, _p(self._vtx), _p(self._xyz), self.nel, nv, 0))
+
+    def graph(self):
+        nnz, ws = C.c_int64(), C.c_int64()
+        lib().rsbgpu_graph_stats(self.h, C.byref(nnz), C.byref(ws))
+        rowptr = np.zeros(self.nel + 1, dtype=np.int64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        w = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        self._chk(lib().rsbgpu_get_graph(self.h, _p(rowptr), _p(col), _p(w)))
+        return rowptr, col[:nnz.value], w[:nnz.value]
+
+    def centroids(self):
+        c = np.zeros(3 * self.nel)
+        self._chk(lib().rsbgpu_get_centroids(self.h, _p(c)))
+        return c.reshape(3, self.nel).T.copy()
+
+    def set_segments(self, cstart, n, p, r0):
+        a = [np.ascontiguousarray(x, dtype=np.int32) for x in (cstart, n, p, r0)]
+        self._chk(lib().rsbgpu_set_segments(self.h, a[0].size, *(_p(x) for x in a)))
+
+    def prepartition(self, rsb_pre=1):
+        self._chk(lib().rsbgpu_prepartition(self.h, rsb_pre))
+
+    def level_laplacian(self):
+        self._chk(lib().rsbgpu_level_laplacian(self.h))
+
+    def level_csr(self):
+        mpad, nnz = C.c_int64(), C.c_int64()
+        self._chk(lib().rsbgpu_get_level_csr(self.h, C.byref(mpad), C.byref(nnz), None, None,
+                                             None, None))
+        rowptr = np.zeros(mpad.value + 1, dtype=np.int64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        w = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        re = np.zeros(max(mpad.value, 1), dtype=np.int32)
+        self._chk(lib().rsbgpu_get_level_csr(self.h, C.byref(mpad), C.byref(nnz), _p(rowptr),
+                                             _p(col), _p(w), _p(re)))
+        return rowptr, col[:nnz.value], w[:nnz.value], re[:mpad.value]
+
+    def fiedler(self, nseg, precision=64, max_iter=50, max_passes=50, tol=1e-5):
+        out = {k: np.zeros(nseg, dtype=np.int32) for k in ("niter", "npass")}
+        out["lambda"] = np.zeros(nseg)
+        out["rnorm"] = np.zeros(3 * nseg)
+        out["resid"] = np.zeros(nseg)
+        fo = FiedlerOut(_p(out["niter"]), _p(out["npass"]), _p(out["lambda"]),
+                        _p(out["rnorm"]), _p(out["resid"]))
+        self._chk(lib().rsbgpu_fiedler(self.h, precision, max_iter, max_passes, tol, 0,
+                                       C.byref(fo)))
+        return out
+
+    def fiedler_sort(self):
+        self._chk(lib().rsbgpu_fiedler_sort(self.h))
+
+    def bisect(self, nlower, sender, scount, find_comps=1):
+        nl = np.ascontiguousarray(nlower, dtype=np.int32).copy()
+        sd = np.ascontiguousarray(sender, dtype=np.int32)
+        sc = np.ascontiguousarray(scount, dtype=np.int32)
+        ncomp = np.zeros(2 * nl.size, dtype=np.int32)
+        moved = np.zeros(nl.size, dtype=np.int32)
+        self._chk(lib().rsbgpu_bisect(self.h, _p(nl), _p(sd), _p(sc), find_comps, _p(ncomp),
+                                      _p(moved)))
+        return nl, ncomp, moved
+
+    def order(self):
+        o = np.zeros(self.nel, dtype=np.int32)
+        self._chk(lib().rsbgpu_get_order(self.h, _p(o)))
+        return o
+
+    def fiedler_values(self):
+        f = np.zeros(self.nel)
+        self._chk(lib().rsbgpu_get_fiedler(self.h, _p(f)))
+        return f
+
+    def write_part(self):
+        part = np.zeros(self.nel, dtype=np.int32)
+        self._chk(lib().rsbgpu_write_part(self.h, _p(part), 0))
+        return part
+
+
+def test_tqli(d, e):
+    """Tridiagonal eigen-solve on the device: (lambda_min_abs, y)."""
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64) if len(e) else np.zeros(1)
+    y = np.zeros(d.size)
+    lam = np.zeros(1)
+    rc = lib().rsbgpu_test_tqli(_p(d), _p(e), d.size, _p(y), _p(lam))
+    if rc:
+        raise ParrsbError(rc, "rsbgpu_test_tqli failed")
+    return lam[0], y

@@ -1,0 +1,468 @@
+// Fiedler sort, bisection, component counts, balance, part output and
+// partition quality.
+#include <algorithm>
+#include <vector>
+
+#include "state.hpp"
+
+// ---------------------------------------------------------------------------
+// Fiedler sort (rsb.c:308): stable sort of every segment with p > 1 by its
+// Fiedler values; other segments keep their order.  Two stable passes: by
+// value (inactive positions get one common key and so keep their order),
+// then by segment.
+__global__ void k_fkeys(const double *__restrict__ fied, const int *__restrict__ order,
+                        const int *__restrict__ segof, const int *__restrict__ seg, int nseg,
+                        int nel, uint64_t *__restrict__ key, int *__restrict__ val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  int s = segof[order[i]];
+  key[i] = seg[2 * nseg + s] > 1 ? dkey(fied[i]) : 0;
+  val[i] = i;
+}
+
+__global__ void k_segkeys(const int *__restrict__ perm, const int *__restrict__ order,
+                          const int *__restrict__ segof, int nel, uint64_t *__restrict__ key) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nel) return;
+  key[j] = (uint64_t)segof[order[perm[j]]];
+}
+
+__global__ void k_permute(const int *__restrict__ perm, const int *__restrict__ order,
+                          const double *__restrict__ fied, int nel, int *__restrict__ order2,
+                          double *__restrict__ fied2) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nel) return;
+  int i = perm[j];
+  order2[j] = order[i];
+  fied2[j] = fied[i];
+}
+
+extern "C" int rsbgpu_fiedler_sort(rsbgpu *g) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel;
+  if (nel == 0) return 0;
+  rsb_phase_begin(g);
+  uint64_t *key = nullptr;
+  int *perm = nullptr;
+  double *fied2 = nullptr;
+  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * (nel + 1)));
+  RSB_TRY(hipMalloc((void **)&perm, sizeof(int) * (nel + 1)));
+  RSB_TRY(hipMalloc((void **)&fied2, sizeof(double) * (nel + 1)));
+  k_fkeys<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_fied, g->d_order, g->d_segof, g->d_seg,
+                                                    g->nseg, nel, key, perm);
+  RSB_LAUNCH_CHECK();
+  int rc = rsb_sort_pairs_u64(g, key, perm, nel, 0, 64);
+  if (rc) return rc;
+  if (g->nseg > 1) {
+    k_segkeys<<<nblocks(nel, 256), 256, 0, g->stream>>>(perm, g->d_order, g->d_segof, nel, key);
+    RSB_LAUNCH_CHECK();
+    int sb = 1;
+    while ((1ll << sb) < g->nseg) sb++;
+    rc = rsb_sort_pairs_u64(g, key, perm, nel, 0, sb);
+    if (rc) return rc;
+  }
+  k_permute<<<nblocks(nel, 256), 256, 0, g->stream>>>(perm, g->d_order, g->d_fied, nel,
+                                                      g->d_order2, fied2);
+  RSB_LAUNCH_CHECK();
+  std::swap(g->d_order, g->d_order2);
+  RSB_TRY(hipMemcpyAsync(g->d_fied, fied2, sizeof(double) * nel, hipMemcpyDeviceToDevice,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  hipFree(key), hipFree(perm), hipFree(fied2);
+  rsb_phase_end(g, PH_SORT);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Half labels: hseg[e] = 2 s + (position >= nlower[s]) for elements of
+// segments with p > 1, -1 otherwise.
+__global__ void k_halves(const int *__restrict__ order, const int *__restrict__ segof,
+                         const int *__restrict__ seg, int nseg, const int *__restrict__ nlower,
+                         int nel, int *__restrict__ hseg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  int e = order[i], s = segof[e];
+  hseg[e] = seg[2 * nseg + s] > 1 ? 2 * s + ((i - seg[s]) >= nlower[s] ? 1 : 0) : -1;
+}
+
+// Lock-free union-find over level rows (parents always have smaller
+// indices, so following parents ends at a root; hooking is a CAS on a root).
+__device__ inline int uf_rep(int *par, int x) {
+  int cur = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (cur != x) {
+    int prev = x, next;
+    while (cur > (next = __hip_atomic_load(&par[cur], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(&par[prev], next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      prev = cur;
+      cur = next;
+    }
+  }
+  return cur;
+}
+
+__device__ inline void uf_union(int *par, int u, int v) {
+  int ru = uf_rep(par, u), rv = uf_rep(par, v);
+  while (ru != rv) {
+    if (ru < rv) {
+      int t = ru;
+      ru = rv;
+      rv = t;
+    }
+    int old = atomicCAS(&par[ru], ru, rv);
+    if (old == ru) break;
+    ru = uf_rep(par, old);
+    rv = uf_rep(par, rv);
+  }
+}
+
+__global__ void k_uf_init(int64_t mpad, int *__restrict__ par) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < mpad) par[i] = (int)i;
+}
+
+__global__ void k_uf_edges(int64_t mpad, const int *__restrict__ row_elem,
+                           const int *__restrict__ hseg, const int64_t *__restrict__ slice_off,
+                           const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
+                           int *__restrict__ par) {
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= mpad) return;
+  int e = row_elem[row];
+  if (e < 0) return;
+  int h = hseg[e];
+  int64_t sl = row >> 6;
+  const uint32_t *q = ell + slice_off[sl] + (row & 63);
+  for (int j = 0; j < slice_w[sl]; j++) {
+    uint32_t a = q[(int64_t)j * RSB_WAVE];
+    int c = (int)(a & 0x0fffffffu);
+    if ((a >> 28) == 0 || c >= row) continue;
+    if (hseg[row_elem[c]] == h) uf_union(par, (int)row, c);
+  }
+}
+
+__global__ void k_uf_count(int64_t mpad, const int *__restrict__ row_elem,
+                           const int *__restrict__ hseg, const int *__restrict__ par,
+                           int *__restrict__ ncomp) {
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= mpad) return;
+  int e = row_elem[row];
+  if (e < 0) return;
+  if (par[row] == row) atomicAdd(&ncomp[hseg[e]], 1);
+}
+
+// Interface flag of the sending half (rsb.c:145-182): an element of the
+// sender that shares a vertex with an element of the other half of the same
+// segment.
+__global__ void k_iface(const int *__restrict__ order, const int *__restrict__ segof,
+                        const int *__restrict__ seg, int nseg, const int *__restrict__ nlower,
+                        const int *__restrict__ sender, const int *__restrict__ hseg,
+                        const int *__restrict__ elem_row, const int *__restrict__ row_elem,
+                        const int64_t *__restrict__ slice_off, const int *__restrict__ slice_w,
+                        const uint32_t *__restrict__ ell, int nel, int *__restrict__ flag,
+                        int *__restrict__ nif) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  int e = order[i], s = segof[e];
+  int f = 0;
+  if (seg[2 * nseg + s] > 1 && sender[s] >= 0) {
+    int side = (i - seg[s]) >= nlower[s] ? 1 : 0;
+    if (side == sender[s]) {
+      int row = elem_row[e], h = hseg[e];
+      int64_t sl = row >> 6;
+      const uint32_t *q = ell + slice_off[sl] + (row & 63);
+      for (int j = 0; j < slice_w[sl] && !f; j++) {
+        uint32_t a = q[(int64_t)j * RSB_WAVE];
+        if ((a >> 28) == 0) continue;
+        if (hseg[row_elem[a & 0x0fffffffu]] != h) f = 1;
+      }
+    }
+  }
+  flag[i] = f;
+  if (f) atomicAdd(&nif[s], 1);
+}
+
+// Pick the interface elements that move (rsb.c:167-200): the upper half
+// offers its smallest Fiedler values first, the lower half its largest; ties
+// by position.  C = exclusive prefix count of flags over positions.
+__global__ void k_pick(const int *__restrict__ order, const int *__restrict__ segof,
+                       const int *__restrict__ seg, int nseg, const int *__restrict__ nlower,
+                       const int *__restrict__ sender, const int *__restrict__ scount,
+                       const int *__restrict__ nif, const double *__restrict__ fied,
+                       const int *__restrict__ flag, const int *__restrict__ C, int nel,
+                       int *__restrict__ lower) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  int e = order[i], s = segof[e];
+  if (seg[2 * nseg + s] <= 1) {
+    lower[i] = 0;
+    return;
+  }
+  int cs = seg[s], n = seg[nseg + s], nl = nlower[s];
+  int side = (i - cs) >= nl ? 1 : 0;
+  int low = side == 0;
+  int snd = sender[s];
+  if (snd >= 0 && flag[i] && nif[s] >= scount[s]) {
+    int rank;
+    if (snd == 1) {
+      rank = C[i] - C[cs + nl];
+    } else {
+      int lo = i, hi = i;
+      while (lo > cs && fied[lo - 1] == fied[i]) lo--;
+      while (hi + 1 < cs + nl && fied[hi + 1] == fied[i]) hi++;
+      rank = (C[cs + nl] - C[hi + 1]) + (C[i] - C[lo]);
+    }
+    if (rank < scount[s]) low = !low;
+  }
+  (void)n;
+  lower[i] = low;
+}
+
+__global__ void k_partition(const int *__restrict__ order, const double *__restrict__ fied,
+                            const int *__restrict__ segof, const int *__restrict__ seg, int nseg,
+                            const int *__restrict__ lower, const int *__restrict__ Lx, int nel,
+                            int *__restrict__ order2, double *__restrict__ fied2,
+                            int *__restrict__ nl_out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  int e = order[i], s = segof[e];
+  int j = i;
+  if (seg[2 * nseg + s] > 1) {
+    int cs = seg[s], n = seg[nseg + s];
+    int before = Lx[i] - Lx[cs], nl = Lx[cs + n] - Lx[cs];
+    j = lower[i] ? cs + before : cs + nl + (i - cs - before);
+    if (i == cs) nl_out[s] = nl;
+  }
+  order2[j] = e;
+  fied2[j] = fied[i];
+}
+
+extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const int *scount,
+                             int find_comps, int *ncomp, int *moved) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel, nseg = g->nseg;
+  if (nel == 0) return 0;
+  int *d_nl = nullptr, *d_snd = nullptr, *d_sc = nullptr, *d_nif = nullptr, *hseg = nullptr;
+  int *d_ncomp = nullptr, *par = nullptr, *flag = nullptr, *C = nullptr, *low = nullptr;
+  int *d_nlo = nullptr;
+  double *fied2 = nullptr;
+  RSB_TRY(hipMalloc((void **)&d_nl, sizeof(int) * nseg * 6));
+  d_snd = d_nl + nseg, d_sc = d_nl + 2 * nseg, d_nif = d_nl + 3 * nseg, d_nlo = d_nl + 4 * nseg;
+  RSB_TRY(hipMalloc((void **)&d_ncomp, sizeof(int) * 2 * nseg));
+  RSB_TRY(hipMalloc((void **)&hseg, sizeof(int) * nel));
+  RSB_TRY(hipMemcpyAsync(d_nl, nlower, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
+  RSB_TRY(hipMemcpyAsync(d_snd, sender, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
+  RSB_TRY(hipMemcpyAsync(d_sc, scount, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
+  RSB_TRY(hipMemsetAsync(d_nif, 0, sizeof(int) * nseg, g->stream));
+  RSB_TRY(hipMemsetAsync(d_ncomp, 0, sizeof(int) * 2 * nseg, g->stream));
+  k_halves<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nseg, d_nl,
+                                                     nel, hseg);
+  RSB_LAUNCH_CHECK();
+  // components of each half before balancing (rsb.c:320-327)
+  if (find_comps && g->mpad > 0) {
+    rsb_phase_begin(g);
+    RSB_TRY(hipMalloc((void **)&par, sizeof(int) * g->mpad));
+    k_uf_init<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, par);
+    k_uf_edges<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(
+        g->mpad, g->d_row_elem, hseg, g->d_slice_off, g->d_slice_w, g->d_ell, par);
+    k_uf_count<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, par,
+                                                             d_ncomp);
+    RSB_LAUNCH_CHECK();
+    rsb_phase_end(g, PH_COMP);
+  }
+  rsb_phase_begin(g);
+  RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * (nel + 1)));
+  RSB_TRY(hipMalloc((void **)&C, sizeof(int) * (nel + 1)));
+  RSB_TRY(hipMalloc((void **)&low, sizeof(int) * (nel + 1)));
+  RSB_TRY(hipMalloc((void **)&fied2, sizeof(double) * (nel + 1)));
+  RSB_TRY(hipMemsetAsync(flag + nel, 0, sizeof(int), g->stream));
+  RSB_TRY(hipMemsetAsync(low + nel, 0, sizeof(int), g->stream));
+  k_iface<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nseg, d_nl,
+                                                    d_snd, hseg, g->d_elem_row, g->d_row_elem,
+                                                    g->d_slice_off, g->d_slice_w, g->d_ell, nel,
+                                                    flag, d_nif);
+  RSB_LAUNCH_CHECK();
+  int rc = rsb_exclusive_scan_i32(g, flag, C, nel + 1);
+  if (rc) return rc;
+  k_pick<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nseg, d_nl,
+                                                   d_snd, d_sc, d_nif, g->d_fied, flag, C, nel,
+                                                   low);
+  RSB_LAUNCH_CHECK();
+  rc = rsb_exclusive_scan_i32(g, low, C, nel + 1);
+  if (rc) return rc;
+  RSB_TRY(hipMemcpyAsync(d_nlo, d_nl, sizeof(int) * nseg, hipMemcpyDeviceToDevice, g->stream));
+  k_partition<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_fied, g->d_segof,
+                                                        g->d_seg, nseg, low, C, nel,
+                                                        g->d_order2, fied2, d_nlo);
+  RSB_LAUNCH_CHECK();
+  std::swap(g->d_order, g->d_order2);
+  RSB_TRY(hipMemcpyAsync(g->d_fied, fied2, sizeof(double) * nel, hipMemcpyDeviceToDevice,
+                         g->stream));
+  std::vector<int> nlo(nseg), nif(nseg);
+  RSB_TRY(hipMemcpyAsync(nlo.data(), d_nlo, sizeof(int) * nseg, hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipMemcpyAsync(nif.data(), d_nif, sizeof(int) * nseg, hipMemcpyDeviceToHost,
+                         g->stream));
+  if (ncomp)
+    RSB_TRY(hipMemcpyAsync(ncomp, d_ncomp, sizeof(int) * 2 * nseg, hipMemcpyDeviceToHost,
+                           g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  rsb_phase_end(g, PH_BAL);
+  for (int s = 0; s < nseg; s++) {
+    int mv = 0;
+    if (g->h_p[s] > 1) {
+      mv = (sender[s] >= 0 && scount[s] > 0 && nif[s] >= scount[s]);
+      nlower[s] = nlo[s];
+    }
+    if (moved) moved[s] = mv;
+  }
+  hipFree(d_nl), hipFree(d_ncomp), hipFree(hseg), hipFree(flag), hipFree(C), hipFree(low),
+      hipFree(fied2);
+  if (par) hipFree(par);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_write_part(const int *__restrict__ segof, const int *__restrict__ seg,
+                             int nseg, int nel, int *__restrict__ part) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  part[e] = seg[3 * nseg + segof[e]];
+}
+
+extern "C" int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel;
+  if (nel == 0) return 0;
+  int *dp = part;
+  if (!part_is_device) RSB_TRY(hipMalloc((void **)&dp, sizeof(int) * nel));
+  k_write_part<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_segof, g->d_seg, g->nseg, nel, dp);
+  RSB_LAUNCH_CHECK();
+  if (!part_is_device) {
+    RSB_TRY(hipMemcpyAsync(part, dp, sizeof(int) * nel, hipMemcpyDeviceToHost, g->stream));
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    hipFree(dp);
+  } else {
+    RSB_TRY(hipStreamSynchronize(g->stream));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Partition quality on the global dual graph.
+// Part ids outside [0, nparts) are reported (bad) and left out.
+__global__ void k_q_check(int nel, const int *__restrict__ part, int nparts,
+                          int *__restrict__ bad) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nel && (part[e] < 0 || part[e] >= nparts)) atomicOr(bad, 1);
+}
+
+__global__ void k_q_edges(int nel, const int64_t *__restrict__ rowptr,
+                          const uint32_t *__restrict__ adj, const int *__restrict__ part,
+                          unsigned long long *__restrict__ cut, int *__restrict__ par,
+                          int *__restrict__ size) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long w = 0, c = 0;
+  if (e < nel) {
+    int pe = part[e];
+    atomicAdd(&size[pe], 1);
+    for (int64_t k = rowptr[e]; k < rowptr[e + 1]; k++) {
+      int j = (int)(adj[k] & 0x0fffffffu);
+      if (j <= e) continue;
+      if (part[j] != pe) w += adj[k] >> 28, c++;
+      else uf_union(par, j, e);
+    }
+  }
+  w = wave_sum(w);
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&cut[0], w), atomicAdd(&cut[1], c);
+}
+
+__global__ void k_q_roots(int nel, const int *__restrict__ part, const int *__restrict__ par,
+                          int *__restrict__ ncomp) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nel && par[e] == e) atomicAdd(&ncomp[part[e]], 1);
+}
+
+__global__ void k_q_vkeys(const long long *__restrict__ vtx, const int *__restrict__ part,
+                          int64_t ns, int nv, uint64_t *__restrict__ key, int *__restrict__ val) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  key[s] = ((uint64_t)vtx[s] << 24) | (uint32_t)part[s / nv];
+  val[s] = 0;
+}
+
+__global__ void k_q_volume(const uint64_t *__restrict__ key, int64_t ns,
+                           unsigned long long *__restrict__ vol) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < ns && (i == 0 || (key[i] >> 24) != (key[i - 1] >> 24)) && (key[i] >> 24) != 0) {
+    unsigned long long k = 0;
+    for (int64_t j = i; j < ns && (key[j] >> 24) == (key[i] >> 24); j++)
+      if (j == i || key[j] != key[j - 1]) k++;
+    v = k * (k - 1);
+  }
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) atomicAdd(vol, v);
+}
+
+extern "C" int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t *out) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel;
+  for (int i = 0; i < 8; i++) out[i] = 0;
+  if (nel == 0) return 0;
+  if (nparts < 1 || nparts >= (1 << 24)) RSB_FAIL(3, "nparts out of range for the quality report");
+  if (!g->d_vtx) RSB_FAIL(3, "quality needs vertex ids");
+  {
+    int *bad = nullptr, hb = 0;
+    RSB_TRY(hipMalloc((void **)&bad, sizeof(int)));
+    RSB_TRY(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
+    k_q_check<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, d_part, nparts, bad);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    hipFree(bad);
+    if (hb) RSB_FAIL(3, "part ids outside [0, %d)", nparts);
+  }
+  unsigned long long *acc = nullptr;
+  int *par = nullptr, *size = nullptr, *ncomp = nullptr;
+  RSB_TRY(hipMalloc((void **)&acc, sizeof(unsigned long long) * 4));
+  RSB_TRY(hipMalloc((void **)&par, sizeof(int) * nel));
+  RSB_TRY(hipMalloc((void **)&size, sizeof(int) * nparts * 2));
+  ncomp = size + nparts;
+  RSB_TRY(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 4, g->stream));
+  RSB_TRY(hipMemsetAsync(size, 0, sizeof(int) * nparts * 2, g->stream));
+  k_uf_init<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, par);
+  k_q_edges<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, g->d_growptr, g->d_gadj, d_part, acc,
+                                                      par, size);
+  k_q_roots<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, d_part, par, ncomp);
+  RSB_LAUNCH_CHECK();
+  int64_t ns = (int64_t)nel * g->nv;
+  uint64_t *key = nullptr;
+  int *val = nullptr;
+  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ns));
+  RSB_TRY(hipMalloc((void **)&val, sizeof(int) * ns));
+  k_q_vkeys<<<nblocks(ns, 256), 256, 0, g->stream>>>(g->d_vtx, d_part, ns, g->nv, key, val);
+  RSB_LAUNCH_CHECK();
+  int rc = rsb_sort_pairs_u64(g, key, val, ns, 0, 64);
+  if (rc) return rc;
+  k_q_volume<<<nblocks(ns, 256), 256, 0, g->stream>>>(key, ns, acc + 2);
+  RSB_LAUNCH_CHECK();
+  std::vector<int> hs(2 * (size_t)nparts);
+  unsigned long long ha[4];
+  RSB_TRY(hipMemcpyAsync(hs.data(), size, sizeof(int) * 2 * nparts, hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipMemcpyAsync(ha, acc, sizeof(ha), hipMemcpyDeviceToHost, g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  int64_t mx = 0, mn = INT64_MAX, tot = 0, cmx = 0, empty = 0;
+  for (int q = 0; q < nparts; q++) {
+    mx = std::max<int64_t>(mx, hs[q]);
+    mn = std::min<int64_t>(mn, hs[q]);
+    tot += hs[nparts + q];
+    cmx = std::max<int64_t>(cmx, hs[nparts + q]);
+    empty += hs[q] == 0;
+  }
+  out[0] = (int64_t)ha[0], out[1] = (int64_t)ha[1], out[2] = mx, out[3] = mn, out[4] = tot;
+  out[5] = cmx, out[6] = (int64_t)ha[2], out[7] = empty;
+  hipFree(acc), hipFree(par), hipFree(size), hipFree(key), hipFree(val);
+  return 0;
+}

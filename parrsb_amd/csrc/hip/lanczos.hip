@@ -1,0 +1,757 @@
+// Batched CG-Lanczos Fiedler solver: every segment of a bisection level runs
+// the reference's restarted CG-Lanczos (fiedler.c:434-621) at once.
+//
+// Rows of the level are grouped in tiles of RSB_TILE rows that never cross a
+// segment, one workgroup per tile.  A segment's scalar reductions (means, dot
+// products) are finished by the last of its tile-blocks to arrive (a
+// counter per segment), in a fixed order, so results do not depend on block
+// scheduling and no extra launch is needed per reduction.  One CG-Lanczos
+// iteration is three launches over all rows of the level:
+//   k_lz_pupdate  p = beta (p - mean_p) + r, basis[it] = r / |r|, sum(p)
+//   k_lz_spmv     w = L p (SELL-64), pap = w . (p - mean_p)
+//   k_lz_rupdate  r = r - alpha w, rtr = r . r, tridiagonal entries, stop test
+// The mean of p is removed lazily (fiedler.c:20-30 `ortho`): w = L p is
+// evaluated in the difference form sum_j w_ij (p_i - p_j), which the mean
+// does not change.  Segments that converged or are between passes skip every
+// launch at the first instruction.
+//
+// Precision: vectors are T (double, or float); every scalar and the
+// tridiagonal eigen-solve are FP64.  With T = float, dot products are
+// compensated (Dot2: exact products and error-free sums in FP32) and their
+// block partials are combined in FP64.
+#include <algorithm>
+#include <vector>
+
+#include "state.hpp"
+
+struct lz_args {
+  const int *seg;        // 5*nseg: cstart, n, p, r0, pstart
+  const int *tile;       // 3*ntiles: seg, row0, row1
+  const int *seg_tiles;  // 2*nseg: first tile, tile count
+  const int *row_elem;
+  int nseg;
+  int *status, *iter, *k, *npass, *niter, *miter;
+  double *sc, *T, *y, *partial;
+  unsigned *cnt;
+  int *running;
+  int mpass;
+  double tol;
+  int miter_max;
+};
+
+#define SC(s, j) a.sc[(size_t)(s) * LZ_NSC + (j)]
+
+template <typename T>
+struct acc_t;
+template <>
+struct acc_t<double> {
+  double v = 0.0;
+  __device__ inline void dot(double x, double y) { v += x * y; }
+  __device__ inline void add(double x) { v += x; }
+  __device__ inline double value() const { return v; }
+};
+template <>
+struct acc_t<float> {
+  float s = 0.f, c = 0.f;
+  __device__ inline void dot(float x, float y) {
+    float p = x * y;
+    float ep = __builtin_fmaf(x, y, -p);  // exact product error
+    float t, et;
+    two_sum(s, p, t, et);
+    s = t;
+    c += ep + et;
+  }
+  __device__ inline void add(float x) {
+    float t, et;
+    two_sum(s, x, t, et);
+    s = t;
+    c += et;
+  }
+  __device__ inline double value() const { return (double)s + (double)c; }
+};
+
+// Tile prologue: returns false if the block has nothing to do.
+struct tile_t {
+  int s, row0, row1, pstart, n, first, count;
+};
+__device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t) {
+  int ti = blockIdx.x;
+  t.s = a.tile[3 * ti];
+  if (a.status[t.s] != want_status) return false;
+  t.row0 = a.tile[3 * ti + 1];
+  t.row1 = a.tile[3 * ti + 2];
+  t.n = a.seg[a.nseg + t.s];
+  t.pstart = a.seg[4 * a.nseg + t.s];
+  t.first = a.seg_tiles[2 * t.s];
+  t.count = a.seg_tiles[2 * t.s + 1];
+  return true;
+}
+
+// Publish NV block partials, and in the segment's last block reduce all of
+// them (fixed order).  Returns true in that block, with tot[] in thread 0.
+template <int NV>
+__device__ inline bool seg_reduce(const lz_args &a, const tile_t &t, double (&v)[NV],
+                                  double (&tot)[NV]) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  block_sum<NV>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) a.partial[(size_t)blockIdx.x * 4 + k] = v[k];
+  if (!last_block_of(&a.cnt[t.s], (unsigned)t.count)) return false;
+  double w[NV];
+  for (int k = 0; k < NV; k++) w[k] = 0.0;
+  for (int i = threadIdx.x; i < t.count; i += blockDim.x)
+    for (int k = 0; k < NV; k++) w[k] += load_acquire(&a.partial[(size_t)(t.first + i) * 4 + k]);
+  block_sum<NV>(w, red);
+  for (int k = 0; k < NV; k++) tot[k] = w[k];
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Start vector (fiedler.c:570-585): x_i = i + 1, plus 1000 n on the first half
+// of the pre-partition order; then mean removed and normalised (FP64).
+// mode 0: reference start; mode 1: caller-supplied per-element start.
+__device__ inline double start_value(int mode, int i, int n, const float *start, int e) {
+  if (mode == 1) return (double)start[e];
+  double v = i + 1.0;
+  if (i < n / 2) v += (double)(1000 * (long long)n);
+  return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_start(lz_args a, int phase, int mode,
+                                                        const float *start, T *f) {
+  tile_t t;
+  if (!tile_of(a, LZ_SETUP, t)) return;
+  double m = phase > 0 ? SC(t.s, SC_M0) : 0.0;
+  double sc = phase > 1 ? SC(t.s, SC_N0) : 0.0;
+  double v[1] = {0.0};
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    int i = row - t.pstart;
+    double x = 0.0;
+    if (i < t.n) {
+      x = start_value(mode, i, t.n, start, a.row_elem[row]);
+      if (phase == 0) v[0] += x;
+      else if (phase == 1) v[0] += (x - m) * (x - m);
+    }
+    if (phase == 2) f[row] = i < t.n ? (T)((x - m) * sc) : (T)0;
+  }
+  if (phase == 2) return;
+  double tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    if (phase == 0) SC(t.s, SC_M0) = tot[0] / t.n;
+    else SC(t.s, SC_N0) = 1.0 / sqrt(tot[0]);
+  }
+}
+
+// Pass setup (fiedler.c:441-456): r = f - mean_f; then r -= mean(r);
+// rtr, |r|, target tol |r|.
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_setup(lz_args a, int phase, const T *f, T *r,
+                                                        T *p) {
+  tile_t t;
+  if (!tile_of(a, LZ_SETUP, t)) return;
+  acc_t<T> acc;
+  if (phase == 0) {
+    T mf = (T)SC(t.s, SC_MEANF);
+    for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+      int i = row - t.pstart;
+      T x = i < t.n ? (T)(f[row] - mf) : (T)0;
+      r[row] = x;
+      p[row] = (T)0;
+      acc.add(x);
+    }
+  } else {
+    T mr = (T)SC(t.s, SC_TMP);
+    for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+      int i = row - t.pstart;
+      if (i < t.n) {
+        T x = r[row] - mr;
+        r[row] = x;
+        acc.dot(x, x);
+      }
+    }
+  }
+  double v[1] = {acc.value()}, tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    int s = t.s;
+    if (phase == 0) {
+      SC(s, SC_TMP) = tot[0] / t.n;
+    } else {
+      double rtr = tot[0], rn = sqrt(rtr);
+      SC(s, SC_RTR) = rtr;
+      SC(s, SC_RNORM) = rn;
+      SC(s, SC_RNORM0) = rn;
+      SC(s, SC_RTOL) = rn * a.tol;
+      SC(s, SC_RNI) = 1.0 / rn;
+      SC(s, SC_RTZ1) = rtr;  // iteration 0: rtz2 = 1 (initial rtz1), rtz1 = rtr
+      SC(s, SC_RTZ2) = 1.0;
+      SC(s, SC_BETA) = 0.0;
+      SC(s, SC_PAP) = 0.0;
+      SC(s, SC_MEANP) = 0.0;
+      a.iter[s] = 0;
+      a.status[s] = LZ_RUNNING;
+      atomicAdd(&a.running[0], 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_pupdate(lz_args a, const T *__restrict__ r,
+                                                          T *__restrict__ p,
+                                                          T *__restrict__ basis, int64_t mpad) {
+  tile_t t;
+  if (!tile_of(a, LZ_RUNNING, t)) return;
+  const int s = t.s, it = a.iter[s];
+  const T beta = (T)SC(s, SC_BETA), mp = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
+  T *b = basis + (size_t)it * mpad;
+  acc_t<T> acc;
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    int i = row - t.pstart;
+    T rv = r[row];
+    T pn = (i < t.n) ? (it == 0 ? rv : beta * (p[row] - mp) + rv) : (T)0;
+    p[row] = pn;
+    b[row] = rv * rni;
+    acc.add(pn);
+  }
+  double v[1] = {acc.value()}, tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) SC(s, SC_MEANP) = tot[0] / t.n;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_spmv(lz_args a, const T *__restrict__ p,
+                                                       T *__restrict__ w,
+                                                       const int64_t *__restrict__ slice_off,
+                                                       const int *__restrict__ slice_w,
+                                                       const uint32_t *__restrict__ ell) {
+  tile_t t;
+  if (!tile_of(a, LZ_RUNNING, t)) return;
+  const int s = t.s;
+  const T mp = (T)SC(s, SC_MEANP);
+  acc_t<T> acc;
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    const int sl = row >> 6, lane = row & 63;
+    const uint32_t *e = ell + slice_off[sl] + lane;
+    const int wd = slice_w[sl];
+    const T pi = p[row];
+    T sum = (T)0;
+    for (int j = 0; j < wd; j++) {
+      uint32_t q = e[(size_t)j * RSB_WAVE];
+      T pj = p[q & 0x0fffffffu];
+      sum += (T)(q >> 28) * (pi - pj);
+    }
+    w[row] = sum;
+    if (row - t.pstart < t.n) acc.dot(sum, pi - mp);
+  }
+  double v[1] = {acc.value()}, tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    SC(s, SC_PAPOLD) = SC(s, SC_PAP);
+    SC(s, SC_PAP) = tot[0];
+    SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / tot[0];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restrict__ r,
+                                                          const T *__restrict__ w) {
+  tile_t t;
+  if (!tile_of(a, LZ_RUNNING, t)) return;
+  const int s = t.s;
+  const T alpha = (T)SC(s, SC_ALPHA);
+  acc_t<T> acc;
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    T x = r[row] - alpha * w[row];
+    r[row] = x;
+    acc.dot(x, x);
+  }
+  double v[1] = {acc.value()}, tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    // fiedler.c:486-509
+    const int it = a.iter[s], mi = a.miter[s];
+    double rtr = tot[0], rnorm = sqrt(rtr);
+    double rtz1 = SC(s, SC_RTZ1), rtz2 = SC(s, SC_RTZ2), beta = SC(s, SC_BETA);
+    double pap = SC(s, SC_PAP), pap_old = SC(s, SC_PAPOLD);
+    double *Td = a.T + (size_t)s * 2 * a.miter_max, *Tu = Td + a.miter_max;
+    if (it == 0) {
+      Td[0] = pap / rtz1;
+    } else {
+      Td[it] = (beta * beta * pap_old + pap) / rtz1;
+      Tu[it - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
+    }
+    SC(s, SC_RTR) = rtr;
+    SC(s, SC_RNORM) = rnorm;
+    SC(s, SC_RNI) = 1.0 / rnorm;
+    if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
+      a.k[s] = it + 1;
+      a.status[s] = LZ_PASSDONE;
+      atomicSub(&a.running[0], 1);
+    } else {
+      a.iter[s] = it + 1;
+      SC(s, SC_RTZ2) = rtz1;
+      SC(s, SC_RTZ1) = rtr;
+      SC(s, SC_BETA) = rtr / rtz1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal eigen-solve of each finished pass: implicit QL with shifts, the
+// steps of fiedler.c:328-432 (FP64, contraction off so every operation rounds
+// as written).  One wavefront per segment: every lane runs the scalar
+// recurrence on the shared d/e arrays; the plane rotations of the
+// eigenvector matrix are spread over the lanes (lane = matrix row).  The
+// coefficients y of the eigenvector with the smallest |eigenvalue| are kept.
+#pragma clang fp contract(off)
+__device__ inline double sgn_abs(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+__device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, double *lam_out) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n * n; i += RSB_WAVE) z[i] = 0.0;
+  __syncthreads();
+  for (int i = lane; i < n; i += RSB_WAVE) z[i * n + i] = 1.0;
+  __syncthreads();
+  int failed = 0;
+  for (int l = 0; l < n && !failed; l++) {
+    int sweeps = 0, m;
+    while (true) {
+      for (m = l; m < n - 1; m++) {
+        double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) / dd < 1e-12) break;
+      }
+      if (m == l) break;
+      if (sweeps++ == 30) {
+        failed = 1;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = sqrt(g * g + 1.0);
+      g = d[m] - d[l] + e[l] / (g + sgn_abs(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      for (int i = m - 1; i >= l; i--) {
+        double f = s * e[i], b = c * e[i];
+        double ei1;
+        if (fabs(f) >= fabs(g)) {
+          c = g / f;
+          r = sqrt(c * c + 1.0);
+          ei1 = f * r;
+          s = 1.0 / r;
+          c = c * s;
+        } else {
+          s = f / g;
+          r = sqrt(s * s + 1.0);
+          ei1 = g * r;
+          c = 1.0 / r;
+          s = s * c;
+        }
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        double di1 = g + p;
+        g = c * r - b;
+        __syncthreads();
+        if (lane == 0) e[i + 1] = ei1, d[i + 1] = di1;
+        for (int k = lane; k < n; k += RSB_WAVE) {
+          double zk1 = z[k * n + i + 1], zk = z[k * n + i];
+          z[k * n + i + 1] = s * zk + c * zk1;
+          z[k * n + i] = c * zk - s * zk1;
+        }
+        __syncthreads();
+      }
+      double dl = d[l] - p;
+      __syncthreads();
+      if (lane == 0) d[l] = dl, e[l] = g, e[m] = 0.0;
+      __syncthreads();
+    }
+  }
+  // eigenvalue of smallest magnitude (first on ties, fiedler.c:538-545)
+  int imin = 0;
+  double emin = fabs(d[0]);
+  for (int i = 1; i < n; i++)
+    if (fabs(d[i]) < emin) emin = fabs(d[i]), imin = i;
+  if (failed) {
+    // the reference leaves the rotations untransposed and unnormalised
+    for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[imin * n + j];
+  } else {
+    // eigenvector imin = column imin of z, normalised (sum in row order)
+    if (lane == 0) {
+      double nrm = 0.0;
+      for (int i = 0; i < n; i++) nrm += z[i * n + imin] * z[i * n + imin];
+      if (nrm > 0.0) nrm = sqrt(fabs(nrm));
+      e[0] = 1.0 / nrm;
+    }
+    __syncthreads();
+    double scale = e[0];
+    for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[j * n + imin] * scale;
+  }
+  if (lane == 0) *lam_out = d[imin];
+  __syncthreads();
+}
+#pragma clang fp contract(fast)
+
+__global__ void __launch_bounds__(RSB_WAVE) k_lz_tqli(lz_args a) {
+  const int s = blockIdx.x;
+  if (a.status[s] != LZ_PASSDONE) return;
+  extern __shared__ double lds[];
+  const int n = a.k[s], M = a.miter_max;
+  double *d = lds, *e = d + M, *z = e + M;
+  const double *Td = a.T + (size_t)s * 2 * M, *Tu = Td + M;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) {
+    d[i] = Td[i];
+    e[i] = (i < n - 1) ? Tu[i] : 0.0;
+  }
+  __syncthreads();
+  tqli_wave(n, d, e, z, a.y + (size_t)s * M, &SC(s, SC_LAMBDA));
+}
+
+// Stand-alone tridiagonal solve for the parity tests.
+__global__ void __launch_bounds__(RSB_WAVE) k_tqli_test(int n, const double *dd,
+                                                        const double *ee, double *y,
+                                                        double *lam) {
+  extern __shared__ double lds[];
+  double *d = lds, *e = d + n, *z = e + n;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) d[i] = dd[i], e[i] = i < n - 1 ? ee[i] : 0.0;
+  __syncthreads();
+  tqli_wave(n, d, e, z, y, lam);
+}
+
+// Ritz vector (fiedler.c:547-553): f = sum_j basis_j y_j (FP64 accumulation,
+// j ascending), then its mean (ortho).  Decides restart or finish.
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__restrict__ basis,
+                                                       int64_t mpad, T *__restrict__ f) {
+  tile_t t;
+  if (!tile_of(a, LZ_PASSDONE, t)) return;
+  const int s = t.s, k = a.k[s];
+  const double *y = a.y + (size_t)s * a.miter_max;
+  double v[1] = {0.0};
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    int i = row - t.pstart;
+    double x = 0.0;
+    if (i < t.n)
+      for (int j = 0; j < k; j++) x += (double)basis[(size_t)j * mpad + row] * y[j];
+    f[row] = (T)x;
+    v[0] += (double)(T)x;
+  }
+  double tot[1];
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    SC(s, SC_MEANF) = tot[0] / t.n;
+    int np = a.npass[s] + 1;
+    a.npass[s] = np;
+    a.niter[s] = (np - 1) * a.miter[s] + k;
+    if (k == a.miter[s] && np < a.mpass) {
+      a.status[s] = LZ_SETUP;
+      atomicAdd(&a.running[1], 1);
+    } else {
+      a.status[s] = LZ_FINISHED;
+    }
+  }
+}
+
+// Final normalisation (fiedler.c:606-616) and scatter to compact positions.
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_finish(lz_args a, int phase,
+                                                         const T *__restrict__ f,
+                                                         double *__restrict__ fied,
+                                                         double *__restrict__ xrow) {
+  tile_t t;
+  if (!tile_of(a, LZ_FINISHED, t)) return;
+  const int s = t.s;
+  const double mf = SC(s, SC_MEANF);
+  if (phase == 1) {
+    const double ni = SC(s, SC_NORMF);
+    const int cstart = a.seg[s];
+    for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+      int i = row - t.pstart;
+      double x = i < t.n ? ((double)f[row] - mf) * ni : 0.0;
+      xrow[row] = x;
+      if (i < t.n) fied[cstart + i] = x;
+    }
+    return;
+  }
+  double v[1] = {0.0}, tot[1];
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    int i = row - t.pstart;
+    if (i < t.n) {
+      double x = (double)f[row] - mf;
+      v[0] += x * x;
+    }
+  }
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) SC(s, SC_NORMF) = 1.0 / sqrt(tot[0]);
+}
+
+// Eigen-residual of the final vector x (FP64): lambda_R = x.Lx, then
+// |Lx - lambda_R x| / lambda_R.
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_resid(lz_args a, int phase,
+                                                        const double *__restrict__ x,
+                                                        const int64_t *__restrict__ slice_off,
+                                                        const int *__restrict__ slice_w,
+                                                        const uint32_t *__restrict__ ell) {
+  tile_t t;
+  if (!tile_of(a, LZ_FINISHED, t)) return;
+  const int s = t.s;
+  const double lam = phase ? SC(s, SC_RES) : 0.0;
+  double v[1] = {0.0}, tot[1];
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    const int sl = row >> 6, lane = row & 63;
+    const uint32_t *e = ell + slice_off[sl] + lane;
+    const double xi = x[row];
+    double sum = 0.0;
+    for (int j = 0; j < slice_w[sl]; j++) {
+      uint32_t q = e[(size_t)j * RSB_WAVE];
+      sum += (double)(q >> 28) * (xi - x[q & 0x0fffffffu]);
+    }
+    if (phase == 0) v[0] += sum * xi;
+    else {
+      double d = sum - lam * xi;
+      v[0] += d * d;
+    }
+  }
+  if (!seg_reduce<1>(a, t, v, tot)) return;
+  if (threadIdx.x == 0) {
+    if (phase == 0) SC(s, SC_RES) = tot[0];
+    else SC(s, SC_TMP) = sqrt(tot[0]) / lam;
+  }
+}
+
+__global__ void k_lz_init_state(lz_args a, int miter_opt) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nseg) return;
+  int n = a.seg[a.nseg + s], p = a.seg[2 * a.nseg + s];
+  a.status[s] = (p > 1 && n >= 2) ? LZ_SETUP : LZ_IDLE;
+  a.miter[s] = n < miter_opt ? n : miter_opt;  // fiedler.c:522
+  a.npass[s] = 0, a.niter[s] = 0, a.iter[s] = 0, a.k[s] = 0;
+  a.cnt[s] = 0;
+  for (int j = 0; j < LZ_NSC; j++) SC(s, j) = 0.0;
+}
+
+// Segments that cannot be bisected spectrally (fewer than 2 elements) keep
+// Fiedler value 0.
+__global__ void k_lz_zero_small(lz_args a, double *fied) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nseg) return;
+  int n = a.seg[a.nseg + s], p = a.seg[2 * a.nseg + s];
+  if (p > 1 && n == 1) fied[a.seg[s]] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init_mode,
+                       rsbgpu_fiedler_out *out) {
+  const int nseg = g->nseg, ntiles = g->ntiles;
+  const int64_t mpad = g->mpad;
+  int miter_max = 1;
+  for (int s = 0; s < nseg; s++)
+    if (g->h_p[s] > 1 && g->h_n[s] >= 2) miter_max = std::max(miter_max, std::min(miter_opt, g->h_n[s]));
+  if (miter_max > 128) RSB_FAIL(3, "rsb_max_iter > 128 is not supported");
+  // buffers
+  size_t vb = (size_t)mpad * sizeof(T);
+  if (vb > g->vec_bytes) {
+    for (void **q : {&g->d_r, &g->d_p, &g->d_w, &g->d_f}) {
+      if (*q) RSB_TRY(hipFree(*q));
+      *q = nullptr;
+      RSB_TRY(hipMalloc(q, vb + 256));
+    }
+    g->vec_bytes = vb;
+  }
+  size_t bb = vb * (size_t)miter_max;
+  if (bb > g->basis_bytes) {
+    if (g->d_basis) RSB_TRY(hipFree(g->d_basis));
+    g->d_basis = nullptr;
+    RSB_TRY(hipMalloc(&g->d_basis, bb + 256));
+    g->basis_bytes = bb;
+  }
+  double *xrow = nullptr;
+  RSB_TRY(hipMalloc((void **)&xrow, sizeof(double) * (mpad + 1)));
+  lz_state &L = g->lz;
+  if (nseg > g->lz_nseg_cap || miter_max > g->lz_miter_cap) {
+    // iter, k, npass, niter and miter are carved from the status block
+    for (void *q : {(void *)L.status, (void *)L.sc, (void *)L.T, (void *)L.y, (void *)L.cnt})
+      if (q) RSB_TRY(hipFree(q));
+    int cap = std::max(nseg, 1);
+    RSB_TRY(hipMalloc((void **)&L.status, sizeof(int) * cap * 6));
+    L.iter = L.status + cap, L.k = L.status + 2 * cap, L.npass = L.status + 3 * cap;
+    L.niter = L.status + 4 * cap;
+    RSB_TRY(hipMalloc((void **)&L.sc, sizeof(double) * cap * LZ_NSC));
+    RSB_TRY(hipMalloc((void **)&L.T, sizeof(double) * cap * 2 * miter_max));
+    RSB_TRY(hipMalloc((void **)&L.y, sizeof(double) * cap * miter_max));
+    RSB_TRY(hipMalloc((void **)&L.cnt, sizeof(unsigned) * cap));
+    g->lz_nseg_cap = cap, g->lz_miter_cap = miter_max;
+  }
+  if (ntiles > g->lz_tile_cap) {
+    if (L.partial) RSB_TRY(hipFree(L.partial));
+    RSB_TRY(hipMalloc((void **)&L.partial, sizeof(double) * 4 * std::max(ntiles, 1)));
+    g->lz_tile_cap = ntiles;
+  }
+  if (!L.running) RSB_TRY(hipMalloc((void **)&L.running, sizeof(int) * 2));
+  lz_args a;
+  a.seg = g->d_seg, a.tile = g->d_tile, a.seg_tiles = g->d_seg_tiles, a.row_elem = g->d_row_elem;
+  a.nseg = nseg;
+  a.status = L.status, a.iter = L.iter, a.k = L.k, a.npass = L.npass, a.niter = L.niter;
+  a.miter = L.status + 5 * std::max(g->lz_nseg_cap, 1);
+  a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
+  a.running = L.running;
+  a.mpass = mpass, a.tol = tol, a.miter_max = miter_max;
+  T *r = (T *)g->d_r, *p = (T *)g->d_p, *w = (T *)g->d_w, *f = (T *)g->d_f,
+    *basis = (T *)g->d_basis;
+  hipStream_t st = g->stream;
+
+  RSB_TRY(hipMemsetAsync(L.running, 0, sizeof(int) * 2, st));
+  k_lz_init_state<<<nblocks(nseg, 256), 256, 0, st>>>(a, miter_opt);
+  RSB_LAUNCH_CHECK();
+  k_lz_zero_small<<<nblocks(nseg, 256), 256, 0, st>>>(a, g->d_fied);
+  RSB_LAUNCH_CHECK();
+  if (ntiles == 0) return 0;
+  for (int ph = 0; ph < 3; ph++) {
+    k_lz_start<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, ph, init_mode, g->d_start, f);
+    RSB_LAUNCH_CHECK();
+  }
+  // f holds the normalised start vector; SC_MEANF is 0 for the first pass.
+  const size_t tq_lds = sizeof(double) * (2 * miter_max + miter_max * miter_max);
+  for (int pass = 0; pass < mpass; pass++) {
+    k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, r, p);
+    RSB_LAUNCH_CHECK();
+    k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipEventRecord(g->lz_ev0, st));
+    int it, nrec = 0;
+    if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
+      size_t old = g->spmv_ev.size();
+      g->spmv_ev.resize(2 * miter_max);
+      for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
+    }
+    for (it = 0; it < miter_max; it++) {
+      k_lz_pupdate<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, p, basis, mpad);
+      if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
+      k_lz_spmv<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, p, w, g->d_slice_off, g->d_slice_w,
+                                                 g->d_ell);
+      if (g->prof_spmv) {
+        RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
+        nrec++;
+      }
+      k_lz_rupdate<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, w);
+      RSB_LAUNCH_CHECK();
+      g->lz_launches++;
+      if ((it & 7) == 7 && it + 1 < miter_max) {
+        int run = 0;
+        RSB_TRY(hipMemcpyAsync(&run, L.running, sizeof(int), hipMemcpyDeviceToHost, st));
+        RSB_TRY(hipStreamSynchronize(st));
+        if (run == 0) break;
+      }
+    }
+    RSB_TRY(hipEventRecord(g->lz_ev1, st));
+    RSB_TRY(hipEventSynchronize(g->lz_ev1));
+    float ms = 0.f;
+    RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
+    g->lz_ms += ms;
+    for (int q = 0; q < nrec; q++) {
+      RSB_TRY(hipEventElapsedTime(&ms, g->spmv_ev[2 * q], g->spmv_ev[2 * q + 1]));
+      g->spmv_ms += ms;
+    }
+    g->spmv_launches += nrec;
+    RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
+    k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
+    RSB_LAUNCH_CHECK();
+    k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
+    RSB_LAUNCH_CHECK();
+    int again = 0;
+    RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    if (again == 0) break;
+  }
+  k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, g->d_fied, xrow);
+  RSB_LAUNCH_CHECK();
+  k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, g->d_fied, xrow);
+  RSB_LAUNCH_CHECK();
+  if (out && out->resid) {
+    k_lz_resid<<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, xrow, g->d_slice_off, g->d_slice_w, g->d_ell);
+    k_lz_resid<<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, xrow, g->d_slice_off, g->d_slice_w, g->d_ell);
+    RSB_LAUNCH_CHECK();
+  }
+  // read back per-segment results
+  std::vector<int> ni(nseg), np(nseg);
+  std::vector<double> sc((size_t)nseg * LZ_NSC);
+  RSB_TRY(hipMemcpyAsync(ni.data(), a.niter, sizeof(int) * nseg, hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipMemcpyAsync(np.data(), a.npass, sizeof(int) * nseg, hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipMemcpyAsync(sc.data(), a.sc, sizeof(double) * nseg * LZ_NSC,
+                         hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipStreamSynchronize(st));
+  (void)hipFree(xrow);
+  // Algorithmic bytes, per iteration and segment of n rows and nnz
+  // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 9 s n vector
+  // streams for the whole iteration; 4(n+1) + 4 nnz + 2 s n for the SpMV
+  // (x read once, w written).  Counted iterations, real (unpadded) sizes.
+  for (int s = 0; s < nseg; s++) {
+    if (g->h_p[s] > 1 && g->h_n[s] >= 2) {
+      double n = g->h_n[s], nz = (double)g->h_seg_nnz[s], sz = sizeof(T);
+      g->lz_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 9 * sz * n);
+      g->spmv_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 2 * sz * n);
+    }
+    if (out) {
+      if (out->niter) out->niter[s] = ni[s];
+      if (out->npass) out->npass[s] = np[s];
+      if (out->lambda) out->lambda[s] = sc[(size_t)s * LZ_NSC + SC_LAMBDA];
+      if (out->rnorm) {
+        out->rnorm[3 * s] = sc[(size_t)s * LZ_NSC + SC_RNORM0];
+        out->rnorm[3 * s + 1] = sc[(size_t)s * LZ_NSC + SC_RTOL];
+        out->rnorm[3 * s + 2] = sc[(size_t)s * LZ_NSC + SC_RNORM];
+      }
+      if (out->resid) out->resid[s] = sc[(size_t)s * LZ_NSC + SC_TMP];
+    }
+  }
+  return 0;
+}
+
+extern "C" int rsbgpu_set_start(rsbgpu *g, const float *d_start) {
+  RSB_TRY(hipSetDevice(g->device));
+  RSB_TRY(dev_alloc(g->d_start, (size_t)g->nel));
+  RSB_TRY(hipMemcpyAsync(g->d_start, d_start, sizeof(float) * (size_t)g->nel,
+                         hipMemcpyDeviceToDevice, g->stream));
+  return 0;
+}
+
+extern "C" int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_passes,
+                              double tol, int init_mode, rsbgpu_fiedler_out *out) {
+  RSB_TRY(hipSetDevice(g->device));
+  if (max_iter < 1 || max_passes < 1) RSB_FAIL(3, "rsb_max_iter and rsb_max_passes must be >= 1");
+  if (init_mode == 1 && !g->d_start) RSB_FAIL(3, "no start vector set");
+  rsb_phase_begin(g);
+  int rc;
+  if (precision == 32) rc = run_lanczos<float>(g, max_iter, max_passes, tol, init_mode, out);
+  else rc = run_lanczos<double>(g, max_iter, max_passes, tol, init_mode, out);
+  rsb_phase_end(g, PH_LANCZOS);
+  return rc;
+}
+
+extern "C" int rsbgpu_test_tqli(const double *d, const double *e, int n, double *y,
+                                double *lam) {
+  double *dd = nullptr, *de = nullptr, *dy = nullptr, *dl = nullptr;
+  if (n < 1 || n > 128) return 3;
+  hipError_t err = hipMalloc((void **)&dd, sizeof(double) * n);
+  if (err == hipSuccess) err = hipMalloc((void **)&de, sizeof(double) * n);
+  if (err == hipSuccess) err = hipMalloc((void **)&dy, sizeof(double) * n);
+  if (err == hipSuccess) err = hipMalloc((void **)&dl, sizeof(double));
+  if (err == hipSuccess) err = hipMemcpy(dd, d, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemset(de, 0, sizeof(double) * n);
+  if (err == hipSuccess && n > 1)
+    err = hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    size_t lds = sizeof(double) * (2 * n + n * n);
+    k_tqli_test<<<1, RSB_WAVE, lds, 0>>>(n, dd, de, dy, dl);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err == hipSuccess) err = hipMemcpy(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(lam, dl, sizeof(double), hipMemcpyDeviceToHost);
+  if (dd) (void)hipFree(dd);
+  if (de) (void)hipFree(de);
+  if (dy) (void)hipFree(dy);
+  if (dl) (void)hipFree(dl);
+  return err == hipSuccess ? 0 : 2;
+}

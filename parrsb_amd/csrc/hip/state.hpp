@@ -1,0 +1,173 @@
+// Device state behind the opaque rsbgpu handle (internal to the HIP layer).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "rsbgpu.h"
+#include "common.hpp"
+
+#define RSB_TILE 1024   // Lanczos rows per block (4 rows per thread)
+#define RSB_BLOCK 256
+#define RSB_MAXC 64     // max candidate neighbours gathered per element
+
+// Phases timed with events (rsbgpu_phase_times).
+enum { PH_SETUP = 0, PH_PRE, PH_LAP, PH_LANCZOS, PH_TQLI, PH_SORT, PH_COMP, PH_BAL, PH_N };
+
+// Per-segment Lanczos state (device arrays of length nseg).
+struct lz_state {
+  int *status;     // LZ_* below
+  int *iter;       // iteration index within the pass
+  int *k;          // iterations of the finished pass
+  int *npass, *niter;
+  double *sc;      // LZ_NSC scalars per segment
+  double *T;       // 2*miter per segment: diag[miter], upper[miter]
+  double *y;       // miter per segment: Ritz coefficients
+  double *partial; // 4 per tile
+  unsigned *cnt;   // last-block counters, one per segment
+  int *running;    // count of RUNNING segments
+};
+enum { LZ_IDLE = 0, LZ_SETUP = 1, LZ_RUNNING = 2, LZ_PASSDONE = 3, LZ_FINISHED = 4 };
+// scalar slots
+enum {
+  SC_RTR = 0, SC_RTZ1, SC_RTZ2, SC_PAP, SC_PAPOLD, SC_BETA, SC_ALPHA, SC_MEANP,
+  SC_RNI, SC_RNORM, SC_RTOL, SC_RNORM0, SC_MEANF, SC_LAMBDA, SC_NORMF, SC_TMP, SC_RES,
+  SC_M0, SC_N0,
+  LZ_NSC
+};
+
+struct rsbgpu {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char err[512] = {0};
+
+  // mesh
+  int nel = 0, nv = 0, ndim = 0;
+  const long long *d_vtx = nullptr;
+  const double *d_xyz = nullptr;
+  long long *own_vtx = nullptr;
+  double *own_xyz = nullptr;
+  double *d_coord = nullptr;  // SoA x[nel], y[nel], z[nel]
+  uint32_t *d_rank = nullptr; // SoA dense coordinate ranks per axis
+  int rank_bits = 1;
+  int has_coord = 0;
+
+  // weighted dual graph of the whole mesh (CSR, columns ascending)
+  int64_t *d_growptr = nullptr;
+  uint32_t *d_gadj = nullptr; // col | weight << 28
+  int64_t gnnz = 0, gwsum = 0;
+
+  // level state
+  int *d_order = nullptr, *d_order2 = nullptr; // element at compact position
+  int *d_segof = nullptr;                      // segment of each element
+  int nseg = 0;
+  std::vector<int> h_cstart, h_n, h_p, h_r0, h_pstart;
+  int *d_seg = nullptr;  // 5*nseg: cstart, n, p, r0, pstart
+  int64_t mpad = 0;
+  int *d_row_elem = nullptr, *d_elem_row = nullptr;
+  // SELL-64 level Laplacian
+  int64_t nslices = 0, lnnz_pad = 0, lnnz = 0;
+  int64_t *d_slice_off = nullptr;
+  int *d_slice_w = nullptr;
+  uint32_t *d_ell = nullptr;
+  int *d_rowlen = nullptr;
+  int64_t *d_seg_nnz = nullptr;
+  std::vector<int64_t> h_seg_nnz;
+  // tiles of the level (a tile never straddles a segment)
+  int ntiles = 0;
+  int *d_tile = nullptr;       // 3*ntiles: seg, row0, row1
+  int *d_seg_tiles = nullptr;  // 2*nseg: first tile, tile count
+  std::vector<int> h_tile, h_seg_tiles;
+
+  // Lanczos vectors (padded rows)
+  void *d_r = nullptr, *d_p = nullptr, *d_w = nullptr, *d_f = nullptr, *d_basis = nullptr;
+  size_t vec_bytes = 0, basis_bytes = 0;
+  double *d_fied = nullptr;     // Fiedler value per compact position
+  float *d_start = nullptr;     // optional start vector per element
+  lz_state lz{};
+  int lz_nseg_cap = 0, lz_tile_cap = 0, lz_miter_cap = 0;
+
+  // sort scratch
+  void *d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  uint64_t *d_k0 = nullptr, *d_k1 = nullptr;
+  int *d_v1 = nullptr;
+  size_t sort_cap = 0;
+
+  // generic scratch
+  int *d_iscratch = nullptr;
+  size_t iscratch_n = 0;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double ph_ms[PH_N] = {0};
+  double lz_bytes = 0, lz_ms = 0;      // iteration kernels: model bytes, loop time
+  long long lz_launches = 0;           // iterations launched (3 kernels each)
+  double spmv_bytes = 0, spmv_ms = 0;  // SpMV kernel alone: model bytes, kernel time
+  long long spmv_launches = 0;
+  int prof_spmv = 0;                   // bracket every SpMV launch with events
+  std::vector<hipEvent_t> spmv_ev;
+  hipEvent_t lz_ev0 = nullptr, lz_ev1 = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// error helpers
+#define RSB_TRY(expr)                                                           \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      snprintf(g->err, sizeof(g->err), "%s:%d: %s: %s", __FILE__, __LINE__,     \
+               #expr, hipGetErrorString(e_));                                   \
+      return 2;                                                                 \
+    }                                                                           \
+  } while (0)
+
+#define RSB_LAUNCH_CHECK() RSB_TRY(hipGetLastError())
+
+#define RSB_FAIL(code, ...)                                                     \
+  do {                                                                          \
+    snprintf(g->err, sizeof(g->err), __VA_ARGS__);                             \
+    return (code);                                                              \
+  } while (0)
+
+// grow-only device allocation
+template <typename T>
+inline hipError_t dev_ensure(T *&p, size_t &cap_elems, size_t need) {
+  if (need <= cap_elems && p) return hipSuccess;
+  if (p) {
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return e;
+    p = nullptr;
+  }
+  size_t n = need ? need : 1;
+  hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+  if (e == hipSuccess) cap_elems = n;
+  return e;
+}
+
+template <typename T>
+inline hipError_t dev_alloc(T *&p, size_t n) {
+  if (p) {
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return e;
+    p = nullptr;
+  }
+  return hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+}
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// sort helpers (sort.hip)
+int rsb_sort_pairs_u64(rsbgpu *g, uint64_t *keys, int *vals, int64_t n, int begin_bit,
+                       int end_bit);
+int rsb_sort_pairs_u32(rsbgpu *g, uint32_t *keys, int *vals, int64_t n, int begin_bit,
+                       int end_bit);
+int rsb_exclusive_scan_i64(rsbgpu *g, const int64_t *in, int64_t *out, int64_t n);
+int rsb_exclusive_scan_i32(rsbgpu *g, const int *in, int *out, int64_t n);
+
+// phase timing
+void rsb_phase_begin(rsbgpu *g);
+void rsb_phase_end(rsbgpu *g, int phase);

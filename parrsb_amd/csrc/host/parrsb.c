@@ -1,0 +1,317 @@
+/*
+ * Public entry points: parrsb_part_mesh (drop-in for parrsb.c:853-906) and
+ * the device-pointer variant.  Host code in C; the device work goes through
+ * the rsbgpu C ABI.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "parrsb_host.h"
+
+/* MPI is used only to learn the size of `comm`, and only if the process has
+ * MPI loaded and initialised; the library itself does not link MPI. */
+#if defined(PARRSB_NO_MPI)
+int MPI_Initialized(int *flag) __attribute__((weak));
+int MPI_Comm_size(MPI_Comm comm, int *size) __attribute__((weak));
+#else
+#pragma weak MPI_Initialized
+#pragma weak MPI_Comm_size
+#endif
+
+/* Defaults of parrsb.c:26-44, plus nparts = 0 (one part per rank). */
+parrsb_options parrsb_default_options = {
+    .partitioner = 0,
+    .tagged = 0,
+    .levels = 2,
+    .find_disconnected_comps = 1,
+    .repair = 0,
+    .verbose_level = 1,
+    .profile_level = 0,
+    .rsb_algo = 0,
+    .rsb_pre = 1,
+    .rsb_max_iter = 50,
+    .rsb_max_passes = 50,
+    .rsb_tol = 1e-5,
+    .rsb_dump_stats = 0,
+    .rsb_mg_grammian = 0,
+    .rsb_mg_factor = 2,
+    .nparts = 0};
+
+static parrsb_amd_report last_report;
+static char last_error[512];
+static int cfg_precision = 64, cfg_profiling = 0;
+
+const char *parrsb_amd_last_error(void) { return last_error; }
+
+int parrsb_amd_last_report(parrsb_amd_report *rep) {
+  if (!rep) return 1;
+  *rep = last_report;
+  return 0;
+}
+
+int parrsb_amd_set_precision(int bits) {
+  if (bits != 32 && bits != 64) return 3;
+  cfg_precision = bits;
+  return 0;
+}
+
+int parrsb_amd_set_profiling(int on) {
+  cfg_profiling = on != 0;
+  return 0;
+}
+
+static void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(last_error, sizeof(last_error), fmt, ap);
+  va_end(ap);
+}
+
+/* PARRSB_* environment overrides written into the caller's struct
+ * (parrsb.c:48-77), plus PARRSB_NPARTS. */
+static void update_options(parrsb_options *o) {
+#define UPD(field, name, is_int)                 \
+  do {                                           \
+    const char *v_ = getenv(name);               \
+    if (v_) {                                    \
+      if (is_int) o->field = atoi(v_);           \
+      else o->field = atof(v_);                  \
+    }                                            \
+  } while (0)
+  UPD(partitioner, "PARRSB_PARTITIONER", 1);
+  UPD(tagged, "PARRSB_TAGGED", 1);
+  UPD(levels, "PARRSB_LEVELS", 1);
+  UPD(find_disconnected_comps, "PARRSB_FIND_DISCONNECTED_COMPONENTS", 1);
+  UPD(repair, "PARRSB_REPAIR", 1);
+  UPD(verbose_level, "PARRSB_VERBOSE_LEVEL", 1);
+  UPD(profile_level, "PARRSB_PROFILE_LEVEL", 1);
+  UPD(rsb_algo, "PARRSB_RSB_ALGO", 1);
+  UPD(rsb_pre, "PARRSB_RSB_PRE", 1);
+  UPD(rsb_max_iter, "PARRSB_RSB_MAX_ITER", 1);
+  UPD(rsb_max_passes, "PARRSB_RSB_MAX_PASSES", 1);
+  UPD(rsb_tol, "PARRSB_RSB_TOL", 0);
+  UPD(rsb_mg_grammian, "PARRSB_RSB_MG_GRAMMIAN", 1);
+  UPD(rsb_mg_factor, "PARRSB_RSB_MG_FACTOR", 1);
+  UPD(nparts, "PARRSB_NPARTS", 1);
+#undef UPD
+}
+
+static void print_options(const parrsb_options *o) {
+  if (o->verbose_level <= 0) return;
+#define PO(field, name, fmt) printf("%s = " fmt "\n", name, o->field)
+  PO(partitioner, "PARRSB_PARTITIONER", "%d");
+  PO(tagged, "PARRSB_TAGGED", "%d");
+  PO(levels, "PARRSB_LEVELS", "%d");
+  PO(find_disconnected_comps, "PARRSB_FIND_DISCONNECTED_COMPONENTS", "%d");
+  PO(repair, "PARRSB_REPAIR", "%d");
+  PO(verbose_level, "PARRSB_VERBOSE_LEVEL", "%d");
+  PO(profile_level, "PARRSB_PROFILE_LEVEL", "%d");
+  PO(rsb_algo, "PARRSB_RSB_ALGO", "%d");
+  PO(rsb_pre, "PARRSB_RSB_PRE", "%d");
+  PO(rsb_max_iter, "PARRSB_RSB_MAX_ITER", "%d");
+  PO(rsb_max_passes, "PARRSB_RSB_MAX_PASSES", "%d");
+  PO(rsb_tol, "PARRSB_RSB_TOL", "%lf");
+  PO(rsb_mg_grammian, "PARRSB_RSB_MG_GRAMMIAN", "%d");
+  PO(rsb_mg_factor, "PARRSB_RSB_MG_FACTOR", "%d");
+  PO(nparts, "PARRSB_NPARTS", "%d");
+#undef PO
+  fflush(stdout);
+}
+
+static double now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static int device_index(void) {
+  const char *v = getenv("PARRSB_AMD_DEVICE");
+  return v ? atoi(v) : 0;
+}
+
+static int precision_bits(void) {
+  const char *v = getenv("PARRSB_AMD_PRECISION");
+  if (v) return atoi(v) == 32 ? 32 : 64;
+  return cfg_precision;
+}
+
+/* Warnings of check_rsb_partition (rsb.c:61-111). */
+static void check_partition(const parrsb_amd_report *r, const parrsb_options *o) {
+  for (int l = 0; l < r->nlevels && l < PARRSB_AMD_MAX_LEVELS; l++) {
+    if (r->nconverged_fail[l] > 0)
+      fprintf(stderr,
+              "Warning: Lanczos did not reach the target residual after %d x %d iterations "
+              "in Level=%d (%d segments)!\n",
+              o->rsb_max_passes, o->rsb_max_iter, l, r->nconverged_fail[l]);
+    if (o->find_disconnected_comps == 1 && r->ncomp_max[l] > 1)
+      fprintf(stderr,
+              "Warning: Partition created %d/%d (min/max) disconnected components in "
+              "Level=%d!\n",
+              r->ncomp_min[l], r->ncomp_max[l], l);
+  }
+  fflush(stderr);
+}
+
+static void print_profile(const parrsb_amd_report *r) {
+  printf("phase ms: setup %.3f pre %.3f laplacian %.3f lanczos %.3f sort %.3f components "
+         "%.3f balance %.3f\n",
+         r->phase_ms[0], r->phase_ms[1], r->phase_ms[2], r->phase_ms[3], r->phase_ms[5],
+         r->phase_ms[6], r->phase_ms[7]);
+  for (int l = 0; l < r->nlevels && l < PARRSB_AMD_MAX_LEVELS; l++)
+    printf("level=%02d segments=%d RSB_FIEDLER_CALC_NITER=%d/%d RSB_COMPONENTS_NCOMP=%d/%d "
+           "lambda=%.6e/%.6e\n",
+           l, r->nsegs[l], r->niter_min[l], r->niter_max[l], r->ncomp_min[l], r->ncomp_max[l],
+           r->lambda_min[l], r->lambda_max[l]);
+  fflush(stdout);
+}
+
+/* Shared body of the host- and device-pointer entry points. */
+static int run(int *part, const long long *vtx, const double *xyz, int nel, int nv,
+               parrsb_options *options, int nparts, int on_device) {
+  double t0 = now();
+  memset(&last_report, 0, sizeof(last_report));
+  last_error[0] = 0;
+  if (vtx == NULL && xyz == NULL) {
+    set_error("both vertices and coordinates are NULL");
+    return 4;
+  }
+  if (nv != 4 && nv != 8) {
+    set_error("nv must be 4 or 8");
+    return 3;
+  }
+  if (options->partitioner == 2 || (options->partitioner == 0 && options->rsb_pre == 2)) {
+    set_error("RIB is not provided by this build");
+    return 3;
+  }
+  if (options->partitioner == 0 && options->rsb_algo != 0) {
+    set_error("only the Lanczos solver (rsb_algo = 0) is provided");
+    return 3;
+  }
+  if (vtx == NULL && options->partitioner == 0) {
+    set_error("RSB needs vertex ids");
+    return 3;
+  }
+  if (xyz == NULL) options->rsb_pre = 0; /* parrsb.c:250 */
+  if (options->levels > 2) options->levels = 2; /* parrsb.c:228-229 */
+  if (nparts < 1) {
+    set_error("number of parts must be >= 1");
+    return 3;
+  }
+  rsbgpu *g = NULL;
+  int rc = rsbgpu_open(&g, device_index());
+  if (rc) {
+    set_error("no usable GPU (device %d)", device_index());
+    return 2;
+  }
+  rsbgpu_set_profiling(g, cfg_profiling);
+  rc = rsbgpu_set_mesh(g, vtx, xyz, nel, nv, on_device);
+  parrsb_amd_report *rep = &last_report;
+  rep->nparts = nparts, rep->nel = nel, rep->precision = precision_bits();
+  if (!rc) {
+    int64_t nnz = 0, ws = 0;
+    rsbgpu_graph_stats(g, &nnz, &ws);
+    rep->graph_nnz = nnz, rep->graph_wsum = ws;
+    if (options->partitioner == 1) {
+      rc = rcb_run(g, nel, nparts);
+    } else {
+      rsb_params prm;
+      prm.nparts = nparts, prm.rsb_pre = options->rsb_pre;
+      prm.max_iter = options->rsb_max_iter, prm.max_passes = options->rsb_max_passes;
+      prm.find_comps = options->find_disconnected_comps, prm.precision = precision_bits();
+      prm.verbose = options->verbose_level, prm.tol = options->rsb_tol;
+      prm.residuals = 1;
+      rc = rsb_run(g, nel, &prm, rep);
+    }
+  }
+  if (!rc) rc = rsbgpu_write_part(g, part, on_device);
+  if (rc) set_error("%s", rsbgpu_error(g));
+  rsbgpu_phase_times(g, rep->phase_ms);
+  double tr[6];
+  rsbgpu_lanczos_traffic(g, tr);
+  rep->lanczos_bytes = tr[0], rep->lanczos_ms = tr[1], rep->lanczos_iters = tr[2];
+  rep->spmv_bytes = tr[3], rep->spmv_ms = tr[4], rep->spmv_launches = tr[5];
+  rsbgpu_close(g);
+  rep->seconds = now() - t0;
+  if (rc) return rc;
+  if (options->partitioner == 0) check_partition(rep, options);
+  if (options->verbose_level > 0) {
+    static const char *algo[3] = {"RSB", "RCB", "RIB"};
+    printf("par%s finished in %g seconds.\n", algo[options->partitioner % 3], rep->seconds);
+    fflush(stdout);
+  }
+  if (options->profile_level > 0) print_profile(rep);
+  return 0;
+}
+
+int parrsb_part_mesh(int *part, const long long *const vtx, const double *const xyz,
+                     const int *const tag, const int nel, const int nv,
+                     parrsb_options *const options, MPI_Comm comm) {
+  update_options(options);
+  int np = 1;
+  if (MPI_Initialized && MPI_Comm_size) {
+    int init = 0;
+    MPI_Initialized(&init);
+    if (init) MPI_Comm_size(comm, &np);
+  }
+  if (options->verbose_level > 0) {
+    printf("Running parRSB ..., nv = %d, nelg = %d\n", nv, nel);
+    fflush(stdout);
+  }
+  print_options(options);
+  if (options->tagged == 1 && !tag) { /* parrsb.c:873-877 */
+    set_error("tagged partitioning requested but tag array is NULL");
+    return 1;
+  }
+  if (options->tagged == 1) {
+    set_error("tagged partitioning is not provided by this build");
+    return 3;
+  }
+  if (np > 1) {
+    set_error("all elements must be passed from one rank (communicator size %d)", np);
+    return 3;
+  }
+  int nparts = options->nparts > 0 ? options->nparts : np;
+  return run(part, vtx, xyz, nel, nv, options, nparts, 0);
+}
+
+int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *d_xyz,
+                             int nel, int nv, parrsb_options *options) {
+  update_options(options);
+  if (options->tagged == 1) {
+    set_error("tagged partitioning is not provided by this build");
+    return 3;
+  }
+  return run(d_part, d_vtx, d_xyz, nel, nv, options, options->nparts, 1);
+}
+
+static int quality(const int *part, const long long *vtx, int nel, int nv, int nparts,
+                   int64_t *out, int on_device) {
+  rsbgpu *g = NULL;
+  if (rsbgpu_open(&g, device_index())) return 2;
+  int rc = rsbgpu_set_mesh(g, vtx, NULL, nel, nv, on_device);
+  const int *dp = part;
+  int *tmp = NULL;
+  if (!rc && !on_device) {
+    rc = rsbgpu_upload_ints(g, part, nel, &tmp);
+    dp = tmp;
+  }
+  if (!rc) rc = rsbgpu_quality(g, dp, nparts, out);
+  if (rc) set_error("%s", rsbgpu_error(g));
+  if (tmp) rsbgpu_free(tmp);
+  rsbgpu_close(g);
+  return rc;
+}
+
+int parrsb_amd_partition_quality(const int *part, const long long *vtx, int nel, int nv,
+                                 int nparts, int64_t *out) {
+  return quality(part, vtx, nel, nv, nparts, out, 0);
+}
+
+int parrsb_amd_partition_quality_dev(const int *d_part, const long long *d_vtx, int nel,
+                                     int nv, int nparts, int64_t *out) {
+  return quality(d_part, d_vtx, nel, nv, nparts, out, 1);
+}

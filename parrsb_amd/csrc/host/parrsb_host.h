@@ -1,0 +1,19 @@
+/* Internal declarations of the C host layer. */
+#ifndef PARRSB_AMD_HOST_H
+#define PARRSB_AMD_HOST_H
+
+#include "parrsb_amd.h"
+#include "rsbgpu.h"
+
+typedef struct {
+  int nparts, rsb_pre, max_iter, max_passes, find_comps, precision, verbose, residuals;
+  double tol;
+} rsb_params;
+
+long long rsb_lower_count(long long n, int p);
+long long rsb_balance_expected(long long n, int p, int bin);
+int rsb_ncuts(int nparts);
+int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep);
+int rcb_run(rsbgpu *g, int nel, int nparts);
+
+#endif
