@@ -6,13 +6,14 @@
 // products) are finished by the last of its tile-blocks to arrive (a
 // counter per segment), in a fixed order, so results do not depend on block
 // scheduling and no extra launch is needed per reduction.  One CG-Lanczos
-// iteration is three launches over all rows of the level:
-//   k_lz_pupdate  p = beta (p - mean_p) + r, basis[it] = r / |r|, sum(p)
-//   k_lz_spmv     w = L p (SELL-64), pap = w . (p - mean_p)
+// iteration is two launches over all rows of the level:
+//   k_lz_apply    p = beta (p_old - mean) + r formed on the fly for each row
+//                 and its neighbours, w = L p (SELL-64), basis[it] = r / |r|,
+//                 pap = w . (p - mean(p))
 //   k_lz_rupdate  r = r - alpha w, rtr = r . r, tridiagonal entries, stop test
 // The mean of p is removed lazily (fiedler.c:20-30 `ortho`): w = L p is
 // evaluated in the difference form sum_j w_ij (p_i - p_j), which the mean
-// does not change.  Segments that converged or are between passes skip every
+// does not change, and the stored p keeps its mean, subtracted when used.  Segments that converged or are between passes skip every
 // launch at the first instruction.
 //
 // Precision: vectors are T (double, or float); every scalar and the
@@ -198,60 +199,99 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_setup(lz_args a, int phase, co
 }
 
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_pupdate(lz_args a, const T *__restrict__ r,
-                                                          T *__restrict__ p,
-                                                          T *__restrict__ basis, int64_t mpad) {
-  tile_t t;
-  if (!tile_of(a, LZ_RUNNING, t)) return;
-  const int s = t.s, it = a.iter[s];
-  const T beta = (T)SC(s, SC_BETA), mp = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
-  T *b = basis + (size_t)it * mpad;
-  acc_t<T> acc;
-  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
-    int i = row - t.pstart;
-    T rv = r[row];
-    T pn = (i < t.n) ? (it == 0 ? rv : beta * (p[row] - mp) + rv) : (T)0;
-    p[row] = pn;
-    b[row] = rv * rni;
-    acc.add(pn);
+// One row of L q in the difference form sum_j w_ij (q_i - q_j), where q is
+// formed on the fly from the previous search direction and the residual:
+// q = beta (p - m) + r (q = r on the first iteration of a pass).  The packed
+// entries of a SELL-64 slice are read 8 at a time so that each thread keeps 8
+// gathers in flight.
+template <typename T, bool FIRST>
+__device__ inline T qval(const T *__restrict__ p, const T *__restrict__ r, uint32_t c, T beta,
+                         T m) {
+  return FIRST ? r[c] : beta * (p[c] - m) + r[c];
+}
+
+template <typename T, bool FIRST>
+__device__ inline T lap_row_q(const uint32_t *__restrict__ e, int wd, T qi,
+                              const T *__restrict__ p, const T *__restrict__ r, T beta, T m) {
+  T sum = (T)0;
+  int j = 0;
+  for (; j + 8 <= wd; j += 8) {
+    uint32_t q[8];
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) q[u] = __builtin_nontemporal_load(e + (size_t)(j + u) * RSB_WAVE);
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = qval<T, FIRST>(p, r, q[u] & 0x0fffffffu, beta, m);
+#pragma unroll
+    for (int u = 0; u < 8; u++) sum += (T)(q[u] >> 28) * (qi - v[u]);
   }
-  double v[1] = {acc.value()}, tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) SC(s, SC_MEANP) = tot[0] / t.n;
+  for (; j < wd; j++) {
+    uint32_t q = __builtin_nontemporal_load(e + (size_t)j * RSB_WAVE);
+    sum += (T)(q >> 28) * (qi - qval<T, FIRST>(p, r, q & 0x0fffffffu, beta, m));
+  }
+  return sum;
+}
+
+// Fused search-direction update and Laplacian apply (fiedler.c:460-475):
+//   q = beta (p_old - mean_p) + r,  w = L q,  basis[it] = r / |r|
+// and the segment sums of q, w.q and w, from which the last block forms
+// mean_p and pap = w.(q - mean_p), then alpha = rtz1 / pap.
+template <typename T, bool FIRST>
+__device__ inline void lz_apply_rows(const lz_args &a, const tile_t &t, const T *__restrict__ r,
+                                     const T *__restrict__ pold, T *__restrict__ pnew,
+                                     T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
+                                     const int64_t *__restrict__ slice_off,
+                                     const int *__restrict__ slice_w,
+                                     const uint32_t *__restrict__ ell, acc_t<T> &sq,
+                                     acc_t<T> &swq, acc_t<T> &sw) {
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    const bool real = row - t.pstart < t.n;
+    const T ri = r[row];
+    T qi = (T)0, wi = (T)0;
+    if (real) {
+      qi = FIRST ? ri : beta * (pold[row] - m) + ri;
+      const int sl = row >> 6;
+      wi = lap_row_q<T, FIRST>(ell + slice_off[sl] + (row & 63), slice_w[sl], qi, pold, r, beta,
+                               m);
+      sq.add(qi);
+      swq.dot(wi, qi);
+      sw.add(wi);
+    }
+    pnew[row] = qi;
+    w[row] = wi;
+    b[row] = ri * rni;
+  }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_spmv(lz_args a, const T *__restrict__ p,
-                                                       T *__restrict__ w,
-                                                       const int64_t *__restrict__ slice_off,
-                                                       const int *__restrict__ slice_w,
-                                                       const uint32_t *__restrict__ ell) {
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ r,
+                                                        const T *__restrict__ pold,
+                                                        T *__restrict__ pnew, T *__restrict__ w,
+                                                        T *__restrict__ basis, int64_t mpad,
+                                                        const int64_t *__restrict__ slice_off,
+                                                        const int *__restrict__ slice_w,
+                                                        const uint32_t *__restrict__ ell) {
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t)) return;
-  const int s = t.s;
-  const T mp = (T)SC(s, SC_MEANP);
-  acc_t<T> acc;
-  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
-    const int sl = row >> 6, lane = row & 63;
-    const uint32_t *e = ell + slice_off[sl] + lane;
-    const int wd = slice_w[sl];
-    const T pi = p[row];
-    T sum = (T)0;
-    for (int j = 0; j < wd; j++) {
-      uint32_t q = e[(size_t)j * RSB_WAVE];
-      T pj = p[q & 0x0fffffffu];
-      sum += (T)(q >> 28) * (pi - pj);
-    }
-    w[row] = sum;
-    if (row - t.pstart < t.n) acc.dot(sum, pi - mp);
-  }
-  double v[1] = {acc.value()}, tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
+  const int s = t.s, it = a.iter[s];
+  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
+  acc_t<T> sq, swq, sw;
+  T *b = basis + (size_t)it * mpad;
+  if (it == 0)
+    lz_apply_rows<T, true>(a, t, r, pold, pnew, w, b, beta, m, rni, slice_off, slice_w, ell, sq,
+                           swq, sw);
+  else
+    lz_apply_rows<T, false>(a, t, r, pold, pnew, w, b, beta, m, rni, slice_off, slice_w, ell, sq,
+                            swq, sw);
+  double v[3] = {sq.value(), swq.value(), sw.value()}, tot[3];
+  if (!seg_reduce<3>(a, t, v, tot)) return;
   if (threadIdx.x == 0) {
+    double mq = tot[0] / t.n;
+    double pap = tot[1] - mq * tot[2];
+    SC(s, SC_MEANP) = mq;
     SC(s, SC_PAPOLD) = SC(s, SC_PAP);
-    SC(s, SC_PAP) = tot[0];
-    SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / tot[0];
+    SC(s, SC_PAP) = pap;
+    SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / pap;
   }
 }
 
@@ -553,7 +593,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // buffers
   size_t vb = (size_t)mpad * sizeof(T);
   if (vb > g->vec_bytes) {
-    for (void **q : {&g->d_r, &g->d_p, &g->d_w, &g->d_f}) {
+    for (void **q : {&g->d_r, &g->d_p, &g->d_p2, &g->d_w, &g->d_f}) {
       if (*q) RSB_TRY(hipFree(*q));
       *q = nullptr;
       RSB_TRY(hipMalloc(q, vb + 256));
@@ -598,7 +638,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
   a.running = L.running;
   a.mpass = mpass, a.tol = tol, a.miter_max = miter_max;
-  T *r = (T *)g->d_r, *p = (T *)g->d_p, *w = (T *)g->d_w, *f = (T *)g->d_f,
+  T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
   hipStream_t st = g->stream;
 
@@ -627,10 +667,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
     }
     for (it = 0; it < miter_max; it++) {
-      k_lz_pupdate<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, p, basis, mpad);
+      T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
       if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-      k_lz_spmv<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, p, w, g->d_slice_off, g->d_slice_w,
-                                                 g->d_ell);
+      k_lz_apply<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, pold, pnew, w, basis, mpad,
+                                                  g->d_slice_off, g->d_slice_w, g->d_ell);
       if (g->prof_spmv) {
         RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
         nrec++;
@@ -685,13 +725,14 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   (void)hipFree(xrow);
   // Algorithmic bytes, per iteration and segment of n rows and nnz
   // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 9 s n vector
-  // streams for the whole iteration; 4(n+1) + 4 nnz + 2 s n for the SpMV
-  // (x read once, w written).  Counted iterations, real (unpadded) sizes.
+  // streams for the whole iteration; for k_lz_apply alone 4(n+1) + 4 nnz +
+  // 5 s n (p_old and r read once, p_new, w and the basis vector written).
+  // Counted iterations, real (unpadded) sizes.
   for (int s = 0; s < nseg; s++) {
     if (g->h_p[s] > 1 && g->h_n[s] >= 2) {
       double n = g->h_n[s], nz = (double)g->h_seg_nnz[s], sz = sizeof(T);
       g->lz_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 9 * sz * n);
-      g->spmv_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 2 * sz * n);
+      g->spmv_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 5 * sz * n);
     }
     if (out) {
       if (out->niter) out->niter[s] = ni[s];

@@ -86,6 +86,7 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_seg_tiles);
   FREE(g->d_r);
   FREE(g->d_p);
+  FREE(g->d_p2);
   FREE(g->d_w);
   FREE(g->d_f);
   FREE(g->d_basis);

@@ -135,19 +135,41 @@ __global__ void k_node_extent_init(int K, uint64_t *__restrict__ ext) {
   }
 }
 
-__global__ void k_node_extent(const int *__restrict__ order, const int *__restrict__ nodeof,
-                              const int *__restrict__ frozen, const double *__restrict__ coord,
-                              int nel, int ndim, int K, unsigned long long *__restrict__ ext) {
+__global__ void __launch_bounds__(256) k_node_extent(const int *__restrict__ order,
+                                                     const int *__restrict__ nodeof,
+                                                     const int *__restrict__ frozen,
+                                                     const double *__restrict__ coord, int nel,
+                                                     int ndim, int K,
+                                                     unsigned long long *__restrict__ ext) {
+  __shared__ uint64_t smin[4][3], smax[4][3];
+  __shared__ int block_node;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool act = i < nel;
   int node = act ? nodeof[i] : -1;
   if (act && frozen[node]) act = false;
   int e = act ? order[i] : 0;
-  // Reduce runs of equal node within the wave before the atomics.
-  const int lane = threadIdx.x & (RSB_WAVE - 1);
+  // The whole block inside one node (large nodes): reduce in the block and
+  // issue one atomic per extent.
+  if (threadIdx.x == 0) {
+    int last = min((int)(blockIdx.x * blockDim.x + blockDim.x), nel) - 1;
+    block_node = (act && nodeof[last] == node) ? node : -1;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & (RSB_WAVE - 1), wid = threadIdx.x / RSB_WAVE;
+  const bool whole = block_node >= 0;
   for (int d = 0; d < ndim; d++) {
     uint64_t kv = act ? dkey(coord[(size_t)d * nel + e]) : 0;
     uint64_t mn = act ? kv : ~0ull, mx = act ? kv : 0ull;
+    if (whole) {
+      for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
+        uint64_t a = __shfl_xor(mn, o, RSB_WAVE), b = __shfl_xor(mx, o, RSB_WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+      }
+      if (lane == 0) smin[wid][d] = mn, smax[wid][d] = mx;
+      continue;
+    }
+    // runs of equal node within the wave, then one atomic per run
     for (int o = 1; o < RSB_WAVE; o <<= 1) {
       int nb = __shfl_down(node, o, RSB_WAVE);
       uint64_t a = __shfl_down(mn, o, RSB_WAVE), b = __shfl_down(mx, o, RSB_WAVE);
@@ -162,6 +184,18 @@ __global__ void k_node_extent(const int *__restrict__ order, const int *__restri
       atomicMin(&ext[(size_t)(2 * d) * K + node], (unsigned long long)mn);
       atomicMax(&ext[(size_t)(2 * d + 1) * K + node], (unsigned long long)mx);
     }
+  }
+  if (!whole) return;
+  __syncthreads();
+  if (threadIdx.x < ndim) {
+    int d = threadIdx.x;
+    uint64_t mn = smin[0][d], mx = smax[0][d];
+    for (int w = 1; w < (int)blockDim.x / RSB_WAVE; w++) {
+      mn = smin[w][d] < mn ? smin[w][d] : mn;
+      mx = smax[w][d] > mx ? smax[w][d] : mx;
+    }
+    atomicMin(&ext[(size_t)(2 * d) * K + block_node], (unsigned long long)mn);
+    atomicMax(&ext[(size_t)(2 * d + 1) * K + block_node], (unsigned long long)mx);
   }
 }
 

@@ -83,7 +83,8 @@ struct rsbgpu {
   std::vector<int> h_tile, h_seg_tiles;
 
   // Lanczos vectors (padded rows)
-  void *d_r = nullptr, *d_p = nullptr, *d_w = nullptr, *d_f = nullptr, *d_basis = nullptr;
+  void *d_r = nullptr, *d_p = nullptr, *d_p2 = nullptr, *d_w = nullptr, *d_f = nullptr;
+  void *d_basis = nullptr;
   size_t vec_bytes = 0, basis_bytes = 0;
   double *d_fied = nullptr;     // Fiedler value per compact position
   float *d_start = nullptr;     // optional start vector per element
