@@ -2,11 +2,13 @@
 // the reference's restarted CG-Lanczos (fiedler.c:434-621) at once.
 //
 // Rows of the level are grouped in tiles of RSB_TILE rows that never cross a
-// segment, one workgroup per tile.  A segment's scalar reductions (means, dot
-// products) are finished by the last of its tile-blocks to arrive (a
-// counter per segment), in a fixed order, so results do not depend on block
-// scheduling and no extra launch is needed per reduction.  One CG-Lanczos
-// iteration is two launches over all rows of the level:
+// segment, one workgroup per tile.  Each workgroup writes the partial sums of
+// its tile (fixed reduction tree), so results do not depend on block
+// scheduling.  The segment totals and scalar updates are done by a small
+// k_lz_fin launch after each reduction (one workgroup per segment), so no
+// workgroup ever waits on another inside a launch.  One CG-Lanczos
+// iteration is two launches over all rows of the level, each followed by
+// its k_lz_fin:
 //   k_lz_apply    p = beta (p_old - mean) + r formed on the fly for each row
 //                 and its neighbours, w = L p (SELL-64), basis[it] = r / |r|,
 //                 pap = w . (p - mean(p))
@@ -88,23 +90,15 @@ __device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t) {
   return true;
 }
 
-// Publish NV block partials, and in the segment's last block reduce all of
-// them (fixed order).  Returns true in that block, with tot[] in thread 0.
+// Write the block's partial sums (a fixed reduction tree) for the tile.
+// The segment totals are formed by k_lz_fin in the next launch: no
+// cross-workgroup handshake inside a kernel, so no device-scope fences.
 template <int NV>
-__device__ inline bool seg_reduce(const lz_args &a, const tile_t &t, double (&v)[NV],
-                                  double (&tot)[NV]) {
+__device__ inline void publish(const lz_args &a, double (&v)[NV]) {
   __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   block_sum<NV>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) a.partial[(size_t)blockIdx.x * 4 + k] = v[k];
-  if (!last_block_of(&a.cnt[t.s], (unsigned)t.count)) return false;
-  double w[NV];
-  for (int k = 0; k < NV; k++) w[k] = 0.0;
-  for (int i = threadIdx.x; i < t.count; i += blockDim.x)
-    for (int k = 0; k < NV; k++) w[k] += load_acquire(&a.partial[(size_t)(t.first + i) * 4 + k]);
-  block_sum<NV>(w, red);
-  for (int k = 0; k < NV; k++) tot[k] = w[k];
-  return true;
+    for (int k = 0; k < 4; k++) a.partial[(size_t)blockIdx.x * 4 + k] = k < NV ? v[k] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -137,12 +131,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_start(lz_args a, int phase, in
     if (phase == 2) f[row] = i < t.n ? (T)((x - m) * sc) : (T)0;
   }
   if (phase == 2) return;
-  double tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    if (phase == 0) SC(t.s, SC_M0) = tot[0] / t.n;
-    else SC(t.s, SC_N0) = 1.0 / sqrt(tot[0]);
-  }
+  publish<1>(a, v);
 }
 
 // Pass setup (fiedler.c:441-456): r = f - mean_f; then r -= mean(r);
@@ -173,29 +162,8 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_setup(lz_args a, int phase, co
       }
     }
   }
-  double v[1] = {acc.value()}, tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    int s = t.s;
-    if (phase == 0) {
-      SC(s, SC_TMP) = tot[0] / t.n;
-    } else {
-      double rtr = tot[0], rn = sqrt(rtr);
-      SC(s, SC_RTR) = rtr;
-      SC(s, SC_RNORM) = rn;
-      SC(s, SC_RNORM0) = rn;
-      SC(s, SC_RTOL) = rn * a.tol;
-      SC(s, SC_RNI) = 1.0 / rn;
-      SC(s, SC_RTZ1) = rtr;  // iteration 0: rtz2 = 1 (initial rtz1), rtz1 = rtr
-      SC(s, SC_RTZ2) = 1.0;
-      SC(s, SC_BETA) = 0.0;
-      SC(s, SC_PAP) = 0.0;
-      SC(s, SC_MEANP) = 0.0;
-      a.iter[s] = 0;
-      a.status[s] = LZ_RUNNING;
-      atomicAdd(&a.running[0], 1);
-    }
-  }
+  double v[1] = {acc.value()};
+  publish<1>(a, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,36 +231,107 @@ __device__ inline void lz_apply_rows(const lz_args &a, const tile_t &t, const T 
   }
 }
 
+// LDS-tiled rows (tiles.hip): q for the tile's rows and its halo is staged
+// in LDS once; every neighbour is then read from LDS through a 16-bit local
+// index, 8 entries per 16-byte load.
+struct tiled_args {
+  const int64_t *slice_off;
+  const int *slice_w;
+  const uint32_t *ell;
+  const int *nhalo;
+  const int64_t *halo_off;
+  const int *halo;
+  const int64_t *off16;
+  const int *w16;
+  const uint16_t *e16;
+};
+
+#define TB_LOCAL 4096
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ inline void acc_entry(uint32_t e, T qi, const T *ql, T &sum) {
+  sum += (T)(e >> 12) * (qi - ql[e & 0xfffu]);
+}
+
+template <typename T, bool FIRST>
+__device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const tiled_args &ta,
+                                      int nh, T *ql, T *rl, const T *__restrict__ r,
+                                      const T *__restrict__ pold, T *__restrict__ pnew,
+                                      T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
+                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+  const int nrows = t.row1 - t.row0;
+  const int64_t ho = ta.halo_off[blockIdx.x];
+  const int nreal = min(nrows, t.pstart + t.n - t.row0);
+  for (int idx = threadIdx.x; idx < nrows + nh; idx += blockDim.x) {
+    const int gr = idx < nrows ? t.row0 + idx : ta.halo[ho + idx - nrows];
+    const T rg = r[gr];
+    T q = FIRST ? rg : beta * (pold[gr] - m) + rg;
+    if (idx < nrows) {
+      rl[idx] = rg;
+      if (idx >= nreal) q = (T)0;
+    }
+    ql[idx] = q;
+  }
+  __syncthreads();
+  for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
+    const int li = row - t.row0, sl = row >> 6;
+    const u32x4 *ep = (const u32x4 *)(ta.e16 + ta.off16[sl] + (row & 63) * 8);
+    const int nb = ta.w16[sl] >> 3;
+    const T qi = ql[li];
+    T sum = (T)0;
+    for (int jb = 0; jb < nb; jb++) {
+      const u32x4 v = __builtin_nontemporal_load(ep + (size_t)jb * RSB_WAVE);
+      acc_entry(v.x & 0xffffu, qi, ql, sum);
+      acc_entry(v.x >> 16, qi, ql, sum);
+      acc_entry(v.y & 0xffffu, qi, ql, sum);
+      acc_entry(v.y >> 16, qi, ql, sum);
+      acc_entry(v.z & 0xffffu, qi, ql, sum);
+      acc_entry(v.z >> 16, qi, ql, sum);
+      acc_entry(v.w & 0xffffu, qi, ql, sum);
+      acc_entry(v.w >> 16, qi, ql, sum);
+    }
+    pnew[row] = qi;
+    w[row] = sum;
+    b[row] = rl[li] * rni;
+    if (li < nreal) {
+      sq.add(qi);
+      swq.dot(sum, qi);
+      sw.add(sum);
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ r,
                                                         const T *__restrict__ pold,
                                                         T *__restrict__ pnew, T *__restrict__ w,
                                                         T *__restrict__ basis, int64_t mpad,
-                                                        const int64_t *__restrict__ slice_off,
-                                                        const int *__restrict__ slice_w,
-                                                        const uint32_t *__restrict__ ell) {
+                                                        tiled_args ta) {
+  __shared__ T ql[TB_LOCAL];
+  __shared__ T rl[RSB_TILE];
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t)) return;
   const int s = t.s, it = a.iter[s];
   const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
   acc_t<T> sq, swq, sw;
   T *b = basis + (size_t)it * mpad;
-  if (it == 0)
-    lz_apply_rows<T, true>(a, t, r, pold, pnew, w, b, beta, m, rni, slice_off, slice_w, ell, sq,
-                           swq, sw);
-  else
-    lz_apply_rows<T, false>(a, t, r, pold, pnew, w, b, beta, m, rni, slice_off, slice_w, ell, sq,
-                            swq, sw);
-  double v[3] = {sq.value(), swq.value(), sw.value()}, tot[3];
-  if (!seg_reduce<3>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    double mq = tot[0] / t.n;
-    double pap = tot[1] - mq * tot[2];
-    SC(s, SC_MEANP) = mq;
-    SC(s, SC_PAPOLD) = SC(s, SC_PAP);
-    SC(s, SC_PAP) = pap;
-    SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / pap;
+  const int nh = ta.nhalo[blockIdx.x];
+  if (nh >= 0) {
+    if (it == 0)
+      lz_apply_tiled<T, true>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw);
+    else
+      lz_apply_tiled<T, false>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                               sw);
+  } else if (it == 0) {
+    lz_apply_rows<T, true>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off, ta.slice_w,
+                           ta.ell, sq, swq, sw);
+  } else {
+    lz_apply_rows<T, false>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off, ta.slice_w,
+                            ta.ell, sq, swq, sw);
   }
+  double v[3] = {sq.value(), swq.value(), sw.value()};
+  publish<3>(a, v);
 }
 
 template <typename T>
@@ -308,35 +347,8 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
     r[row] = x;
     acc.dot(x, x);
   }
-  double v[1] = {acc.value()}, tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    // fiedler.c:486-509
-    const int it = a.iter[s], mi = a.miter[s];
-    double rtr = tot[0], rnorm = sqrt(rtr);
-    double rtz1 = SC(s, SC_RTZ1), rtz2 = SC(s, SC_RTZ2), beta = SC(s, SC_BETA);
-    double pap = SC(s, SC_PAP), pap_old = SC(s, SC_PAPOLD);
-    double *Td = a.T + (size_t)s * 2 * a.miter_max, *Tu = Td + a.miter_max;
-    if (it == 0) {
-      Td[0] = pap / rtz1;
-    } else {
-      Td[it] = (beta * beta * pap_old + pap) / rtz1;
-      Tu[it - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
-    }
-    SC(s, SC_RTR) = rtr;
-    SC(s, SC_RNORM) = rnorm;
-    SC(s, SC_RNI) = 1.0 / rnorm;
-    if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
-      a.k[s] = it + 1;
-      a.status[s] = LZ_PASSDONE;
-      atomicSub(&a.running[0], 1);
-    } else {
-      a.iter[s] = it + 1;
-      SC(s, SC_RTZ2) = rtz1;
-      SC(s, SC_RTZ1) = rtr;
-      SC(s, SC_BETA) = rtr / rtz1;
-    }
-  }
+  double v[1] = {acc.value()};
+  publish<1>(a, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -477,20 +489,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
     f[row] = (T)x;
     v[0] += (double)(T)x;
   }
-  double tot[1];
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    SC(s, SC_MEANF) = tot[0] / t.n;
-    int np = a.npass[s] + 1;
-    a.npass[s] = np;
-    a.niter[s] = (np - 1) * a.miter[s] + k;
-    if (k == a.miter[s] && np < a.mpass) {
-      a.status[s] = LZ_SETUP;
-      atomicAdd(&a.running[1], 1);
-    } else {
-      a.status[s] = LZ_FINISHED;
-    }
-  }
+  publish<1>(a, v);
 }
 
 // Final normalisation (fiedler.c:606-616) and scatter to compact positions.
@@ -514,7 +513,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_finish(lz_args a, int phase,
     }
     return;
   }
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
     int i = row - t.pstart;
     if (i < t.n) {
@@ -522,8 +521,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_finish(lz_args a, int phase,
       v[0] += x * x;
     }
   }
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) SC(s, SC_NORMF) = 1.0 / sqrt(tot[0]);
+  publish<1>(a, v);
 }
 
 // Eigen-residual of the final vector x (FP64): lambda_R = x.Lx, then
@@ -537,7 +535,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_resid(lz_args a, int phase,
   if (!tile_of(a, LZ_FINISHED, t)) return;
   const int s = t.s;
   const double lam = phase ? SC(s, SC_RES) : 0.0;
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
     const int sl = row >> 6, lane = row & 63;
     const uint32_t *e = ell + slice_off[sl] + lane;
@@ -553,10 +551,112 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_resid(lz_args a, int phase,
       v[0] += d * d;
     }
   }
-  if (!seg_reduce<1>(a, t, v, tot)) return;
-  if (threadIdx.x == 0) {
-    if (phase == 0) SC(s, SC_RES) = tot[0];
-    else SC(s, SC_TMP) = sqrt(tot[0]) / lam;
+  publish<1>(a, v);
+}
+
+// ---------------------------------------------------------------------------
+// Segment finalisation: the sums of a segment's tile partials (fixed order)
+// and the scalar updates of the phase that produced them.  One workgroup per
+// segment; segments not in the phase's state leave at once.
+enum {
+  FIN_START0 = 0, FIN_START1, FIN_SETUP0, FIN_SETUP1, FIN_APPLY, FIN_RUPDATE, FIN_RITZ,
+  FIN_FINISH, FIN_RES0, FIN_RES1
+};
+
+__device__ inline int fin_status(int op) {
+  switch (op) {
+    case FIN_START0: case FIN_START1: case FIN_SETUP0: case FIN_SETUP1: return LZ_SETUP;
+    case FIN_APPLY: case FIN_RUPDATE: return LZ_RUNNING;
+    case FIN_RITZ: return LZ_PASSDONE;
+    default: return LZ_FINISHED;
+  }
+}
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  const int s = blockIdx.x;
+  if (a.status[s] != fin_status(op)) return;
+  const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
+  double tot[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < count; i += blockDim.x)
+    for (int k = 0; k < 3; k++) tot[k] += a.partial[(size_t)(first + i) * 4 + k];
+  block_sum<3>(tot, red);
+  if (threadIdx.x != 0) return;
+  const double n = a.seg[a.nseg + s];
+  switch (op) {
+    case FIN_START0: SC(s, SC_M0) = tot[0] / n; break;
+    case FIN_START1: SC(s, SC_N0) = 1.0 / sqrt(tot[0]); break;
+    case FIN_SETUP0: SC(s, SC_TMP) = tot[0] / n; break;
+    case FIN_SETUP1: {  // fiedler.c:441-456
+      double rtr = tot[0], rn = sqrt(rtr);
+      SC(s, SC_RTR) = rtr;
+      SC(s, SC_RNORM) = rn;
+      SC(s, SC_RNORM0) = rn;
+      SC(s, SC_RTOL) = rn * a.tol;
+      SC(s, SC_RNI) = 1.0 / rn;
+      SC(s, SC_RTZ1) = rtr;  // iteration 0: rtz2 = 1 (initial rtz1), rtz1 = rtr
+      SC(s, SC_RTZ2) = 1.0;
+      SC(s, SC_BETA) = 0.0;
+      SC(s, SC_PAP) = 0.0;
+      SC(s, SC_MEANP) = 0.0;
+      a.iter[s] = 0;
+      a.status[s] = LZ_RUNNING;
+      atomicAdd(&a.running[0], 1);
+      break;
+    }
+    case FIN_APPLY: {  // mean of p, pap = w.(p - mean), alpha (fiedler.c:470-478)
+      double mq = tot[0] / n;
+      double pap = tot[1] - mq * tot[2];
+      SC(s, SC_MEANP) = mq;
+      SC(s, SC_PAPOLD) = SC(s, SC_PAP);
+      SC(s, SC_PAP) = pap;
+      SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / pap;
+      break;
+    }
+    case FIN_RUPDATE: {  // fiedler.c:486-509
+      const int it = a.iter[s], mi = a.miter[s];
+      double rtr = tot[0], rnorm = sqrt(rtr);
+      double rtz1 = SC(s, SC_RTZ1), rtz2 = SC(s, SC_RTZ2), beta = SC(s, SC_BETA);
+      double pap = SC(s, SC_PAP), pap_old = SC(s, SC_PAPOLD);
+      double *Td = a.T + (size_t)s * 2 * a.miter_max, *Tu = Td + a.miter_max;
+      if (it == 0) {
+        Td[0] = pap / rtz1;
+      } else {
+        Td[it] = (beta * beta * pap_old + pap) / rtz1;
+        Tu[it - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
+      }
+      SC(s, SC_RTR) = rtr;
+      SC(s, SC_RNORM) = rnorm;
+      SC(s, SC_RNI) = 1.0 / rnorm;
+      if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
+        a.k[s] = it + 1;
+        a.status[s] = LZ_PASSDONE;
+        atomicSub(&a.running[0], 1);
+      } else {
+        a.iter[s] = it + 1;
+        SC(s, SC_RTZ2) = rtz1;
+        SC(s, SC_RTZ1) = rtr;
+        SC(s, SC_BETA) = rtr / rtz1;
+      }
+      break;
+    }
+    case FIN_RITZ: {  // mean of the Ritz vector; restart or finish (fiedler.c:529-553)
+      const int k = a.k[s];
+      SC(s, SC_MEANF) = tot[0] / n;
+      int np = a.npass[s] + 1;
+      a.npass[s] = np;
+      a.niter[s] = (np - 1) * a.miter[s] + k;
+      if (k == a.miter[s] && np < a.mpass) {
+        a.status[s] = LZ_SETUP;
+        atomicAdd(&a.running[1], 1);
+      } else {
+        a.status[s] = LZ_FINISHED;
+      }
+      break;
+    }
+    case FIN_FINISH: SC(s, SC_NORMF) = 1.0 / sqrt(tot[0]); break;
+    case FIN_RES0: SC(s, SC_RES) = tot[0]; break;
+    case FIN_RES1: SC(s, SC_TMP) = sqrt(tot[0]) / SC(s, SC_RES); break;
   }
 }
 
@@ -641,6 +741,13 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
   hipStream_t st = g->stream;
+#define FIN(op)                                           \
+  do {                                                    \
+    k_lz_fin<<<nseg, RSB_BLOCK, 0, st>>>(a, (op));        \
+    RSB_LAUNCH_CHECK();                                   \
+  } while (0)
+  tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell, g->d_tnhalo, g->d_thalo_off,
+                g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
 
   RSB_TRY(hipMemsetAsync(L.running, 0, sizeof(int) * 2, st));
   k_lz_init_state<<<nblocks(nseg, 256), 256, 0, st>>>(a, miter_opt);
@@ -651,14 +758,17 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   for (int ph = 0; ph < 3; ph++) {
     k_lz_start<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, ph, init_mode, g->d_start, f);
     RSB_LAUNCH_CHECK();
+    if (ph < 2) FIN(ph == 0 ? FIN_START0 : FIN_START1);
   }
   // f holds the normalised start vector; SC_MEANF is 0 for the first pass.
   const size_t tq_lds = sizeof(double) * (2 * miter_max + miter_max * miter_max);
   for (int pass = 0; pass < mpass; pass++) {
     k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, r, p);
     RSB_LAUNCH_CHECK();
+    FIN(FIN_SETUP0);
     k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
     RSB_LAUNCH_CHECK();
+    FIN(FIN_SETUP1);
     RSB_TRY(hipEventRecord(g->lz_ev0, st));
     int it, nrec = 0;
     if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
@@ -669,14 +779,15 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     for (it = 0; it < miter_max; it++) {
       T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
       if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-      k_lz_apply<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, pold, pnew, w, basis, mpad,
-                                                  g->d_slice_off, g->d_slice_w, g->d_ell);
+      k_lz_apply<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, pold, pnew, w, basis, mpad, ta);
       if (g->prof_spmv) {
         RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
         nrec++;
       }
+      FIN(FIN_APPLY);
       k_lz_rupdate<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, w);
       RSB_LAUNCH_CHECK();
+      FIN(FIN_RUPDATE);
       g->lz_launches++;
       if ((it & 7) == 7 && it + 1 < miter_max) {
         int run = 0;
@@ -700,6 +811,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     RSB_LAUNCH_CHECK();
     k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
     RSB_LAUNCH_CHECK();
+    FIN(FIN_RITZ);
     int again = 0;
     RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     RSB_TRY(hipStreamSynchronize(st));
@@ -707,12 +819,16 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   }
   k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, g->d_fied, xrow);
   RSB_LAUNCH_CHECK();
+  FIN(FIN_FINISH);
   k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, g->d_fied, xrow);
   RSB_LAUNCH_CHECK();
   if (out && out->resid) {
     k_lz_resid<<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, xrow, g->d_slice_off, g->d_slice_w, g->d_ell);
+    RSB_LAUNCH_CHECK();
+    FIN(FIN_RES0);
     k_lz_resid<<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, xrow, g->d_slice_off, g->d_slice_w, g->d_ell);
     RSB_LAUNCH_CHECK();
+    FIN(FIN_RES1);
   }
   // read back per-segment results
   std::vector<int> ni(nseg), np(nseg);

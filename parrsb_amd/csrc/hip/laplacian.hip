@@ -152,6 +152,8 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
                                                           g->d_slice_w, g->d_ell);
   RSB_LAUNCH_CHECK();
   hipFree(dnnz), hipFree(bytes);
+  rc = rsb_build_tiles(g);
+  if (rc) return rc;
   rsb_phase_end(g, PH_LAP);
   return 0;
 }

@@ -80,6 +80,12 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_slice_w);
   FREE(g->d_ell);
   FREE(g->d_rowlen);
+  FREE(g->d_tnhalo);
+  FREE(g->d_halo);
+  FREE(g->d_w16);
+  FREE(g->d_thalo_off);
+  FREE(g->d_off16);
+  FREE(g->d_e16);
   FREE(g->d_seg_nnz);
   for (hipEvent_t ev : g->spmv_ev) hipEventDestroy(ev);
   FREE(g->d_tile);

@@ -74,6 +74,12 @@ struct rsbgpu {
   int *d_slice_w = nullptr;
   uint32_t *d_ell = nullptr;
   int *d_rowlen = nullptr;
+  // LDS-tiled form (tiles.hip): halo rows per tile and 16-bit local entries
+  int *d_tnhalo = nullptr, *d_halo = nullptr, *d_w16 = nullptr;
+  int64_t *d_thalo_off = nullptr, *d_off16 = nullptr;
+  uint16_t *d_e16 = nullptr;
+  int64_t nhalo_total = 0, e16_count = 0;
+  int nwide = 0;
   int64_t *d_seg_nnz = nullptr;
   std::vector<int64_t> h_seg_nnz;
   // tiles of the level (a tile never straddles a segment)
@@ -160,6 +166,8 @@ inline hipError_t dev_alloc(T *&p, size_t n) {
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int rsb_build_tiles(rsbgpu *g);  // tiles.hip
 
 // sort helpers (sort.hip)
 int rsb_sort_pairs_u64(rsbgpu *g, uint64_t *keys, int *vals, int64_t n, int begin_bit,

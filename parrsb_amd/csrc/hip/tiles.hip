@@ -1,0 +1,233 @@
+// LDS-tiled form of the level Laplacian, used by the Lanczos apply kernel.
+//
+// A tile is the block of up to RSB_TILE consecutive rows one workgroup
+// owns.  Its "halo" is the set of rows outside the tile that its rows
+// reference; the apply kernel loads tile + halo values into LDS once and
+// reads every neighbour from LDS.  Entries are re-encoded relative to that
+// LDS window as 16-bit words: local index (12 bits) | weight (4 bits), so a
+// tile plus its halo may hold at most 4096 rows; a tile whose halo does not
+// fit is marked (nhalo = -1) and its rows use the global 32-bit form.
+//
+// Layout of the 16-bit entries: per 64-row slice, its width rounded up to a
+// multiple of 8; entry j of lane l at (j / 8) * 512 + l * 8 + (j % 8), so one
+// 16-byte load per lane fetches 8 consecutive entries of its row.
+#include <stdlib.h>
+
+#include <vector>
+
+#include "state.hpp"
+
+#define TB_HASH 8192  // LDS hash slots per tile (power of two)
+#define TB_LOCAL 4096 // local index range (12 bits)
+
+__device__ inline unsigned tb_hash(unsigned c) { return (c * 2654435761u) & (TB_HASH - 1); }
+
+// Insert c into the LDS set; returns false if the set is full.
+__device__ inline bool tb_insert(int *keys, int c) {
+  unsigned h = tb_hash((unsigned)c);
+  for (int probe = 0; probe < TB_HASH; probe++) {
+    int old = atomicCAS(&keys[h], -1, c);
+    if (old == -1 || old == c) return true;
+    h = (h + 1) & (TB_HASH - 1);
+  }
+  return false;
+}
+
+__device__ inline int tb_find(const int *keys, int c) {
+  unsigned h = tb_hash((unsigned)c);
+  for (int probe = 0; probe < TB_HASH; probe++) {
+    int k = keys[h];
+    if (k == c) return (int)h;
+    if (k == -1) return -1;
+    h = (h + 1) & (TB_HASH - 1);
+  }
+  return -1;
+}
+
+// Collect the external columns of tile blockIdx.x into the LDS set.
+__device__ inline bool tb_collect(int *keys, int row0, int row1,
+                                  const int64_t *__restrict__ slice_off,
+                                  const int *__restrict__ slice_w,
+                                  const uint32_t *__restrict__ ell) {
+  for (int i = threadIdx.x; i < TB_HASH; i += blockDim.x) keys[i] = -1;
+  __syncthreads();
+  bool ok = true;
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int sl = row >> 6;
+    const uint32_t *e = ell + slice_off[sl] + (row & 63);
+    for (int j = 0; j < slice_w[sl]; j++) {
+      uint32_t q = e[(size_t)j * RSB_WAVE];
+      int c = (int)(q & 0x0fffffffu);
+      if ((q >> 28) == 0 || (c >= row0 && c < row1)) continue;
+      if (!tb_insert(keys, c)) ok = false;
+    }
+  }
+  __shared__ int any_fail;
+  if (threadIdx.x == 0) any_fail = 0;
+  __syncthreads();
+  if (!ok) atomicOr(&any_fail, 1);
+  __syncthreads();
+  return any_fail == 0;
+}
+
+// Block-wide exclusive scan of per-thread counts (deterministic).
+__device__ inline int tb_scan(int v, int *total) {
+  __shared__ int s[RSB_BLOCK];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+    int t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int incl = s[threadIdx.x];
+  *total = s[blockDim.x - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// Pass A: number of halo rows per tile (-1 if it does not fit).
+__global__ void __launch_bounds__(RSB_BLOCK) k_tile_count(const int *__restrict__ tile,
+                                                          const int64_t *__restrict__ slice_off,
+                                                          const int *__restrict__ slice_w,
+                                                          const uint32_t *__restrict__ ell,
+                                                          int *__restrict__ nhalo) {
+  __shared__ int keys[TB_HASH];
+  const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
+  bool ok = tb_collect(keys, row0, row1, slice_off, slice_w, ell);
+  int cnt = 0;
+  const int per = TB_HASH / RSB_BLOCK;
+  for (int k = 0; k < per; k++) cnt += keys[threadIdx.x * per + k] >= 0;
+  int total;
+  tb_scan(cnt, &total);
+  if (threadIdx.x == 0) nhalo[t] = (ok && (row1 - row0) + total <= TB_LOCAL) ? total : -1;
+}
+
+// Pass B: halo lists (in slot order) and the 16-bit entries.
+__global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
+    const int *__restrict__ tile, const int64_t *__restrict__ slice_off,
+    const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
+    const int *__restrict__ nhalo, const int64_t *__restrict__ halo_off,
+    const int64_t *__restrict__ off16, const int *__restrict__ w16, int *__restrict__ halo,
+    uint16_t *__restrict__ e16) {
+  __shared__ int keys[TB_HASH];
+  __shared__ short local[TB_HASH];
+  const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
+  const int nrows = row1 - row0;
+  const bool fits = nhalo[t] >= 0;
+  if (fits) {
+    tb_collect(keys, row0, row1, slice_off, slice_w, ell);
+    const int per = TB_HASH / RSB_BLOCK;
+    int cnt = 0;
+    for (int k = 0; k < per; k++) cnt += keys[threadIdx.x * per + k] >= 0;
+    int total;
+    int base = tb_scan(cnt, &total);
+    for (int k = 0; k < per; k++) {
+      int slot = threadIdx.x * per + k;
+      if (keys[slot] >= 0) {
+        local[slot] = (short)(nrows + base);
+        halo[halo_off[t] + base] = keys[slot];
+        base++;
+      }
+    }
+    __syncthreads();
+  }
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int sl = row >> 6, lane = row & 63;
+    const uint32_t *e = ell + slice_off[sl] + lane;
+    uint16_t *o = e16 + off16[sl] + lane * 8;
+    const int w = slice_w[sl], w8 = w16[sl];
+    const uint16_t self = (uint16_t)(row - row0);
+    for (int j = 0; j < w8; j++) {
+      uint16_t v = self;  // padding: own row, weight 0
+      if (fits && j < w) {
+        uint32_t q = e[(size_t)j * RSB_WAVE];
+        int c = (int)(q & 0x0fffffffu);
+        uint32_t wt = q >> 28;
+        if (wt) {
+          int li = c - row0;
+          if (c < row0 || c >= row1) {
+            int slot = tb_find(keys, c);
+            li = slot >= 0 ? local[slot] : -1;
+          }
+          // every external column was inserted above, so li >= 0
+          if (li >= 0) v = (uint16_t)(li | (wt << 12));
+        }
+      }
+      o[(j >> 3) * 512 + (j & 7)] = v;
+    }
+  }
+}
+
+__global__ void k_w16(int64_t nslices, const int *__restrict__ w, int *__restrict__ w8,
+                      int64_t *__restrict__ bytes) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslices) {
+    int v = (w[i] + 7) / 8 * 8;
+    w8[i] = v;
+    bytes[i] = (int64_t)v * RSB_WAVE;
+  }
+  if (i == nslices) bytes[i] = 0;
+}
+
+__global__ void k_all_wide(int n, int *__restrict__ nh) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) nh[i] = -1;
+}
+
+__global__ void k_nhalo64(int n, const int *__restrict__ nh, int64_t *__restrict__ o) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = nh[i] > 0 ? nh[i] : 0;
+  if (i == n) o[i] = 0;
+}
+
+int rsb_build_tiles(rsbgpu *g) {
+  const int nt = g->ntiles;
+  const int64_t ns = g->nslices;
+  if (nt == 0) return 0;
+  RSB_TRY(dev_alloc(g->d_tnhalo, (size_t)nt + 1));
+  RSB_TRY(dev_alloc(g->d_thalo_off, (size_t)nt + 1));
+  RSB_TRY(dev_alloc(g->d_w16, (size_t)ns + 1));
+  RSB_TRY(dev_alloc(g->d_off16, (size_t)ns + 1));
+  int64_t *tmp = nullptr;
+  RSB_TRY(hipMalloc((void **)&tmp, sizeof(int64_t) * (std::max<int64_t>(ns, nt) + 1)));
+  k_tile_count<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w,
+                                                 g->d_ell, g->d_tnhalo);
+  RSB_LAUNCH_CHECK();
+  const char *off = getenv("PARRSB_AMD_NO_TILES");  // every tile on the global path
+  if (off && atoi(off) == 1) {
+    k_all_wide<<<nblocks(nt, 256), 256, 0, g->stream>>>(nt, g->d_tnhalo);
+    RSB_LAUNCH_CHECK();
+  }
+  k_nhalo64<<<nblocks(nt + 1, 256), 256, 0, g->stream>>>(nt, g->d_tnhalo, tmp);
+  RSB_LAUNCH_CHECK();
+  int rc = rsb_exclusive_scan_i64(g, tmp, g->d_thalo_off, nt + 1);
+  if (rc) return rc;
+  k_w16<<<nblocks(ns + 1, 256), 256, 0, g->stream>>>(ns, g->d_slice_w, g->d_w16, tmp);
+  RSB_LAUNCH_CHECK();
+  rc = rsb_exclusive_scan_i64(g, tmp, g->d_off16, ns + 1);
+  if (rc) return rc;
+  int64_t nh = 0, n16 = 0;
+  RSB_TRY(hipMemcpyAsync(&nh, g->d_thalo_off + nt, sizeof(int64_t), hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipMemcpyAsync(&n16, g->d_off16 + ns, sizeof(int64_t), hipMemcpyDeviceToHost,
+                         g->stream));
+  std::vector<int> nhv(nt);
+  RSB_TRY(hipMemcpyAsync(nhv.data(), g->d_tnhalo, sizeof(int) * nt, hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  g->nhalo_total = nh;
+  g->e16_count = n16;
+  g->nwide = 0;
+  for (int v : nhv) g->nwide += v < 0;
+  RSB_TRY(dev_alloc(g->d_halo, (size_t)nh + 1));
+  RSB_TRY(dev_alloc(g->d_e16, (size_t)n16 + 8));
+  k_tile_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w, g->d_ell,
+                                                g->d_tnhalo, g->d_thalo_off, g->d_off16,
+                                                g->d_w16, g->d_halo, g->d_e16);
+  RSB_LAUNCH_CHECK();
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  hipFree(tmp);
+  return 0;
+}

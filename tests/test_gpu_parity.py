@@ -132,9 +132,14 @@ def _segments_after_rcb(vtx, xyz, nv, p):
     return d
 
 
-@pytest.mark.parametrize("dims", [(9, 6, 5), (12, 10, 7)])
-def test_lanczos_fp64_matches_oracle(oracle, dims):
-    flush(f"lanczos fp64 {dims}")
+@pytest.mark.parametrize("tiles", ["lds", "global"])
+@pytest.mark.parametrize("dims", [(9, 6, 5), (12, 10, 7), (40, 30, 20)])
+def test_lanczos_fp64_matches_oracle(oracle, dims, tiles, monkeypatch):
+    """Both forms of the Laplacian apply: LDS tiles with halos, and the
+    global 32-bit path (forced for every tile with PARRSB_AMD_NO_TILES=1)."""
+    flush(f"lanczos fp64 {dims} {tiles}")
+    if tiles == "global":
+        monkeypatch.setenv("PARRSB_AMD_NO_TILES", "1")
     vtx, xyz, nv = M.box_hex(*dims, seed=4)
     with _segments_after_rcb(vtx, xyz, nv, 2) as d:
         order = d.order()
