@@ -63,6 +63,17 @@ def run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, steps, war
     return dt, reps
 
 
+def measured_traffic(desc, precision):
+    """HBM bytes per k_lz_apply launch from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None if no summary exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{desc}_fp{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def cpu_baseline(cfg):
     """Oracle (single-threaded C, FP64) on a bounded sample of the workload:
     a sub-mesh of the same family, partitioned to the same elements per part."""
@@ -95,7 +106,7 @@ def cpu_baseline(cfg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
@@ -142,14 +153,17 @@ def main():
                                "batched FP64 (reference arithmetic)"),
                    "start_vector": "reference (RCB position); learned warm start not used",
                    "inputs": "device-resident (HBM)"},
-        "roofline": {"bound": "hbm", "kernel": "k_lz_spmv",
+        "roofline": {"bound": "hbm", "kernel": "k_lz_apply",
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": spmv_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(desc, args.precision),
                      "bytes_per_launch": rep["spmv_bytes"] / launches,
                      "avg_launch_ms": rep["spmv_ms"] / launches,
+                     "launches": rep["spmv_launches"],
                      "iteration_achieved_gbs": it_gbs},
         "quality": q,
-        "levels": {k: rep[k] for k in ("niter_max", "npass_max", "ncomp_max", "resid_max")},
+        "levels": {k: rep[k] for k in ("niter_max", "niter_mean", "npass_max", "ncomp_max",
+                                       "resid_med", "resid_max", "level_ms")},
         "phase_ms": rep["phase_ms"],
     }
     if not args.no_compare and args.precision == 32:

@@ -41,6 +41,9 @@ typedef struct {
   int nconverged_fail[PARRSB_AMD_MAX_LEVELS]; /* segments at max_iter*max_passes */
   double lambda_min[PARRSB_AMD_MAX_LEVELS], lambda_max[PARRSB_AMD_MAX_LEVELS];
   double resid_max[PARRSB_AMD_MAX_LEVELS]; /* max ||Lf - lambda f|| / lambda */
+  double resid_med[PARRSB_AMD_MAX_LEVELS]; /* median over the level's segments */
+  double niter_mean[PARRSB_AMD_MAX_LEVELS]; /* element-weighted mean iterations */
+  double level_ms[PARRSB_AMD_MAX_LEVELS];  /* wall time of the level */
 } parrsb_amd_report;
 
 /* Same contract as parrsb_part_mesh, with every array in GPU memory of the

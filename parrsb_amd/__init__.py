@@ -54,7 +54,9 @@ class Report(C.Structure):
                 ("npass_max", C.c_int * ML), ("ncomp_min", C.c_int * ML),
                 ("ncomp_max", C.c_int * ML), ("nbalanced", C.c_int * ML),
                 ("nconverged_fail", C.c_int * ML), ("lambda_min", C.c_double * ML),
-                ("lambda_max", C.c_double * ML), ("resid_max", C.c_double * ML)]
+                ("lambda_max", C.c_double * ML), ("resid_max", C.c_double * ML),
+                ("resid_med", C.c_double * ML), ("niter_mean", C.c_double * ML),
+                ("level_ms", C.c_double * ML)]
 
     def as_dict(self):
         d = {}

@@ -14,12 +14,26 @@
  * The split arithmetic (how many elements the lower ranks hold after a sort,
  * how many each half should hold) is done here, on the host.
  */
+#define _POSIX_C_SOURCE 200809L
 #include <float.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <time.h>
+
 #include "parrsb_host.h"
+
+static double wall(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static int cmp_dbl(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
 
 /* Elements on the lower ceil(p/2) virtual ranks after n elements are spread
  * over p ranks: the first p - (n mod p) ranks get floor(n/p), the others one
@@ -86,6 +100,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   A.cstart[0] = 0, A.n[0] = nel, A.p[0] = P, A.r0[0] = 0;
   rep->nlevels = ncuts;
   for (int cut = 0; cut < ncuts; cut++) {
+    double tl = wall();
     TRY(rsbgpu_set_segments(g, nseg, A.cstart, A.n, A.p, A.r0));
     TRY(rsbgpu_prepartition(g, prm->rsb_pre));
     TRY(rsbgpu_level_laplacian(g));
@@ -106,7 +121,8 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     /* level statistics */
     int lv = cut < PARRSB_AMD_MAX_LEVELS ? cut : PARRSB_AMD_MAX_LEVELS - 1;
     int itmin = 1 << 30, itmax = 0, pmax = 0, cmin = 1 << 30, cmax = 0, nb = 0, ncut = 0, nfail = 0;
-    double lmin = DBL_MAX, lmax = -DBL_MAX, rmax = 0.0;
+    double lmin = DBL_MAX, lmax = -DBL_MAX, rmax = 0.0, work = 0.0, nwork = 0.0;
+    int nres = 0;
     for (int s = 0; s < nseg; s++) {
       if (A.p[s] <= 1) continue;
       ncut++;
@@ -116,7 +132,12 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
         pmax = npass[s] > pmax ? npass[s] : pmax;
         lmin = lam[s] < lmin ? lam[s] : lmin;
         lmax = lam[s] > lmax ? lam[s] : lmax;
-        if (prm->residuals && res[s] > rmax) rmax = res[s];
+        if (prm->residuals) {
+          if (res[s] > rmax) rmax = res[s];
+          rn[nres++] = res[s]; /* rn is free after the Lanczos call */
+        }
+        work += (double)niter[s] * A.n[s];
+        nwork += A.n[s];
         int mi = prm->max_iter < A.n[s] ? prm->max_iter : A.n[s];
         if (niter[s] == mi * prm->max_passes) nfail++;
       }
@@ -138,6 +159,11 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     rep->nconverged_fail[lv] = nfail;
     rep->lambda_min[lv] = ncut ? lmin : 0.0, rep->lambda_max[lv] = ncut ? lmax : 0.0;
     rep->resid_max[lv] = rmax;
+    if (nres > 0) {
+      qsort(rn, (size_t)nres, sizeof(double), cmp_dbl);
+      rep->resid_med[lv] = rn[nres / 2];
+    }
+    rep->niter_mean[lv] = nwork > 0 ? work / nwork : 0.0;
     if (prm->verbose > 1)
       printf("rsb: cut %d/%d segments=%d niter=%d..%d ncomp=%d..%d balanced=%d\n", cut + 1,
              ncuts, ncut, rep->niter_min[lv], itmax, rep->ncomp_min[lv], cmax, nb);
@@ -156,6 +182,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     }
     segtab t = A;
     A = B, B = t, nseg = nn;
+    rep->level_ms[lv] = 1e3 * (wall() - tl);
   }
   TRY(rsbgpu_set_segments(g, nseg, A.cstart, A.n, A.p, A.r0));
 done:
