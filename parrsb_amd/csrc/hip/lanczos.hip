@@ -40,6 +40,8 @@ struct lz_args {
   int mpass;
   double tol;
   int miter_max;
+  const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
+  const int *act_segs;   // if set: blockIdx.x -> segment, for k_lz_fin
 };
 
 #define SC(s, j) a.sc[(size_t)(s) * LZ_NSC + (j)]
@@ -75,10 +77,11 @@ struct acc_t<float> {
 
 // Tile prologue: returns false if the block has nothing to do.
 struct tile_t {
-  int s, row0, row1, pstart, n, first, count;
+  int ti, s, row0, row1, pstart, n, first, count;
 };
 __device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t) {
-  int ti = blockIdx.x;
+  const int ti = a.act_tiles ? a.act_tiles[blockIdx.x] : blockIdx.x;
+  t.ti = ti;
   t.s = a.tile[3 * ti];
   if (a.status[t.s] != want_status) return false;
   t.row0 = a.tile[3 * ti + 1];
@@ -94,11 +97,11 @@ __device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t) {
 // The segment totals are formed by k_lz_fin in the next launch: no
 // cross-workgroup handshake inside a kernel, so no device-scope fences.
 template <int NV>
-__device__ inline void publish(const lz_args &a, double (&v)[NV]) {
+__device__ inline void publish(const lz_args &a, const tile_t &t, double (&v)[NV]) {
   __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   block_sum<NV>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 4; k++) a.partial[(size_t)blockIdx.x * 4 + k] = k < NV ? v[k] : 0.0;
+    for (int k = 0; k < 4; k++) a.partial[(size_t)t.ti * 4 + k] = k < NV ? v[k] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_start(lz_args a, int phase, in
     if (phase == 2) f[row] = i < t.n ? (T)((x - m) * sc) : (T)0;
   }
   if (phase == 2) return;
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // Pass setup (fiedler.c:441-456): r = f - mean_f; then r -= mean(r);
@@ -163,7 +166,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_setup(lz_args a, int phase, co
     }
   }
   double v[1] = {acc.value()};
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +264,7 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
                                       T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
   const int nrows = t.row1 - t.row0;
-  const int64_t ho = ta.halo_off[blockIdx.x];
+  const int64_t ho = ta.halo_off[t.ti];
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   for (int idx = threadIdx.x; idx < nrows + nh; idx += blockDim.x) {
     const int gr = idx < nrows ? t.row0 + idx : ta.halo[ho + idx - nrows];
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
   const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
   acc_t<T> sq, swq, sw;
   T *b = basis + (size_t)it * mpad;
-  const int nh = ta.nhalo[blockIdx.x];
+  const int nh = ta.nhalo[t.ti];
   if (nh >= 0) {
     if (it == 0)
       lz_apply_tiled<T, true>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw);
@@ -331,7 +334,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
                             ta.ell, sq, swq, sw);
   }
   double v[3] = {sq.value(), swq.value(), sw.value()};
-  publish<3>(a, v);
+  publish<3>(a, t, v);
 }
 
 template <typename T>
@@ -348,15 +351,17 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
     acc.dot(x, x);
   }
   double v[1] = {acc.value()};
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // ---------------------------------------------------------------------------
 // Tridiagonal eigen-solve of each finished pass: implicit QL with shifts, the
 // steps of fiedler.c:328-432 (FP64, contraction off so every operation rounds
-// as written).  One wavefront per segment: every lane runs the scalar
-// recurrence on the shared d/e arrays; the plane rotations of the
-// eigenvector matrix are spread over the lanes (lane = matrix row).  The
+// as written).  One wavefront per segment: every lane runs the same scalar
+// recurrence and writes the same values to the shared d/e arrays, so each
+// lane only ever reads values it wrote itself and the sweep needs no
+// barrier; the plane rotations of the eigenvector matrix are spread over the
+// lanes (lane = matrix row, each lane touches only its own rows).  The
 // coefficients y of the eigenvector with the smallest |eigenvalue| are kept.
 #pragma clang fp contract(off)
 __device__ inline double sgn_abs(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
@@ -405,21 +410,20 @@ __device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, dou
         p = s * r;
         double di1 = g + p;
         g = c * r - b;
-        __syncthreads();
-        if (lane == 0) e[i + 1] = ei1, d[i + 1] = di1;
+        e[i + 1] = ei1;
+        d[i + 1] = di1;
         for (int k = lane; k < n; k += RSB_WAVE) {
           double zk1 = z[k * n + i + 1], zk = z[k * n + i];
           z[k * n + i + 1] = s * zk + c * zk1;
           z[k * n + i] = c * zk - s * zk1;
         }
-        __syncthreads();
       }
-      double dl = d[l] - p;
-      __syncthreads();
-      if (lane == 0) d[l] = dl, e[l] = g, e[m] = 0.0;
-      __syncthreads();
+      d[l] = d[l] - p;
+      e[l] = g;
+      e[m] = 0.0;
     }
   }
+  __syncthreads();  // rows of z written by other lanes are read below
   // eigenvalue of smallest magnitude (first on ties, fiedler.c:538-545)
   int imin = 0;
   double emin = fabs(d[0]);
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
     f[row] = (T)x;
     v[0] += (double)(T)x;
   }
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // Final normalisation (fiedler.c:606-616) and scatter to compact positions.
@@ -521,7 +525,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_finish(lz_args a, int phase,
       v[0] += x * x;
     }
   }
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // Eigen-residual of the final vector x (FP64): lambda_R = x.Lx, then
@@ -551,7 +555,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_resid(lz_args a, int phase,
       v[0] += d * d;
     }
   }
-  publish<1>(a, v);
+  publish<1>(a, t, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,7 +578,7 @@ __device__ inline int fin_status(int op) {
 
 __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
-  const int s = blockIdx.x;
+  const int s = a.act_segs ? a.act_segs[blockIdx.x] : blockIdx.x;
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
   double tot[3] = {0.0, 0.0, 0.0};
@@ -660,6 +664,16 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   }
 }
 
+// Lists of the tiles and segments in a given state (order irrelevant: every
+// tile writes its partial at its own index and k_lz_fin sums them in tile
+// order).
+__global__ void k_lz_compact(lz_args a, int ntiles, int want, int *__restrict__ tiles,
+                             int *__restrict__ segs, int *__restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntiles && a.status[a.tile[3 * i]] == want) tiles[atomicAdd(&counts[0], 1)] = i;
+  if (i < a.nseg && a.status[i] == want) segs[atomicAdd(&counts[1], 1)] = i;
+}
+
 __global__ void k_lz_init_state(lz_args a, int miter_opt) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.nseg) return;
@@ -731,6 +745,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   }
   if (!L.running) RSB_TRY(hipMalloc((void **)&L.running, sizeof(int) * 2));
   lz_args a;
+  a.act_tiles = nullptr, a.act_segs = nullptr;
   a.seg = g->d_seg, a.tile = g->d_tile, a.seg_tiles = g->d_seg_tiles, a.row_elem = g->d_row_elem;
   a.nseg = nseg;
   a.status = L.status, a.iter = L.iter, a.k = L.k, a.npass = L.npass, a.niter = L.niter;
@@ -746,6 +761,22 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     k_lz_fin<<<nseg, RSB_BLOCK, 0, st>>>(a, (op));        \
     RSB_LAUNCH_CHECK();                                   \
   } while (0)
+  // compacted lists of running tiles / segments for the iteration launches
+  int *act = nullptr;
+  RSB_TRY(hipMalloc((void **)&act, sizeof(int) * ((size_t)ntiles + nseg + 2)));
+  int *act_t = act, *act_s = act + ntiles, *act_n = act + ntiles + nseg;
+  int nact[2] = {ntiles, nseg};
+  auto compact = [&](int want) -> int {
+    RSB_TRY(hipMemsetAsync(act_n, 0, sizeof(int) * 2, st));
+    k_lz_compact<<<nblocks(std::max(ntiles, nseg), 256), 256, 0, st>>>(a, ntiles, want, act_t,
+                                                                        act_s, act_n);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipMemcpyAsync(nact, act_n, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    return 0;
+  };
+  lz_args ai = a;  // iteration launches over the running lists
+  ai.act_tiles = act_t, ai.act_segs = act_s;
   tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell, g->d_tnhalo, g->d_thalo_off,
                 g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
 
@@ -769,6 +800,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
     RSB_LAUNCH_CHECK();
     FIN(FIN_SETUP1);
+    if (compact(LZ_RUNNING)) return 2;
     RSB_TRY(hipEventRecord(g->lz_ev0, st));
     int it, nrec = 0;
     if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
@@ -778,22 +810,22 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     }
     for (it = 0; it < miter_max; it++) {
       T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
+      if (nact[0] == 0) break;
       if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-      k_lz_apply<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, pold, pnew, w, basis, mpad, ta);
+      k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta);
       if (g->prof_spmv) {
         RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
         nrec++;
       }
-      FIN(FIN_APPLY);
-      k_lz_rupdate<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, r, w);
+      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
+      k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
+      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
       RSB_LAUNCH_CHECK();
-      FIN(FIN_RUPDATE);
       g->lz_launches++;
       if ((it & 7) == 7 && it + 1 < miter_max) {
-        int run = 0;
-        RSB_TRY(hipMemcpyAsync(&run, L.running, sizeof(int), hipMemcpyDeviceToHost, st));
-        RSB_TRY(hipStreamSynchronize(st));
-        if (run == 0) break;
+        // drop segments that finished their pass from the launch lists
+        if (compact(LZ_RUNNING)) return 2;
+        if (nact[1] == 0) break;
       }
     }
     RSB_TRY(hipEventRecord(g->lz_ev1, st));
@@ -830,6 +862,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     RSB_LAUNCH_CHECK();
     FIN(FIN_RES1);
   }
+  (void)hipFree(act);
   // read back per-segment results
   std::vector<int> ni(nseg), np(nseg);
   std::vector<double> sc((size_t)nseg * LZ_NSC);
