@@ -119,6 +119,28 @@ int rsbgpu_get_centroids(rsbgpu *g, double *coord);
  * out[6] comm volume, out[7] empty parts. */
 int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t *out);
 
+/* Learned warm start (a message-passing network over the level graph, see
+ * lws.hip for the architecture).  No counterpart in the reference: it only
+ * replaces the start vector of fiedler.c:570-585, and the reference start is
+ * the fallback.  Parameters are LWS_NPARAM floats in device memory. */
+int rsbgpu_lws_nparams(void);
+int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params);
+/* Network output for every row of the current level (after
+ * rsbgpu_level_laplacian), d_rows[mpad]. */
+int rsbgpu_lws_predict(rsbgpu *g, const float *d_params, float level, float *d_rows);
+/* One training step on the current level, after a plain rsbgpu_fiedler and
+ * before rsbgpu_fiedler_sort: the loss 1 - cos^2(prediction, Fiedler vector)
+ * averaged over the level's segments, its gradient added to d_grad. */
+int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_grad, float level,
+                          double *loss);
+/* Adam update of the parameters (device arrays of LWS_NPARAM floats). */
+int rsbgpu_lws_adam(float *d_params, const float *d_grad, float *d_m, float *d_v, int step,
+                    float lr);
+/* Parity hook: forward + backward on one segment given on the host. */
+int rsbgpu_lws_test(int M, const double *coord, float level, const int *rowptr, const int *col,
+                    const int *wgt, const float *t, const float *params, float *y, float *grad,
+                    double *loss);
+
 /* Small helpers for callers holding host data: copy n ints to a new device
  * buffer (freed with rsbgpu_free). */
 int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out);

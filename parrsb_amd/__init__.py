@@ -118,6 +118,12 @@ SIGNATURES = {
     "rsbgpu_get_centroids": (I, [P, P]),
     "rsbgpu_quality": (I, [P, P, I, P]),
     "rsbgpu_upload_ints": (I, [P, P, I, C.POINTER(P)]),
+    "rsbgpu_lws_nparams": (I, []),
+    "rsbgpu_lws_set_params": (I, [P, P]),
+    "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
+    "rsbgpu_lws_train_step": (I, [P, P, P, C.c_float, C.POINTER(C.c_double)]),
+    "rsbgpu_lws_adam": (I, [P, P, P, P, I, C.c_float]),
+    "rsbgpu_lws_test": (I, [I, P, C.c_float, P, P, P, P, P, P, P, C.POINTER(C.c_double)]),
     "rsbgpu_free": (None, [P]),
 }
 
@@ -341,3 +347,22 @@ def test_tqli(d, e):
     if rc:
         raise ParrsbError(rc, "rsbgpu_test_tqli failed")
     return lam[0], y
+
+
+def lws_test(coord, level, rowptr, col, wgt, t, params):
+    """Network forward + backward on one segment on the device (parity hook)."""
+    coord = np.ascontiguousarray(coord, dtype=np.float64)  # 3 x M
+    M = coord.shape[1]
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    wgt = np.ascontiguousarray(wgt, dtype=np.int32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    y = np.zeros(M, dtype=np.float32)
+    grad = np.zeros(params.size, dtype=np.float32)
+    loss = C.c_double()
+    rc = lib().rsbgpu_lws_test(M, _p(coord), level, _p(rowptr), _p(col), _p(wgt), _p(t),
+                               _p(params), _p(y), _p(grad), C.byref(loss))
+    if rc:
+        raise ParrsbError(rc, "rsbgpu_lws_test failed")
+    return y, grad, loss.value

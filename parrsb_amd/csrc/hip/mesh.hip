@@ -98,6 +98,7 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_basis);
   FREE(g->d_fied);
   FREE(g->d_start);
+  FREE(g->d_lws_ws);
   FREE(g->lz.status);  // iter, k, npass, niter, miter live in the same block
   FREE(g->lz.sc);
   FREE(g->lz.T);

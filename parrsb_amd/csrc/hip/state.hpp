@@ -93,7 +93,10 @@ struct rsbgpu {
   void *d_basis = nullptr;
   size_t vec_bytes = 0, basis_bytes = 0;
   double *d_fied = nullptr;     // Fiedler value per compact position
-  float *d_start = nullptr;     // optional start vector per element
+  float *d_start = nullptr;     // optional start vector per padded row
+  const float *lws_params = nullptr;  // learned warm start parameters (device)
+  void *d_lws_ws = nullptr;
+  size_t lws_ws_bytes = 0;
   lz_state lz{};
   int lz_nseg_cap = 0, lz_tile_cap = 0, lz_miter_cap = 0;
 
@@ -168,6 +171,7 @@ inline hipError_t dev_alloc(T *&p, size_t n) {
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 int rsb_build_tiles(rsbgpu *g);  // tiles.hip
+int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows);  // lws.hip
 
 // sort helpers (sort.hip)
 int rsb_sort_pairs_u64(rsbgpu *g, uint64_t *keys, int *vals, int64_t n, int begin_bit,
