@@ -44,6 +44,8 @@ typedef struct {
   double resid_med[PARRSB_AMD_MAX_LEVELS]; /* median over the level's segments */
   double niter_mean[PARRSB_AMD_MAX_LEVELS]; /* element-weighted mean iterations */
   double level_ms[PARRSB_AMD_MAX_LEVELS];  /* wall time of the level */
+  int nfallback[PARRSB_AMD_MAX_LEVELS];    /* learned start: segments that fell back */
+  int warm_start;                          /* 1 if the learned start was used */
 } parrsb_amd_report;
 
 /* Same contract as parrsb_part_mesh, with every array in GPU memory of the
@@ -66,6 +68,17 @@ int parrsb_amd_set_precision(int bits);
 
 /* 1 = time every SpMV launch with events (roofline measurements). */
 int parrsb_amd_set_profiling(int on);
+
+/* Learned warm start for later RSB calls: `params` are the network's
+ * parrsb_amd_warm_start_nparams() FP32 parameters (host memory, copied), or
+ * NULL to switch it off.  Each Fiedler solve then starts from the network's
+ * prediction, which gets `passes` restarted passes of rsb_max_iter
+ * iterations; a segment that misses rsb_tol within them is solved again from
+ * the reference start.  PARRSB_AMD_WARM_START=<file> loads parameters from
+ * a file written by parrsb_amd.lws_train (PARRSB_AMD_WARM_PASSES sets
+ * `passes`). */
+int parrsb_amd_warm_start_nparams(void);
+int parrsb_amd_set_warm_start(const float *params, int passes);
 
 /* Quality of a partition (host arrays): out[0] weighted edge cut, out[1] cut
  * edges, out[2] max part size, out[3] min part size, out[4] components over

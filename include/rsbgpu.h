@@ -31,6 +31,9 @@ typedef struct {
   double *lambda;   /* Ritz value of the last pass */
   double *rnorm;    /* 3 per segment: initial, target, final CG residual */
   double *resid;    /* relative eigen-residual ||Lf - lambda f|| / lambda */
+  int *fallback;    /* init_mode 2: 0 if the learned start met tol, else 1 +
+                       the iterations of the learned attempt (the segment
+                       was solved again from the reference start) */
 } rsbgpu_fiedler_out;
 
 int rsbgpu_open(rsbgpu **g, int device);
@@ -77,7 +80,12 @@ int rsbgpu_level_laplacian(rsbgpu *g);
 
 /* Fiedler vectors of every segment with p > 1.  precision: 64 or 32.
  * init_mode: 0 = the reference's start vector from the pre-partition
- * position; 1 = a start vector set by rsbgpu_set_start (per element).
+ * position; 1 = a start vector set by rsbgpu_set_start (per element);
+ * 2 = the learned warm start (rsbgpu_lws_set_params): the network's
+ * prediction gets the passes set there (max_iter iterations each); a
+ * segment that misses tol within them, or whose prediction is constant or
+ * not finite, starts over from the reference start with the full
+ * max_passes budget, and niter counts both attempts.
  * The result is held on the device per compact position. */
 int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_passes, double tol,
                    int init_mode, rsbgpu_fiedler_out *out);
@@ -124,13 +132,15 @@ int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t *out);
  * replaces the start vector of fiedler.c:570-585, and the reference start is
  * the fallback.  Parameters are LWS_NPARAM floats in device memory. */
 int rsbgpu_lws_nparams(void);
-int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params);
+/* Parameters and inputs of init_mode 2: the bisection level (0 = first
+ * cut) and the passes the learned start may use before the fallback. */
+int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level, int passes);
 /* Network output for every row of the current level (after
  * rsbgpu_level_laplacian), d_rows[mpad]. */
 int rsbgpu_lws_predict(rsbgpu *g, const float *d_params, float level, float *d_rows);
 /* One training step on the current level, after a plain rsbgpu_fiedler and
  * before rsbgpu_fiedler_sort: the loss 1 - cos^2(prediction, Fiedler vector)
- * averaged over the level's segments, its gradient added to d_grad. */
+ * averaged over the level's segments, its gradient written to d_grad. */
 int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_grad, float level,
                           double *loss);
 /* Adam update of the parameters (device arrays of LWS_NPARAM floats). */
@@ -141,9 +151,11 @@ int rsbgpu_lws_test(int M, const double *coord, float level, const int *rowptr, 
                     const int *wgt, const float *t, const float *params, float *y, float *grad,
                     double *loss);
 
-/* Small helpers for callers holding host data: copy n ints to a new device
- * buffer (freed with rsbgpu_free). */
+/* Small helpers for callers holding host data: copy n ints (or `bytes`
+ * bytes) to a new device buffer (freed with rsbgpu_free). */
 int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out);
+int rsbgpu_upload(rsbgpu *g, const void *host, size_t bytes, void **dev_out);
+int rsbgpu_download(rsbgpu *g, const void *dev, size_t bytes, void *host);
 void rsbgpu_free(void *dev);
 
 #ifdef __cplusplus

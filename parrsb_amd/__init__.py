@@ -56,7 +56,8 @@ class Report(C.Structure):
                 ("nconverged_fail", C.c_int * ML), ("lambda_min", C.c_double * ML),
                 ("lambda_max", C.c_double * ML), ("resid_max", C.c_double * ML),
                 ("resid_med", C.c_double * ML), ("niter_mean", C.c_double * ML),
-                ("level_ms", C.c_double * ML)]
+                ("level_ms", C.c_double * ML), ("nfallback", C.c_int * ML),
+                ("warm_start", C.c_int)]
 
     def as_dict(self):
         d = {}
@@ -75,7 +76,7 @@ class Report(C.Structure):
 
 class FiedlerOut(C.Structure):
     _fields_ = [("niter", C.c_void_p), ("npass", C.c_void_p), ("lam", C.c_void_p),
-                ("rnorm", C.c_void_p), ("resid", C.c_void_p)]
+                ("rnorm", C.c_void_p), ("resid", C.c_void_p), ("fallback", C.c_void_p)]
 
 
 P = C.c_void_p
@@ -91,6 +92,8 @@ SIGNATURES = {
     "parrsb_amd_last_error": (C.c_char_p, []),
     "parrsb_amd_set_precision": (I, [I]),
     "parrsb_amd_set_profiling": (I, [I]),
+    "parrsb_amd_warm_start_nparams": (I, []),
+    "parrsb_amd_set_warm_start": (I, [P, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
     "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
     # rsbgpu.h
@@ -118,8 +121,10 @@ SIGNATURES = {
     "rsbgpu_get_centroids": (I, [P, P]),
     "rsbgpu_quality": (I, [P, P, I, P]),
     "rsbgpu_upload_ints": (I, [P, P, I, C.POINTER(P)]),
+    "rsbgpu_upload": (I, [P, P, C.c_size_t, C.POINTER(P)]),
+    "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
     "rsbgpu_lws_nparams": (I, []),
-    "rsbgpu_lws_set_params": (I, [P, P]),
+    "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
     "rsbgpu_lws_train_step": (I, [P, P, P, C.c_float, C.POINTER(C.c_double)]),
     "rsbgpu_lws_adam": (I, [P, P, P, P, I, C.c_float]),
@@ -297,16 +302,48 @@ class Device:
                                              _p(col), _p(w), _p(re)))
         return rowptr, col[:nnz.value], w[:nnz.value], re[:mpad.value]
 
-    def fiedler(self, nseg, precision=64, max_iter=50, max_passes=50, tol=1e-5):
-        out = {k: np.zeros(nseg, dtype=np.int32) for k in ("niter", "npass")}
+    def fiedler(self, nseg, precision=64, max_iter=50, max_passes=50, tol=1e-5, init_mode=0):
+        """init_mode 2 needs lws_set_params first; out["fallback"][s] is 0 if
+        the learned start met tol, else 1 + its iterations."""
+        out = {k: np.zeros(nseg, dtype=np.int32) for k in ("niter", "npass", "fallback")}
         out["lambda"] = np.zeros(nseg)
         out["rnorm"] = np.zeros(3 * nseg)
         out["resid"] = np.zeros(nseg)
         fo = FiedlerOut(_p(out["niter"]), _p(out["npass"]), _p(out["lambda"]),
-                        _p(out["rnorm"]), _p(out["resid"]))
-        self._chk(lib().rsbgpu_fiedler(self.h, precision, max_iter, max_passes, tol, 0,
+                        _p(out["rnorm"]), _p(out["resid"]), _p(out["fallback"]))
+        self._chk(lib().rsbgpu_fiedler(self.h, precision, max_iter, max_passes, tol, init_mode,
                                        C.byref(fo)))
         return out
+
+    # learned warm start (device parameter arrays given as integer pointers,
+    # e.g. torch tensor data_ptr())
+    def lws_set_params(self, d_params, level, passes=1):
+        self._chk(lib().rsbgpu_lws_set_params(self.h, d_params, level, passes))
+
+    def lws_predict(self, d_params, level, d_rows):
+        self._chk(lib().rsbgpu_lws_predict(self.h, d_params, level, d_rows))
+
+    def upload(self, a):
+        """Copy a numpy array to a new device buffer; returns its address
+        (free with free())."""
+        a = np.ascontiguousarray(a)
+        d = P()
+        self._chk(lib().rsbgpu_upload(self.h, _p(a), a.nbytes, C.byref(d)))
+        return d.value
+
+    def download(self, d, like):
+        out = np.empty_like(like)
+        self._chk(lib().rsbgpu_download(self.h, d, out.nbytes, _p(out)))
+        return out
+
+    @staticmethod
+    def free(d):
+        lib().rsbgpu_free(d)
+
+    def lws_train_step(self, d_params, d_grad, level):
+        loss = C.c_double()
+        self._chk(lib().rsbgpu_lws_train_step(self.h, d_params, d_grad, level, C.byref(loss)))
+        return loss.value
 
     def fiedler_sort(self):
         self._chk(lib().rsbgpu_fiedler_sort(self.h))
@@ -335,6 +372,23 @@ class Device:
         part = np.zeros(self.nel, dtype=np.int32)
         self._chk(lib().rsbgpu_write_part(self.h, _p(part), 0))
         return part
+
+
+def warm_start_nparams() -> int:
+    return lib().parrsb_amd_warm_start_nparams()
+
+
+def set_warm_start(params, passes=1):
+    """Learned warm start for later part_mesh calls (None switches it off)."""
+    if params is None:
+        lib().parrsb_amd_set_warm_start(None, 1)
+        return
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    if params.size != warm_start_nparams():
+        raise ValueError(f"expected {warm_start_nparams()} parameters, got {params.size}")
+    rc = lib().parrsb_amd_set_warm_start(_p(params), passes)
+    if rc:
+        raise ParrsbError(rc, "bad warm-start parameters")
 
 
 def test_tqli(d, e):

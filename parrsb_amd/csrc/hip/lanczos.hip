@@ -38,6 +38,7 @@ struct lz_args {
   unsigned *cnt;
   int *running;
   int mpass;
+  int mode;  // start vector: 0 reference, 1 caller per element, 2 learned per row
   double tol;
   int miter_max;
   const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
@@ -107,9 +108,12 @@ __device__ inline void publish(const lz_args &a, const tile_t &t, double (&v)[NV
 // ---------------------------------------------------------------------------
 // Start vector (fiedler.c:570-585): x_i = i + 1, plus 1000 n on the first half
 // of the pre-partition order; then mean removed and normalised (FP64).
-// mode 0: reference start; mode 1: caller-supplied per-element start.
-__device__ inline double start_value(int mode, int i, int n, const float *start, int e) {
+// mode 0: reference start; mode 1: caller-supplied per-element start;
+// mode 2: per-row start (the learned warm start's prediction).
+__device__ inline double start_value(int mode, int i, int n, const float *start, int e,
+                                     int row) {
   if (mode == 1) return (double)start[e];
+  if (mode == 2) return (double)start[row];
   double v = i + 1.0;
   if (i < n / 2) v += (double)(1000 * (long long)n);
   return v;
@@ -127,7 +131,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_start(lz_args a, int phase, in
     int i = row - t.pstart;
     double x = 0.0;
     if (i < t.n) {
-      x = start_value(mode, i, t.n, start, a.row_elem[row]);
+      x = start_value(mode, i, t.n, start, a.row_elem[row], row);
       if (phase == 0) v[0] += x;
       else if (phase == 1) v[0] += (x - m) * (x - m);
     }
@@ -589,7 +593,16 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   const double n = a.seg[a.nseg + s];
   switch (op) {
     case FIN_START0: SC(s, SC_M0) = tot[0] / n; break;
-    case FIN_START1: SC(s, SC_N0) = 1.0 / sqrt(tot[0]); break;
+    case FIN_START1:
+      if (a.mode == 2 && !(tot[0] > 0.0 && tot[0] < INFINITY)) {
+        // constant or non-finite prediction: no pass, straight to the fallback
+        SC(s, SC_N0) = 0.0;
+        a.k[s] = a.miter[s];
+        a.status[s] = LZ_FINISHED;
+      } else {
+        SC(s, SC_N0) = 1.0 / sqrt(tot[0]);
+      }
+      break;
     case FIN_SETUP0: SC(s, SC_TMP) = tot[0] / n; break;
     case FIN_SETUP1: {  // fiedler.c:441-456
       double rtr = tot[0], rn = sqrt(rtr);
@@ -685,6 +698,21 @@ __global__ void k_lz_init_state(lz_args a, int miter_opt) {
   for (int j = 0; j < LZ_NSC; j++) SC(s, j) = 0.0;
 }
 
+// Learned start: segments that missed tol within their passes (k == miter
+// on the last pass, as the restart rule of fiedler.c:529-553 reads it) start
+// over from the reference start; the iterations spent so far are kept.
+__global__ void k_lz_fallback(lz_args a, int *__restrict__ count) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.nseg) return;
+  if (a.status[s] != LZ_FINISHED || a.k[s] != a.miter[s]) return;
+  SC(s, SC_WARMIT) = a.niter[s];
+  SC(s, SC_FALLBACK) = 1.0;
+  SC(s, SC_MEANF) = 0.0;
+  a.npass[s] = 0, a.niter[s] = 0, a.iter[s] = 0, a.k[s] = 0;
+  a.status[s] = LZ_SETUP;
+  atomicAdd(count, 1);
+}
+
 // Segments that cannot be bisected spectrally (fewer than 2 elements) keep
 // Fiedler value 0.
 __global__ void k_lz_zero_small(lz_args a, double *fied) {
@@ -752,7 +780,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.miter = L.status + 5 * std::max(g->lz_nseg_cap, 1);
   a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
   a.running = L.running;
-  a.mpass = mpass, a.tol = tol, a.miter_max = miter_max;
+  a.mpass = mpass, a.mode = init_mode, a.tol = tol, a.miter_max = miter_max;
   T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
   hipStream_t st = g->stream;
@@ -785,69 +813,100 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   RSB_LAUNCH_CHECK();
   k_lz_zero_small<<<nblocks(nseg, 256), 256, 0, st>>>(a, g->d_fied);
   RSB_LAUNCH_CHECK();
-  if (ntiles == 0) return 0;
-  for (int ph = 0; ph < 3; ph++) {
-    k_lz_start<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, ph, init_mode, g->d_start, f);
-    RSB_LAUNCH_CHECK();
-    if (ph < 2) FIN(ph == 0 ? FIN_START0 : FIN_START1);
+  if (ntiles == 0) {
+    (void)hipFree(act);
+    (void)hipFree(xrow);
+    return 0;
   }
-  // f holds the normalised start vector; SC_MEANF is 0 for the first pass.
-  const size_t tq_lds = sizeof(double) * (2 * miter_max + miter_max * miter_max);
-  for (int pass = 0; pass < mpass; pass++) {
-    k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, r, p);
-    RSB_LAUNCH_CHECK();
-    FIN(FIN_SETUP0);
-    k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
-    RSB_LAUNCH_CHECK();
-    FIN(FIN_SETUP1);
-    if (compact(LZ_RUNNING)) return 2;
-    RSB_TRY(hipEventRecord(g->lz_ev0, st));
-    int it, nrec = 0;
-    if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
-      size_t old = g->spmv_ev.size();
-      g->spmv_ev.resize(2 * miter_max);
-      for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
-    }
-    for (it = 0; it < miter_max; it++) {
-      T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
-      if (nact[0] == 0) break;
-      if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-      k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta);
-      if (g->prof_spmv) {
-        RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
-        nrec++;
-      }
-      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
-      k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
-      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+  // start vector of the segments in LZ_SETUP, normalised into f
+  auto start = [&](int mode, const float *sv) -> int {
+    a.mode = ai.mode = mode;
+    for (int ph = 0; ph < 3; ph++) {
+      k_lz_start<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, ph, mode, sv, f);
       RSB_LAUNCH_CHECK();
-      g->lz_launches++;
-      if ((it & 7) == 7 && it + 1 < miter_max) {
-        // drop segments that finished their pass from the launch lists
-        if (compact(LZ_RUNNING)) return 2;
-        if (nact[1] == 0) break;
+      if (ph < 2) FIN(ph == 0 ? FIN_START0 : FIN_START1);
+    }
+    return 0;
+  };
+  // restarted CG-Lanczos passes of the segments in LZ_SETUP (f holds the
+  // normalised start vector; SC_MEANF is 0 for the first pass)
+  const size_t tq_lds = sizeof(double) * (2 * miter_max + miter_max * miter_max);
+  auto passes = [&](int np_max) -> int {
+    a.mpass = ai.mpass = np_max;
+    for (int pass = 0; pass < np_max; pass++) {
+      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, r, p);
+      RSB_LAUNCH_CHECK();
+      FIN(FIN_SETUP0);
+      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
+      RSB_LAUNCH_CHECK();
+      FIN(FIN_SETUP1);
+      if (compact(LZ_RUNNING)) return 2;
+      RSB_TRY(hipEventRecord(g->lz_ev0, st));
+      int it, nrec = 0;
+      if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
+        size_t old = g->spmv_ev.size();
+        g->spmv_ev.resize(2 * miter_max);
+        for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
       }
+      for (it = 0; it < miter_max; it++) {
+        T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
+        if (nact[0] == 0) break;
+        if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
+        k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta);
+        if (g->prof_spmv) {
+          RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
+          nrec++;
+        }
+        k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
+        k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
+        k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+        RSB_LAUNCH_CHECK();
+        g->lz_launches++;
+        if ((it & 7) == 7 && it + 1 < miter_max) {
+          // drop segments that finished their pass from the launch lists
+          if (compact(LZ_RUNNING)) return 2;
+          if (nact[1] == 0) break;
+        }
+      }
+      RSB_TRY(hipEventRecord(g->lz_ev1, st));
+      RSB_TRY(hipEventSynchronize(g->lz_ev1));
+      float ms = 0.f;
+      RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
+      g->lz_ms += ms;
+      for (int q = 0; q < nrec; q++) {
+        RSB_TRY(hipEventElapsedTime(&ms, g->spmv_ev[2 * q], g->spmv_ev[2 * q + 1]));
+        g->spmv_ms += ms;
+      }
+      g->spmv_launches += nrec;
+      RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
+      k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
+      RSB_LAUNCH_CHECK();
+      k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
+      RSB_LAUNCH_CHECK();
+      FIN(FIN_RITZ);
+      int again = 0;
+      RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+      RSB_TRY(hipStreamSynchronize(st));
+      if (again == 0) break;
     }
-    RSB_TRY(hipEventRecord(g->lz_ev1, st));
-    RSB_TRY(hipEventSynchronize(g->lz_ev1));
-    float ms = 0.f;
-    RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
-    g->lz_ms += ms;
-    for (int q = 0; q < nrec; q++) {
-      RSB_TRY(hipEventElapsedTime(&ms, g->spmv_ev[2 * q], g->spmv_ev[2 * q + 1]));
-      g->spmv_ms += ms;
-    }
-    g->spmv_launches += nrec;
+    return 0;
+  };
+  int rc = 0;
+  if (init_mode == 2) {
+    // learned start (per row, written into w, which the passes overwrite
+    // only after the start has been normalised into f), then the fallback
+    float *srow = (float *)w;
+    if ((rc = rsb_lws_predict(g, g->lws_params, g->lws_level, srow))) return rc;
+    if ((rc = start(2, srow)) || (rc = passes(std::min(g->lws_passes, mpass)))) return rc;
     RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
-    k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
+    k_lz_fallback<<<nblocks(nseg, 256), 256, 0, st>>>(a, L.running + 1);
     RSB_LAUNCH_CHECK();
-    k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
-    RSB_LAUNCH_CHECK();
-    FIN(FIN_RITZ);
-    int again = 0;
-    RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    int nfb = 0;
+    RSB_TRY(hipMemcpyAsync(&nfb, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     RSB_TRY(hipStreamSynchronize(st));
-    if (again == 0) break;
+    if (nfb > 0 && ((rc = start(0, nullptr)) || (rc = passes(mpass)))) return rc;
+  } else {
+    if ((rc = start(init_mode, g->d_start)) || (rc = passes(mpass))) return rc;
   }
   k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, g->d_fied, xrow);
   RSB_LAUNCH_CHECK();
@@ -878,6 +937,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // 5 s n (p_old and r read once, p_new, w and the basis vector written).
   // Counted iterations, real (unpadded) sizes.
   for (int s = 0; s < nseg; s++) {
+    const double *scs = &sc[(size_t)s * LZ_NSC];
+    ni[s] += (int)scs[SC_WARMIT];
     if (g->h_p[s] > 1 && g->h_n[s] >= 2) {
       double n = g->h_n[s], nz = (double)g->h_seg_nnz[s], sz = sizeof(T);
       g->lz_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 9 * sz * n);
@@ -893,6 +954,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         out->rnorm[3 * s + 2] = sc[(size_t)s * LZ_NSC + SC_RNORM];
       }
       if (out->resid) out->resid[s] = sc[(size_t)s * LZ_NSC + SC_TMP];
+      if (out->fallback) out->fallback[s] = scs[SC_FALLBACK] != 0.0 ? 1 + (int)scs[SC_WARMIT] : 0;
     }
   }
   return 0;
@@ -911,6 +973,8 @@ extern "C" int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_pa
   RSB_TRY(hipSetDevice(g->device));
   if (max_iter < 1 || max_passes < 1) RSB_FAIL(3, "rsb_max_iter and rsb_max_passes must be >= 1");
   if (init_mode == 1 && !g->d_start) RSB_FAIL(3, "no start vector set");
+  if (init_mode == 2 && !g->lws_params) RSB_FAIL(3, "no warm-start parameters set");
+  if (init_mode < 0 || init_mode > 2) RSB_FAIL(3, "unknown init_mode %d", init_mode);
   rsb_phase_begin(g);
   int rc;
   if (precision == 32) rc = run_lanczos<float>(g, max_iter, max_passes, tol, init_mode, out);

@@ -517,8 +517,12 @@ int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows) {
 
 extern "C" int rsbgpu_lws_nparams(void) { return LWS_NPARAM; }
 
-extern "C" int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params) {
+extern "C" int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level,
+                                     int passes) {
+  if (passes < 1) RSB_FAIL(3, "learned start needs at least one pass");
   g->lws_params = d_params;
+  g->lws_level = level;
+  g->lws_passes = passes;
   return 0;
 }
 
@@ -535,8 +539,9 @@ extern "C" int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_
   const int64_t M = g->mpad;
   const int nt = g->ntiles, nseg = g->nseg;
   *loss_out = 0.0;
-  if (nt == 0) return 0;
   hipStream_t st = g->stream;
+  RSB_TRY(hipMemsetAsync(d_grad, 0, sizeof(float) * LWS_NPARAM, st));
+  if (nt == 0) return hipStreamSynchronize(st) == hipSuccess ? 0 : 2;
   lws_ws w;
   int rc = lws_ws_get(g, true, w);
   if (rc) return rc;

@@ -489,6 +489,21 @@ extern "C" int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_o
   return 0;
 }
 
+extern "C" int rsbgpu_upload(rsbgpu *g, const void *host, size_t bytes, void **dev_out) {
+  RSB_TRY(hipSetDevice(g->device));
+  void *d = nullptr;
+  RSB_TRY(hipMalloc(&d, bytes > 0 ? bytes : 1));
+  if (bytes > 0) RSB_TRY(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+  *dev_out = d;
+  return 0;
+}
+
+extern "C" int rsbgpu_download(rsbgpu *g, const void *dev, size_t bytes, void *host) {
+  RSB_TRY(hipSetDevice(g->device));
+  if (bytes > 0) RSB_TRY(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 extern "C" void rsbgpu_free(void *dev) {
   if (dev) hipFree(dev);
 }

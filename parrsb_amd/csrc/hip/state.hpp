@@ -35,7 +35,7 @@ enum { LZ_IDLE = 0, LZ_SETUP = 1, LZ_RUNNING = 2, LZ_PASSDONE = 3, LZ_FINISHED =
 enum {
   SC_RTR = 0, SC_RTZ1, SC_RTZ2, SC_PAP, SC_PAPOLD, SC_BETA, SC_ALPHA, SC_MEANP,
   SC_RNI, SC_RNORM, SC_RTOL, SC_RNORM0, SC_MEANF, SC_LAMBDA, SC_NORMF, SC_TMP, SC_RES,
-  SC_M0, SC_N0,
+  SC_M0, SC_N0, SC_WARMIT, SC_FALLBACK,
   LZ_NSC
 };
 
@@ -95,6 +95,8 @@ struct rsbgpu {
   double *d_fied = nullptr;     // Fiedler value per compact position
   float *d_start = nullptr;     // optional start vector per padded row
   const float *lws_params = nullptr;  // learned warm start parameters (device)
+  float lws_level = 0.f;              // bisection level fed to the network
+  int lws_passes = 1;                 // passes granted to the learned start
   void *d_lws_ws = nullptr;
   size_t lws_ws_bytes = 0;
   lz_state lz{};

@@ -44,6 +44,9 @@ parrsb_options parrsb_default_options = {
 static parrsb_amd_report last_report;
 static char last_error[512];
 static int cfg_precision = 64, cfg_profiling = 0;
+static float *cfg_warm = NULL; /* host copy of the learned-start parameters */
+static int cfg_warm_passes = 1;
+static char cfg_warm_file[512];
 
 const char *parrsb_amd_last_error(void) { return last_error; }
 
@@ -69,6 +72,54 @@ static void set_error(const char *fmt, ...) {
   va_start(ap, fmt);
   vsnprintf(last_error, sizeof(last_error), fmt, ap);
   va_end(ap);
+}
+
+int parrsb_amd_warm_start_nparams(void) { return rsbgpu_lws_nparams(); }
+
+int parrsb_amd_set_warm_start(const float *params, int passes) {
+  if (params && passes < 1) return 3;
+  free(cfg_warm);
+  cfg_warm = NULL;
+  cfg_warm_file[0] = 0;
+  if (!params) return 0;
+  size_t n = (size_t)rsbgpu_lws_nparams();
+  cfg_warm = (float *)malloc(n * sizeof(float));
+  if (!cfg_warm) return 2;
+  memcpy(cfg_warm, params, n * sizeof(float));
+  cfg_warm_passes = passes;
+  return 0;
+}
+
+/* Parameter file: "LWS1", int32 count, count float32 values (little endian,
+ * as written by parrsb_amd.lws_train). */
+static int load_warm_file(const char *path) {
+  if (cfg_warm && strcmp(path, cfg_warm_file) == 0) return 0;
+  FILE *fp = fopen(path, "rb");
+  if (!fp) {
+    set_error("cannot open warm-start file %s", path);
+    return 3;
+  }
+  char magic[4];
+  int32_t n = 0;
+  int ok = fread(magic, 1, 4, fp) == 4 && memcmp(magic, "LWS1", 4) == 0 &&
+           fread(&n, sizeof(n), 1, fp) == 1 && n == rsbgpu_lws_nparams();
+  float *buf = ok ? (float *)malloc(sizeof(float) * (size_t)n) : NULL;
+  ok = ok && buf && fread(buf, sizeof(float), (size_t)n, fp) == (size_t)n;
+  fclose(fp);
+  if (!ok) {
+    free(buf);
+    set_error("warm-start file %s is not a parameter file of this network", path);
+    return 3;
+  }
+  const char *v = getenv("PARRSB_AMD_WARM_PASSES");
+  int rc = parrsb_amd_set_warm_start(buf, v ? atoi(v) : cfg_warm_passes);
+  free(buf);
+  if (rc) {
+    set_error("bad PARRSB_AMD_WARM_PASSES");
+    return 3;
+  }
+  snprintf(cfg_warm_file, sizeof(cfg_warm_file), "%s", path);
+  return 0;
 }
 
 /* PARRSB_* environment overrides written into the caller's struct
@@ -201,6 +252,11 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
     set_error("number of parts must be >= 1");
     return 3;
   }
+  const char *wf = getenv("PARRSB_AMD_WARM_START");
+  if (wf && *wf) {
+    int wrc = load_warm_file(wf);
+    if (wrc) return wrc;
+  }
   rsbgpu *g = NULL;
   int rc = rsbgpu_open(&g, device_index());
   if (rc) {
@@ -224,10 +280,19 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
       prm.find_comps = options->find_disconnected_comps, prm.precision = precision_bits();
       prm.verbose = options->verbose_level, prm.tol = options->rsb_tol;
       prm.residuals = 1;
-      rc = rsb_run(g, nel, &prm, rep);
+      prm.warm = NULL, prm.warm_passes = cfg_warm_passes;
+      void *dw = NULL;
+      if (cfg_warm) {
+        rc = rsbgpu_upload(g, cfg_warm, sizeof(float) * (size_t)rsbgpu_lws_nparams(), &dw);
+        prm.warm = (const float *)dw;
+        rep->warm_start = 1;
+      }
+      if (!rc) rc = rsb_run(g, nel, &prm, rep);
+      if (!rc) rc = rsbgpu_write_part(g, part, on_device);
+      rsbgpu_free(dw);
     }
   }
-  if (!rc) rc = rsbgpu_write_part(g, part, on_device);
+  if (!rc && options->partitioner == 1) rc = rsbgpu_write_part(g, part, on_device);
   if (rc) set_error("%s", rsbgpu_error(g));
   rsbgpu_phase_times(g, rep->phase_ms);
   double tr[6];

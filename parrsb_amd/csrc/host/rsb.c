@@ -92,7 +92,9 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   double *lam = (double *)calloc((size_t)P, sizeof(double));
   double *rn = (double *)calloc(3 * (size_t)P, sizeof(double));
   double *res = (double *)calloc((size_t)P, sizeof(double));
-  if (!ok || !nl || !snd || !scnt || !moved || !ncomp || !niter || !npass || !lam || !rn || !res) {
+  int *fb = (int *)calloc((size_t)P, sizeof(int));
+  if (!ok || !nl || !snd || !scnt || !moved || !ncomp || !niter || !npass || !lam || !rn || !res ||
+      !fb) {
     rc = 2;
     goto done;
   }
@@ -104,8 +106,14 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     TRY(rsbgpu_set_segments(g, nseg, A.cstart, A.n, A.p, A.r0));
     TRY(rsbgpu_prepartition(g, prm->rsb_pre));
     TRY(rsbgpu_level_laplacian(g));
-    rsbgpu_fiedler_out fo = {niter, npass, lam, rn, prm->residuals ? res : NULL};
-    TRY(rsbgpu_fiedler(g, prm->precision, prm->max_iter, prm->max_passes, prm->tol, 0, &fo));
+    rsbgpu_fiedler_out fo = {niter, npass, lam, rn, prm->residuals ? res : NULL, fb};
+    int mode = 0;
+    if (prm->warm) {
+      TRY(rsbgpu_lws_set_params(g, prm->warm, (float)cut, prm->warm_passes));
+      mode = 2;
+    }
+    memset(fb, 0, sizeof(int) * (size_t)nseg);
+    TRY(rsbgpu_fiedler(g, prm->precision, prm->max_iter, prm->max_passes, prm->tol, mode, &fo));
     TRY(rsbgpu_fiedler_sort(g));
     for (int s = 0; s < nseg; s++) {
       snd[s] = -1, scnt[s] = 0, nl[s] = A.n[s];
@@ -139,7 +147,9 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
         work += (double)niter[s] * A.n[s];
         nwork += A.n[s];
         int mi = prm->max_iter < A.n[s] ? prm->max_iter : A.n[s];
-        if (niter[s] == mi * prm->max_passes) nfail++;
+        /* iterations of the reference-start solve (a fallback counts the
+           learned attempt's fb[s] - 1 iterations in front of them) */
+        if (niter[s] - (fb[s] ? fb[s] - 1 : 0) == mi * prm->max_passes) nfail++;
       }
       if (prm->find_comps) {
         for (int h = 0; h < 2; h++) {
@@ -149,6 +159,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
         }
       }
       nb += moved[s];
+      rep->nfallback[lv] += fb[s] > 0;
     }
     rep->nsegs[lv] = ncut;
     rep->niter_min[lv] = ncut ? itmin : 0, rep->niter_max[lv] = itmax;
@@ -188,7 +199,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
 done:
   segtab_free(&A), segtab_free(&B);
   free(nl), free(snd), free(scnt), free(moved), free(ncomp), free(niter), free(npass), free(lam),
-      free(rn), free(res);
+      free(rn), free(res), free(fb);
   return rc;
 }
 

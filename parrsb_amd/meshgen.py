@@ -16,6 +16,10 @@ Workloads used by the tests and the benchmark:
   box_hex(128, 128, 64, jitter=0.1, seed=2)    1,048,576 hexes, 1,024 parts
   holed_quads(2048, 2048, seed=3)              ~3.98M quads, 4,096 parts
   warped_box_hex(256, seed=5)                  16,777,216 hexes, 65,536 parts
+  pipe_hex(16, 128, 1024, seed=7)              2,097,152 hexes, 2,048 parts
+                                               (held-out learned-start mesh)
+  warm_start_mesh(i)                           20k-200k hexes, the training set
+                                               of the learned warm start
 """
 from __future__ import annotations
 
@@ -83,6 +87,111 @@ def warped_box_hex(n: int = 256, seed: int = 5, amp: float = 0.05):
     del x
     xyz = X[vtx - 1]
     return vtx, xyz, 8
+
+
+def _lattice(nx: int, ny: int, nz: int, wrap_y: bool = False):
+    """Hex connectivity of an nx x ny x nz lattice and the lattice index
+    (i, j, k) of every vertex id.  With wrap_y the j direction is periodic
+    (j = ny is j = 0), as around a pipe."""
+    my = ny if wrap_y else ny + 1
+    i = np.arange(nx, dtype=np.int64)
+    j = np.arange(ny, dtype=np.int64)
+    k = np.arange(nz, dtype=np.int64)
+    kk, jj, ii = np.meshgrid(k, j, i, indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    sx, sy = nx + 1, (nx + 1) * my
+    vtx = np.empty((ii.size, 8), dtype=np.int64)
+    for c, (dx, dy, dz) in enumerate(_HEX):
+        vtx[:, c] = 1 + (ii + dx) + sx * ((jj + dy) % my) + sy * (kk + dz)
+    nvert = sx * my * (nz + 1)
+    vid = np.arange(nvert, dtype=np.int64)
+    ijk = np.stack([vid % sx, (vid // sx) % my, vid // sy], axis=-1).astype(np.float64)
+    return vtx, ijk
+
+
+def _warp(X, rng, amp, size):
+    """X_d += size * sum_e a_de sin(2 pi k_de u_e + phi_de), u = X scaled to
+    [0, 1] per axis, a_de in [0, amp], k_de in {1, 2}."""
+    lo, hi = X.min(0), X.max(0)
+    u = (X - lo) / np.where(hi > lo, hi - lo, 1.0)
+    a = rng.uniform(0.0, amp, size=(3, 3))
+    kf = rng.integers(1, 3, size=(3, 3)).astype(np.float64)
+    ph = rng.uniform(0.0, 2 * np.pi, size=(3, 3))
+    Y = X.copy()
+    for d in range(3):
+        for e in range(3):
+            Y[:, d] += size * a[d, e] * np.sin(2 * np.pi * kf[d, e] * u[:, e] + ph[d, e])
+    return Y
+
+
+def pipe_hex(nr: int, nt: int, nz: int, r_in: float = 0.5, r_out: float = 1.0,
+             length: float | None = None, seed: int = 7, amp: float = 0.05):
+    """Annular pipe section: nr cells across the wall, nt around (periodic),
+    nz along the axis; length defaults to the one that makes the cells about
+    as long as they are wide around the mean radius.  Vertex positions are
+    warped by a seeded sinusoidal map of amplitude <= amp times the outer
+    diameter."""
+    rng = np.random.default_rng(seed)
+    if length is None:
+        length = nz * 2 * np.pi * 0.5 * (r_in + r_out) / nt
+    vtx, ijk = _lattice(nr, nt, nz, wrap_y=True)
+    r = r_in + (r_out - r_in) * ijk[:, 0] / nr
+    th = 2 * np.pi * ijk[:, 1] / nt
+    X = np.stack([r * np.cos(th), r * np.sin(th), length * ijk[:, 2] / nz], axis=-1)
+    X = _warp(X, rng, amp, 2 * r_out)
+    return vtx, X[vtx - 1], 8
+
+
+def channel_hex(nx: int, ny: int, nz: int, width: float = 1.0, bend: float = 1.0,
+                seed: int = 11, amp: float = 0.05):
+    """Rectangular channel of nx cells along its centreline (cells square in
+    plan), ny across and nz deep, bent through `bend` radians along a circular
+    arc, then warped like pipe_hex."""
+    rng = np.random.default_rng(seed)
+    vtx, ijk = _lattice(nx, ny, nz)
+    h = width / ny
+    s = ijk[:, 0] * h
+    u = ijk[:, 1] * h - 0.5 * width
+    w = ijk[:, 2] * h
+    L = nx * h
+    if abs(bend) > 1e-9:
+        R = L / bend
+        X = np.stack([(R + u) * np.sin(s / R), R - (R + u) * np.cos(s / R), w], axis=-1)
+    else:
+        X = np.stack([s, u, w], axis=-1)
+    X = _warp(X, rng, amp, width)
+    return vtx, X[vtx - 1], 8
+
+
+def warm_start_mesh(i: int):
+    """Mesh i of the learned warm start's training set: a seeded annular pipe
+    section or bent channel of 20k-200k hexes, aspect ratio 1-16, warped with
+    amplitude <= 0.2.  Returns (vtx, xyz, nv, info)."""
+    rng = np.random.default_rng(1000003 * (i + 1))
+    target = int(np.exp(rng.uniform(np.log(20000), np.log(200000))))
+    aspect = float(np.exp(rng.uniform(0.0, np.log(16.0))))
+    amp = float(rng.uniform(0.0, 0.2))
+    seed = int(rng.integers(1 << 30))
+    if rng.uniform() < 0.5:
+        # pipe: length / outer diameter = aspect; cells ~cubic
+        r_in = float(rng.uniform(0.3, 0.8))
+        nr_over_nt = (1 - r_in) / (np.pi * (1 + r_in))
+        nz_over_nt = 2 * aspect / (np.pi * (1 + r_in))
+        nt = max(8, int(round((target / (nr_over_nt * nz_over_nt)) ** (1 / 3))))
+        nr = max(1, int(round(nr_over_nt * nt)))
+        nz = max(1, int(round(nz_over_nt * nt)))
+        vtx, xyz, nv = pipe_hex(nr, nt, nz, r_in=r_in, r_out=1.0, length=2.0 * aspect,
+                                seed=seed, amp=amp)
+        info = dict(kind="pipe", nr=nr, nt=nt, nz=nz, r_in=r_in, aspect=aspect, amp=amp)
+    else:
+        depth = float(rng.uniform(0.2, 1.0))  # depth / width
+        ny = max(2, int(round((target / (aspect * depth)) ** (1 / 3))))
+        nx, nz = max(1, int(round(aspect * ny))), max(1, int(round(depth * ny)))
+        bend = float(rng.uniform(0.0, np.pi))
+        vtx, xyz, nv = channel_hex(nx, ny, nz, bend=bend, seed=seed, amp=amp)
+        info = dict(kind="channel", nx=nx, ny=ny, nz=nz, bend=bend, aspect=aspect, amp=amp)
+    info["nel"] = int(vtx.shape[0])
+    return vtx, xyz, nv, info
 
 
 def quad_grid(nx: int, ny: int, h: float | None = None):
