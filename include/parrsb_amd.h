@@ -46,6 +46,10 @@ typedef struct {
   double level_ms[PARRSB_AMD_MAX_LEVELS];  /* wall time of the level */
   int nfallback[PARRSB_AMD_MAX_LEVELS];    /* learned start: segments that fell back */
   int warm_start;                          /* 1 if the learned start was used */
+  /* RSB_NEIGHBORS (rsb.c:342-343): parts adjacent to a rank's part after the
+     cut, min / max over segments, mean over ranks */
+  int nbrs_min[PARRSB_AMD_MAX_LEVELS], nbrs_max[PARRSB_AMD_MAX_LEVELS];
+  double nbrs_mean[PARRSB_AMD_MAX_LEVELS];
 } parrsb_amd_report;
 
 /* Same contract as parrsb_part_mesh, with every array in GPU memory of the

@@ -139,7 +139,8 @@ int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level, int pas
  * rsbgpu_level_laplacian), d_rows[mpad]. */
 int rsbgpu_lws_predict(rsbgpu *g, const float *d_params, float level, float *d_rows);
 /* One training step on the current level, after a plain rsbgpu_fiedler and
- * before rsbgpu_fiedler_sort: the loss 1 - cos^2(prediction, Fiedler vector)
+ * before rsbgpu_fiedler_sort: the loss R(y) / lambda - 1 per segment (R the
+ * Rayleigh quotient of the prediction y, lambda the segment's Fiedler value)
  * averaged over the level's segments, its gradient written to d_grad. */
 int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_grad, float level,
                           double *loss);
@@ -148,8 +149,12 @@ int rsbgpu_lws_adam(float *d_params, const float *d_grad, float *d_m, float *d_v
                     float lr);
 /* Parity hook: forward + backward on one segment given on the host. */
 int rsbgpu_lws_test(int M, const double *coord, float level, const int *rowptr, const int *col,
-                    const int *wgt, const float *t, const float *params, float *y, float *grad,
+                    const int *wgt, double lam, const float *params, float *y, float *grad,
                     double *loss);
+
+/* Number of other current segments that share a vertex with each segment
+ * (statistics.c:17-113), nbrs[nseg] on the host. */
+int rsbgpu_neighbors(rsbgpu *g, int *nbrs);
 
 /* Small helpers for callers holding host data: copy n ints (or `bytes`
  * bytes) to a new device buffer (freed with rsbgpu_free). */

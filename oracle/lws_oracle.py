@@ -1,8 +1,9 @@
 """numpy restatement of the learned-warm-start network (test infrastructure
-only): features, forward, the 1 - cos^2 loss and the backward pass, with
-optional BF16 rounding at the points where the device kernels round (layer
-activations, aggregates, weights, gradients fed to the matrix units).  Its
-backward pass is pinned by central finite differences in tests/test_lws.py.
+only): features, forward, the Rayleigh-quotient loss and the backward pass,
+with optional BF16 rounding at the points where the device kernels round
+(layer activations, aggregates, weights, gradients fed to the matrix units).
+Its backward pass is pinned by central finite differences in
+tests/test_lws.py.
 
 Parameter layout (floats): W0 [64 x 8] | b0 [64] | 4 x (W_l [64 x 128] |
 b_l [64]) | w_o [64] | b_o.
@@ -93,17 +94,23 @@ def forward(P, x, A, emulate_bf16=False):
     return y, dict(h=h, a=a, dinv=dinv, x=x)
 
 
-def loss(y, t):
-    yc, tc = y - y.mean(), t - t.mean()
-    S, Q, T = yc @ tc, yc @ yc, tc @ tc
-    return 1.0 - S * S / (Q * T)
+def laplacian(A):
+    return np.diag(A.sum(1)) - A
 
 
-def dloss(y, t):
-    yc, tc = y - y.mean(), t - t.mean()
-    S, Q, T = yc @ tc, yc @ yc, tc @ tc
-    g = -2.0 * S * tc / (Q * T) + 2.0 * S * S * yc / (Q * Q * T)
-    return g - g.mean()
+def loss(y, A, lam):
+    """R(y) / lam - 1, R the Rayleigh quotient of the graph Laplacian of A
+    on the centred prediction."""
+    yc = y - y.mean()
+    return (yc @ laplacian(A) @ yc) / (yc @ yc) / lam - 1.0
+
+
+def dloss(y, A, lam):
+    yc = y - y.mean()
+    Ly = laplacian(A) @ y
+    Q = yc @ yc
+    R = (y @ Ly) / Q
+    return 2.0 * (Ly - R * yc) / (Q * lam)
 
 
 def backward(P, cache, A, dy, emulate_bf16=False):

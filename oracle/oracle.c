@@ -142,6 +142,117 @@ void oracle_rcb_order(const double *coord, int ndim, int *order, int n, int p) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Dominant eigenvector of a symmetric n x n matrix (n = 2, 3) by cyclic
+ * Jacobi rotations (Golub & Van Loan, Matrix Computations, 4th ed.,
+ * Alg. 8.5.1 sym.schur2 and 8.5.3 cyclic Jacobi).  The eigenvector of the
+ * largest diagonal entry after convergence (lowest index on ties), signed so
+ * that its largest-magnitude component (lowest index on ties) is positive. */
+void oracle_sym_dominant(const double *Ain, int n, double *ev) {
+  double A[3][3], V[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      A[i][j] = (i < n && j < n) ? Ain[i * 3 + j] : 0.0;
+      V[i][j] = i == j ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 32; sweep++) {
+    int rotated = 0;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = A[p][q];
+        if (apq == 0.0) continue;
+        double scale = fabs(A[p][p]) + fabs(A[q][q]);
+        if (fabs(apq) <= DBL_EPSILON * DBL_EPSILON * scale) continue;
+        double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+        double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                              : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+        double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        /* A <- J^T A J, V <- V J with J_pp = J_qq = c, J_pq = s, J_qp = -s */
+        for (int k = 0; k < n; k++) {
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        A[p][q] = A[q][p] = 0.0;
+        for (int k = 0; k < n; k++) {
+          double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+        rotated = 1;
+      }
+    if (!rotated) break;
+  }
+  int im = 0;
+  for (int i = 1; i < n; i++)
+    if (A[i][i] > A[im][im]) im = i;
+  int dm = 0;
+  for (int d = 1; d < n; d++)
+    if (fabs(V[d][im]) > fabs(V[dm][im])) dm = d;
+  double sg = V[dm][im] < 0.0 ? -1.0 : 1.0;
+  for (int d = 0; d < 3; d++) ev[d] = d < n ? sg * V[d][im] : 0.0;
+}
+
+/* RIB axis of the elements order[0..n) (rib.c:6-57): mean of the centroids
+ * (summed in order, then divided), centred covariance (summed in order),
+ * dominant eigenvector.  The reference takes <= 100 power steps from a
+ * random start (fiedler.c:38-73), which fixes neither the sign nor, to better
+ * than ~1e-6, the direction; this restatement takes the exact dominant
+ * eigenvector with a fixed sign.  In 2D the covariance is 2 x 2 (the
+ * reference passes the 3 x 3 array with N = 2, rib.c:47). */
+void oracle_rib_axis(const double *coord, int ndim, const int *order, int n, double *mean,
+                     double *ev) {
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < ndim; d++) m[d] += coord[3 * (size_t)order[i] + d];
+  for (int d = 0; d < ndim; d++) m[d] /= n;
+  double I[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < n; i++) {
+    const double *c = coord + 3 * (size_t)order[i];
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ndim; d++) x[d] = c[d] - m[d];
+    for (int a = 0; a < ndim; a++)
+      for (int b = 0; b < ndim; b++) I[a * 3 + b] += x[a] * x[b];
+  }
+  oracle_sym_dominant(I, ndim, ev);
+  for (int d = 0; d < 3; d++) mean[d] = m[d];
+}
+
+static void rib_rec(const double *coord, int ndim, int *order, double *key, int n, int p) {
+  if (n <= 1) return;
+  double m[3], ev[3];
+  oracle_rib_axis(coord, ndim, order, n, m, ev);
+  for (int i = 0; i < n; i++) {
+    const double *c = coord + 3 * (size_t)order[i];
+    double k = 0.0;
+    for (int d = 0; d < ndim; d++) k += (c[d] - m[d]) * ev[d];
+    key[i] = k;
+  }
+  stable_sort_by_pos_key(order, key, n);
+  /* split as RCB: virtual-rank boundary for p > 1 (rib.c:73-105), half for
+     the local recursion (rib.c:59-71) */
+  if (p > 1) {
+    int nl = (int)oracle_lower_count(n, p);
+    rib_rec(coord, ndim, order, key, nl, (p + 1) / 2);
+    rib_rec(coord, ndim, order + nl, key + nl, n - nl, p / 2);
+  } else {
+    int mid = n / 2;
+    rib_rec(coord, ndim, order, key, mid, 1);
+    rib_rec(coord, ndim, order + mid, key + mid, n - mid, 1);
+  }
+}
+
+void oracle_rib_order(const double *coord, int ndim, int *order, int n, int p) {
+  double *key = xmalloc(sizeof(double) * (size_t)n);
+  rib_rec(coord, ndim, order, key, n, p);
+  free(key);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Slot grouping: slots (e, j) of a sub-mesh grouped by vertex id.  Vertex
  * id 0 is "no connection" in gather-scatter: such a slot forms its own
  * group. */
@@ -608,7 +719,12 @@ static int cmp_ll(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-static int max_neighbors(const long long *vtx, int nv, const int *segof, int nel, int nseg) {
+/* Neighbouring segments after a cut (statistics.c:17-113): for every
+ * segment, the number of other segments sharing a vertex with it; min / max
+ * over segments and the mean over virtual ranks (each rank reports its
+ * segment's count, metrics.c:53-62). */
+static void neighbor_stats(const long long *vtx, int nv, const int *segof, int nel, int nseg,
+                           const int *segp, int *mn_out, int *mx_out, double *mean_out) {
   size_t ns = (size_t)nel * nv;
   vseg_t *a = xmalloc(sizeof(vseg_t) * ns);
   for (size_t s = 0; s < ns; s++) a[s].vid = vtx[s], a[s].seg = segof[s / nv];
@@ -631,10 +747,15 @@ static int max_neighbors(const long long *vtx, int nv, const int *segof, int nel
   int *cnt = calloc((size_t)nseg, sizeof(int));
   for (size_t i = 0; i < np; i++)
     if (i == 0 || pairs[i] != pairs[i - 1]) cnt[pairs[i] / nseg]++;
-  int mx = 0;
-  for (int s = 0; s < nseg; s++) mx = cnt[s] > mx ? cnt[s] : mx;
+  int mx = 0, mn = 1 << 30;
+  double sum = 0.0, ranks = 0.0;
+  for (int s = 0; s < nseg; s++) {
+    mx = cnt[s] > mx ? cnt[s] : mx;
+    mn = cnt[s] < mn ? cnt[s] : mn;
+    sum += (double)cnt[s] * segp[s], ranks += segp[s];
+  }
+  *mn_out = nseg ? mn : 0, *mx_out = mx, *mean_out = ranks > 0 ? sum / ranks : 0.0;
   free(a), free(pairs), free(cnt);
-  return mx;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -679,6 +800,8 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       /* pre-partition (rsb.c:282-291) */
       if (rsb_pre == 1) {
         oracle_rcb_order(coord, ndim, el, n, p);
+      } else if (rsb_pre == 2) {
+        oracle_rib_order(coord, ndim, el, n, p);
       } else {
         for (int i = 0; i < n; i++) fv[i] = (double)el[i];
         stable_sort_by_pos_key(el, fv, n);
@@ -756,9 +879,14 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       rep->nsegs[cut] = ncut;
       rep->nbalanced[cut] = nbal;
       rep->lambda_min[cut] = ncut ? lmin : 0, rep->lambda_max[cut] = ncut ? lmax : 0;
-      for (int s = 0; s < nseg; s++)
+      int *segp = xmalloc(sizeof(int) * (size_t)nseg);
+      for (int s = 0; s < nseg; s++) {
+        segp[s] = segs[s].p;
         for (int i = 0; i < segs[s].n; i++) segof[order[segs[s].start + i]] = s;
-      rep->nbrs_max[cut] = max_neighbors(vtx, nv, segof, nel, nseg);
+      }
+      neighbor_stats(vtx, nv, segof, nel, nseg, segp, &rep->nbrs_min[cut], &rep->nbrs_max[cut],
+                     &rep->nbrs_mean[cut]);
+      free(segp);
     }
   }
   for (int s = 0; s < nseg; s++)

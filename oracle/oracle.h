@@ -11,6 +11,7 @@
  * representation:
  *   - element distribution after a global sort       sort.c:48-70, 106-117
  *   - RCB order with virtual-rank split points        rcb.c:7-137
+ *   - RIB order (deterministic axis)                  rib.c:6-111
  *   - Lanczos start vector from the global position   fiedler.c:563-585
  *   - CG-Lanczos, TQLI, Ritz restarts                 fiedler.c:13-30, 323-561
  *   - shared-vertex Laplacian (gather-scatter form)   laplacian.c:159-205
@@ -33,7 +34,7 @@ extern "C" {
 
 typedef struct {
   int find_disconnected_comps;
-  int rsb_pre; /* 0 none (sort by global id), 1 RCB */
+  int rsb_pre; /* 0 none (sort by global id), 1 RCB, 2 RIB */
   int max_iter, max_passes;
   double tol;
   int nparts;
@@ -50,6 +51,8 @@ typedef struct {
   int nsegs[ORACLE_MAX_LEVELS];
   int nbalanced[ORACLE_MAX_LEVELS]; /* segments where interface elements moved */
   double lambda_min[ORACLE_MAX_LEVELS], lambda_max[ORACLE_MAX_LEVELS];
+  int nbrs_min[ORACLE_MAX_LEVELS];    /* neighbouring segments after the cut */
+  double nbrs_mean[ORACLE_MAX_LEVELS]; /* mean over virtual ranks */
 } oracle_report;
 
 /* Centroids: FP64 mean of the nv corners, summed in corner order
@@ -70,6 +73,15 @@ int64_t oracle_balance_expected(int64_t n, int p, int bin);
  * the current order and is permuted in place into the RCB order for p
  * virtual ranks.  coord is indexed by element (x3). */
 void oracle_rcb_order(const double *coord, int ndim, int *order, int n, int p);
+
+/* RIB order of a segment (rib.c:6-111) with the deterministic axis of
+ * oracle_rib_axis; same split points as RCB. */
+void oracle_rib_order(const double *coord, int ndim, int *order, int n, int p);
+/* Mean and dominant covariance eigenvector of the centroids order[0..n). */
+void oracle_rib_axis(const double *coord, int ndim, const int *order, int n, double *mean,
+                     double *ev);
+/* Dominant eigenvector of a symmetric n x n (n <= 3) matrix stored 3 x 3. */
+void oracle_sym_dominant(const double *A, int n, double *ev);
 
 /* Weighted dual graph of the whole mesh: w_ij = number of shared vertex ids
  * (mat.c:17-61, 203-249).  CSR with columns sorted ascending, no diagonal.

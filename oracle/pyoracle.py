@@ -47,7 +47,8 @@ class Report(C.Structure):
                 ("ncomp_min", C.c_int * ML), ("ncomp_max", C.c_int * ML),
                 ("nbrs_max", C.c_int * ML), ("nsegs", C.c_int * ML),
                 ("nbalanced", C.c_int * ML),
-                ("lambda_min", C.c_double * ML), ("lambda_max", C.c_double * ML)]
+                ("lambda_min", C.c_double * ML), ("lambda_max", C.c_double * ML),
+                ("nbrs_min", C.c_int * ML), ("nbrs_mean", C.c_double * ML)]
 
     def as_dict(self):
         n = self.nlevels
@@ -59,6 +60,9 @@ P = C.c_void_p
 
 def _declare(L):
     L.oracle_centroids.argtypes = [P, C.c_int, C.c_int, P]
+    L.oracle_rib_order.argtypes = [P, C.c_int, P, C.c_int, C.c_int]
+    L.oracle_rib_axis.argtypes = [P, C.c_int, P, C.c_int, P, P]
+    L.oracle_sym_dominant.argtypes = [P, C.c_int, P]
     L.oracle_rank_count.argtypes = [C.c_int64, C.c_int, C.c_int]
     L.oracle_rank_count.restype = C.c_int64
     L.oracle_lower_count.argtypes = [C.c_int64, C.c_int]
@@ -119,6 +123,29 @@ def rcb_order(coord, ndim, order, p):
     order = np.array(order, dtype=np.int32, copy=True)
     lib().oracle_rcb_order(_p(coord), ndim, _p(order), order.size, p)
     return order
+
+
+def rib_order(coord, ndim, order, p):
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    order = np.array(order, dtype=np.int32, copy=True)
+    lib().oracle_rib_order(_p(coord), ndim, _p(order), order.size, p)
+    return order
+
+
+def rib_axis(coord, ndim, order):
+    coord = np.ascontiguousarray(coord, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    m, ev = np.zeros(3), np.zeros(3)
+    lib().oracle_rib_axis(_p(coord), ndim, _p(order), order.size, _p(m), _p(ev))
+    return m, ev
+
+
+def sym_dominant(A, n):
+    A3 = np.zeros((3, 3))
+    A3[:n, :n] = A
+    ev = np.zeros(3)
+    lib().oracle_sym_dominant(_p(np.ascontiguousarray(A3)), n, _p(ev))
+    return ev[:n]
 
 
 def dual_graph(vtx, nv):

@@ -57,7 +57,8 @@ class Report(C.Structure):
                 ("lambda_max", C.c_double * ML), ("resid_max", C.c_double * ML),
                 ("resid_med", C.c_double * ML), ("niter_mean", C.c_double * ML),
                 ("level_ms", C.c_double * ML), ("nfallback", C.c_int * ML),
-                ("warm_start", C.c_int)]
+                ("warm_start", C.c_int), ("nbrs_min", C.c_int * ML),
+                ("nbrs_max", C.c_int * ML), ("nbrs_mean", C.c_double * ML)]
 
     def as_dict(self):
         d = {}
@@ -123,12 +124,14 @@ SIGNATURES = {
     "rsbgpu_upload_ints": (I, [P, P, I, C.POINTER(P)]),
     "rsbgpu_upload": (I, [P, P, C.c_size_t, C.POINTER(P)]),
     "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
+    "rsbgpu_neighbors": (I, [P, P]),
     "rsbgpu_lws_nparams": (I, []),
     "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
     "rsbgpu_lws_train_step": (I, [P, P, P, C.c_float, C.POINTER(C.c_double)]),
     "rsbgpu_lws_adam": (I, [P, P, P, P, I, C.c_float]),
-    "rsbgpu_lws_test": (I, [I, P, C.c_float, P, P, P, P, P, P, P, C.POINTER(C.c_double)]),
+    "rsbgpu_lws_test": (I, [I, P, C.c_float, P, P, P, C.c_double, P, P, P,
+                            C.POINTER(C.c_double)]),
     "rsbgpu_free": (None, [P]),
 }
 
@@ -345,6 +348,11 @@ class Device:
         self._chk(lib().rsbgpu_lws_train_step(self.h, d_params, d_grad, level, C.byref(loss)))
         return loss.value
 
+    def neighbors(self, nseg):
+        nb = np.zeros(nseg, dtype=np.int32)
+        self._chk(lib().rsbgpu_neighbors(self.h, _p(nb)))
+        return nb
+
     def fiedler_sort(self):
         self._chk(lib().rsbgpu_fiedler_sort(self.h))
 
@@ -403,19 +411,18 @@ def test_tqli(d, e):
     return lam[0], y
 
 
-def lws_test(coord, level, rowptr, col, wgt, t, params):
+def lws_test(coord, level, rowptr, col, wgt, lam, params):
     """Network forward + backward on one segment on the device (parity hook)."""
     coord = np.ascontiguousarray(coord, dtype=np.float64)  # 3 x M
     M = coord.shape[1]
     rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
     col = np.ascontiguousarray(col, dtype=np.int32)
     wgt = np.ascontiguousarray(wgt, dtype=np.int32)
-    t = np.ascontiguousarray(t, dtype=np.float32)
     params = np.ascontiguousarray(params, dtype=np.float32)
     y = np.zeros(M, dtype=np.float32)
     grad = np.zeros(params.size, dtype=np.float32)
     loss = C.c_double()
-    rc = lib().rsbgpu_lws_test(M, _p(coord), level, _p(rowptr), _p(col), _p(wgt), _p(t),
+    rc = lib().rsbgpu_lws_test(M, _p(coord), level, _p(rowptr), _p(col), _p(wgt), lam,
                                _p(params), _p(y), _p(grad), C.byref(loss))
     if rc:
         raise ParrsbError(rc, "rsbgpu_lws_test failed")

@@ -294,68 +294,78 @@ __global__ void k_lws_w(const float *__restrict__ P, unsigned short *__restrict_
 // ---------------------------------------------------------------------------
 
 // ---------------------------------------------------------------------------
-// Training: loss 1 - cos^2 between the centred prediction and the centred
-// Fiedler vector of each segment, averaged over segments.
-__global__ void k_lws_target(const int *__restrict__ tile, int ntiles, const int *__restrict__ seg,
-                             int nseg, const double *__restrict__ fied, float *__restrict__ t) {
-  // one block per tile; t[row] = Fiedler value of the row (0 on padding)
-  const int ti = blockIdx.x, s = tile[3 * ti];
-  const int row0 = tile[3 * ti + 1], row1 = tile[3 * ti + 2];
-  const int cstart = seg[s], n = seg[nseg + s], pstart = seg[4 * nseg + s];
-  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
-    const int i = row - pstart;
-    t[row] = i < n ? (float)fied[cstart + i] : 0.f;
+// Training loss of a segment: its Rayleigh quotient relative to the Fiedler
+// value the plain solver found, R(y) / lambda - 1 with R(y) = y.Ly / yc.yc
+// (yc = y - mean; L annihilates constants, so y.Ly = yc.L yc).  It is
+// smallest on the Fiedler eigenspace (any vector of a degenerate one) and
+// weighs each eigencomponent of the error by lambda_k / lambda_2: the
+// high-frequency error that restarted CG-Lanczos is slowest to remove costs
+// the most.  Averaged over the level's segments.
+
+// ly = L y on every row of the level (difference form, SELL-64 global entries)
+__global__ void k_lws_lap(int64_t mpad, const int64_t *__restrict__ slice_off,
+                          const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
+                          const float *__restrict__ y, float *__restrict__ ly) {
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= mpad) return;
+  const int64_t sl = row >> 6;
+  const uint32_t *e = ell + slice_off[sl] + (row & 63);
+  const float yi = y[row];
+  float acc = 0.f;
+  for (int j = 0; j < slice_w[sl]; j++) {
+    const uint32_t q = e[(size_t)j * RSB_WAVE];
+    acc += (float)(q >> 28) * (yi - y[q & 0x0fffffffu]);
   }
+  ly[row] = acc;
 }
 
-// sums per segment: k = 0: y, 1: t; k = 2: yc.tc, 3: yc.yc, 4: tc.tc
+// sums per segment: phase 0: S[8s] = sum y; phase 1: S[8s+2] = y.Ly,
+// S[8s+3] = yc.yc
 __global__ void k_lws_sums(const int *__restrict__ tile, const int *__restrict__ seg, int nseg,
-                           const float *__restrict__ y, const float *__restrict__ t, int phase,
+                           const float *__restrict__ y, const float *__restrict__ ly, int phase,
                            double *__restrict__ S) {
   const int ti = blockIdx.x, s = tile[3 * ti];
   const int row0 = tile[3 * ti + 1], row1 = tile[3 * ti + 2];
   const int n = seg[nseg + s], pstart = seg[4 * nseg + s];
-  const double my = phase ? S[s * 8 + 0] / n : 0.0, mt = phase ? S[s * 8 + 1] / n : 0.0;
-  double v[3] = {0, 0, 0};
+  const double my = phase ? S[s * 8 + 0] / n : 0.0;
+  double v[2] = {0, 0};
   for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
     if (row - pstart >= n) continue;
     if (phase == 0) {
-      v[0] += y[row], v[1] += t[row];
+      v[0] += y[row];
     } else {
-      const double a = y[row] - my, b = t[row] - mt;
-      v[0] += a * b, v[1] += a * a, v[2] += b * b;
+      const double a = y[row] - my;
+      v[0] += (double)y[row] * ly[row], v[1] += a * a;
     }
   }
-  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 3];
-  block_sum<3>(v, red);
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 2];
+  block_sum<2>(v, red);
   if (threadIdx.x == 0) {
-    if (phase == 0) atomicAdd(&S[s * 8 + 0], v[0]), atomicAdd(&S[s * 8 + 1], v[1]);
-    else atomicAdd(&S[s * 8 + 2], v[0]), atomicAdd(&S[s * 8 + 3], v[1]), atomicAdd(&S[s * 8 + 4], v[2]);
+    if (phase == 0) atomicAdd(&S[s * 8 + 0], v[0]);
+    else atomicAdd(&S[s * 8 + 2], v[0]), atomicAdd(&S[s * 8 + 3], v[1]);
   }
 }
 
-// dL/dy for L_s = 1 - S^2 / (Q T): centred gradient; the loss of segment s
-// goes to S[s*8+5] (weighted by 1/nact via `scale`).
+// dL/dy = 2 (Ly - R yc) / (Q lambda), times `scale` (1 / segments); the loss
+// of segment s goes to S[8s+5].  lam: the segment's Fiedler value (lz sc).
 __global__ void k_lws_dy(const int *__restrict__ tile, const int *__restrict__ seg, int nseg,
-                         const float *__restrict__ y, const float *__restrict__ t,
+                         const float *__restrict__ y, const float *__restrict__ ly,
+                         const double *__restrict__ lam, int lam_stride,
                          double *__restrict__ S, float scale, float *__restrict__ dy) {
   const int ti = blockIdx.x, s = tile[3 * ti];
   const int row0 = tile[3 * ti + 1], row1 = tile[3 * ti + 2];
   const int n = seg[nseg + s], pstart = seg[4 * nseg + s];
-  const double my = S[s * 8 + 0] / n, mt = S[s * 8 + 1] / n;
-  const double Sxy = S[s * 8 + 2], Q = S[s * 8 + 3], T = S[s * 8 + 4];
-  const bool ok = Q > 0 && T > 0;
+  const double my = S[s * 8 + 0] / n, Ry = S[s * 8 + 2], Q = S[s * 8 + 3];
+  const double lm = lam[(size_t)s * lam_stride];
+  // lambda ~ 0: disconnected segment, no usable scale
+  const bool ok = Q > 0 && lm > 1e-12;
+  const double R = ok ? Ry / Q : 0.0;
   for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
     float g = 0.f;
-    if (ok && row - pstart < n) {
-      const double a = y[row] - my, b = t[row] - mt;
-      // d/dyc, then centred again (mean of the gradient removed)
-      double gy = -2.0 * Sxy * b / (Q * T) + 2.0 * Sxy * Sxy * a / (Q * Q * T);
-      g = (float)(gy * scale);
-    }
+    if (ok && row - pstart < n) g = (float)(scale * 2.0 * (ly[row] - R * (y[row] - my)) / (Q * lm));
     dy[row] = g;
   }
-  if (threadIdx.x == 0 && row0 == pstart && ok) S[s * 8 + 5] = 1.0 - Sxy * Sxy / (Q * T);
+  if (threadIdx.x == 0 && row0 == pstart) S[s * 8 + 5] = ok ? R / lm - 1.0 : 0.0;
 }
 
 // dh4 = dy w_o (BF16); dw_o += sum dy h4, db_o += sum dy
@@ -546,15 +556,17 @@ extern "C" int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_
   int rc = lws_ws_get(g, true, w);
   if (rc) return rc;
   if ((rc = lws_forward(g, d_params, w, level))) return rc;
-  // loss and dL/dy
+  // loss and dL/dy (w.t holds L y)
+  if (!g->lz.sc || g->lz_nseg_cap < nseg) RSB_FAIL(3, "training needs the level's Fiedler values");
   RSB_TRY(hipMemsetAsync(w.S, 0, sizeof(double) * nseg * 8, st));
-  k_lws_target<<<nt, 256, 0, st>>>(g->d_tile, nt, g->d_seg, nseg, g->d_fied, w.t);
+  k_lws_lap<<<nblocks(M, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell, w.y, w.t);
   k_lws_sums<<<nt, RSB_BLOCK, 0, st>>>(g->d_tile, g->d_seg, nseg, w.y, w.t, 0, w.S);
   k_lws_sums<<<nt, RSB_BLOCK, 0, st>>>(g->d_tile, g->d_seg, nseg, w.y, w.t, 1, w.S);
   int nact = 0;
   for (int s = 0; s < nseg; s++) nact += (g->h_p[s] > 1 && g->h_n[s] >= 2);
   const float scale = nact ? 1.f / nact : 0.f;
-  k_lws_dy<<<nt, 256, 0, st>>>(g->d_tile, g->d_seg, nseg, w.y, w.t, w.S, scale, w.dy);
+  k_lws_dy<<<nt, 256, 0, st>>>(g->d_tile, g->d_seg, nseg, w.y, w.t, g->lz.sc + SC_LAMBDA, LZ_NSC,
+                               w.S, scale, w.dy);
   RSB_LAUNCH_CHECK();
   // backward
   k_lws_out_bwd<<<256, 256, 0, st>>>(M, w.dy, w.h[LWS_L], d_params, w.g0, d_grad);
@@ -597,9 +609,9 @@ extern "C" int rsbgpu_lws_adam(float *d_params, const float *d_grad, float *d_m,
 
 // Parity hook: forward and backward of the network on one segment given on
 // the host: coord [3 x M] (SoA), level, its graph as CSR (rowptr[M+1], col,
-// w), target t[M], params; writes y[M], grad and the loss.
+// w), its Fiedler value lam, params; writes y[M], grad and the loss.
 extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const int *rowptr,
-                               const int *col, const int *wgt, const float *t,
+                               const int *col, const int *wgt, double lam,
                                const float *params, float *y, float *grad, double *loss) {
   rsbgpu *g = nullptr;
   if (rsbgpu_open(&g, 0)) return 2;
@@ -629,8 +641,8 @@ extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const
     g->nel = M, g->ndim = 3, g->nseg = 1, g->mpad = mp, g->nslices = ns;
     g->ntiles = (int)(tiles.size() / 3);
     g->h_p = {2}, g->h_n = {M}, g->h_cstart = {0}, g->h_r0 = {0}, g->h_pstart = {0};
-    std::vector<double> coord(coord_in, coord_in + 3 * (size_t)M), fied(M);
-    for (int r = 0; r < M; r++) fied[r] = t[r];
+    std::vector<double> coord(coord_in, coord_in + 3 * (size_t)M), sc(LZ_NSC, 0.0);
+    sc[SC_LAMBDA] = lam;
     std::vector<int> segof(M, 0);
 #define UP(dst, src, n, T)                                                               \
   if (!rc && dev_alloc(dst, (size_t)(n)) != hipSuccess) rc = 2;                         \
@@ -644,7 +656,8 @@ extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const
     UP(g->d_tile, tiles.data(), tiles.size(), int);
     UP(g->d_seg_tiles, segt.data(), 2, int);
     UP(g->d_coord, coord.data(), 3 * (size_t)M, double);
-    UP(g->d_fied, fied.data(), M, double);
+    UP(g->lz.sc, sc.data(), LZ_NSC, double);
+    g->lz_nseg_cap = 1;
     UP(g->d_segof, segof.data(), M, int);
 #undef UP
   }

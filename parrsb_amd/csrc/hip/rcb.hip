@@ -11,6 +11,7 @@
 // (equal coordinates share a rank), so ties stay ties and a stable sort keeps
 // the current order among them.
 #include <algorithm>
+#include <cfloat>
 
 #include "state.hpp"
 
@@ -340,6 +341,334 @@ static int run_rcb(rsbgpu *g) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// RIB rounds (rib.c:6-111): the same node tree as RCB, but every node is
+// sorted by the projection of its centroids onto the dominant eigenvector of
+// their centred covariance.  The reference finds that vector with <= 100
+// power steps from a random start (fiedler.c:38-73), so its sign and, to
+// ~1e-6, its direction vary from run to run; here it is the exact dominant
+// eigenvector (cyclic Jacobi), signed so that its largest component is
+// positive.  Sums are compensated (two-sum), so node means and covariances
+// are nearly independent of the summation order.  In 2D the covariance is
+// 2 x 2 (the reference passes its 3 x 3 array with N = 2, rib.c:47).
+#pragma clang fp contract(off)
+#define RIB_SMALL 256  // nodes up to this size: one thread per node
+#define RIB_CHUNK 2048 // larger nodes: chunks of this many positions
+
+struct dsum {
+  double s = 0.0, c = 0.0;
+  __device__ inline void add(double x) {
+    double t = s + x, bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  __device__ inline void merge(double os, double oc) {
+    add(os);
+    c += oc;
+  }
+  __device__ inline double value() const { return s + c; }
+};
+
+// Block-wide merge of NQ compensated sums, fixed tree; result in thread 0.
+template <int NQ>
+__device__ inline void block_dsum(dsum (&v)[NQ], double *red) {
+  const int lane = threadIdx.x & (RSB_WAVE - 1), wid = threadIdx.x / RSB_WAVE;
+  const int nw = blockDim.x / RSB_WAVE;
+  for (int o = RSB_WAVE / 2; o > 0; o >>= 1)
+    for (int q = 0; q < NQ; q++) {
+      double os = __shfl_xor(v[q].s, o, RSB_WAVE), oc = __shfl_xor(v[q].c, o, RSB_WAVE);
+      v[q].merge(os, oc);
+    }
+  if (lane == 0)
+    for (int q = 0; q < NQ; q++) red[(wid * NQ + q) * 2] = v[q].s, red[(wid * NQ + q) * 2 + 1] = v[q].c;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NQ; q++) {
+      dsum t;
+      for (int w = 0; w < nw; w++) t.merge(red[(w * NQ + q) * 2], red[(w * NQ + q) * 2 + 1]);
+      v[q] = t;
+    }
+  __syncthreads();
+}
+
+// Dominant eigenvector of the symmetric n x n (n = 2, 3) matrix A (row-major
+// 3 x 3) by cyclic Jacobi rotations (Golub & Van Loan, Matrix Computations,
+// Alg. 8.5.1 and 8.5.3); ties between eigenvalues go to the lower index.
+__device__ inline void rib_dominant(double (&A)[3][3], int n, double *ev) {
+  double V[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+  for (int sweep = 0; sweep < 32; sweep++) {
+    bool rotated = false;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p][q];
+        if (apq == 0.0) continue;
+        const double scale = fabs(A[p][p]) + fabs(A[q][q]);
+        if (fabs(apq) <= DBL_EPSILON * DBL_EPSILON * scale) continue;
+        const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                    : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - sn * akq;
+          A[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - sn * aqk;
+          A[q][k] = sn * apk + c * aqk;
+        }
+        A[p][q] = A[q][p] = 0.0;
+        for (int k = 0; k < n; k++) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - sn * vkq;
+          V[k][q] = sn * vkp + c * vkq;
+        }
+        rotated = true;
+      }
+    if (!rotated) break;
+  }
+  int im = 0;
+  for (int i = 1; i < n; i++)
+    if (A[i][i] > A[im][im]) im = i;
+  int dm = 0;
+  for (int d = 1; d < n; d++)
+    if (fabs(V[d][im]) > fabs(V[dm][im])) dm = d;
+  const double sg = V[dm][im] < 0.0 ? -1.0 : 1.0;
+  for (int d = 0; d < 3; d++) ev[d] = d < n ? sg * V[d][im] : 0.0;
+}
+
+__device__ inline void rib_axis_from_cov(const double *cov6, int ndim, double *ev) {
+  // cov6: xx, xy, xz, yy, yz, zz
+  double A[3][3] = {{cov6[0], cov6[1], cov6[2]},
+                    {cov6[1], cov6[3], cov6[4]},
+                    {cov6[2], cov6[4], cov6[5]}};
+  rib_dominant(A, ndim, ev);
+}
+
+// Small nodes: one thread computes mean, covariance and axis of its node.
+// ax[6k..6k+2] = mean, ax[6k+3..6k+5] = axis.
+__global__ void k_rib_small(int K, const nodes_t nd, const int *__restrict__ order,
+                            const double *__restrict__ coord, int nel, int ndim,
+                            double *__restrict__ ax) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K || nd.frozen[k] || nd.len[k] > RIB_SMALL) return;
+  const int st = nd.start[k], n = nd.len[k];
+  dsum m[3];
+  for (int i = 0; i < n; i++) {
+    const int e = order[st + i];
+    for (int d = 0; d < ndim; d++) m[d].add(coord[(size_t)d * nel + e]);
+  }
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < ndim; d++) mean[d] = m[d].value() / n;
+  dsum cv[6];
+  for (int i = 0; i < n; i++) {
+    const int e = order[st + i];
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ndim; d++) x[d] = coord[(size_t)d * nel + e] - mean[d];
+    cv[0].add(x[0] * x[0]), cv[1].add(x[0] * x[1]), cv[2].add(x[0] * x[2]);
+    cv[3].add(x[1] * x[1]), cv[4].add(x[1] * x[2]), cv[5].add(x[2] * x[2]);
+  }
+  double c6[6];
+  for (int q = 0; q < 6; q++) c6[q] = cv[q].value();
+  double ev[3];
+  rib_axis_from_cov(c6, ndim, ev);
+  for (int d = 0; d < 3; d++) ax[6 * (size_t)k + d] = mean[d], ax[6 * (size_t)k + 3 + d] = ev[d];
+}
+
+__global__ void k_rib_chunk_count(int K, const nodes_t nd, int *__restrict__ cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const int n = nd.len[k];
+  cnt[k] = (!nd.frozen[k] && n > RIB_SMALL) ? (n + RIB_CHUNK - 1) / RIB_CHUNK : 0;
+}
+
+__global__ void k_rib_chunk_map(int K, const int *__restrict__ cnt,
+                                const int *__restrict__ cfirst, int *__restrict__ cnode) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  for (int c = 0; c < cnt[k]; c++) cnode[cfirst[k] + c] = k;
+}
+
+// Partial sums of one chunk of a large node: PH 0 the coordinates, PH 1 the
+// centred products (xx, xy, xz, yy, yz, zz).  part[(chunk * 6 + q) * 2 + {0,1}].
+template <int PH>
+__global__ void __launch_bounds__(256) k_rib_chunk_sums(const int *__restrict__ cnode,
+                                                        const int *__restrict__ cfirst,
+                                                        const nodes_t nd,
+                                                        const int *__restrict__ order,
+                                                        const double *__restrict__ coord,
+                                                        int nel, int ndim,
+                                                        const double *__restrict__ ax,
+                                                        double *__restrict__ part) {
+  constexpr int NQ = PH == 0 ? 3 : 6;
+  __shared__ double red[(256 / RSB_WAVE) * NQ * 2];
+  const int c = blockIdx.x, k = cnode[c];
+  const int st = nd.start[k], n = nd.len[k];
+  const int i0 = (c - cfirst[k]) * RIB_CHUNK, i1 = min(n, i0 + RIB_CHUNK);
+  double mean[3] = {0.0, 0.0, 0.0};
+  if (PH == 1)
+    for (int d = 0; d < ndim; d++) mean[d] = ax[6 * (size_t)k + d];
+  dsum v[NQ];
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const int e = order[st + i];
+    double x[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ndim; d++) x[d] = coord[(size_t)d * nel + e] - mean[d];
+    if (PH == 0) {
+      for (int d = 0; d < 3; d++) v[d].add(x[d]);
+    } else {
+      v[0].add(x[0] * x[0]), v[1].add(x[0] * x[1]), v[2].add(x[0] * x[2]);
+      v[3 % NQ].add(x[1] * x[1]), v[4 % NQ].add(x[1] * x[2]), v[5 % NQ].add(x[2] * x[2]);
+    }
+  }
+  block_dsum<NQ>(v, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NQ; q++) {
+      part[((size_t)c * 6 + q) * 2] = v[q].s;
+      part[((size_t)c * 6 + q) * 2 + 1] = v[q].c;
+    }
+}
+
+// Large nodes: merge the chunk partials in chunk order; PH 0 -> mean,
+// PH 1 -> axis.
+template <int PH>
+__global__ void k_rib_large_fin(int K, const nodes_t nd, const int *__restrict__ cnt,
+                                const int *__restrict__ cfirst, int ndim,
+                                const double *__restrict__ part, double *__restrict__ ax) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K || cnt[k] == 0) return;
+  constexpr int NQ = PH == 0 ? 3 : 6;
+  dsum v[NQ];
+  for (int c = cfirst[k]; c < cfirst[k] + cnt[k]; c++)
+    for (int q = 0; q < NQ; q++) v[q].merge(part[((size_t)c * 6 + q) * 2], part[((size_t)c * 6 + q) * 2 + 1]);
+  if (PH == 0) {
+    for (int d = 0; d < 3; d++) ax[6 * (size_t)k + d] = d < ndim ? v[d].value() / nd.len[k] : 0.0;
+  } else {
+    double c6[6];
+    for (int q = 0; q < 6; q++) c6[q] = v[q % NQ].value();
+    double ev[3];
+    rib_axis_from_cov(c6, ndim, ev);
+    for (int d = 0; d < 3; d++) ax[6 * (size_t)k + 3 + d] = ev[d];
+  }
+}
+
+// Projection keys; frozen nodes get key 0 (their order is kept by the stable
+// sorts).  val = current position.
+__global__ void k_rib_keys(int nel, int ndim, const int *__restrict__ order,
+                           const int *__restrict__ nodeof, const int *__restrict__ frozen,
+                           const double *__restrict__ coord, const double *__restrict__ ax,
+                           uint64_t *__restrict__ key, int *__restrict__ val) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  const int k = nodeof[i], e = order[i];
+  double p = 0.0;
+  if (!frozen[k])
+    for (int d = 0; d < ndim; d++)
+      p += (coord[(size_t)d * nel + e] - ax[6 * (size_t)k + d]) * ax[6 * (size_t)k + 3 + d];
+  key[i] = frozen[k] ? 0ull : dkey(p);
+  val[i] = i;
+}
+
+// After the stable sort by projection: node of each moved position, and its
+// element.
+__global__ void k_rib_node_keys(int nel, const int *__restrict__ pos,
+                                const int *__restrict__ nodeof, const int *__restrict__ order,
+                                uint32_t *__restrict__ key2, int *__restrict__ val2) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nel) return;
+  const int i = pos[j];
+  key2[j] = (uint32_t)nodeof[i];
+  val2[j] = order[i];
+}
+#pragma clang fp contract(fast)
+
+static int run_rib(rsbgpu *g) {
+  const int nel = g->nel, nseg = g->nseg;
+  int rounds = 0;
+  for (int s = 0; s < nseg; s++)
+    if (g->h_p[s] > 1) rounds = std::max(rounds, rcb_depth(g->h_n[s], g->h_p[s]));
+  if (rounds == 0) return 0;
+  size_t cap = (size_t)nseg + (size_t)nel * (size_t)std::min(rounds, 4) / 2 + (size_t)nel + 16;
+  size_t ccap = (size_t)nel / RIB_SMALL + (size_t)nel / RIB_CHUNK + (size_t)nseg + 16;
+  // node tables (two generations) and per-node counts, per-position arrays,
+  // and the chunk tables of the large nodes
+  int *buf = nullptr;
+  RSB_TRY(hipMalloc((void **)&buf, sizeof(int) * (cap * 13 + ccap + 3 * (size_t)nel)));
+  nodes_t A{buf, buf + cap, buf + 2 * cap, buf + 3 * cap};
+  nodes_t B{buf + 4 * cap, buf + 5 * cap, buf + 6 * cap, buf + 7 * cap};
+  int *cnt = buf + 8 * cap, *first = buf + 9 * cap, *split = buf + 10 * cap;
+  int *ccnt = buf + 11 * cap, *cfirst = buf + 12 * cap, *cnode = buf + 13 * cap;
+  int *nodeof = cnode + ccap, *pos = nodeof + nel, *val2 = pos + nel;
+  double *ax = nullptr, *part = nullptr;
+  uint64_t *key = nullptr;
+  RSB_TRY(hipMalloc((void **)&ax, sizeof(double) * 6 * cap));
+  RSB_TRY(hipMalloc((void **)&part, sizeof(double) * 12 * ccap));
+  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ((size_t)nel + 1)));
+  uint32_t *key2 = (uint32_t *)key;  // the second sort reuses the key array
+  hipStream_t st = g->stream;
+  int K = nseg;
+  k_nodes_from_segments<<<nblocks(nseg, 256), 256, 0, st>>>(g->d_seg, nseg, A);
+  RSB_LAUNCH_CHECK();
+  k_nodeof_init<<<nblocks(nel, 256), 256, 0, st>>>(g->d_seg, nseg, nel, nodeof);
+  RSB_LAUNCH_CHECK();
+  for (int r = 0; r < rounds; r++) {
+    // axis of every node
+    k_rib_small<<<nblocks(K, 256), 256, 0, st>>>(K, A, g->d_order, g->d_coord, nel, g->ndim,
+                                                 ax);
+    k_rib_chunk_count<<<nblocks(K, 256), 256, 0, st>>>(K, A, ccnt);
+    RSB_LAUNCH_CHECK();
+    int rc = rsb_exclusive_scan_i32(g, ccnt, cfirst, K);
+    if (rc) return rc;
+    int tail[2];
+    RSB_TRY(hipMemcpyAsync(&tail[0], cfirst + K - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipMemcpyAsync(&tail[1], ccnt + K - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    const int C = tail[0] + tail[1];
+    if ((size_t)C > ccap) RSB_FAIL(2, "RIB chunk table overflow (%d > %zu)", C, ccap);
+    if (C > 0) {
+      k_rib_chunk_map<<<nblocks(K, 256), 256, 0, st>>>(K, ccnt, cfirst, cnode);
+      k_rib_chunk_sums<0><<<C, 256, 0, st>>>(cnode, cfirst, A, g->d_order, g->d_coord, nel,
+                                             g->ndim, ax, part);
+      k_rib_large_fin<0><<<nblocks(K, 256), 256, 0, st>>>(K, A, ccnt, cfirst, g->ndim, part, ax);
+      k_rib_chunk_sums<1><<<C, 256, 0, st>>>(cnode, cfirst, A, g->d_order, g->d_coord, nel,
+                                             g->ndim, ax, part);
+      k_rib_large_fin<1><<<nblocks(K, 256), 256, 0, st>>>(K, A, ccnt, cfirst, g->ndim, part, ax);
+      RSB_LAUNCH_CHECK();
+    }
+    // stable sort by projection, then stable sort by node
+    k_rib_keys<<<nblocks(nel, 256), 256, 0, st>>>(nel, g->ndim, g->d_order, nodeof, A.frozen,
+                                                  g->d_coord, ax, key, pos);
+    RSB_LAUNCH_CHECK();
+    if ((rc = rsb_sort_pairs_u64(g, key, pos, nel, 0, 64))) return rc;
+    k_rib_node_keys<<<nblocks(nel, 256), 256, 0, st>>>(nel, pos, nodeof, g->d_order, key2,
+                                                       val2);
+    RSB_LAUNCH_CHECK();
+    int node_bits = 1;
+    while ((1ll << node_bits) < K) node_bits++;
+    if ((rc = rsb_sort_pairs_u32(g, key2, val2, nel, 0, node_bits))) return rc;
+    RSB_TRY(hipMemcpyAsync(g->d_order, val2, sizeof(int) * (size_t)nel,
+                           hipMemcpyDeviceToDevice, st));
+    // children, as RCB
+    k_child_count<<<nblocks(K, 256), 256, 0, st>>>(K, A, cnt);
+    RSB_LAUNCH_CHECK();
+    if ((rc = rsb_exclusive_scan_i32(g, cnt, first, K))) return rc;
+    RSB_TRY(hipMemcpyAsync(&tail[0], first + K - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipMemcpyAsync(&tail[1], cnt + K - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    const int K2 = tail[0] + tail[1];
+    if ((size_t)K2 > cap) RSB_FAIL(2, "RIB node table overflow (%d > %zu)", K2, cap);
+    k_children<<<nblocks(K, 256), 256, 0, st>>>(K, A, first, B, split);
+    RSB_LAUNCH_CHECK();
+    k_nodeof_next<<<nblocks(nel, 256), 256, 0, st>>>(nel, first, split, nodeof);
+    RSB_LAUNCH_CHECK();
+    K = K2;
+    std::swap(A, B);
+  }
+  RSB_TRY(hipStreamSynchronize(st));
+  hipFree(buf), hipFree(ax), hipFree(part), hipFree(key);
+  return 0;
+}
+
 // rsb_pre == 0: stable sort of each active segment by global id (= input
 // index), rsb.c:284-287.  Inactive segments keep their order.
 __global__ void k_gid_keys(const int *__restrict__ order, const int *__restrict__ segof,
@@ -374,6 +703,9 @@ extern "C" int rsbgpu_prepartition(rsbgpu *g, int rsb_pre) {
   if (rsb_pre == 1) {
     if (!g->has_coord) RSB_FAIL(3, "RCB needs coordinates");
     rc = run_rcb(g);
+  } else if (rsb_pre == 2) {
+    if (!g->has_coord) RSB_FAIL(3, "RIB needs coordinates");
+    rc = run_rib(g);
   } else if (rsb_pre == 0) {
     rc = run_gid(g);
   } else {

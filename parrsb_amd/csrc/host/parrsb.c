@@ -214,9 +214,9 @@ static void print_profile(const parrsb_amd_report *r) {
          r->phase_ms[6], r->phase_ms[7]);
   for (int l = 0; l < r->nlevels && l < PARRSB_AMD_MAX_LEVELS; l++)
     printf("level=%02d segments=%d RSB_FIEDLER_CALC_NITER=%d/%d RSB_COMPONENTS_NCOMP=%d/%d "
-           "lambda=%.6e/%.6e\n",
+           "RSB_NEIGHBORS=%g/%d/%d lambda=%.6e/%.6e\n",
            l, r->nsegs[l], r->niter_min[l], r->niter_max[l], r->ncomp_min[l], r->ncomp_max[l],
-           r->lambda_min[l], r->lambda_max[l]);
+           r->nbrs_mean[l], r->nbrs_min[l], r->nbrs_max[l], r->lambda_min[l], r->lambda_max[l]);
   fflush(stdout);
 }
 
@@ -234,8 +234,12 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
     set_error("nv must be 4 or 8");
     return 3;
   }
-  if (options->partitioner == 2 || (options->partitioner == 0 && options->rsb_pre == 2)) {
-    set_error("RIB is not provided by this build");
+  if (options->partitioner < 0 || options->partitioner > 2) {
+    set_error("partitioner must be 0 (RSB), 1 (RCB) or 2 (RIB)");
+    return 3;
+  }
+  if (options->partitioner != 0 && xyz == NULL) {
+    set_error("RCB and RIB need coordinates");
     return 3;
   }
   if (options->partitioner == 0 && options->rsb_algo != 0) {
@@ -271,8 +275,8 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
     int64_t nnz = 0, ws = 0;
     rsbgpu_graph_stats(g, &nnz, &ws);
     rep->graph_nnz = nnz, rep->graph_wsum = ws;
-    if (options->partitioner == 1) {
-      rc = rcb_run(g, nel, nparts);
+    if (options->partitioner == 1 || options->partitioner == 2) {
+      rc = rcb_run(g, nel, nparts, options->partitioner == 1 ? 1 : 2);
     } else {
       rsb_params prm;
       prm.nparts = nparts, prm.rsb_pre = options->rsb_pre;
@@ -292,7 +296,7 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
       rsbgpu_free(dw);
     }
   }
-  if (!rc && options->partitioner == 1) rc = rsbgpu_write_part(g, part, on_device);
+  if (!rc && options->partitioner != 0) rc = rsbgpu_write_part(g, part, on_device);
   if (rc) set_error("%s", rsbgpu_error(g));
   rsbgpu_phase_times(g, rep->phase_ms);
   double tr[6];

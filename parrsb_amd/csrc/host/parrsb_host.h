@@ -16,6 +16,7 @@ long long rsb_lower_count(long long n, int p);
 long long rsb_balance_expected(long long n, int p, int bin);
 int rsb_ncuts(int nparts);
 int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep);
-int rcb_run(rsbgpu *g, int nel, int nparts);
+/* partitioner 1 (pre = 1, RCB) or 2 (pre = 2, RIB) */
+int rcb_run(rsbgpu *g, int nel, int nparts, int pre);
 
 #endif

@@ -61,6 +61,24 @@ typedef struct {
   int *cstart, *n, *p, *r0;
 } segtab;
 
+/* RSB_NEIGHBORS of the segments just set (rsb.c:342-343, statistics.c:17-113):
+ * min / max over segments, mean over the virtual ranks. */
+static int level_neighbors(rsbgpu *g, int nseg, const int *p, int *nb, parrsb_amd_report *rep,
+                           int lv) {
+  int rc = rsbgpu_neighbors(g, nb);
+  if (rc) return rc;
+  int mn = 1 << 30, mx = 0;
+  double sum = 0.0, ranks = 0.0;
+  for (int s = 0; s < nseg; s++) {
+    mn = nb[s] < mn ? nb[s] : mn;
+    mx = nb[s] > mx ? nb[s] : mx;
+    sum += (double)nb[s] * p[s], ranks += p[s];
+  }
+  rep->nbrs_min[lv] = nseg ? mn : 0, rep->nbrs_max[lv] = mx;
+  rep->nbrs_mean[lv] = ranks > 0 ? sum / ranks : 0.0;
+  return 0;
+}
+
 static int segtab_alloc(segtab *t, int cap) {
   t->cstart = (int *)calloc((size_t)cap, sizeof(int));
   t->n = (int *)calloc((size_t)cap, sizeof(int));
@@ -92,9 +110,9 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   double *lam = (double *)calloc((size_t)P, sizeof(double));
   double *rn = (double *)calloc(3 * (size_t)P, sizeof(double));
   double *res = (double *)calloc((size_t)P, sizeof(double));
-  int *fb = (int *)calloc((size_t)P, sizeof(int));
+  int *fb = (int *)calloc((size_t)P, sizeof(int)), *nbr = (int *)calloc((size_t)P, sizeof(int));
   if (!ok || !nl || !snd || !scnt || !moved || !ncomp || !niter || !npass || !lam || !rn || !res ||
-      !fb) {
+      !fb || !nbr) {
     rc = 2;
     goto done;
   }
@@ -104,6 +122,8 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   for (int cut = 0; cut < ncuts; cut++) {
     double tl = wall();
     TRY(rsbgpu_set_segments(g, nseg, A.cstart, A.n, A.p, A.r0));
+    if (cut > 0 && cut - 1 < PARRSB_AMD_MAX_LEVELS)
+      TRY(level_neighbors(g, nseg, A.p, nbr, rep, cut - 1));
     TRY(rsbgpu_prepartition(g, prm->rsb_pre));
     TRY(rsbgpu_level_laplacian(g));
     rsbgpu_fiedler_out fo = {niter, npass, lam, rn, prm->residuals ? res : NULL, fb};
@@ -196,16 +216,18 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     rep->level_ms[lv] = 1e3 * (wall() - tl);
   }
   TRY(rsbgpu_set_segments(g, nseg, A.cstart, A.n, A.p, A.r0));
+  if (ncuts > 0 && ncuts - 1 < PARRSB_AMD_MAX_LEVELS)
+    TRY(level_neighbors(g, nseg, A.p, nbr, rep, ncuts - 1));
 done:
   segtab_free(&A), segtab_free(&B);
   free(nl), free(snd), free(scnt), free(moved), free(ncomp), free(niter), free(npass), free(lam),
-      free(rn), free(res), free(fb);
+      free(rn), free(res), free(fb), free(nbr);
   return rc;
 }
 
-/* partitioner = 1: RCB alone (parrsb.c:277).  The element order after the
- * virtual-rank levels of RCB gives each rank its share; the ranks' ranges
- * follow from the same split arithmetic. */
+/* partitioner = 1: RCB alone, 2: RIB alone (parrsb.c:275-280).  The element
+ * order after the virtual-rank levels gives each rank its share; the ranks'
+ * ranges follow from the same split arithmetic. */
 static void rcb_ranges(int *cstart, int *n, int start, int len, int p, int r0) {
   if (p == 1) {
     cstart[r0] = start, n[r0] = len;
@@ -216,7 +238,7 @@ static void rcb_ranges(int *cstart, int *n, int start, int len, int p, int r0) {
   rcb_ranges(cstart, n, start + l, len - l, p / 2, r0 + (p + 1) / 2);
 }
 
-int rcb_run(rsbgpu *g, int nel, int P) {
+int rcb_run(rsbgpu *g, int nel, int P, int pre) {
   int rc = 0, one_c = 0, one_p = P, one_r = 0;
   int *cs = (int *)calloc((size_t)P, sizeof(int)), *n = (int *)calloc((size_t)P, sizeof(int));
   int *p = (int *)calloc((size_t)P, sizeof(int)), *r0 = (int *)calloc((size_t)P, sizeof(int));
@@ -225,7 +247,7 @@ int rcb_run(rsbgpu *g, int nel, int P) {
     goto done;
   }
   TRY(rsbgpu_set_segments(g, 1, &one_c, &nel, &one_p, &one_r));
-  TRY(rsbgpu_prepartition(g, 1));
+  TRY(rsbgpu_prepartition(g, pre));
   rcb_ranges(cs, n, 0, nel, P, 0);
   for (int r = 0; r < P; r++) p[r] = 1, r0[r] = r;
   TRY(rsbgpu_set_segments(g, P, cs, n, p, r0));

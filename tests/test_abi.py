@@ -106,9 +106,11 @@ def test_input_errors_before_device(monkeypatch):
     assert _call(None, None, 8, opts) == 4           # both NULL (parrsb.c:244-249)
     vtx = np.arange(1, 9, dtype=np.int64)
     assert _call(vtx, None, 5, opts) == 3            # nv must be 4 or 8
-    opts = pa.default_options(verbose_level=0, nparts=2, rsb_pre=2)
     xyz = np.zeros(24)
-    assert _call(vtx, xyz, 8, opts) == 3             # RIB not provided
+    opts = pa.default_options(verbose_level=0, nparts=2, partitioner=3)
+    assert _call(vtx, xyz, 8, opts) == 3             # no such partitioner
+    opts = pa.default_options(verbose_level=0, nparts=2, partitioner=2)
+    assert _call(vtx, None, 8, opts) == 3            # RIB needs coordinates
     opts = pa.default_options(verbose_level=0, nparts=2, tagged=1)
     tag = np.zeros(1, dtype=np.int32)
     assert _call(vtx, xyz, 8, opts, tag) == 3        # tagged mode not provided

@@ -301,3 +301,44 @@ def test_rcb_partitioner_matches_oracle_ranks(oracle):
         ranges(start + l, length - l, p // 2, r0 + (p + 1) // 2)
     ranges(0, n, P, 0)
     assert np.array_equal(part, ref)
+
+
+def test_neighbors_match_brute_force(oracle):
+    """RSB_NEIGHBORS on arbitrary segments: distinct other segments sharing a
+    vertex, against a numpy count over the incidence structure."""
+    vtx, xyz, nv = M.holed_quads(48, 40, seed=11, frac=0.12, rmin=2, rmax=6)
+    n = vtx.shape[0]
+    rng = np.random.default_rng(3)
+    cuts = np.sort(rng.choice(np.arange(1, n), size=11, replace=False))
+    cs = np.concatenate([[0], cuts]).astype(np.int32)
+    ln = np.diff(np.concatenate([cs, [n]])).astype(np.int32)
+    with pa.Device() as d:
+        d.set_mesh(vtx, xyz, nv)
+        d.set_segments(cs, ln, np.full(cs.size, 2), np.arange(cs.size) * 2)
+        order = d.order()
+        got = d.neighbors(cs.size)
+    seg = np.empty(n, dtype=np.int64)
+    for s, (a, l) in enumerate(zip(cs, ln)):
+        seg[order[a:a + l]] = s
+    pairs = set()
+    by_v = {}
+    for e in range(n):
+        for v in vtx[e]:
+            by_v.setdefault(int(v), set()).add(int(seg[e]))
+    for ss in by_v.values():
+        for a in ss:
+            for b in ss:
+                if a != b:
+                    pairs.add((a, b))
+    ref = np.zeros(cs.size, dtype=np.int32)
+    for a, _ in pairs:
+        ref[a] += 1
+    assert np.array_equal(got, ref)
+
+
+def test_neighbors_report_16cube():
+    vtx, xyz, nv = M.box_hex(16, 16, 16, jitter=0.1, seed=1)
+    pa.part_mesh(vtx, xyz, nv, 8)
+    r = pa.last_report()
+    assert r["nbrs_min"] == r["nbrs_max"] == [1, 3, 7]
+    assert r["nbrs_mean"] == [1.0, 3.0, 7.0]

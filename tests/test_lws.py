@@ -1,7 +1,7 @@
-"""Learned warm start: the numpy oracle's backward pass is pinned by finite
-differences (CPU); the device network (BF16 MFMA forward and backward) is
-checked against the oracle with BF16 rounding emulated at the same points
-(GPU)."""
+"""Learned warm start: the numpy oracle's loss is pinned by closed forms and
+its backward pass by finite differences (CPU); the device network (BF16 MFMA
+forward and backward) is checked against the oracle with BF16 rounding
+emulated at the same points (GPU)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -28,11 +28,11 @@ def test_oracle_backward_matches_finite_differences():
     Ad = A.toarray()
     rowlen = np.diff(A.indptr)
     x = W.features(coord, rowlen, 3)
+    lam = np.linalg.eigvalsh(W.laplacian(Ad))[1]
     rng = np.random.default_rng(1)
-    t = np.cos(np.pi * x[:, 0]) + 0.1 * rng.standard_normal(x.shape[0])
     P = W.init_params(2)
     y, cache = W.forward(P, x, Ad)
-    G = W.backward(P, cache, Ad, W.dloss(y, t))
+    G = W.backward(P, cache, Ad, W.dloss(y, Ad, lam))
     idx = np.concatenate([rng.choice(np.arange(W.OFF["W0"], W.OFF["b0"]), 8),
                           rng.choice(np.arange(W.OFF["W1"], W.OFF["b1"]), 8),
                           rng.choice(np.arange(W.OFF["W4"], W.OFF["b4"]), 8),
@@ -42,16 +42,25 @@ def test_oracle_backward_matches_finite_differences():
         Pp, Pm = P.copy(), P.copy()
         Pp[i] += eps
         Pm[i] -= eps
-        fd = (W.loss(W.forward(Pp, x, Ad)[0], t) - W.loss(W.forward(Pm, x, Ad)[0], t)) / (2 * eps)
-        assert abs(fd - G[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, G[i])
+        fd = (W.loss(W.forward(Pp, x, Ad)[0], Ad, lam) -
+              W.loss(W.forward(Pm, x, Ad)[0], Ad, lam)) / (2 * eps)
+        assert abs(fd - G[i]) <= 1e-6 * (1 + abs(G).max()) + 1e-4 * abs(fd), (i, fd, G[i])
 
 
-def test_loss_is_sign_and_scale_invariant():
+def test_loss_closed_forms():
+    """Zero on the Fiedler vector, lambda_k / lambda_2 - 1 on the k-th
+    eigenvector, invariant under y -> a y + b, and the gradient vanishes on
+    eigenvectors (stationary points of the Rayleigh quotient)."""
+    A, _ = segment_graph((4, 3, 3))
+    Ad = A.toarray()
+    w, V = np.linalg.eigh(W.laplacian(Ad))
+    lam = w[1]
+    assert abs(W.loss(V[:, 1], Ad, lam)) < 1e-10
+    assert abs(W.loss(V[:, 5], Ad, lam) - (w[5] / w[1] - 1)) < 1e-9
     rng = np.random.default_rng(3)
-    t = rng.standard_normal(50)
-    y = rng.standard_normal(50)
-    assert abs(W.loss(y, t) - W.loss(-3.0 * y + 7.0, t)) < 1e-12
-    assert abs(W.loss(2.0 * t + 1.0, t)) < 1e-12
+    y = rng.standard_normal(Ad.shape[0])
+    assert abs(W.loss(y, Ad, lam) - W.loss(-3.0 * y + 7.0, Ad, lam)) < 1e-9
+    assert np.abs(W.dloss(V[:, 3], Ad, lam)).max() < 1e-9
 
 
 @pytest.mark.gpu
@@ -62,19 +71,17 @@ def test_device_network_matches_oracle(dims):
     Ad = A.toarray()
     rowlen = np.diff(A.indptr)
     level = 5.0
-    rng = np.random.default_rng(4)
-    t = np.cos(np.pi * W.features(coord, rowlen, level)[:, 0]) + 0.05 * rng.standard_normal(coord.shape[0])
-    t = (t - t.mean()) / np.linalg.norm(t - t.mean())
+    lam = float(np.linalg.eigvalsh(W.laplacian(Ad))[1])
     P = W.init_params(5).astype(np.float32).astype(np.float64)
     y, grad, L = pa.lws_test(coord.T.copy(), level, A.indptr, A.indices, A.data.astype(np.int32),
-                             t.astype(np.float32), P)
+                             lam, P)
     x = W.features(coord, rowlen, level)
     yo, cache = W.forward(P, x, Ad, emulate_bf16=True)
     # forward: BF16 activations, FP32 accumulation in a different order
     assert np.max(np.abs(y - yo)) <= 2e-2 * np.max(np.abs(yo)) + 1e-3
-    Lo = W.loss(yo, t)
+    Lo = W.loss(yo, Ad, lam)
     assert abs(L - Lo) <= 2e-2 * max(Lo, 1e-3)
-    Go = W.backward(P, cache, Ad, W.dloss(yo, t), emulate_bf16=True)
+    Go = W.backward(P, cache, Ad, W.dloss(yo, Ad, lam), emulate_bf16=True)
     cos = grad @ Go / (np.linalg.norm(grad) * np.linalg.norm(Go))
     assert cos > 0.995, cos
     assert np.linalg.norm(grad - Go) <= 0.05 * np.linalg.norm(Go)

@@ -287,6 +287,87 @@ def test_rcb_ties_keep_current_order(oracle):
     assert out.tolist() == [3, 1, 2, 0]
 
 
+# ---------------------------------------------------------------- RIB
+def test_sym_dominant_matches_eigh(oracle):
+    """Jacobi dominant eigenvector vs LAPACK (numpy.linalg.eigh), with the
+    sign fixed by the largest component; 2 x 2 and 3 x 3, random and
+    nearly diagonal matrices."""
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        n = 2 + trial % 2
+        B = rng.standard_normal((n, n)) * (1e-6 if trial % 7 == 0 else 1.0)
+        A = B @ B.T + (np.diag(rng.uniform(0, 3, n)) if trial % 5 == 0 else 0)
+        w, V = np.linalg.eigh(A)
+        if w[-1] - w[-2] < 1e-6 * abs(w[-1]):
+            continue
+        v = V[:, -1] * np.sign(V[np.argmax(np.abs(V[:, -1])), -1])
+        ev = oracle.sym_dominant(A, n)
+        assert np.allclose(ev, v, atol=1e-12), (A, ev, v)
+
+
+def test_sym_dominant_degenerate_picks_lowest_axis(oracle):
+    assert oracle.sym_dominant(np.eye(3) * 2.0, 3).tolist() == [1.0, 0.0, 0.0]
+    assert oracle.sym_dominant(np.diag([1.0, 3.0, 3.0]), 3).tolist() == [0.0, 1.0, 0.0]
+
+
+def test_rib_axis_is_principal_axis(oracle):
+    """Points stretched along a known direction: the axis is that direction
+    (sign: largest component positive) and the mean is the sample mean."""
+    rng = np.random.default_rng(3)
+    d = np.array([0.48, -0.6, 0.64])
+    d /= np.linalg.norm(d)
+    pts = rng.standard_normal((400, 1)) * 5.0 * d + rng.standard_normal((400, 3)) * 0.2 + 1.5
+    m, ev = oracle.rib_axis(pts, 3, np.arange(400, dtype=np.int32))
+    assert np.allclose(m, pts.mean(0), rtol=1e-13)
+    C = (pts - pts.mean(0)).T @ (pts - pts.mean(0))
+    w, V = np.linalg.eigh(C)
+    v = V[:, -1] * np.sign(V[np.argmax(np.abs(V[:, -1])), -1])
+    assert np.allclose(ev, v, atol=1e-12)
+    assert abs(abs(ev @ d) - 1) < 1e-3
+
+
+def _check_rib(coord, ndim, order, p, oracle):
+    """Every node is split at its virtual-rank boundary along the principal
+    axis of its own centroids (numpy eigh): keys left <= keys right."""
+    n = order.size
+    if n <= 1:
+        return
+    c = coord[order][:, :ndim]
+    x = c - c.mean(0)
+    w, V = np.linalg.eigh(x.T @ x)
+    key = x @ V[:, -1]
+    nl = oracle.lower_count(n, p) if p > 1 else n // 2
+    lo, hi = key[:nl], key[nl:]
+    # either orientation of the axis; sign is the oracle's convention
+    tol = 1e-9 * (np.abs(key).max() + 1)
+    assert lo.max(initial=-np.inf) <= hi.min(initial=np.inf) + tol or \
+        lo.min(initial=np.inf) >= hi.max(initial=-np.inf) - tol
+    _check_rib(coord, ndim, order[:nl], (p + 1) // 2 if p > 1 else 1, oracle)
+    _check_rib(coord, ndim, order[nl:], p // 2 if p > 1 else 1, oracle)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 5, 8, 13])
+def test_rib_split_structure(oracle, p):
+    rng = np.random.default_rng(100 + p)
+    for ndim in (2, 3):
+        coord = np.zeros((150, 3))
+        coord[:, :ndim] = rng.uniform(0, 1, (150, ndim)) @ rng.standard_normal((ndim, ndim))
+        start = rng.permutation(150).astype(np.int32)
+        out = oracle.rib_order(coord, ndim, start, p)
+        assert sorted(out.tolist()) == list(range(150))
+        _check_rib(coord, ndim, out, p, oracle)
+
+
+def test_rib_on_axis_aligned_box_is_rcb_like(oracle):
+    """A box elongated along y: the first RIB split is a y split, as RCB's."""
+    vtx, xyz, nv = M.box_hex(6, 14, 5, jitter=0.1, seed=2)
+    c = oracle.centroids(xyz, nv)
+    n = c.shape[0]
+    o = oracle.rib_order(c, 3, np.arange(n, dtype=np.int32), 2)
+    nl = oracle.lower_count(n, 2)
+    assert c[o[:nl], 1].max() < c[o[nl:], 1].min()
+
+
 # ---------------------------------------------------------------- full path
 def test_part_mesh_16cube_is_octant_split(oracle):
     vtx, xyz, nv = M.box_hex(16, 16, 16, jitter=0.1, seed=1)
@@ -298,6 +379,10 @@ def test_part_mesh_16cube_is_octant_split(oracle):
     assert q["edge_cut"] == sec(16, 16) + 2 * sec(16, 8) + 4 * sec(8, 8) == 11164
     assert q["max_part"] == q["min_part"] == 512
     assert q["components"] == 8 and q["max_components"] == 1
+    # neighbouring parts after each cut: halves touch 1, quarters (sharing
+    # the centre line) 3, octants (sharing the centre vertex) 7
+    assert rep["nbrs_min"] == rep["nbrs_max"] == [1, 3, 7]
+    assert rep["nbrs_mean"] == [1.0, 3.0, 7.0]
     assert rep["ncomp_max"] == [1, 1, 1]
     c = oracle.centroids(xyz, nv)
     for r in range(8):
