@@ -3,10 +3,13 @@
 A step is one complete partitioning call (centroids, dual graph, every
 bisection level, part ids) on a mesh already resident in HBM.  Default
 workload: 256^3 warped hex box (16,777,216 elements) into 65,536 parts with
-batched FP32 Lanczos; the plain FP64 GPU path is timed beside it on the same
-mesh, and the CPU oracle on a bounded sample.
+batched FP32 Lanczos and the learned warm start (the committed network
+parameters, parrsb_amd/lws_weights.bin); beside it, on the same mesh, the
+batched FP32 path from the reference start and the plain FP64 GPU path
+(reference arithmetic and start), and the CPU oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5]
+                    [--warm-start PATH|none]
 """
 import argparse
 import json
@@ -44,9 +47,10 @@ def make_mesh(name):
 
 
 def run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, steps, warmup,
-              profile):
+              profile, warm=None):
     pa.set_precision(precision)
     pa.set_profiling(profile)
+    pa.set_warm_start(warm, 1)
     for i in range(warmup):
         pa.part_mesh_dev(d_part.data_ptr(), d_vtx.data_ptr(), d_xyz.data_ptr(), nel, nv, P)
         log(f"warmup {i + 1}/{warmup} fp{precision}: {pa.last_report()['seconds']:.3f} s")
@@ -58,8 +62,10 @@ def run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, steps, war
         reps.append(pa.last_report())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    log(f"timed {steps} step(s) fp{precision}: {dt:.3f} s")
+    log(f"timed {steps} step(s) fp{precision}{' learned' if warm is not None else ''}: "
+        f"{dt:.3f} s")
     pa.set_profiling(False)
+    pa.set_warm_start(None)
     return dt, reps
 
 
@@ -112,6 +118,8 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--warm-start", default=os.path.join(ROOT, "parrsb_amd", "lws_weights.bin"),
+                    help="network parameter file of the learned start, or 'none'")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build partitions on one GPU; only --gpus 1 is supported",
@@ -132,9 +140,14 @@ def main():
     del xyz
     torch.cuda.synchronize()
     log(f"mesh resident: {nel} elements, nv={nv}, {P} parts")
+    warm = None
+    if args.warm_start != "none" and os.path.exists(args.warm_start):
+        from parrsb_amd import lws_train
+        warm = lws_train.load_params(args.warm_start)
+        log(f"learned warm start: {args.warm_start}")
 
     dt, reps = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision,
-                         args.steps, args.warmup, True)
+                         args.steps, args.warmup, True, warm)
     q = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
     rep = reps[-1]
     ms = dt / args.steps * 1e3
@@ -151,7 +164,9 @@ def main():
                    "lanczos": ("batched FP32 vectors, compensated dots, FP64 scalars and "
                                "tridiagonal solve" if args.precision == 32 else
                                "batched FP64 (reference arithmetic)"),
-                   "start_vector": "reference (RCB position); learned warm start not used",
+                   "start_vector": ("learned warm start (one pass, then the reference start "
+                                    "where it misses rsb_tol)" if warm is not None else
+                                    "reference (RCB position)"),
                    "inputs": "device-resident (HBM)"},
         "roofline": {"bound": "hbm", "kernel": "k_lz_apply",
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -163,9 +178,18 @@ def main():
                      "iteration_achieved_gbs": it_gbs},
         "quality": q,
         "levels": {k: rep[k] for k in ("niter_max", "niter_mean", "npass_max", "ncomp_max",
-                                       "resid_med", "resid_max", "level_ms")},
+                                       "nfallback", "nbrs_mean", "resid_med", "resid_max",
+                                       "level_ms")},
         "phase_ms": rep["phase_ms"],
     }
+    if not args.no_compare and warm is not None:
+        dtr, repsr = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision, 1,
+                               0, False)
+        qr = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
+        out["reference_start"] = {"value": nel / dtr / 1e6, "ms": dtr * 1e3, "quality": qr,
+                                  "niter_mean": repsr[-1]["niter_mean"]}
+        out["speedup_vs_reference_start"] = value / (nel / dtr / 1e6)
+        out["edge_cut_vs_reference_start"] = q["edge_cut"] / max(qr["edge_cut"], 1)
     if not args.no_compare and args.precision == 32:
         dt64, reps64 = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 64, 1, 0, False)
         q64 = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)

@@ -75,14 +75,16 @@ int parrsb_amd_set_profiling(int on);
 
 /* Learned warm start for later RSB calls: `params` are the network's
  * parrsb_amd_warm_start_nparams() FP32 parameters (host memory, copied), or
- * NULL to switch it off.  Each Fiedler solve then starts from the network's
+ * NULL to switch it off.  The Fiedler solve of every segment of at most
+ * `max_n` elements (0: every segment) then starts from the network's
  * prediction, which gets `passes` restarted passes of rsb_max_iter
  * iterations; a segment that misses rsb_tol within them is solved again from
- * the reference start.  PARRSB_AMD_WARM_START=<file> loads parameters from
- * a file written by parrsb_amd.lws_train (PARRSB_AMD_WARM_PASSES sets
- * `passes`). */
+ * the reference start.  Larger segments take the reference start.
+ * PARRSB_AMD_WARM_START=<file> loads parameters from a file written by
+ * parrsb_amd.lws_train (PARRSB_AMD_WARM_PASSES and PARRSB_AMD_WARM_MAX_N set
+ * `passes` and `max_n`, default 1 and 2048). */
 int parrsb_amd_warm_start_nparams(void);
-int parrsb_amd_set_warm_start(const float *params, int passes);
+int parrsb_amd_set_warm_start(const float *params, int passes, int max_n);
 
 /* Quality of a partition (host arrays): out[0] weighted edge cut, out[1] cut
  * edges, out[2] max part size, out[3] min part size, out[4] components over

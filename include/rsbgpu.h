@@ -31,9 +31,10 @@ typedef struct {
   double *lambda;   /* Ritz value of the last pass */
   double *rnorm;    /* 3 per segment: initial, target, final CG residual */
   double *resid;    /* relative eigen-residual ||Lf - lambda f|| / lambda */
-  int *fallback;    /* init_mode 2: 0 if the learned start met tol, else 1 +
-                       the iterations of the learned attempt (the segment
-                       was solved again from the reference start) */
+  int *fallback;    /* init_mode 2: 0 if the learned start met tol; 1 + the
+                       iterations of the learned attempt if the segment was
+                       solved again from the reference start; -1 if it was
+                       above the size limit and took the reference start */
 } rsbgpu_fiedler_out;
 
 int rsbgpu_open(rsbgpu **g, int device);
@@ -133,8 +134,11 @@ int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t *out);
  * the fallback.  Parameters are LWS_NPARAM floats in device memory. */
 int rsbgpu_lws_nparams(void);
 /* Parameters and inputs of init_mode 2: the bisection level (0 = first
- * cut) and the passes the learned start may use before the fallback. */
-int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level, int passes);
+ * cut), the passes the learned start may use before the fallback, and the
+ * largest segment (elements) it is tried on (0: all; larger segments take
+ * the reference start with the full budget). */
+int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level, int passes,
+                          int max_n);
 /* Network output for every row of the current level (after
  * rsbgpu_level_laplacian), d_rows[mpad]. */
 int rsbgpu_lws_predict(rsbgpu *g, const float *d_params, float level, float *d_rows);

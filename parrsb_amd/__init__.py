@@ -94,7 +94,7 @@ SIGNATURES = {
     "parrsb_amd_set_precision": (I, [I]),
     "parrsb_amd_set_profiling": (I, [I]),
     "parrsb_amd_warm_start_nparams": (I, []),
-    "parrsb_amd_set_warm_start": (I, [P, I]),
+    "parrsb_amd_set_warm_start": (I, [P, I, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
     "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
     # rsbgpu.h
@@ -126,7 +126,7 @@ SIGNATURES = {
     "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
     "rsbgpu_neighbors": (I, [P, P]),
     "rsbgpu_lws_nparams": (I, []),
-    "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I]),
+    "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
     "rsbgpu_lws_train_step": (I, [P, P, P, C.c_float, C.POINTER(C.c_double)]),
     "rsbgpu_lws_adam": (I, [P, P, P, P, I, C.c_float]),
@@ -320,8 +320,8 @@ class Device:
 
     # learned warm start (device parameter arrays given as integer pointers,
     # e.g. torch tensor data_ptr())
-    def lws_set_params(self, d_params, level, passes=1):
-        self._chk(lib().rsbgpu_lws_set_params(self.h, d_params, level, passes))
+    def lws_set_params(self, d_params, level, passes=1, max_n=0):
+        self._chk(lib().rsbgpu_lws_set_params(self.h, d_params, level, passes, max_n))
 
     def lws_predict(self, d_params, level, d_rows):
         self._chk(lib().rsbgpu_lws_predict(self.h, d_params, level, d_rows))
@@ -382,19 +382,25 @@ class Device:
         return part
 
 
+# Largest segment the learned start is tried on by default (elements): above
+# it the reference start needs fewer iterations than the network's
+# prediction under the reference's stop rule.
+WARM_MAX_N = 2048
+
+
 def warm_start_nparams() -> int:
     return lib().parrsb_amd_warm_start_nparams()
 
 
-def set_warm_start(params, passes=1):
+def set_warm_start(params, passes=1, max_n=WARM_MAX_N):
     """Learned warm start for later part_mesh calls (None switches it off)."""
     if params is None:
-        lib().parrsb_amd_set_warm_start(None, 1)
+        lib().parrsb_amd_set_warm_start(None, 1, 0)
         return
     params = np.ascontiguousarray(params, dtype=np.float32)
     if params.size != warm_start_nparams():
         raise ValueError(f"expected {warm_start_nparams()} parameters, got {params.size}")
-    rc = lib().parrsb_amd_set_warm_start(_p(params), passes)
+    rc = lib().parrsb_amd_set_warm_start(_p(params), passes, max_n)
     if rc:
         raise ParrsbError(rc, "bad warm-start parameters")
 

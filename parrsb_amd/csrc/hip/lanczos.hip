@@ -39,6 +39,7 @@ struct lz_args {
   int *running;
   int mpass;
   int mode;  // start vector: 0 reference, 1 caller per element, 2 learned per row
+  int max_n; // mode 2: segments with more rows take the reference start
   double tol;
   int miter_max;
   const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
@@ -594,7 +595,13 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   switch (op) {
     case FIN_START0: SC(s, SC_M0) = tot[0] / n; break;
     case FIN_START1:
-      if (a.mode == 2 && !(tot[0] > 0.0 && tot[0] < INFINITY)) {
+      if (a.mode == 2 && a.max_n > 0 && a.seg[a.nseg + s] > a.max_n) {
+        // above the size limit: no learned attempt, reference start next
+        SC(s, SC_N0) = 0.0;
+        SC(s, SC_FALLBACK) = -1.0;
+        a.k[s] = a.miter[s];
+        a.status[s] = LZ_FINISHED;
+      } else if (a.mode == 2 && !(tot[0] > 0.0 && tot[0] < INFINITY)) {
         // constant or non-finite prediction: no pass, straight to the fallback
         SC(s, SC_N0) = 0.0;
         a.k[s] = a.miter[s];
@@ -705,8 +712,9 @@ __global__ void k_lz_fallback(lz_args a, int *__restrict__ count) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.nseg) return;
   if (a.status[s] != LZ_FINISHED || a.k[s] != a.miter[s]) return;
-  SC(s, SC_WARMIT) = a.niter[s];
-  SC(s, SC_FALLBACK) = 1.0;
+  const bool skipped = SC(s, SC_FALLBACK) < 0.0;  // above the size limit
+  SC(s, SC_WARMIT) = skipped ? 0.0 : a.niter[s];
+  SC(s, SC_FALLBACK) = skipped ? -1.0 : 1.0;
   SC(s, SC_MEANF) = 0.0;
   a.npass[s] = 0, a.niter[s] = 0, a.iter[s] = 0, a.k[s] = 0;
   a.status[s] = LZ_SETUP;
@@ -781,6 +789,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
   a.running = L.running;
   a.mpass = mpass, a.mode = init_mode, a.tol = tol, a.miter_max = miter_max;
+  a.max_n = g->lws_max_n;
   T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
   hipStream_t st = g->stream;
@@ -954,7 +963,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         out->rnorm[3 * s + 2] = sc[(size_t)s * LZ_NSC + SC_RNORM];
       }
       if (out->resid) out->resid[s] = sc[(size_t)s * LZ_NSC + SC_TMP];
-      if (out->fallback) out->fallback[s] = scs[SC_FALLBACK] != 0.0 ? 1 + (int)scs[SC_WARMIT] : 0;
+      if (out->fallback)
+        out->fallback[s] = scs[SC_FALLBACK] < 0.0   ? -1
+                           : scs[SC_FALLBACK] > 0.0 ? 1 + (int)scs[SC_WARMIT]
+                                                    : 0;
     }
   }
   return 0;

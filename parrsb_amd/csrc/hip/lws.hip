@@ -528,11 +528,13 @@ int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows) {
 extern "C" int rsbgpu_lws_nparams(void) { return LWS_NPARAM; }
 
 extern "C" int rsbgpu_lws_set_params(rsbgpu *g, const float *d_params, float level,
-                                     int passes) {
+                                     int passes, int max_n) {
   if (passes < 1) RSB_FAIL(3, "learned start needs at least one pass");
+  if (max_n < 0) RSB_FAIL(3, "negative segment-size limit");
   g->lws_params = d_params;
   g->lws_level = level;
   g->lws_passes = passes;
+  g->lws_max_n = max_n;
   return 0;
 }
 

@@ -97,6 +97,7 @@ struct rsbgpu {
   const float *lws_params = nullptr;  // learned warm start parameters (device)
   float lws_level = 0.f;              // bisection level fed to the network
   int lws_passes = 1;                 // passes granted to the learned start
+  int lws_max_n = 0;                  // larger segments skip it (0: no limit)
   void *d_lws_ws = nullptr;
   size_t lws_ws_bytes = 0;
   lz_state lz{};

@@ -45,7 +45,7 @@ static parrsb_amd_report last_report;
 static char last_error[512];
 static int cfg_precision = 64, cfg_profiling = 0;
 static float *cfg_warm = NULL; /* host copy of the learned-start parameters */
-static int cfg_warm_passes = 1;
+static int cfg_warm_passes = 1, cfg_warm_max_n = 2048;
 static char cfg_warm_file[512];
 
 const char *parrsb_amd_last_error(void) { return last_error; }
@@ -76,8 +76,8 @@ static void set_error(const char *fmt, ...) {
 
 int parrsb_amd_warm_start_nparams(void) { return rsbgpu_lws_nparams(); }
 
-int parrsb_amd_set_warm_start(const float *params, int passes) {
-  if (params && passes < 1) return 3;
+int parrsb_amd_set_warm_start(const float *params, int passes, int max_n) {
+  if (params && (passes < 1 || max_n < 0)) return 3;
   free(cfg_warm);
   cfg_warm = NULL;
   cfg_warm_file[0] = 0;
@@ -87,6 +87,7 @@ int parrsb_amd_set_warm_start(const float *params, int passes) {
   if (!cfg_warm) return 2;
   memcpy(cfg_warm, params, n * sizeof(float));
   cfg_warm_passes = passes;
+  cfg_warm_max_n = max_n;
   return 0;
 }
 
@@ -111,11 +112,12 @@ static int load_warm_file(const char *path) {
     set_error("warm-start file %s is not a parameter file of this network", path);
     return 3;
   }
-  const char *v = getenv("PARRSB_AMD_WARM_PASSES");
-  int rc = parrsb_amd_set_warm_start(buf, v ? atoi(v) : cfg_warm_passes);
+  const char *v = getenv("PARRSB_AMD_WARM_PASSES"), *m = getenv("PARRSB_AMD_WARM_MAX_N");
+  int rc = parrsb_amd_set_warm_start(buf, v ? atoi(v) : cfg_warm_passes,
+                                     m ? atoi(m) : cfg_warm_max_n);
   free(buf);
   if (rc) {
-    set_error("bad PARRSB_AMD_WARM_PASSES");
+    set_error("bad PARRSB_AMD_WARM_PASSES or PARRSB_AMD_WARM_MAX_N");
     return 3;
   }
   snprintf(cfg_warm_file, sizeof(cfg_warm_file), "%s", path);
@@ -284,7 +286,7 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
       prm.find_comps = options->find_disconnected_comps, prm.precision = precision_bits();
       prm.verbose = options->verbose_level, prm.tol = options->rsb_tol;
       prm.residuals = 1;
-      prm.warm = NULL, prm.warm_passes = cfg_warm_passes;
+      prm.warm = NULL, prm.warm_passes = cfg_warm_passes, prm.warm_max_n = cfg_warm_max_n;
       void *dw = NULL;
       if (cfg_warm) {
         rc = rsbgpu_upload(g, cfg_warm, sizeof(float) * (size_t)rsbgpu_lws_nparams(), &dw);

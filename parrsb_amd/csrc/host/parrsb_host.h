@@ -9,7 +9,7 @@ typedef struct {
   int nparts, rsb_pre, max_iter, max_passes, find_comps, precision, verbose, residuals;
   double tol;
   const float *warm;  /* device parameters of the learned start, or NULL */
-  int warm_passes;
+  int warm_passes, warm_max_n;
 } rsb_params;
 
 long long rsb_lower_count(long long n, int p);

@@ -129,7 +129,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     rsbgpu_fiedler_out fo = {niter, npass, lam, rn, prm->residuals ? res : NULL, fb};
     int mode = 0;
     if (prm->warm) {
-      TRY(rsbgpu_lws_set_params(g, prm->warm, (float)cut, prm->warm_passes));
+      TRY(rsbgpu_lws_set_params(g, prm->warm, (float)cut, prm->warm_passes, prm->warm_max_n));
       mode = 2;
     }
     memset(fb, 0, sizeof(int) * (size_t)nseg);

@@ -77,7 +77,8 @@ def balance_expected(n, p, b):
 def run_levels(dev, nel, nparts, on_level=None, precision=64, max_iter=50, max_passes=50,
                tol=1e-5):
     """The RSB level loop through the step-by-step device API; on_level(cut,
-    nseg, out) runs after each level's Fiedler solve, before the sort."""
+    sizes, out) runs after each level's Fiedler solve, before the sort
+    (sizes: elements of the level's segments that are bisected)."""
     segs = [(0, nel, nparts, 0)]
     ncuts = meshgen.ncuts(nparts)
     for cut in range(ncuts):
@@ -87,7 +88,7 @@ def run_levels(dev, nel, nparts, on_level=None, precision=64, max_iter=50, max_p
         dev.level_laplacian()
         out = dev.fiedler(len(segs), precision, max_iter, max_passes, tol)
         if on_level is not None:
-            on_level(cut, len(segs), out)
+            on_level(cut, n[p > 1], out)
         dev.fiedler_sort()
         nl = n.copy()
         snd = np.full(len(segs), -1, np.int32)
@@ -119,7 +120,10 @@ def parts_for(nel, per_part=1000):
 
 
 def train(meshes, steps_per_level=2, lr=1e-3, seed=0, init=None, out=None, log=print,
-          per_part=1000):
+          per_part=1000, max_n=0):
+    """Adam on the levels of meshgen.warm_start_mesh(i) for i in meshes; with
+    max_n > 0 only levels whose segments have at most max_n elements (where
+    the start is used) are trained on."""
     P0 = np.asarray(init if init is not None else init_params(seed), dtype=np.float32)
     holder = Device()  # owns the parameter, gradient and moment buffers
     z = np.zeros_like(P0)
@@ -132,7 +136,9 @@ def train(meshes, steps_per_level=2, lr=1e-3, seed=0, init=None, out=None, log=p
         nparts = parts_for(info["nel"], per_part)
         losses = []
 
-        def on_level(cut, nseg, o):
+        def on_level(cut, sizes, o):
+            if max_n > 0 and (sizes.size == 0 or sizes.max() > max_n):
+                return
             for _ in range(steps_per_level):
                 loss = dev.lws_train_step(dP, dG, float(cut))
                 step[0] += 1
@@ -144,10 +150,11 @@ def train(meshes, steps_per_level=2, lr=1e-3, seed=0, init=None, out=None, log=p
         with Device() as dev:
             dev.set_mesh(vtx, xyz, nv)
             run_levels(dev, info["nel"], nparts, on_level)
-        hist.append(dict(mesh=i, nel=info["nel"], kind=info["kind"], nparts=nparts,
-                         loss_first=float(np.mean(losses[::steps_per_level])),
-                         loss_last=float(np.mean(losses[steps_per_level - 1::steps_per_level]))))
-        if (idx + 1) % 20 == 0 or idx + 1 == len(meshes):
+        if losses:
+            hist.append(dict(mesh=i, nel=info["nel"], kind=info["kind"], nparts=nparts,
+                             loss_first=float(np.mean(losses[::steps_per_level])),
+                             loss_last=float(np.mean(losses[steps_per_level - 1::steps_per_level]))))
+        if ((idx + 1) % 20 == 0 or idx + 1 == len(meshes)) and hist:
             recent = hist[-20:]
             log(f"lws_train: {idx + 1}/{len(meshes)} meshes, {step[0]} steps, "
                 f"{time.time() - t0:.0f} s, loss {np.mean([h['loss_first'] for h in recent]):.4f}")
@@ -162,7 +169,7 @@ def train(meshes, steps_per_level=2, lr=1e-3, seed=0, init=None, out=None, log=p
     return P, hist
 
 
-def evaluate(params, mesh, nparts, passes=1, precision=32):
+def evaluate(params, mesh, nparts, passes=1, precision=32, max_n=0):
     """Partition `mesh` = (vtx, xyz, nv) with the plain and with the learned
     start through parrsb_part_mesh; returns the reports' iteration counts,
     Lanczos time, fallbacks and partition quality of both."""
@@ -171,7 +178,7 @@ def evaluate(params, mesh, nparts, passes=1, precision=32):
     res = {}
     for name, P in (("plain", None), ("learned", params)):
         pa.set_precision(precision)
-        pa.set_warm_start(P, passes)
+        pa.set_warm_start(P, passes, max_n)
         part, _ = pa.part_mesh(vtx, xyz, nv, nparts)
         r = pa.last_report()
         q = pa.quality(part, vtx, nv, nparts)
@@ -199,13 +206,15 @@ def main(argv=None):
     ap.add_argument("--eval", default=None,
                     help="held-out evaluation: 'pipe:nr,nt,nz,nparts' after training")
     ap.add_argument("--passes", type=int, default=1)
+    ap.add_argument("--max-n", type=int, default=2048,
+                    help="train on and evaluate with segments of at most this many elements")
     ap.add_argument("--history", default=None, help="write the per-mesh losses (JSON)")
     a = ap.parse_args(argv)
     lo, hi = (int(x) for x in a.meshes.split(":"))
     log = lambda m: (print(m), sys.stdout.flush())
     init = load_params(a.init) if a.init else None
     if hi > lo:
-        P, hist = train(range(lo, hi), a.steps, a.lr, a.seed, init, a.out, log)
+        P, hist = train(range(lo, hi), a.steps, a.lr, a.seed, init, a.out, log, max_n=a.max_n)
         if a.history:
             with open(a.history, "w") as f:
                 json.dump(hist, f)
@@ -215,7 +224,7 @@ def main(argv=None):
         kind, dims = a.eval.split(":")
         nr, nt, nz, nparts = (int(x) for x in dims.split(","))
         mesh = meshgen.pipe_hex(nr, nt, nz, seed=7)
-        res = evaluate(P, mesh, nparts, a.passes)
+        res = evaluate(P, mesh, nparts, a.passes, max_n=a.max_n)
         log(json.dumps(res))
 
 

@@ -102,7 +102,7 @@ def test_failed_warm_start_falls_back_to_reference_start(kind):
         pa.set_warm_start(None)
         part0, _ = pa.part_mesh(vtx, xyz, nv, 16)
         r0 = pa.last_report()
-        pa.set_warm_start(P, 1)
+        pa.set_warm_start(P, 1, 0)
         part1, _ = pa.part_mesh(vtx, xyz, nv, 16)
         r1 = pa.last_report()
     finally:
@@ -131,7 +131,12 @@ def test_warm_start_levels_report_fallback_iterations():
         plain = dev.fiedler(1, 64, 50, 50, 1e-5)
         dev.lws_set_params(dP, 0.0, 1)
         warm = dev.fiedler(1, 64, 50, 50, 1e-5, init_mode=2)
+        # above the size limit: the reference start, nothing else
+        dev.lws_set_params(dP, 0.0, 1, 100)
+        gated = dev.fiedler(1, 64, 50, 50, 1e-5, init_mode=2)
         dev.free(dP)
+    assert gated["fallback"][0] == -1
+    assert gated["niter"][0] == plain["niter"][0] and gated["lambda"][0] == plain["lambda"][0]
     fb = int(warm["fallback"][0])
     if fb:
         # the fallback is the plain solve, bit for bit
