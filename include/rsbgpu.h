@@ -56,6 +56,9 @@ int rsbgpu_set_profiling(rsbgpu *g, int spmv_events);
 /* Parity hook: tridiagonal eigen-solve of one (d, e) on the device; y gets
  * the coefficients of the eigenvector of smallest |eigenvalue|. */
 int rsbgpu_test_tqli(const double *d, const double *e, int n, double *y, double *lam);
+/* The FP32 path's eigen-solve (multisection + inverse iteration) on the same
+ * input: eigenpair of smallest |eigenvalue|, vector signed y[0] >= 0. */
+int rsbgpu_test_eigmin(const double *d, const double *e, int n, double *y, double *lam);
 
 /* Load the mesh.  If on_device != 0 the pointers are device pointers that
  * stay valid for the lifetime of the handle; else they are copied.  Computes

@@ -105,6 +105,7 @@ SIGNATURES = {
     "rsbgpu_lanczos_traffic": (I, [P, P]),
     "rsbgpu_set_profiling": (I, [P, I]),
     "rsbgpu_test_tqli": (I, [P, P, I, P, P]),
+    "rsbgpu_test_eigmin": (I, [P, P, I, P, P]),
     "rsbgpu_set_mesh": (I, [P, P, P, I, I, I]),
     "rsbgpu_graph_stats": (I, [P, P, P]),
     "rsbgpu_set_segments": (I, [P, I, P, P, P, P]),
@@ -405,13 +406,15 @@ def set_warm_start(params, passes=1, max_n=WARM_MAX_N):
         raise ParrsbError(rc, "bad warm-start parameters")
 
 
-def test_tqli(d, e):
-    """Tridiagonal eigen-solve on the device: (lambda_min_abs, y)."""
+def test_tqli(d, e, fast=False):
+    """Tridiagonal eigen-solve on the device: (lambda_min_abs, y); fast=True
+    runs the FP32 path's multisection solve instead of the QL solve."""
     d = np.ascontiguousarray(d, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.float64) if len(e) else np.zeros(1)
     y = np.zeros(d.size)
     lam = np.zeros(1)
-    rc = lib().rsbgpu_test_tqli(_p(d), _p(e), d.size, _p(y), _p(lam))
+    fn = lib().rsbgpu_test_eigmin if fast else lib().rsbgpu_test_tqli
+    rc = fn(_p(d), _p(e), d.size, _p(y), _p(lam))
     if rc:
         raise ParrsbError(rc, "rsbgpu_test_tqli failed")
     return lam[0], y

@@ -25,6 +25,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <cfloat>
+
 #include "state.hpp"
 
 struct lz_args {
@@ -363,39 +365,76 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
 // Tridiagonal eigen-solve of each finished pass: implicit QL with shifts, the
 // steps of fiedler.c:328-432 (FP64, contraction off so every operation rounds
 // as written).  One wavefront per segment: every lane runs the same scalar
-// recurrence and writes the same values to the shared d/e arrays, so each
-// lane only ever reads values it wrote itself and the sweep needs no
-// barrier; the plane rotations of the eigenvector matrix are spread over the
-// lanes (lane = matrix row, each lane touches only its own rows).  The
-// coefficients y of the eigenvector with the smallest |eigenvalue| are kept.
+// recurrence; the diagonal and off-diagonal are held in registers (below) so
+// the serial chain never waits on LDS; the plane rotations of the
+// eigenvector matrix (in LDS) are spread over the lanes (lane = matrix row,
+// each lane touches only its own rows).  The coefficients y of the
+// eigenvector with the smallest |eigenvalue| are kept.
 #pragma clang fp contract(off)
 __device__ inline double sgn_abs(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// d and e live in registers: lane k holds entries k and k + 64 (n <= 128).
+// The serial QL chain reads them with readlane (a uniform index), the owning
+// lane updates them, and the deflation scan (first m >= l with
+// |e_m| / (|d_m| + |d_m+1|) < 1e-12) tests every m at once and takes the
+// first hit from a ballot: the same tests, in parallel, give the same m.
+__device__ inline double lane_get(double v, int i) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), i);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
+  return __longlong_as_double(
+      (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo));
+}
+struct lanes2 {
+  double a, b;  // entries lane and lane + 64
+  __device__ inline double get(int i) const { return i < RSB_WAVE ? lane_get(a, i) : lane_get(b, i - RSB_WAVE); }
+  __device__ inline void set(int i, double x, int lane) {
+    if (i < RSB_WAVE) {
+      if (lane == i) a = x;
+    } else if (lane == i - RSB_WAVE) {
+      b = x;
+    }
+  }
+};
 
 __device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, double *lam_out) {
   const int lane = threadIdx.x;
   for (int i = lane; i < n * n; i += RSB_WAVE) z[i] = 0.0;
   __syncthreads();
   for (int i = lane; i < n; i += RSB_WAVE) z[i * n + i] = 1.0;
+  lanes2 D, E;
+  D.a = lane < n ? d[lane] : 0.0;
+  D.b = lane + RSB_WAVE < n ? d[lane + RSB_WAVE] : 0.0;
+  E.a = lane < n ? e[lane] : 0.0;
+  E.b = lane + RSB_WAVE < n ? e[lane + RSB_WAVE] : 0.0;
   __syncthreads();
   int failed = 0;
   for (int l = 0; l < n && !failed; l++) {
     int sweeps = 0, m;
     while (true) {
-      for (m = l; m < n - 1; m++) {
-        double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) / dd < 1e-12) break;
-      }
+      // d[m + 1] next to d[m]: lane 63's neighbour is entry 64
+      double dn_a = __shfl_down(D.a, 1, RSB_WAVE);
+      const double d64 = lane_get(D.b, 0);
+      if (lane == RSB_WAVE - 1) dn_a = d64;
+      const double dn_b = __shfl_down(D.b, 1, RSB_WAVE);
+      const int ma = lane, mb = lane + RSB_WAVE;
+      const bool ta = ma >= l && ma < n - 1 && fabs(E.a) / (fabs(D.a) + fabs(dn_a)) < 1e-12;
+      const bool tb = mb >= l && mb < n - 1 && fabs(E.b) / (fabs(D.b) + fabs(dn_b)) < 1e-12;
+      const unsigned long long ba = __ballot(ta), bb = __ballot(tb);
+      m = ba ? __ffsll((long long)ba) - 1 : bb ? RSB_WAVE + __ffsll((long long)bb) - 1 : n - 1;
       if (m == l) break;
       if (sweeps++ == 30) {
         failed = 1;
         break;
       }
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      const double dl = D.get(l), el = E.get(l);
+      double g = (D.get(l + 1) - dl) / (2.0 * el);
       double r = sqrt(g * g + 1.0);
-      g = d[m] - d[l] + e[l] / (g + sgn_abs(r, g));
+      g = D.get(m) - dl + el / (g + sgn_abs(r, g));
       double s = 1.0, c = 1.0, p = 0.0;
       for (int i = m - 1; i >= l; i--) {
-        double f = s * e[i], b = c * e[i];
+        const double ei = E.get(i);
+        double f = s * ei, b = c * ei;
         double ei1;
         if (fabs(f) >= fabs(g)) {
           c = g / f;
@@ -410,30 +449,32 @@ __device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, dou
           c = 1.0 / r;
           s = s * c;
         }
-        g = d[i + 1] - p;
-        r = (d[i] - g) * s + 2.0 * c * b;
+        g = D.get(i + 1) - p;
+        r = (D.get(i) - g) * s + 2.0 * c * b;
         p = s * r;
-        double di1 = g + p;
+        const double di1 = g + p;
         g = c * r - b;
-        e[i + 1] = ei1;
-        d[i + 1] = di1;
+        E.set(i + 1, ei1, lane);
+        D.set(i + 1, di1, lane);
         for (int k = lane; k < n; k += RSB_WAVE) {
           double zk1 = z[k * n + i + 1], zk = z[k * n + i];
           z[k * n + i + 1] = s * zk + c * zk1;
           z[k * n + i] = c * zk - s * zk1;
         }
       }
-      d[l] = d[l] - p;
-      e[l] = g;
-      e[m] = 0.0;
+      D.set(l, D.get(l) - p, lane);
+      E.set(l, g, lane);
+      E.set(m, 0.0, lane);
     }
   }
   __syncthreads();  // rows of z written by other lanes are read below
   // eigenvalue of smallest magnitude (first on ties, fiedler.c:538-545)
   int imin = 0;
-  double emin = fabs(d[0]);
-  for (int i = 1; i < n; i++)
-    if (fabs(d[i]) < emin) emin = fabs(d[i]), imin = i;
+  double emin = fabs(D.get(0));
+  for (int i = 1; i < n; i++) {
+    const double v = fabs(D.get(i));
+    if (v < emin) emin = v, imin = i;
+  }
   if (failed) {
     // the reference leaves the rotations untransposed and unnormalised
     for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[imin * n + j];
@@ -449,10 +490,165 @@ __device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, dou
     double scale = e[0];
     for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[j * n + imin] * scale;
   }
-  if (lane == 0) *lam_out = d[imin];
+  const double lam = D.get(imin);
+  if (lane == 0) *lam_out = lam;
   __syncthreads();
 }
 #pragma clang fp contract(fast)
+
+// Fast variant for the FP32 path: the eigenpair of smallest |eigenvalue| of
+// the tridiagonal T directly, instead of all of them.  Eigenvalue by
+// multisection on Sturm counts (each of the 64 lanes counts the eigenvalues
+// below its own shift; 65 sub-intervals per step), eigenvector by inverse
+// iteration with the LDL^T factorisation of T - a I, a the lower end of the
+// final interval (Parlett, The Symmetric Eigenvalue Problem, 3.3 and 4.3;
+// the count recurrence with the pivot guard of LAPACK dstebz).  Sign: the
+// component along the first Lanczos vector (the pass's start) is made
+// non-negative.  It agrees with the QL solve to rounding, not bit for bit.
+__device__ inline int sturm_count(int n, const lanes2 &D, const lanes2 &E2, double x,
+                                  double pivmin) {
+  int c = 0;
+  double q = D.get(0) - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  c += q < 0.0;
+  for (int i = 1; i < n; i++) {
+    q = (D.get(i) - x) - E2.get(i - 1) / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// eigenvalue number j (ascending) in [lo, hi]; returns the final interval
+__device__ inline void multisect(int n, const lanes2 &D, const lanes2 &E2, double pivmin, int j,
+                                 double &lo, double &hi) {
+  const int lane = threadIdx.x;
+  for (int step = 0; step < 16; step++) {
+    const double w = hi - lo;
+    if (!(w > 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + pivmin)) break;
+    const double x = lo + w * (double)(lane + 1) / (double)(RSB_WAVE + 1);
+    const int c = sturm_count(n, D, E2, x, pivmin);
+    // lanes whose shift is at or below eigenvalue j: count <= j
+    const unsigned long long below = __ballot(c <= j);
+    // shifts increase with the lane: the highest such lane is the new lower
+    // end, the next lane the new upper end
+    const int k = below ? 63 - __clzll((long long)below) : -1;
+    const double xk = k >= 0 ? __shfl(x, k, RSB_WAVE) : lo;
+    const double xk1 = k + 1 < RSB_WAVE ? __shfl(x, k + 1, RSB_WAVE) : hi;
+    lo = xk, hi = xk1;
+  }
+}
+
+__device__ void eig_min_wave(int n, const double *d, const double *e, double *y,
+                             double *lam_out, double *work) {
+  const int lane = threadIdx.x;
+  lanes2 D, E2;
+  D.a = lane < n ? d[lane] : 0.0;
+  D.b = lane + RSB_WAVE < n ? d[lane + RSB_WAVE] : 0.0;
+  const double ea = lane < n - 1 ? e[lane] : 0.0, eb = lane + RSB_WAVE < n - 1 ? e[lane + RSB_WAVE] : 0.0;
+  E2.a = ea * ea, E2.b = eb * eb;
+  // Gershgorin bounds and the pivot guard
+  double glo = DBL_MAX, ghi = -DBL_MAX, emax = 0.0, tnorm = 0.0;
+  for (int i = lane; i < n; i += RSB_WAVE) {
+    const double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < n - 1 ? fabs(e[i]) : 0.0);
+    glo = fmin(glo, d[i] - r), ghi = fmax(ghi, d[i] + r);
+    tnorm = fmax(tnorm, fabs(d[i]) + r);
+    if (i < n - 1) emax = fmax(emax, e[i] * e[i]);
+  }
+  for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, o, RSB_WAVE));
+    ghi = fmax(ghi, __shfl_xor(ghi, o, RSB_WAVE));
+    emax = fmax(emax, __shfl_xor(emax, o, RSB_WAVE));
+    tnorm = fmax(tnorm, __shfl_xor(tnorm, o, RSB_WAVE));
+  }
+  const double pivmin = DBL_MIN * fmax(1.0, emax);
+  const double span = fmax(ghi - glo, DBL_MIN);
+  glo -= 2.0 * DBL_EPSILON * span, ghi += 2.0 * DBL_EPSILON * span;
+  // smallest eigenvalue; if it is negative, compare the eigenvalues either
+  // side of 0 by magnitude
+  double lo = glo, hi = ghi;
+  multisect(n, D, E2, pivmin, 0, lo, hi);
+  double lam = 0.5 * (lo + hi), shift = lo;
+  if (lam < 0.0) {
+    const int k = sturm_count(n, D, E2, 0.0, pivmin);
+    double l1 = glo, h1 = ghi;
+    multisect(n, D, E2, pivmin, k - 1, l1, h1);
+    lam = 0.5 * (l1 + h1), shift = l1;
+    if (k < n) {
+      double l2 = glo, h2 = ghi;
+      multisect(n, D, E2, pivmin, k, l2, h2);
+      const double lam2 = 0.5 * (l2 + h2);
+      if (fabs(lam2) < fabs(lam)) lam = lam2, shift = l2;
+    }
+  }
+  // inverse iteration (two solves) on lane 0: (T - shift I) = L D L^T, a
+  // pivot below eps * |T|_inf replaced by +-eps * |T|_inf; each solve is
+  // rescaled by its largest entry before normalising
+  if (lane == 0) {
+    double *piv = work, *x = work + n;
+    const double tau = DBL_EPSILON * fmax(tnorm, DBL_MIN);
+    double q = d[0] - shift;
+    if (fabs(q) < tau) q = q < 0.0 ? -tau : tau;
+    piv[0] = q;
+    for (int i = 1; i < n; i++) {
+      q = (d[i] - shift) - e[i - 1] * e[i - 1] / q;
+      if (fabs(q) < tau) q = q < 0.0 ? -tau : tau;
+      piv[i] = q;
+    }
+    for (int i = 0; i < n; i++) x[i] = 1.0;
+    for (int it = 0; it < 2; it++) {
+      // L z = x (unit lower, l_i = e_i / piv_i), D w = z, L^T v = w
+      for (int i = 1; i < n; i++) x[i] -= e[i - 1] / piv[i - 1] * x[i - 1];
+      for (int i = 0; i < n; i++) x[i] /= piv[i];
+      for (int i = n - 2; i >= 0; i--) x[i] -= e[i] / piv[i] * x[i + 1];
+      double amax = 0.0;
+      for (int i = 0; i < n; i++) amax = fmax(amax, fabs(x[i]));
+      if (!(amax > 0.0) || amax > DBL_MAX) {
+        for (int i = 0; i < n; i++) x[i] = i == 0 ? 1.0 : 0.0;
+        break;
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < n; i++) {
+        x[i] /= amax;
+        nrm += x[i] * x[i];
+      }
+      nrm = sqrt(nrm);
+      int f = 0;
+      while (f < n - 1 && x[f] == 0.0) f++;
+      const double sc = (x[f] < 0.0 ? -1.0 : 1.0) / nrm;
+      for (int i = 0; i < n; i++) x[i] *= sc;
+    }
+    *lam_out = lam;
+  }
+  __syncthreads();
+  for (int j = lane; j < n; j += RSB_WAVE) y[j] = work[n + j];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(RSB_WAVE) k_lz_eigmin(lz_args a) {
+  const int s = blockIdx.x;
+  if (a.status[s] != LZ_PASSDONE) return;
+  extern __shared__ double lds[];
+  const int n = a.k[s], M = a.miter_max;
+  double *d = lds, *e = d + M, *work = e + M;
+  const double *Td = a.T + (size_t)s * 2 * M, *Tu = Td + M;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) {
+    d[i] = Td[i];
+    e[i] = (i < n - 1) ? Tu[i] : 0.0;
+  }
+  __syncthreads();
+  eig_min_wave(n, d, e, a.y + (size_t)s * M, &SC(s, SC_LAMBDA), work);
+}
+
+__global__ void __launch_bounds__(RSB_WAVE) k_eigmin_test(int n, const double *dd,
+                                                          const double *ee, double *y,
+                                                          double *lam) {
+  extern __shared__ double lds[];
+  double *d = lds, *e = d + n, *work = e + n;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) d[i] = dd[i], e[i] = i < n - 1 ? ee[i] : 0.0;
+  __syncthreads();
+  eig_min_wave(n, d, e, y, lam, work);
+}
 
 __global__ void __launch_bounds__(RSB_WAVE) k_lz_tqli(lz_args a) {
   const int s = blockIdx.x;
@@ -888,7 +1084,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       }
       g->spmv_launches += nrec;
       RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
-      k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
+      if (sizeof(T) == 4)
+        k_lz_eigmin<<<nseg, RSB_WAVE, sizeof(double) * 4 * miter_max, st>>>(a);
+      else
+        k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
       RSB_LAUNCH_CHECK();
       k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
       RSB_LAUNCH_CHECK();
@@ -901,7 +1100,13 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     return 0;
   };
   int rc = 0;
-  if (init_mode == 2) {
+  // learned start: nothing to do if no segment is within the size limit
+  bool learned_any = false;
+  if (init_mode == 2)
+    for (int s = 0; s < nseg; s++)
+      if (g->h_p[s] > 1 && g->h_n[s] >= 2 && (g->lws_max_n == 0 || g->h_n[s] <= g->lws_max_n))
+        learned_any = true;
+  if (init_mode == 2 && learned_any) {
     // learned start (per row, written into w, which the passes overwrite
     // only after the start has been normalised into f), then the fallback
     float *srow = (float *)w;
@@ -915,7 +1120,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     RSB_TRY(hipStreamSynchronize(st));
     if (nfb > 0 && ((rc = start(0, nullptr)) || (rc = passes(mpass)))) return rc;
   } else {
-    if ((rc = start(init_mode, g->d_start)) || (rc = passes(mpass))) return rc;
+    if ((rc = start(init_mode == 2 ? 0 : init_mode, g->d_start)) || (rc = passes(mpass)))
+      return rc;
   }
   k_lz_finish<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, g->d_fied, xrow);
   RSB_LAUNCH_CHECK();
@@ -964,7 +1170,9 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       }
       if (out->resid) out->resid[s] = sc[(size_t)s * LZ_NSC + SC_TMP];
       if (out->fallback)
-        out->fallback[s] = scs[SC_FALLBACK] < 0.0   ? -1
+        out->fallback[s] = (init_mode == 2 && !learned_any && g->h_p[s] > 1 && g->h_n[s] >= 2) ||
+                                   scs[SC_FALLBACK] < 0.0
+                               ? -1
                            : scs[SC_FALLBACK] > 0.0 ? 1 + (int)scs[SC_WARMIT]
                                                     : 0;
     }
@@ -995,8 +1203,21 @@ extern "C" int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_pa
   return rc;
 }
 
+static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, double *lam,
+                            int fast);
+
 extern "C" int rsbgpu_test_tqli(const double *d, const double *e, int n, double *y,
                                 double *lam) {
+  return rsb_test_tridiag(d, e, n, y, lam, 0);
+}
+
+extern "C" int rsbgpu_test_eigmin(const double *d, const double *e, int n, double *y,
+                                  double *lam) {
+  return rsb_test_tridiag(d, e, n, y, lam, 1);
+}
+
+static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, double *lam,
+                            int fast) {
   double *dd = nullptr, *de = nullptr, *dy = nullptr, *dl = nullptr;
   if (n < 1 || n > 128) return 3;
   hipError_t err = hipMalloc((void **)&dd, sizeof(double) * n);
@@ -1008,8 +1229,12 @@ extern "C" int rsbgpu_test_tqli(const double *d, const double *e, int n, double 
   if (err == hipSuccess && n > 1)
     err = hipMemcpy(de, e, sizeof(double) * (n - 1), hipMemcpyHostToDevice);
   if (err == hipSuccess) {
-    size_t lds = sizeof(double) * (2 * n + n * n);
-    k_tqli_test<<<1, RSB_WAVE, lds, 0>>>(n, dd, de, dy, dl);
+    if (fast) {
+      k_eigmin_test<<<1, RSB_WAVE, sizeof(double) * 4 * n, 0>>>(n, dd, de, dy, dl);
+    } else {
+      size_t lds = sizeof(double) * (2 * n + n * n);
+      k_tqli_test<<<1, RSB_WAVE, lds, 0>>>(n, dd, de, dy, dl);
+    }
     err = hipGetLastError();
   }
   if (err == hipSuccess) err = hipDeviceSynchronize();

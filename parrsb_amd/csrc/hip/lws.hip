@@ -42,12 +42,15 @@ __device__ inline unsigned short f2bf(float f) {
 __device__ inline float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 // ---------------------------------------------------------------------------
-// C[M x N] (+)= A[M x K] B[K x N]: A and B BF16 row-major, FP32 accumulate on
-// MFMA 16x16x16.  One wavefront computes a 16 x N strip (N <= 128, K a
-// multiple of 16); B lives in LDS.  Lane layout of v_mfma_f32_16x16x16_bf16:
-// A: lane l holds row l%16, k = 4*(l/16)+0..3; B: lane l holds column l%16,
-// k = 4*(l/16)+0..3; C: lane l holds column l%16, rows 4*(l/16)+0..3.
-// Epilogue: +bias[n], optional relu, output BF16 or FP32.
+// C[M x N] = A[M x K] B[K x N] (+ bias, relu): A and B BF16 row-major, FP32
+// accumulate on MFMA 16x16x16.  A wavefront computes 16 x N strips (N <= 128,
+// K a multiple of 16); B is staged once per workgroup in LDS, transposed
+// (N x K) so that a lane's four k values of one column are one 8-byte read,
+// and the workgroup then walks strips.  Lane layout of
+// v_mfma_f32_16x16x16_bf16: A: lane l holds row l%16, k = 4*(l/16)+0..3;
+// B: lane l holds column l%16, same k; C: lane l holds column l%16, rows
+// 4*(l/16)+0..3.  Epilogue: +bias[n], optional relu, BF16 or FP32 out.
+#define LWS_GEMM_BLOCKS 2048
 template <int N, int K, bool RELU, bool OUT_BF16>
 __global__ void __launch_bounds__(256) k_gemm_rows(int M, const unsigned short *__restrict__ A0,
                                                    const unsigned short *__restrict__ A1,
@@ -55,43 +58,50 @@ __global__ void __launch_bounds__(256) k_gemm_rows(int M, const unsigned short *
                                                    const float *__restrict__ bias,
                                                    void *__restrict__ C, int ldc) {
   // A = [A0 | A1] when A1 is given (each lda wide), else A0 (K wide)
-  __shared__ unsigned short Bs[K * N];
-  for (int i = threadIdx.x; i < K * N; i += blockDim.x) Bs[i] = Bg[i];
+  __shared__ __align__(16) unsigned short Bt[N * K];
+  for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
+    const int k = i / N, n = i % N;
+    Bt[n * K + k] = Bg[i];
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row0 = (blockIdx.x * 4 + wave) * 16;
-  if (row0 >= M) return;
-  const int r = row0 + (lane & 15), kq = (lane >> 4) * 4;
-  f32x4 acc[N / 16];
+  const int kq = (lane >> 4) * 4;
+  float bc[N / 16];
 #pragma unroll
-  for (int nb = 0; nb < N / 16; nb++) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    bf16x4 a = (bf16x4){0, 0, 0, 0};
-    if (r < M) {
-      const unsigned short *ap = (A1 && k0 >= lda) ? A1 + (size_t)r * lda + (k0 - lda) + kq
-                                                   : A0 + (size_t)r * lda + k0 + kq;
-      a = (bf16x4){(short)ap[0], (short)ap[1], (short)ap[2], (short)ap[3]};
+  for (int nb = 0; nb < N / 16; nb++) bc[nb] = bias ? bias[nb * 16 + (lane & 15)] : 0.f;
+  const int nstrips = (M + 15) / 16;
+  for (int strip = blockIdx.x * 4 + wave; strip < nstrips; strip += gridDim.x * 4) {
+    const int row0 = strip * 16, r = row0 + (lane & 15);
+    f32x4 acc[N / 16];
+#pragma unroll
+    for (int nb = 0; nb < N / 16; nb++) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      bf16x4 a = (bf16x4){0, 0, 0, 0};
+      if (r < M) {
+        const unsigned short *ap = (A1 && k0 >= lda) ? A1 + (size_t)r * lda + (k0 - lda) + kq
+                                                     : A0 + (size_t)r * lda + k0 + kq;
+        a = *(const bf16x4 *)ap;
+      }
+#pragma unroll
+      for (int nb = 0; nb < N / 16; nb++) {
+        const int c = nb * 16 + (lane & 15);
+        const bf16x4 b = *(const bf16x4 *)&Bt[c * K + k0 + kq];
+        acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int nb = 0; nb < N / 16; nb++) {
       const int c = nb * 16 + (lane & 15);
-      bf16x4 b = (bf16x4){(short)Bs[(k0 + kq + 0) * N + c], (short)Bs[(k0 + kq + 1) * N + c],
-                          (short)Bs[(k0 + kq + 2) * N + c], (short)Bs[(k0 + kq + 3) * N + c]};
-      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nb], 0, 0, 0);
-    }
-  }
 #pragma unroll
-  for (int nb = 0; nb < N / 16; nb++) {
-    const int c = nb * 16 + (lane & 15);
-    const float bc = bias ? bias[c] : 0.f;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int rr = row0 + (lane >> 4) * 4 + i;
-      if (rr >= M) continue;
-      float v = acc[nb][i] + bc;
-      if (RELU) v = v > 0.f ? v : 0.f;
-      if (OUT_BF16) ((unsigned short *)C)[(size_t)rr * ldc + c] = f2bf(v);
-      else ((float *)C)[(size_t)rr * ldc + c] = v;
+      for (int i = 0; i < 4; i++) {
+        const int rr = row0 + kq + i;
+        if (rr >= M) continue;
+        float v = acc[nb][i] + bc[nb];
+        if (RELU) v = v > 0.f ? v : 0.f;
+        if (OUT_BF16) ((unsigned short *)C)[(size_t)rr * ldc + c] = f2bf(v);
+        else ((float *)C)[(size_t)rr * ldc + c] = v;
+      }
     }
   }
 }
@@ -137,19 +147,32 @@ __global__ void __launch_bounds__(256) k_gemm_wgrad(int M, int rows_per_block,
 }
 
 // ---------------------------------------------------------------------------
-// Features of every real row of the level.
+// Features of every real row of the level.  Bounding box per segment: the 64
+// rows of a wavefront belong to one segment (segments start 64-aligned), so
+// each wavefront reduces its rows first and issues one atomic per bound.
 __global__ void k_lws_bbox(const int *__restrict__ row_elem, const int *__restrict__ segof,
                            const double *__restrict__ coord, int nel, int64_t mpad, int ndim,
                            unsigned long long *__restrict__ box) {
-  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= mpad) return;
-  const int e = row_elem[row];
-  if (e < 0) return;
-  const int s = segof[e];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = row < mpad ? row_elem[row] : -1;
+  // segment of the wavefront: its first row is a real row unless the whole
+  // wavefront is padding past the last segment
+  const int e0 = __shfl(e, 0, RSB_WAVE);
+  if (e0 < 0) return;
+  const int s = segof[e0];
+  const int lane = threadIdx.x & (RSB_WAVE - 1);
   for (int d = 0; d < ndim; d++) {
-    const uint64_t k = dkey(coord[(size_t)d * nel + e]);
-    atomicMin(&box[(size_t)s * 6 + 2 * d], (unsigned long long)k);
-    atomicMax(&box[(size_t)s * 6 + 2 * d + 1], (unsigned long long)k);
+    uint64_t mn = ~0ull, mx = 0ull;
+    if (e >= 0) mn = mx = dkey(coord[(size_t)d * nel + e]);
+    for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
+      const uint64_t a = __shfl_xor(mn, o, RSB_WAVE), b = __shfl_xor(mx, o, RSB_WAVE);
+      mn = a < mn ? a : mn;
+      mx = b > mx ? b : mx;
+    }
+    if (lane == 0) {
+      atomicMin(&box[(size_t)s * 6 + 2 * d], (unsigned long long)mn);
+      atomicMax(&box[(size_t)s * 6 + 2 * d + 1], (unsigned long long)mx);
+    }
   }
 }
 
@@ -491,7 +514,7 @@ static int lws_forward(rsbgpu *g, const float *P, lws_ws &w, float level) {
     k_lws_agg<<<nblocks(M * 8, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell,
                                                    w.h[l], LWS_H, 1, 0, w.dinv, nullptr, 0,
                                                    w.a[l]);
-    k_gemm_rows<LWS_H, 2 * LWS_H, true, true><<<nblocks(M, 64), 256, 0, st>>>(
+    k_gemm_rows<LWS_H, 2 * LWS_H, true, true><<<std::min<unsigned>(nblocks(M, 64), LWS_GEMM_BLOCKS), 256, 0, st>>>(
         (int)M, w.h[l], w.a[l], LWS_H, w.wt + (size_t)l * 2 * LWS_H * LWS_H, P + LWS_BL(l),
         w.h[l + 1], LWS_H);
     RSB_LAUNCH_CHECK();
@@ -578,7 +601,7 @@ extern "C" int rsbgpu_lws_train_step(rsbgpu *g, const float *d_params, float *d_
     k_gemm_wgrad<LWS_H, 2 * LWS_H><<<nblocks(M, 1024), 256, 0, st>>>(
         (int)M, 1024, w.g0, LWS_H, w.h[l], w.a[l], LWS_H, d_grad + LWS_PL(l));
     // d[h|a] = dz W_l  (M x 128)
-    k_gemm_rows<2 * LWS_H, LWS_H, false, true><<<nblocks(M, 64), 256, 0, st>>>(
+    k_gemm_rows<2 * LWS_H, LWS_H, false, true><<<std::min<unsigned>(nblocks(M, 64), LWS_GEMM_BLOCKS), 256, 0, st>>>(
         (int)M, w.g0, nullptr, LWS_H, w.w + (size_t)l * 2 * LWS_H * LWS_H, nullptr, w.dha,
         2 * LWS_H);
     // dh_l = dha[:, :64] + sum_i w_ij dha[i, 64:] / d_i

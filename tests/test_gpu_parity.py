@@ -123,6 +123,49 @@ def test_tridiagonal_solver_bit_exact(oracle):
         assert np.array_equal(y, z[imin]), n
 
 
+def test_fast_tridiagonal_solver_matches_lapack():
+    """The FP32 path's eigen-solve (multisection + inverse iteration) against
+    LAPACK (scipy eigh_tridiagonal): smallest-|lambda| eigenpair to rounding,
+    sign y[0] >= 0; includes a zero off-diagonal (split matrix), a
+    Lanczos-like spectrum spread over 5 decades and a clustered pair."""
+    import scipy.linalg as sla
+    rng = np.random.default_rng(1)
+    cases = []
+    for n in [1, 2, 3, 7, 20, 50, 64, 100, 128]:
+        d = rng.uniform(0.1, 4, n)
+        e = -rng.uniform(0.01, 1.5, max(n - 1, 0))
+        if n > 10:
+            e[5] = 0.0
+        cases.append((d, e))
+    # T = Q^T diag(lam) Q tridiagonalised by Lanczos on a diagonal matrix
+    lam_true = np.concatenate([[1e-4, 1.3e-4], np.geomspace(1e-3, 10.0, 48)])
+    q = rng.standard_normal(50)
+    q /= np.linalg.norm(q)
+    Q = [q]
+    al, be = [], []
+    for j in range(50):
+        v = lam_true * Q[j]
+        al.append(Q[j] @ v)
+        v -= al[j] * Q[j] + (be[j - 1] * Q[j - 1] if j else 0)
+        for u in Q:
+            v -= (u @ v) * u
+        if j < 49:
+            be.append(np.linalg.norm(v))
+            Q.append(v / be[j])
+    cases.append((np.array(al), np.array(be)))
+    for d, e in cases:
+        n = d.size
+        lam, y = pa.test_tqli(d, e, fast=True)
+        w, V = sla.eigh_tridiagonal(d, e) if n > 1 else (d.copy(), np.ones((1, 1)))
+        i = int(np.argmin(np.abs(w)))
+        scale = np.abs(w).max()
+        assert abs(lam - w[i]) <= 1e-13 * scale, (n, lam, w[i])
+        v = V[:, i] * (1 if V[0, i] >= 0 else -1)
+        gap = np.min(np.abs(np.delete(w, i) - w[i])) if n > 1 else scale
+        assert np.abs(y - v).max() <= 1e-14 * scale / gap + 1e-12, n
+        assert y[0] >= 0
+
+
 def _segments_after_rcb(vtx, xyz, nv, p):
     d = pa.Device()
     d.set_mesh(vtx, xyz, nv)

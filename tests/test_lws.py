@@ -150,3 +150,43 @@ def test_warm_start_levels_report_fallback_iterations():
         # it after few iterations with a larger, still small, eigen-residual
         assert warm["niter"][0] < 50
         assert warm["resid"][0] < 2e-2
+
+
+@pytest.mark.gpu
+def test_trained_start_gains_on_small_segments():
+    """The committed network (parrsb_amd/lws_weights.bin, trained on the GPU
+    by parrsb_amd.lws_train) on a held-out warped pipe: at the levels whose
+    segments are within the size gate (<= 2048 elements) the learned start
+    needs fewer Lanczos iterations than the reference start; the levels
+    above the gate are solved exactly as without it; the edge cut stays
+    within 2%."""
+    import os
+    import parrsb_amd as pa
+    from parrsb_amd import lws_train
+    path = os.path.join(os.path.dirname(pa.__file__), "lws_weights.bin")
+    P = lws_train.load_params(path)
+    vtx, xyz, nv = M.pipe_hex(8, 64, 128, seed=4321, amp=0.12)
+    nparts = 64
+    pa.set_precision(32)
+    try:
+        pa.set_warm_start(None)
+        part0, _ = pa.part_mesh(vtx, xyz, nv, nparts)
+        r0 = pa.last_report()
+        pa.set_warm_start(P, 1, 2048)
+        part1, _ = pa.part_mesh(vtx, xyz, nv, nparts)
+        r1 = pa.last_report()
+    finally:
+        pa.set_warm_start(None)
+        pa.set_precision(64)
+    n = vtx.shape[0]
+    gated = [l for l in range(r0["nlevels"]) if n / 2 ** l <= 2048]
+    above = [l for l in range(r0["nlevels"]) if l not in gated]
+    assert gated, "no level within the gate"
+    for l in above:
+        assert r1["niter_mean"][l] == r0["niter_mean"][l]
+    plain = sum(r0["niter_mean"][l] for l in gated)
+    learned = sum(r1["niter_mean"][l] for l in gated)
+    print("gated levels", gated, "niter_mean plain", plain, "learned", learned)
+    assert learned <= 0.95 * plain
+    q0, q1 = pa.quality(part0, vtx, nv, nparts), pa.quality(part1, vtx, nv, nparts)
+    assert q1["edge_cut"] <= 1.02 * q0["edge_cut"]
