@@ -109,7 +109,71 @@ def cpu_baseline(cfg):
             "sample": sample, "seconds": dt}
 
 
+def summarize_traffic(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, desc,
+                      commands):
+    """HBM bytes per launch of k_lz_apply (and k_lz_rupdate) from two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each
+    scaled by what the counter reads for a copy kernel of known size."""
+    import csv
+
+    def per_kernel(path):
+        acc = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                name = r["Kernel_Name"]
+                v = float(r["Counter_Value"]) * 1024.0
+                s, n = acc.get(name, (0.0, 0))
+                acc[name] = (s + v, n + 1)
+        return acc
+
+    def copy_kib(path):
+        vals = [v / 1024.0 / n for k, (v, n) in per_kernel(path).items() if "copyBuffer" in k]
+        return max(vals)
+
+    gib_kib = 1024.0 * 1024.0
+    sf, sw = gib_kib / copy_kib(cal_fetch_csv), gib_kib / copy_kib(cal_write_csv)
+    fe, wr = per_kernel(fetch_csv), per_kernel(write_csv)
+
+    def kern(prefix):
+        kf = [k for k in fe if k.startswith(prefix)][0]
+        kw = [k for k in wr if k.startswith(prefix)][0]
+        f = fe[kf][0] / fe[kf][1] * sf
+        w = wr[kw][0] / wr[kw][1] * sw
+        return {"launches": fe[kf][1], "fetch_bytes_per_launch": f,
+                "write_bytes_per_launch": w, "bytes_per_launch": f + w}
+
+    ap_ = kern("void k_lz_apply<float>")
+    out = {"workload": desc, "kernel": "k_lz_apply<float>", **ap_,
+           "scale_fetch": sf, "scale_write": sw,
+           "calibration": "FETCH_SIZE and WRITE_SIZE of a 1 GiB device-to-device copy "
+                          "(__amd_rocclr_copyBuffer)",
+           "commands": commands, "rupdate": kern("void k_lz_rupdate<float>")}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-calibration":
+        # a 1 GiB device-to-device copy, three times: the reference reading
+        # of FETCH_SIZE / WRITE_SIZE for --summarize-traffic
+        import torch
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {4 * n} bytes x3", flush=True)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-traffic":
+        # bench.py --summarize-traffic OUT FETCH.csv WRITE.csv CALFETCH.csv CALWRITE.csv DESC
+        a = sys.argv[2:]
+        cmds = [f"rocprofv3 --pmc {c} --output-format csv -d <dir> -o c5 -- python3 -u bench.py "
+                "--steps 1 --warmup 0 --no-cpu --no-compare" for c in ("FETCH_SIZE", "WRITE_SIZE")]
+        print(json.dumps(summarize_traffic(a[0], a[1], a[2], a[3], a[4], a[5], cmds)))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)

@@ -258,6 +258,7 @@ struct tiled_args {
 
 #define TB_LOCAL 4096
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 template <typename T>
 __device__ inline void acc_entry(uint32_t e, T qi, const T *ql, T &sum) {
@@ -286,8 +287,9 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
   __syncthreads();
   for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
     const int li = row - t.row0, sl = row >> 6;
-    const u32x4 *ep = (const u32x4 *)(ta.e16 + ta.off16[sl] + (row & 63) * 8);
-    const int nb = ta.w16[sl] >> 3;
+    const uint16_t *sbase = ta.e16 + ta.off16[sl];
+    const u32x4 *ep = (const u32x4 *)(sbase + (row & 63) * 8);
+    const int w16 = ta.w16[sl], nb = w16 >> 3;
     const T qi = ql[li];
     T sum = (T)0;
     for (int jb = 0; jb < nb; jb++) {
@@ -300,6 +302,14 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
       acc_entry(v.z >> 16, qi, ql, sum);
       acc_entry(v.w & 0xffffu, qi, ql, sum);
       acc_entry(v.w >> 16, qi, ql, sum);
+    }
+    if (w16 & 4) {  // last group of 4 entries
+      const u32x2 v =
+          __builtin_nontemporal_load((const u32x2 *)(sbase + nb * 512 + (row & 63) * 4));
+      acc_entry(v.x & 0xffffu, qi, ql, sum);
+      acc_entry(v.x >> 16, qi, ql, sum);
+      acc_entry(v.y & 0xffffu, qi, ql, sum);
+      acc_entry(v.y >> 16, qi, ql, sum);
     }
     pnew[row] = qi;
     w[row] = sum;
@@ -782,9 +792,21 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   const int s = a.act_segs ? a.act_segs[blockIdx.x] : blockIdx.x;
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
+  // 8 tiles per thread in flight (large segments have up to 16k tiles), a
+  // fixed order of summation
   double tot[3] = {0.0, 0.0, 0.0};
-  for (int i = threadIdx.x; i < count; i += blockDim.x)
-    for (int k = 0; k < 3; k++) tot[k] += a.partial[(size_t)(first + i) * 4 + k];
+  const double2 *pp = (const double2 *)a.partial;
+  for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
+    double2 v0[8], v1[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * blockDim.x;
+      v0[u] = i < count ? pp[2 * (size_t)(first + i)] : make_double2(0.0, 0.0);
+      v1[u] = i < count ? pp[2 * (size_t)(first + i) + 1] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) tot[0] += v0[u].x, tot[1] += v0[u].y, tot[2] += v1[u].x;
+  }
   block_sum<3>(tot, red);
   if (threadIdx.x != 0) return;
   const double n = a.seg[a.nseg + s];

@@ -5,30 +5,54 @@
 // foreign segments among its neighbours (most elements emit none), the
 // (segment, foreign segment) pairs are sorted, and distinct pairs are counted
 // per segment.
+#include <algorithm>
+
 #include "state.hpp"
 
-// Distinct segments t != s among the row's neighbours; emit = false counts.
-template <bool EMIT>
+// Distinct segments t != s among the row's neighbours, appended as
+// (s << 32 | t) keys at an atomically taken offset (nothing is written past
+// `cap`; the caller compares the total with it).  The first NB_CACHE
+// neighbours' segments are kept in registers for the duplicate check.
+#define NB_CACHE 32
+__device__ inline int nb_seg(int64_t k, int64_t r0, const int (&seg)[NB_CACHE],
+                             const uint32_t *__restrict__ gadj, const int *__restrict__ segof) {
+  const int64_t kk = k - r0;
+  return kk < NB_CACHE ? seg[kk] : segof[gadj[k] & 0x0fffffffu];
+}
+
 __global__ void k_nb_pairs(int nel, const int64_t *__restrict__ growptr,
                            const uint32_t *__restrict__ gadj, const int *__restrict__ segof,
-                           int *__restrict__ cnt, const int64_t *__restrict__ off,
+                           unsigned long long *__restrict__ total, int64_t cap,
                            uint64_t *__restrict__ keys) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   const int s = segof[e];
   const int64_t r0 = growptr[e], r1 = growptr[e + 1];
-  int c = 0;
-  int64_t o = EMIT ? off[e] : 0;
-  for (int64_t k = r0; k < r1; k++) {
-    const int t = segof[gadj[k] & 0x0fffffffu];
-    if (t == s) continue;
-    bool seen = false;
-    for (int64_t j = r0; j < k && !seen; j++) seen = segof[gadj[j] & 0x0fffffffu] == t;
-    if (seen) continue;
-    if (EMIT) keys[o + c] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
-    c++;
+  int seg[NB_CACHE];
+  int foreign = 0;
+#pragma unroll
+  for (int k = 0; k < NB_CACHE; k++) {
+    seg[k] = r0 + k < r1 ? segof[gadj[r0 + k] & 0x0fffffffu] : s;
+    foreign += seg[k] != s;
   }
-  if (!EMIT) cnt[e] = c;
+  for (int64_t k = r0 + NB_CACHE; k < r1; k++) foreign += segof[gadj[k] & 0x0fffffffu] != s;
+  if (foreign == 0) return;
+  // pass 1 counts the distinct foreign segments, pass 2 writes them
+  unsigned long long o = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int c = 0;
+    for (int64_t k = r0; k < r1; k++) {
+      const int t = nb_seg(k, r0, seg, gadj, segof);
+      if (t == s) continue;
+      bool seen = false;
+      for (int64_t j = r0; j < k && !seen; j++) seen = nb_seg(j, r0, seg, gadj, segof) == t;
+      if (seen) continue;
+      if (pass == 1 && (int64_t)(o + c) < cap)
+        keys[o + c] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
+      c++;
+    }
+    if (pass == 0) o = atomicAdd(total, (unsigned long long)c);
+  }
 }
 
 __global__ void k_nb_count(int64_t n, const uint64_t *__restrict__ keys, int *__restrict__ nb) {
@@ -37,57 +61,49 @@ __global__ void k_nb_count(int64_t n, const uint64_t *__restrict__ keys, int *__
   if (i == 0 || keys[i] != keys[i - 1]) atomicAdd(&nb[keys[i] >> 32], 1);
 }
 
-__global__ void k_widen(int n, const int *__restrict__ in, int64_t *__restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = in[i];
-}
-
 extern "C" int rsbgpu_neighbors(rsbgpu *g, int *nbrs) {
   RSB_TRY(hipSetDevice(g->device));
   const int nel = g->nel, nseg = g->nseg;
   for (int s = 0; s < nseg; s++) nbrs[s] = 0;
   if (nel == 0 || nseg <= 1) return 0;
   hipStream_t st = g->stream;
-  int *cnt = nullptr, *nb = nullptr;
-  int64_t *off = nullptr, *wide = nullptr;
-  RSB_TRY(hipMalloc((void **)&cnt, sizeof(int) * (size_t)nel));
-  RSB_TRY(hipMalloc((void **)&off, sizeof(int64_t) * ((size_t)nel + 1)));
-  RSB_TRY(hipMalloc((void **)&wide, sizeof(int64_t) * (size_t)nel));
+  // one pass appends into a buffer of nel / 2 keys; if it overflows (or an
+  // element has more than NB_CACHE foreign segments) the exact count sizes
+  // a second pass
+  int64_t cap = std::max<int64_t>(nel / 2, 1024);
+  unsigned long long *total = nullptr;
+  uint64_t *keys = nullptr;
+  int *vals = nullptr, *nb = nullptr;
+  RSB_TRY(hipMalloc((void **)&total, sizeof(unsigned long long)));
   RSB_TRY(hipMalloc((void **)&nb, sizeof(int) * (size_t)nseg));
-  k_nb_pairs<false><<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_growptr, g->d_gadj, g->d_segof,
-                                                       cnt, nullptr, nullptr);
-  RSB_LAUNCH_CHECK();
-  // 64-bit offsets: widen the counts and scan them
-  k_widen<<<nblocks(nel, 256), 256, 0, st>>>(nel, cnt, wide);
-  RSB_LAUNCH_CHECK();
-  int rc = rsb_exclusive_scan_i64(g, wide, off, nel);
-  if (rc) return rc;
-  int64_t last_off = 0;
-  int last_cnt = 0;
-  RSB_TRY(hipMemcpyAsync(&last_off, off + nel - 1, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-  RSB_TRY(hipMemcpyAsync(&last_cnt, cnt + nel - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-  RSB_TRY(hipStreamSynchronize(st));
-  const int64_t npairs = last_off + last_cnt;
+  unsigned long long npairs = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    RSB_TRY(hipMalloc((void **)&keys, sizeof(uint64_t) * (size_t)cap));
+    RSB_TRY(hipMemsetAsync(total, 0, sizeof(unsigned long long), st));
+    k_nb_pairs<<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_growptr, g->d_gadj,
+                                                        g->d_segof, total, cap, keys);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipMemcpyAsync(&npairs, total, sizeof(npairs), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    if ((int64_t)npairs <= cap) break;
+    hipFree(keys);
+    keys = nullptr;
+    if (attempt == 1 || npairs >= (1ull << 31))
+      RSB_FAIL(2, "too many neighbour pairs (%llu)", (unsigned long long)npairs);
+    cap = (int64_t)npairs;
+  }
   RSB_TRY(hipMemsetAsync(nb, 0, sizeof(int) * (size_t)nseg, st));
   if (npairs > 0) {
-    if (npairs >= (int64_t)1 << 31) RSB_FAIL(2, "too many neighbour pairs (%lld)", (long long)npairs);
-    uint64_t *keys = nullptr;
-    int *vals = nullptr;
-    RSB_TRY(hipMalloc((void **)&keys, sizeof(uint64_t) * (size_t)npairs));
     RSB_TRY(hipMalloc((void **)&vals, sizeof(int) * (size_t)npairs));
-    k_nb_pairs<true><<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_growptr, g->d_gadj,
-                                                        g->d_segof, nullptr, off, keys);
-    RSB_LAUNCH_CHECK();
     int sb = 1;
     while ((1ll << sb) < nseg) sb++;
-    if ((rc = rsb_sort_pairs_u64(g, keys, vals, npairs, 0, 32 + sb))) return rc;
-    k_nb_count<<<nblocks(npairs, 256), 256, 0, st>>>(npairs, keys, nb);
+    int rc = rsb_sort_pairs_u64(g, keys, vals, (int64_t)npairs, 0, 32 + sb);
+    if (rc) return rc;
+    k_nb_count<<<nblocks((int64_t)npairs, 256), 256, 0, st>>>((int64_t)npairs, keys, nb);
     RSB_LAUNCH_CHECK();
-    RSB_TRY(hipStreamSynchronize(st));
-    hipFree(keys), hipFree(vals);
   }
   RSB_TRY(hipMemcpyAsync(nbrs, nb, sizeof(int) * (size_t)nseg, hipMemcpyDeviceToHost, st));
   RSB_TRY(hipStreamSynchronize(st));
-  hipFree(cnt), hipFree(off), hipFree(wide), hipFree(nb);
+  hipFree(total), hipFree(keys), hipFree(vals), hipFree(nb);
   return 0;
 }

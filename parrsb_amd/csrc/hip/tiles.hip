@@ -8,9 +8,11 @@
 // tile plus its halo may hold at most 4096 rows; a tile whose halo does not
 // fit is marked (nhalo = -1) and its rows use the global 32-bit form.
 //
-// Layout of the 16-bit entries: per 64-row slice, its width rounded up to a
-// multiple of 8; entry j of lane l at (j / 8) * 512 + l * 8 + (j % 8), so one
-// 16-byte load per lane fetches 8 consecutive entries of its row.
+// Layout of the 16-bit entries: per 64-row slice, its width w rounded up to
+// a multiple of 4; the first 8 * (w / 8) entries in groups of 8, entry j of
+// lane l at (j / 8) * 512 + l * 8 + (j % 8), so one 16-byte load per lane
+// fetches 8 consecutive entries of its row; a last group of 4, if any, at
+// (w / 8) * 512 + l * 4 + (j % 4), one 8-byte load per lane.
 #include <stdlib.h>
 
 #include <vector>
@@ -136,8 +138,8 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
   for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
     const int sl = row >> 6, lane = row & 63;
     const uint32_t *e = ell + slice_off[sl] + lane;
-    uint16_t *o = e16 + off16[sl] + lane * 8;
-    const int w = slice_w[sl], w8 = w16[sl];
+    uint16_t *o = e16 + off16[sl];
+    const int w = slice_w[sl], w8 = w16[sl], full = w8 & ~7;
     const uint16_t self = (uint16_t)(row - row0);
     for (int j = 0; j < w8; j++) {
       uint16_t v = self;  // padding: own row, weight 0
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
           if (li >= 0) v = (uint16_t)(li | (wt << 12));
         }
       }
-      o[(j >> 3) * 512 + (j & 7)] = v;
+      o[j < full ? (j >> 3) * 512 + lane * 8 + (j & 7) : full * 64 + lane * 4 + (j - full)] = v;
     }
   }
 }
@@ -164,7 +166,7 @@ __global__ void k_w16(int64_t nslices, const int *__restrict__ w, int *__restric
                       int64_t *__restrict__ bytes) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nslices) {
-    int v = (w[i] + 7) / 8 * 8;
+    int v = (w[i] + 3) / 4 * 4;
     w8[i] = v;
     bytes[i] = (int64_t)v * RSB_WAVE;
   }
