@@ -82,7 +82,7 @@ int parrsb_amd_set_profiling(int on);
  * the reference start.  Larger segments take the reference start.
  * PARRSB_AMD_WARM_START=<file> loads parameters from a file written by
  * parrsb_amd.lws_train (PARRSB_AMD_WARM_PASSES and PARRSB_AMD_WARM_MAX_N set
- * `passes` and `max_n`, default 1 and 2048). */
+ * `passes` and `max_n`, default 1 and 1024). */
 int parrsb_amd_warm_start_nparams(void);
 int parrsb_amd_set_warm_start(const float *params, int passes, int max_n);
 

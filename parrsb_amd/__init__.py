@@ -385,8 +385,9 @@ class Device:
 
 # Largest segment the learned start is tried on by default (elements): above
 # it the reference start needs fewer iterations than the network's
-# prediction under the reference's stop rule.
-WARM_MAX_N = 2048
+# prediction under the reference's stop rule (at 2048 elements a few percent
+# of the segments already fall back).
+WARM_MAX_N = 1024
 
 
 def warm_start_nparams() -> int:

@@ -45,7 +45,7 @@ static parrsb_amd_report last_report;
 static char last_error[512];
 static int cfg_precision = 64, cfg_profiling = 0;
 static float *cfg_warm = NULL; /* host copy of the learned-start parameters */
-static int cfg_warm_passes = 1, cfg_warm_max_n = 2048;
+static int cfg_warm_passes = 1, cfg_warm_max_n = 1024;
 static char cfg_warm_file[512];
 
 const char *parrsb_amd_last_error(void) { return last_error; }
