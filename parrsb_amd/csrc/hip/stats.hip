@@ -11,15 +11,10 @@
 
 // Distinct segments t != s among the row's neighbours, appended as
 // (s << 32 | t) keys at an atomically taken offset (nothing is written past
-// `cap`; the caller compares the total with it).  The first NB_CACHE
-// neighbours' segments are kept in registers for the duplicate check.
-#define NB_CACHE 32
-__device__ inline int nb_seg(int64_t k, int64_t r0, const int (&seg)[NB_CACHE],
-                             const uint32_t *__restrict__ gadj, const int *__restrict__ segof) {
-  const int64_t kk = k - r0;
-  return kk < NB_CACHE ? seg[kk] : segof[gadj[k] & 0x0fffffffu];
-}
-
+// `cap`; the caller compares the total with it).  The distinct values found
+// so far (at most NB_FOUND, the usual case by far) live in registers; a row
+// with more re-checks against its own prefix.
+#define NB_FOUND 8
 __global__ void k_nb_pairs(int nel, const int64_t *__restrict__ growptr,
                            const uint32_t *__restrict__ gadj, const int *__restrict__ segof,
                            unsigned long long *__restrict__ total, int64_t cap,
@@ -28,24 +23,43 @@ __global__ void k_nb_pairs(int nel, const int64_t *__restrict__ growptr,
   if (e >= nel) return;
   const int s = segof[e];
   const int64_t r0 = growptr[e], r1 = growptr[e + 1];
-  int seg[NB_CACHE];
-  int foreign = 0;
+  int found[NB_FOUND];
+  int nf = 0;
+  bool many = false;
+  for (int64_t k = r0; k < r1; k++) {
+    const int t = segof[gadj[k] & 0x0fffffffu];
+    if (t == s) continue;
+    bool seen = false;
 #pragma unroll
-  for (int k = 0; k < NB_CACHE; k++) {
-    seg[k] = r0 + k < r1 ? segof[gadj[r0 + k] & 0x0fffffffu] : s;
-    foreign += seg[k] != s;
+    for (int j = 0; j < NB_FOUND; j++) seen |= j < nf && found[j] == t;
+    if (seen) continue;
+    if (nf < NB_FOUND) {
+#pragma unroll
+      for (int j = 0; j < NB_FOUND; j++)
+        if (j == nf) found[j] = t;
+      nf++;
+    } else {
+      many = true;
+      break;
+    }
   }
-  for (int64_t k = r0 + NB_CACHE; k < r1; k++) foreign += segof[gadj[k] & 0x0fffffffu] != s;
-  if (foreign == 0) return;
-  // pass 1 counts the distinct foreign segments, pass 2 writes them
+  if (nf == 0) return;
+  if (!many) {
+    const unsigned long long o = atomicAdd(total, (unsigned long long)nf);
+#pragma unroll
+    for (int j = 0; j < NB_FOUND; j++)
+      if (j < nf && (int64_t)(o + j) < cap) keys[o + j] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)found[j];
+    return;
+  }
+  // more than NB_FOUND foreign segments: count, then write, against the prefix
   unsigned long long o = 0;
   for (int pass = 0; pass < 2; pass++) {
     int c = 0;
     for (int64_t k = r0; k < r1; k++) {
-      const int t = nb_seg(k, r0, seg, gadj, segof);
+      const int t = segof[gadj[k] & 0x0fffffffu];
       if (t == s) continue;
       bool seen = false;
-      for (int64_t j = r0; j < k && !seen; j++) seen = nb_seg(j, r0, seg, gadj, segof) == t;
+      for (int64_t j = r0; j < k && !seen; j++) seen = segof[gadj[j] & 0x0fffffffu] == t;
       if (seen) continue;
       if (pass == 1 && (int64_t)(o + c) < cap)
         keys[o + c] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
