@@ -59,6 +59,14 @@ typedef struct {
 int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *d_xyz,
                              int nel, int nv, parrsb_options *options);
 
+/* parrsb_part_mesh that also returns seq[i], the order of element i within
+ * part[i] after partitioning (the part/seq form of the reference README,
+ * README.md:46-66): the element's position in its part's final order (for
+ * RSB the Fiedler order of the last cut). */
+int parrsb_part_mesh_seq(int *part, int *seq, const long long *vtx, const double *xyz,
+                         const int *tag, int nel, int nv, parrsb_options *options,
+                         MPI_Comm comm);
+
 /* Copy the report of the last call made in this process. */
 int parrsb_amd_last_report(parrsb_amd_report *rep);
 

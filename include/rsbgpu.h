@@ -112,6 +112,9 @@ int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const int *scount,
 /* part[e] = r0 of the segment holding e.  part_is_device selects whether
  * `part` is a device pointer. */
 int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device);
+/* seq[e] = position of e within its segment in the current order (after the
+ * last cut: the order of the element within its part). */
+int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device);
 
 /* Copy the current order / Fiedler values (per compact position) to the
  * host; used by the parity tests. */

@@ -94,6 +94,7 @@ SIGNATURES = {
     "parrsb_amd_set_precision": (I, [I]),
     "parrsb_amd_set_profiling": (I, [I]),
     "parrsb_amd_warm_start_nparams": (I, []),
+    "parrsb_part_mesh_seq": (I, [P, P, P, P, P, I, I, C.POINTER(Options), I]),
     "parrsb_amd_set_warm_start": (I, [P, I, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
     "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
@@ -126,6 +127,7 @@ SIGNATURES = {
     "rsbgpu_upload": (I, [P, P, C.c_size_t, C.POINTER(P)]),
     "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
     "rsbgpu_neighbors": (I, [P, P]),
+    "rsbgpu_write_seq": (I, [P, P, I]),
     "rsbgpu_lws_nparams": (I, []),
     "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
@@ -202,6 +204,25 @@ def part_mesh(vtx, xyz, nv, nparts, tag=None, comm=0, options=None, **kw):
     if rc:
         raise ParrsbError(rc, last_error())
     return part, opts
+
+
+def part_mesh_seq(vtx, xyz, nv, nparts, tag=None, comm=0, options=None, **kw):
+    """parrsb_part_mesh_seq: (part, seq, options_after_call)."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64) if vtx is not None else None
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64) if xyz is not None else None
+    nel = (vtx.size if vtx is not None else xyz.size // (3 if nv == 8 else 2)) // nv
+    opts = options if options is not None else default_options(verbose_level=0)
+    opts.nparts = nparts
+    for k, v in kw.items():
+        setattr(opts, k, v)
+    part = np.full(nel, -1, dtype=np.int32)
+    seq = np.full(nel, -1, dtype=np.int32)
+    tg = np.ascontiguousarray(tag, dtype=np.int32) if tag is not None else None
+    rc = lib().parrsb_part_mesh_seq(_p(part), _p(seq), _p(vtx), _p(xyz), _p(tg), nel, nv,
+                                    C.byref(opts), comm)
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return part, seq, opts
 
 
 def part_mesh_dev(d_part, d_vtx, d_xyz, nel, nv, nparts, options=None, **kw):

@@ -347,6 +347,34 @@ extern "C" int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device) {
   return 0;
 }
 
+// seq[e] = position of element e within its final segment (its part), in the
+// final element order.
+__global__ void k_write_seq(const int *__restrict__ order, const int *__restrict__ segof,
+                            const int *__restrict__ seg, int nel, int *__restrict__ seq) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  const int e = order[i];
+  seq[e] = i - seg[segof[e]];
+}
+
+extern "C" int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel;
+  if (nel == 0) return 0;
+  int *dp = seq;
+  if (!seq_is_device) RSB_TRY(hipMalloc((void **)&dp, sizeof(int) * nel));
+  k_write_seq<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nel, dp);
+  RSB_LAUNCH_CHECK();
+  if (!seq_is_device) {
+    RSB_TRY(hipMemcpyAsync(seq, dp, sizeof(int) * nel, hipMemcpyDeviceToHost, g->stream));
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    hipFree(dp);
+  } else {
+    RSB_TRY(hipStreamSynchronize(g->stream));
+  }
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Partition quality on the global dual graph.
 // Part ids outside [0, nparts) are reported (bad) and left out.

@@ -223,7 +223,7 @@ static void print_profile(const parrsb_amd_report *r) {
 }
 
 /* Shared body of the host- and device-pointer entry points. */
-static int run(int *part, const long long *vtx, const double *xyz, int nel, int nv,
+static int run(int *part, int *seq, const long long *vtx, const double *xyz, int nel, int nv,
                parrsb_options *options, int nparts, int on_device) {
   double t0 = now();
   memset(&last_report, 0, sizeof(last_report));
@@ -295,10 +295,12 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
       }
       if (!rc) rc = rsb_run(g, nel, &prm, rep);
       if (!rc) rc = rsbgpu_write_part(g, part, on_device);
+      if (!rc && seq) rc = rsbgpu_write_seq(g, seq, on_device);
       rsbgpu_free(dw);
     }
   }
   if (!rc && options->partitioner != 0) rc = rsbgpu_write_part(g, part, on_device);
+  if (!rc && options->partitioner != 0 && seq) rc = rsbgpu_write_seq(g, seq, on_device);
   if (rc) set_error("%s", rsbgpu_error(g));
   rsbgpu_phase_times(g, rep->phase_ms);
   double tr[6];
@@ -318,9 +320,9 @@ static int run(int *part, const long long *vtx, const double *xyz, int nel, int 
   return 0;
 }
 
-int parrsb_part_mesh(int *part, const long long *const vtx, const double *const xyz,
-                     const int *const tag, const int nel, const int nv,
-                     parrsb_options *const options, MPI_Comm comm) {
+static int part_mesh_host(int *part, int *seq, const long long *const vtx,
+                          const double *const xyz, const int *const tag, const int nel,
+                          const int nv, parrsb_options *const options, MPI_Comm comm) {
   update_options(options);
   int np = 1;
   if (MPI_Initialized && MPI_Comm_size) {
@@ -346,7 +348,25 @@ int parrsb_part_mesh(int *part, const long long *const vtx, const double *const 
     return 3;
   }
   int nparts = options->nparts > 0 ? options->nparts : np;
-  return run(part, vtx, xyz, nel, nv, options, nparts, 0);
+  return run(part, seq, vtx, xyz, nel, nv, options, nparts, 0);
+}
+
+int parrsb_part_mesh(int *part, const long long *const vtx, const double *const xyz,
+                     const int *const tag, const int nel, const int nv,
+                     parrsb_options *const options, MPI_Comm comm) {
+  return part_mesh_host(part, NULL, vtx, xyz, tag, nel, nv, options, comm);
+}
+
+/* part + seq form of README.md:46-66: seq[i] = order of element i within
+ * part[i] after partitioning (its position in the final in-part order). */
+int parrsb_part_mesh_seq(int *part, int *seq, const long long *vtx, const double *xyz,
+                         const int *tag, int nel, int nv, parrsb_options *options,
+                         MPI_Comm comm) {
+  if (!seq) {
+    set_error("seq is NULL");
+    return 3;
+  }
+  return part_mesh_host(part, seq, vtx, xyz, tag, nel, nv, options, comm);
 }
 
 int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *d_xyz,
@@ -356,7 +376,7 @@ int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *
     set_error("tagged partitioning is not provided by this build");
     return 3;
   }
-  return run(d_part, d_vtx, d_xyz, nel, nv, options, options->nparts, 1);
+  return run(d_part, NULL, d_vtx, d_xyz, nel, nv, options, options->nparts, 1);
 }
 
 static int quality(const int *part, const long long *vtx, int nel, int nv, int nparts,

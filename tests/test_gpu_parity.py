@@ -385,3 +385,30 @@ def test_neighbors_report_16cube():
     r = pa.last_report()
     assert r["nbrs_min"] == r["nbrs_max"] == [1, 3, 7]
     assert r["nbrs_mean"] == [1.0, 3.0, 7.0]
+
+
+def test_part_mesh_seq_is_order_within_part(oracle):
+    """parrsb_part_mesh_seq: seq numbers each part's elements 0..size-1; with
+    the RCB partitioner it is the position inside the rank's range of the
+    RCB order (oracle), and RSB gives the same part[] as parrsb_part_mesh."""
+    vtx, xyz, nv = M.box_hex(10, 9, 8, seed=2)
+    n = vtx.shape[0]
+    P = 6
+    part, seq, _ = pa.part_mesh_seq(vtx, xyz, nv, P, partitioner=1)
+    order = oracle.rcb_order(oracle.centroids(xyz, nv), 3, np.arange(n, dtype=np.int32), P)
+    ref = np.zeros(n, dtype=np.int32)
+
+    def ranges(start, length, p):
+        if p == 1:
+            ref[order[start:start + length]] = np.arange(length)
+            return
+        l = oracle.lower_count(length, p)
+        ranges(start, l, (p + 1) // 2)
+        ranges(start + l, length - l, p // 2)
+    ranges(0, n, P)
+    assert np.array_equal(seq, ref)
+    part2, seq2, _ = pa.part_mesh_seq(vtx, xyz, nv, P)
+    part3, _ = pa.part_mesh(vtx, xyz, nv, P)
+    assert np.array_equal(part2, part3)
+    for q in range(P):
+        assert np.array_equal(np.sort(seq2[part2 == q]), np.arange(np.count_nonzero(part2 == q)))
