@@ -83,6 +83,20 @@ int parrsb_part_mesh(int *part, const long long *const vtx,
                      const int nel, const int nv, parrsb_options *const options,
                      MPI_Comm comm);
 
+/*
+ * Global vertex ids from corner coordinates (parRSB.h:62-64 of the
+ * reference, con.c:189-295): coord[nel][nv][nDim] in the preprocessor corner
+ * order (nv = 8 for nDim = 3, 4 for nDim = 2) -> vtx[nel][nv], 1-based ids,
+ * symmetric corner order.  Corners closer than tol times their element's
+ * shortest incident edge share an id.  Returns 0, 1 if the element or face
+ * check of the result fails (con-check.c), 2 on a device failure, 3 for
+ * periodic faces (nPeriodicFaces > 0, not provided by this build), a
+ * communicator of more than one rank, or bad sizes.
+ */
+int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
+                     long long *periodicInfo, int nPeriodicFaces, double tol,
+                     MPI_Comm comm);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,10 @@ int parrsb_part_mesh_seq(int *part, int *seq, const long long *vtx, const double
                          const int *tag, int nel, int nv, parrsb_options *options,
                          MPI_Comm comm);
 
+/* parrsb_conn_mesh with device arrays: d_vtx[nel*nv] out, d_coord in. */
+int parrsb_amd_conn_mesh_dev(long long *d_vtx, const double *d_coord, int nel, int ndim,
+                             double tol);
+
 /* Copy the report of the last call made in this process. */
 int parrsb_amd_last_report(parrsb_amd_report *rep);
 

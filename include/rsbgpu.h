@@ -166,6 +166,13 @@ int rsbgpu_lws_test(int M, const double *coord, float level, const int *rowptr, 
  * (statistics.c:17-113), nbrs[nseg] on the host. */
 int rsbgpu_neighbors(rsbgpu *g, int *nbrs);
 
+/* Connectivity (parrsb_conn_mesh): vtx_out[nel*nv] 1-based ids in the
+ * symmetric corner order from coord[nel*nv*ndim] (preprocessor order), on
+ * device `device`; host or device arrays.  0, 1 (element/face check failed),
+ * 2 (device failure, message in msg), 3 (bad input). */
+int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *coord, int nel, int ndim,
+                     double tol, int on_device, char *msg, int msg_len);
+
 /* Small helpers for callers holding host data: copy n ints (or `bytes`
  * bytes) to a new device buffer (freed with rsbgpu_free). */
 int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out);

@@ -958,3 +958,116 @@ void oracle_quality(const int *part, const long long *vtx, int nel, int nv, int 
   out[6] = vol, out[7] = empty;
   free(rowptr), free(col), free(wgt), free(par), free(sz), free(nc), free(a);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Connectivity: global vertex ids from corner coordinates (con.c:189-295,
+ * con-unique-vertices.c:445-505, con-check.c).  One process: every corner
+ * point starts in one segment; ndim times over the ndim axes, each segment
+ * is sorted by that coordinate and split wherever consecutive points differ
+ * by more than tol^2 * min(dx) in squared distance, dx being the point's
+ * smallest squared distance to the corners joined to it by an element edge
+ * (con.c:70-101).  A point's id is the index of its final segment in the
+ * final layout (con.c:120-145).  The reference sorts segments with a heap
+ * sort (ties in unspecified order); this restatement sorts stably. */
+static const int CON_PRE_TO_SYM[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+static const int CON_NEIGHBOR[8][3] = {{1, 2, 4}, {0, 3, 5}, {0, 3, 6}, {1, 2, 7},
+                                       {0, 5, 6}, {1, 4, 7}, {2, 4, 7}, {3, 5, 6}};
+static const int CON_FACES3D[6][4] = {{1, 5, 7, 3}, {2, 4, 8, 6}, {1, 2, 6, 5},
+                                      {3, 7, 8, 4}, {1, 3, 4, 2}, {5, 6, 8, 7}};
+
+int oracle_conn_mesh(long long *vtx, const double *coord, int nel, int ndim, double tol) {
+  const int nv = ndim == 3 ? 8 : 4;
+  const int64_t np = (int64_t)nel * nv;
+  double *x = xmalloc(sizeof(double) * 3 * (size_t)(np ? np : 1));
+  double *dx = xmalloc(sizeof(double) * (size_t)(np ? np : 1));
+  for (int e = 0; e < nel; e++)
+    for (int k = 0; k < nv; k++) {
+      const int j = CON_PRE_TO_SYM[k];
+      for (int d = 0; d < 3; d++)
+        x[3 * ((int64_t)e * nv + k) + d] = d < ndim ? coord[((int64_t)e * nv + j) * ndim + d] : 0.0;
+    }
+  for (int e = 0; e < nel; e++)
+    for (int k = 0; k < nv; k++) {
+      const double *a = x + 3 * ((int64_t)e * nv + k);
+      double m = DBL_MAX;
+      for (int q = 0; q < ndim; q++) {
+        const double *b = x + 3 * ((int64_t)e * nv + CON_NEIGHBOR[k][q]);
+        double s = 0.0;
+        for (int d = 0; d < ndim; d++) s += (a[d] - b[d]) * (a[d] - b[d]);
+        m = s < m ? s : m;
+      }
+      dx[(int64_t)e * nv + k] = m;
+    }
+  int *lay = xmalloc(sizeof(int) * (size_t)(np ? np : 1));   /* layout -> point */
+  int *seg = xmalloc(sizeof(int) * (size_t)(np ? np : 1));   /* layout -> segment */
+  for (int64_t i = 0; i < np; i++) lay[i] = (int)i, seg[i] = 0;
+  const double tol2 = tol * tol;
+  double *key = xmalloc(sizeof(double) * (size_t)(np ? np : 1));
+  for (int t = 0; t < ndim; t++)
+    for (int d = 0; d < ndim; d++) {
+      /* sort every segment by x[d] (segments are contiguous) */
+      int64_t s0 = 0;
+      while (s0 < np) {
+        int64_t s1 = s0 + 1;
+        while (s1 < np && seg[s1] == seg[s0]) s1++;
+        for (int64_t i = s0; i < s1; i++) key[i] = x[3 * (int64_t)lay[i] + d];
+        stable_sort_by_pos_key(lay + s0, key + s0, (int)(s1 - s0));
+        s0 = s1;
+      }
+      /* split and renumber in layout order */
+      int cur = 0;
+      int prev_old = np ? seg[0] : 0;
+      for (int64_t i = 1; i < np; i++) {
+        const int pa = lay[i - 1], pb = lay[i];
+        const double df = x[3 * (int64_t)pb + d] - x[3 * (int64_t)pa + d];
+        const double lim = (dx[pb] < dx[pa] ? dx[pb] : dx[pa]) * tol2;
+        const int old = seg[i];
+        if (old != prev_old || df * df > lim) cur++;
+        prev_old = old;
+        seg[i] = cur;
+      }
+    }
+  int64_t *gid = xmalloc(sizeof(int64_t) * (size_t)(np ? np : 1));
+  for (int64_t i = 0; i < np; i++) gid[lay[i]] = seg[i];
+  for (int64_t p = 0; p < np; p++) vtx[p] = gid[p] + 1;
+  free(x), free(dx), free(lay), free(seg), free(key), free(gid);
+  /* element check (con-check.c:262-288): no repeated id in an element */
+  for (int e = 0; e < nel; e++)
+    for (int a = 0; a < nv; a++)
+      for (int b = a + 1; b < nv; b++)
+        if (vtx[(int64_t)e * nv + a] == vtx[(int64_t)e * nv + b]) return 1;
+  /* face check (con-check.c:197-260): no element holds exactly 3 of the 4
+     corners of a face (quads: faces have 2 corners, never 3) */
+  if (ndim == 3) {
+    /* vertex -> elements holding it */
+    int64_t nvid = 0;
+    for (int64_t p = 0; p < np; p++) nvid = vtx[p] > nvid ? vtx[p] : nvid;
+    int64_t *cnt = calloc((size_t)nvid + 2, sizeof(int64_t));
+    for (int64_t p = 0; p < np; p++) cnt[vtx[p] + 1]++;
+    for (int64_t v = 0; v <= nvid; v++) cnt[v + 1] += cnt[v];
+    int *els = xmalloc(sizeof(int) * (size_t)(np ? np : 1));
+    int64_t *fill = xmalloc(sizeof(int64_t) * (size_t)(nvid + 1));
+    for (int64_t v = 0; v <= nvid; v++) fill[v] = cnt[v];
+    for (int64_t p = 0; p < np; p++) els[fill[vtx[p]]++] = (int)(p / nv);
+    int err = 0;
+    for (int e = 0; e < nel && !err; e++)
+      for (int f = 0; f < 6 && !err; f++) {
+        /* elements of the face's corners, with multiplicity per corner */
+        for (int c = 0; c < 4 && !err; c++) {
+          const long long v = vtx[(int64_t)e * nv + CON_FACES3D[f][c] - 1];
+          for (int64_t l = cnt[v]; l < cnt[v + 1] && !err; l++) {
+            const int el = els[l];
+            int m = 0;
+            for (int c2 = 0; c2 < 4; c2++) {
+              const long long v2 = vtx[(int64_t)e * nv + CON_FACES3D[f][c2] - 1];
+              for (int64_t l2 = cnt[v2]; l2 < cnt[v2 + 1]; l2++) m += els[l2] == el;
+            }
+            if (m == 3) err = 1;
+          }
+        }
+      }
+    free(cnt), free(els), free(fill);
+    if (err) return 1;
+  }
+  return 0;
+}

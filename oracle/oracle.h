@@ -133,6 +133,12 @@ int oracle_components(const long long *vtx, int nv, const int *elems, int n);
 int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel,
                      int nv, const oracle_opts *opts, oracle_report *rep);
 
+/* Connectivity (parrsb_conn_mesh, con.c:189-295): vtx[nel*nv] (symmetric
+ * corner order, 1-based ids) from coord[nel][nv][ndim] (preprocessor corner
+ * order) with relative tolerance tol; returns 1 if the element or face check
+ * fails (con-check.c), 0 otherwise. */
+int oracle_conn_mesh(long long *vtx, const double *coord, int nel, int ndim, double tol);
+
 /* Partition quality.  out[0] weighted edge cut (sum over undirected edges of
  * shared-vertex weight), out[1] unweighted cut edges, out[2] max part size,
  * out[3] min part size, out[4] total components over parts, out[5] max

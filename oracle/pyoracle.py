@@ -61,6 +61,7 @@ P = C.c_void_p
 def _declare(L):
     L.oracle_centroids.argtypes = [P, C.c_int, C.c_int, P]
     L.oracle_rib_order.argtypes = [P, C.c_int, P, C.c_int, C.c_int]
+    L.oracle_conn_mesh.argtypes = [P, P, C.c_int, C.c_int, C.c_double]
     L.oracle_rib_axis.argtypes = [P, C.c_int, P, C.c_int, P, P]
     L.oracle_sym_dominant.argtypes = [P, C.c_int, P]
     L.oracle_rank_count.argtypes = [C.c_int64, C.c_int, C.c_int]
@@ -146,6 +147,16 @@ def sym_dominant(A, n):
     ev = np.zeros(3)
     lib().oracle_sym_dominant(_p(np.ascontiguousarray(A3)), n, _p(ev))
     return ev[:n]
+
+
+def conn_mesh(xyz, ndim, tol=0.2):
+    """(rc, vtx[nel, nv]) from per-corner coordinates xyz[nel, nv, ndim]."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nv = 8 if ndim == 3 else 4
+    nel = xyz.size // (nv * ndim)
+    vtx = np.zeros((nel, nv), dtype=np.int64)
+    rc = lib().oracle_conn_mesh(_p(vtx), _p(xyz), nel, ndim, tol)
+    return rc, vtx
 
 
 def dual_graph(vtx, nv):

@@ -95,6 +95,8 @@ SIGNATURES = {
     "parrsb_amd_set_profiling": (I, [I]),
     "parrsb_amd_warm_start_nparams": (I, []),
     "parrsb_part_mesh_seq": (I, [P, P, P, P, P, I, I, C.POINTER(Options), I]),
+    "parrsb_conn_mesh": (I, [P, P, C.c_uint, C.c_uint, P, I, C.c_double, I]),
+    "parrsb_amd_conn_mesh_dev": (I, [P, P, I, I, C.c_double]),
     "parrsb_amd_set_warm_start": (I, [P, I, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
     "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
@@ -128,6 +130,7 @@ SIGNATURES = {
     "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
     "rsbgpu_neighbors": (I, [P, P]),
     "rsbgpu_write_seq": (I, [P, P, I]),
+    "rsbgpu_conn_mesh": (I, [I, P, P, I, I, C.c_double, I, C.c_char_p, I]),
     "rsbgpu_lws_nparams": (I, []),
     "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
@@ -223,6 +226,19 @@ def part_mesh_seq(vtx, xyz, nv, nparts, tag=None, comm=0, options=None, **kw):
     if rc:
         raise ParrsbError(rc, last_error())
     return part, seq, opts
+
+
+def conn_mesh(xyz, ndim, tol=0.2, comm=0):
+    """parrsb_conn_mesh: vtx[nel, nv] (1-based ids, symmetric corner order)
+    from per-corner coordinates xyz[nel, nv, ndim] (preprocessor order)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nv = 8 if ndim == 3 else 4
+    nel = xyz.size // (nv * ndim)
+    vtx = np.zeros((nel, nv), dtype=np.int64)
+    rc = lib().parrsb_conn_mesh(_p(vtx), _p(xyz), nel, ndim, None, 0, tol, comm)
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return vtx
 
 
 def part_mesh_dev(d_part, d_vtx, d_xyz, nel, nv, nparts, options=None, **kw):

@@ -379,6 +379,58 @@ int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *
   return run(d_part, NULL, d_vtx, d_xyz, nel, nv, options, options->nparts, 1);
 }
 
+int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
+                     long long *periodicInfo, int nPeriodicFaces, double tol, MPI_Comm comm) {
+  last_error[0] = 0;
+  int verbose = 1;
+  const char *val = getenv("PARRSB_VERBOSE_LEVEL");
+  if (val) verbose = atoi(val);
+  int np = 1;
+  if (MPI_Initialized && MPI_Comm_size) {
+    int init = 0;
+    MPI_Initialized(&init);
+    if (init) MPI_Comm_size(comm, &np);
+  }
+  if (np > 1) {
+    set_error("all elements must be passed from one rank (communicator size %d)", np);
+    return 3;
+  }
+  if (nPeriodicFaces > 0 || (periodicInfo && nPeriodicFaces < 0)) {
+    set_error("periodic faces are not provided by this build");
+    return 3;
+  }
+  if (nDim != 2 && nDim != 3) {
+    set_error("nDim must be 2 or 3");
+    return 3;
+  }
+  if (nel > 0 && (!vtx || !coord)) {
+    set_error("vtx and coord are required");
+    return 3;
+  }
+  if (verbose > 0) printf("Running parCon ...\n"), fflush(stdout);
+  double t0 = now();
+  char msg[256] = {0};
+  int rc = rsbgpu_conn_mesh(device_index(), vtx, coord, (int)nel, (int)nDim, tol, 0, msg,
+                            (int)sizeof(msg));
+  if (rc == 1) set_error("connectivity check failed (repeated vertex in an element, or a face "
+                         "sharing exactly three corners with another element)");
+  else if (rc) set_error("%s", msg[0] ? msg : "connectivity failed");
+  if (verbose > 0 && rc == 0)
+    printf("parCon (tol = %e) finished in %g s\n", tol, now() - t0), fflush(stdout);
+  return rc;
+}
+
+int parrsb_amd_conn_mesh_dev(long long *d_vtx, const double *d_coord, int nel, int ndim,
+                             double tol) {
+  last_error[0] = 0;
+  char msg[256] = {0};
+  int rc = rsbgpu_conn_mesh(device_index(), d_vtx, d_coord, nel, ndim, tol, 1, msg,
+                            (int)sizeof(msg));
+  if (rc == 1) set_error("connectivity check failed");
+  else if (rc) set_error("%s", msg[0] ? msg : "connectivity failed");
+  return rc;
+}
+
 static int quality(const int *part, const long long *vtx, int nel, int nv, int nparts,
                    int64_t *out, int on_device) {
   rsbgpu *g = NULL;
