@@ -71,12 +71,19 @@ extern parrsb_options parrsb_default_options;
  *   vtx[nel*nv]          global vertex ids (non-zero), element-major
  *   xyz[nel*nv*ndim]     per-vertex coordinates (ndim = 3 if nv == 8 else 2),
  *                        or NULL (then options->rsb_pre is set to 0)
- *   tag[nel]             only used when options->tagged == 1
+ *   tag[nel]             only used when options->tagged == 1: the elements
+ *                        of each distinct tag are partitioned on their own
+ *                        into P / ntags parts, the i-th smallest tag taking
+ *                        parts [i * P / ntags, (i + 1) * P / ntags)
+ *                        (parrsb.c:379-561)
  *   options              read and MUTATED: PARRSB_* environment overrides,
- *                        levels clamp, rsb_pre = 0 when xyz == NULL
+ *                        levels clamp, rsb_pre = 0 when xyz == NULL,
+ *                        verbose_level = profile_level = 0 in tagged mode
  * Returns 0 on success, 1 if tagged == 1 and tag == NULL (parrsb.c:873-877),
- * 2 on a device failure, 3 on unsupported input (see header comment),
- * 4 if both vtx and xyz are NULL (the reference aborts, parrsb.c:244-249).
+ * 2 on a device failure, 3 on unsupported input (see header comment; also
+ * P not a multiple of the number of tags, where the reference exits,
+ * parrsb.c:436-444), 4 if both vtx and xyz are NULL (the reference aborts,
+ * parrsb.c:244-249).
  */
 int parrsb_part_mesh(int *part, const long long *const vtx,
                      const double *const xyz, const int *const tag,

@@ -116,6 +116,14 @@ int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device);
  * last cut: the order of the element within its part). */
 int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device);
 
+/* Tagged partitioning (parrsb.c:379-452): reorder the element order so that
+ * the elements of each distinct tag are contiguous, tags ascending, elements
+ * of one tag in their previous order (stable).  *ngroups = number of distinct
+ * tags; counts[0..ngroups) = elements per tag, filled only when
+ * ngroups <= max_groups (the caller checks). */
+int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device, int max_groups,
+                        int *ngroups, int *counts);
+
 /* Copy the current order / Fiedler values (per compact position) to the
  * host; used by the parity tests. */
 int rsbgpu_get_order(rsbgpu *g, int *order_host);

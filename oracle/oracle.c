@@ -896,6 +896,59 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   return 0;
 }
 
+/* Tagged partitioning, parrsb.c:379-561.  The distinct tags, ascending, get
+ * numbers 0..ntags-1 (379-421); nparts must be a multiple of ntags (423-431);
+ * each tag's elements, in input order, are partitioned on their own into
+ * chunk = nparts / ntags parts (parrsb_part_mesh_v0 on a communicator of
+ * chunk ranks, 532-541) and tag number t adds t * chunk to its part ids
+ * (543-547).  Returns 3 when nparts is not a multiple of ntags. */
+typedef struct {
+  int tag, idx;
+} ti_t;
+
+static int cmp_ti(const void *a, const void *b) {
+  const ti_t *x = a, *y = b;
+  if (x->tag != y->tag) return (x->tag > y->tag) - (x->tag < y->tag);
+  return (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+int oracle_part_mesh_tagged(int *part, const long long *vtx, const double *xyz,
+                            const int *tag, int nel, int nv, const oracle_opts *o) {
+  if (nel <= 0) return 0;
+  ti_t *t = xmalloc(sizeof(ti_t) * (size_t)nel);
+  for (int i = 0; i < nel; i++) t[i].tag = tag[i], t[i].idx = i;
+  qsort(t, (size_t)nel, sizeof(ti_t), cmp_ti);
+  int ntags = 1;
+  for (int i = 1; i < nel; i++) ntags += t[i].tag != t[i - 1].tag;
+  if (o->nparts % ntags != 0) {
+    free(t);
+    return 3;
+  }
+  const int chunk = o->nparts / ntags, ndim = nv == 8 ? 3 : 2;
+  oracle_opts so = *o;
+  so.nparts = chunk;
+  long long *lv = xmalloc(sizeof(long long) * (size_t)nel * nv);
+  double *lx = xyz ? xmalloc(sizeof(double) * (size_t)nel * nv * ndim) : NULL;
+  int *lp = xmalloc(sizeof(int) * (size_t)nel);
+  int rc = 0;
+  for (int s = 0, tn = 0; s < nel && !rc; tn++) {
+    int e = s + 1;
+    while (e < nel && t[e].tag == t[s].tag) e++;
+    const int m = e - s;
+    for (int j = 0; j < m; j++) {
+      const int i = t[s + j].idx;
+      memcpy(lv + (size_t)j * nv, vtx + (size_t)i * nv, sizeof(long long) * nv);
+      if (lx) memcpy(lx + (size_t)j * nv * ndim, xyz + (size_t)i * nv * ndim,
+                     sizeof(double) * nv * ndim);
+    }
+    rc = oracle_part_mesh(lp, lv, lx, m, nv, &so, NULL);
+    for (int j = 0; j < m && !rc; j++) part[t[s + j].idx] = lp[j] + tn * chunk;
+    s = e;
+  }
+  free(t), free(lv), free(lx), free(lp);
+  return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 typedef struct {
   long long vid;

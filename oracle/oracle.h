@@ -133,6 +133,12 @@ int oracle_components(const long long *vtx, int nv, const int *elems, int n);
 int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel,
                      int nv, const oracle_opts *opts, oracle_report *rep);
 
+/* Tagged partitioning (parrsb.c:379-561): each distinct tag partitioned on
+ * its own into nparts / ntags parts, offset by its rank among the tags.
+ * Returns 3 when nparts is not a multiple of the number of tags. */
+int oracle_part_mesh_tagged(int *part, const long long *vtx, const double *xyz,
+                            const int *tag, int nel, int nv, const oracle_opts *opts);
+
 /* Connectivity (parrsb_conn_mesh, con.c:189-295): vtx[nel*nv] (symmetric
  * corner order, 1-based ids) from coord[nel][nv][ndim] (preprocessor corner
  * order) with relative tolerance tol; returns 1 if the element or face check

@@ -92,6 +92,8 @@ def _declare(L):
     L.oracle_part_mesh.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(Opts),
                                    C.POINTER(Report)]
     L.oracle_part_mesh.restype = C.c_int
+    L.oracle_part_mesh_tagged.argtypes = [P, P, P, P, C.c_int, C.c_int, C.POINTER(Opts)]
+    L.oracle_part_mesh_tagged.restype = C.c_int
     L.oracle_quality.argtypes = [P, P, C.c_int, C.c_int, C.c_int, P]
 
 
@@ -244,6 +246,24 @@ def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
     if rc != 0:
         raise RuntimeError(f"oracle_part_mesh returned {rc}")
     return part, rep.as_dict()
+
+
+def part_mesh_tagged(vtx, xyz, nv, nparts, tag, rsb_pre=1, find_comps=1, max_iter=50,
+                     max_passes=50, tol=1e-5):
+    """Tagged partitioning; returns part (raises on rc != 0)."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    nel = vtx.size // nv
+    xp = None
+    if xyz is not None:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        xp = _p(xyz)
+    tag = np.ascontiguousarray(tag, dtype=np.int32)
+    part = np.full(nel, -1, dtype=np.int32)
+    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts)
+    rc = lib().oracle_part_mesh_tagged(_p(part), _p(vtx), xp, _p(tag), nel, nv, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"oracle_part_mesh_tagged returned {rc}")
+    return part
 
 
 QUALITY_KEYS = ("edge_cut", "cut_edges", "max_part", "min_part", "components",

@@ -130,6 +130,7 @@ SIGNATURES = {
     "rsbgpu_download": (I, [P, P, C.c_size_t, P]),
     "rsbgpu_neighbors": (I, [P, P]),
     "rsbgpu_write_seq": (I, [P, P, I]),
+    "rsbgpu_group_by_tag": (I, [P, P, I, I, P, P]),
     "rsbgpu_conn_mesh": (I, [I, P, P, I, I, C.c_double, I, C.c_char_p, I]),
     "rsbgpu_lws_nparams": (I, []),
     "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I, I]),
@@ -385,6 +386,18 @@ class Device:
         loss = C.c_double()
         self._chk(lib().rsbgpu_lws_train_step(self.h, d_params, d_grad, level, C.byref(loss)))
         return loss.value
+
+    def group_by_tag(self, tag, max_groups):
+        """Stable grouping of the element order by tag; returns per-tag counts
+        (tags ascending) or raises when there are more than max_groups tags."""
+        tag = np.ascontiguousarray(tag, dtype=np.int32)
+        ng = C.c_int()
+        counts = np.zeros(max(max_groups, 1), dtype=np.int32)
+        self._chk(lib().rsbgpu_group_by_tag(self.h, _p(tag), 0, max_groups, C.byref(ng),
+                                            _p(counts)))
+        if ng.value > max_groups:
+            raise ValueError(f"{ng.value} tags > {max_groups}")
+        return counts[:ng.value]
 
     def neighbors(self, nseg):
         nb = np.zeros(nseg, dtype=np.int32)

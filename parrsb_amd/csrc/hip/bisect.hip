@@ -376,6 +376,66 @@ extern "C" int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device) {
 }
 
 // ---------------------------------------------------------------------------
+// Tagged partitioning (parrsb.c:379-561): the elements of each distinct tag
+// become one group, groups in ascending tag order, elements of a group in
+// their current order.  The (order-preserving) unsigned image of the tag is
+// the key of a stable radix sort of the element order; group heads are the
+// positions where the sorted key changes.
+__global__ void k_tag_keys(int nel, const int *__restrict__ order, const int *__restrict__ tag,
+                           uint32_t *__restrict__ key) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nel) key[i] = (uint32_t)tag[order[i]] ^ 0x80000000u;
+}
+
+__global__ void k_tag_heads(int nel, const uint32_t *__restrict__ key, int cap,
+                            int *__restrict__ nheads, int *__restrict__ heads) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel || (i > 0 && key[i] == key[i - 1])) return;
+  const int o = atomicAdd(nheads, 1);
+  if (o < cap) heads[o] = i;
+}
+
+extern "C" int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device, int max_groups,
+                                   int *ngroups, int *counts) {
+  RSB_TRY(hipSetDevice(g->device));
+  const int nel = g->nel;
+  *ngroups = 0;
+  if (nel == 0) return 0;
+  if (max_groups < 1) RSB_FAIL(3, "max_groups must be >= 1");
+  hipStream_t st = g->stream;
+  const int *dt = tag;
+  int *tmp = nullptr, *heads = nullptr, *nh = nullptr;
+  uint32_t *key = nullptr;
+  if (!tag_is_device) {
+    RSB_TRY(hipMalloc((void **)&tmp, sizeof(int) * (size_t)nel));
+    RSB_TRY(hipMemcpyAsync(tmp, tag, sizeof(int) * (size_t)nel, hipMemcpyHostToDevice, st));
+    dt = tmp;
+  }
+  RSB_TRY(hipMalloc((void **)&key, sizeof(uint32_t) * (size_t)nel));
+  RSB_TRY(hipMalloc((void **)&heads, sizeof(int) * (size_t)max_groups));
+  RSB_TRY(hipMalloc((void **)&nh, sizeof(int)));
+  k_tag_keys<<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_order, dt, key);
+  RSB_LAUNCH_CHECK();
+  int rc = rsb_sort_pairs_u32(g, key, g->d_order, nel, 0, 32);
+  if (rc) return rc;
+  RSB_TRY(hipMemsetAsync(nh, 0, sizeof(int), st));
+  k_tag_heads<<<nblocks(nel, 256), 256, 0, st>>>(nel, key, max_groups, nh, heads);
+  RSB_LAUNCH_CHECK();
+  int n = 0;
+  RSB_TRY(hipMemcpyAsync(&n, nh, sizeof(int), hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipStreamSynchronize(st));
+  *ngroups = n;
+  if (n <= max_groups) {
+    std::vector<int> h((size_t)n);
+    RSB_TRY(hipMemcpy(h.data(), heads, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+    std::sort(h.begin(), h.end());
+    for (int i = 0; i < n; i++) counts[i] = (i + 1 < n ? h[i + 1] : nel) - h[i];
+  }
+  hipFree(tmp), hipFree(key), hipFree(heads), hipFree(nh);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Partition quality on the global dual graph.
 // Part ids outside [0, nparts) are reported (bad) and left out.
 __global__ void k_q_check(int nel, const int *__restrict__ part, int nparts,

@@ -223,9 +223,10 @@ static void print_profile(const parrsb_amd_report *r) {
 }
 
 /* Shared body of the host- and device-pointer entry points. */
-static int run(int *part, int *seq, const long long *vtx, const double *xyz, int nel, int nv,
-               parrsb_options *options, int nparts, int on_device) {
+static int run(int *part, int *seq, const long long *vtx, const double *xyz, const int *tag,
+               int nel, int nv, parrsb_options *options, int nparts, int on_device) {
   double t0 = now();
+  const int verbose = options->verbose_level, profile = options->profile_level;
   memset(&last_report, 0, sizeof(last_report));
   last_error[0] = 0;
   if (vtx == NULL && xyz == NULL) {
@@ -273,12 +274,32 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, int
   rc = rsbgpu_set_mesh(g, vtx, xyz, nel, nv, on_device);
   parrsb_amd_report *rep = &last_report;
   rep->nparts = nparts, rep->nel = nel, rep->precision = precision_bits();
+  /* tagged partitioning (parrsb.c:379-561): each distinct tag is partitioned
+     on its own into nparts / ntags parts; tag i (ascending) owns parts
+     [i * nparts / ntags, (i + 1) * nparts / ntags) */
+  int ngroups = 0, *gcount = NULL;
+  if (!rc && tag) {
+    gcount = (int *)calloc((size_t)nparts, sizeof(int));
+    if (!gcount) rc = 2;
+    if (!rc) rc = rsbgpu_group_by_tag(g, tag, on_device, nparts, &ngroups, gcount);
+    if (!rc && verbose > 0) printf("Num tags: %d\n", ngroups), fflush(stdout);
+    if (!rc && (ngroups > nparts || nparts % ngroups != 0)) {
+      free(gcount);
+      rsbgpu_close(g);
+      set_error("number of parts must be a multiple of number of tags: parts = %d, tags = %d",
+                nparts, ngroups);
+      return 3;
+    }
+    if (!rc && verbose > 0) printf("Processes per tag: %d\n", nparts / ngroups), fflush(stdout);
+    /* the reference zeroes both on the caller's struct (parrsb.c:535-536) */
+    options->verbose_level = 0, options->profile_level = 0;
+  }
   if (!rc) {
     int64_t nnz = 0, ws = 0;
     rsbgpu_graph_stats(g, &nnz, &ws);
     rep->graph_nnz = nnz, rep->graph_wsum = ws;
     if (options->partitioner == 1 || options->partitioner == 2) {
-      rc = rcb_run(g, nel, nparts, options->partitioner == 1 ? 1 : 2);
+      rc = rcb_run(g, nel, nparts, options->partitioner == 1 ? 1 : 2, ngroups, gcount);
     } else {
       rsb_params prm;
       prm.nparts = nparts, prm.rsb_pre = options->rsb_pre;
@@ -287,6 +308,7 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, int
       prm.verbose = options->verbose_level, prm.tol = options->rsb_tol;
       prm.residuals = 1;
       prm.warm = NULL, prm.warm_passes = cfg_warm_passes, prm.warm_max_n = cfg_warm_max_n;
+      prm.ngroups = ngroups, prm.group_count = gcount;
       void *dw = NULL;
       if (cfg_warm) {
         rc = rsbgpu_upload(g, cfg_warm, sizeof(float) * (size_t)rsbgpu_lws_nparams(), &dw);
@@ -308,15 +330,16 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, int
   rep->lanczos_bytes = tr[0], rep->lanczos_ms = tr[1], rep->lanczos_iters = tr[2];
   rep->spmv_bytes = tr[3], rep->spmv_ms = tr[4], rep->spmv_launches = tr[5];
   rsbgpu_close(g);
+  free(gcount);
   rep->seconds = now() - t0;
   if (rc) return rc;
   if (options->partitioner == 0) check_partition(rep, options);
-  if (options->verbose_level > 0) {
+  if (verbose > 0) {
     static const char *algo[3] = {"RSB", "RCB", "RIB"};
     printf("par%s finished in %g seconds.\n", algo[options->partitioner % 3], rep->seconds);
     fflush(stdout);
   }
-  if (options->profile_level > 0) print_profile(rep);
+  if (profile > 0 && !tag) print_profile(rep);
   return 0;
 }
 
@@ -339,16 +362,13 @@ static int part_mesh_host(int *part, int *seq, const long long *const vtx,
     set_error("tagged partitioning requested but tag array is NULL");
     return 1;
   }
-  if (options->tagged == 1) {
-    set_error("tagged partitioning is not provided by this build");
-    return 3;
-  }
   if (np > 1) {
     set_error("all elements must be passed from one rank (communicator size %d)", np);
     return 3;
   }
   int nparts = options->nparts > 0 ? options->nparts : np;
-  return run(part, seq, vtx, xyz, nel, nv, options, nparts, 0);
+  return run(part, seq, vtx, xyz, options->tagged == 1 ? tag : NULL, nel, nv, options, nparts,
+             0);
 }
 
 int parrsb_part_mesh(int *part, const long long *const vtx, const double *const xyz,
@@ -373,10 +393,10 @@ int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *
                              int nel, int nv, parrsb_options *options) {
   update_options(options);
   if (options->tagged == 1) {
-    set_error("tagged partitioning is not provided by this build");
+    set_error("tagged partitioning takes host arrays (parrsb_part_mesh)");
     return 3;
   }
-  return run(d_part, NULL, d_vtx, d_xyz, nel, nv, options, options->nparts, 1);
+  return run(d_part, NULL, d_vtx, d_xyz, NULL, nel, nv, options, options->nparts, 1);
 }
 
 int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,

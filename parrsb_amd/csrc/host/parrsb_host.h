@@ -10,13 +10,18 @@ typedef struct {
   double tol;
   const float *warm;  /* device parameters of the learned start, or NULL */
   int warm_passes, warm_max_n;
+  /* tagged partitioning: ngroups > 1 groups of group_count[i] consecutive
+     elements (ascending tag), each partitioned into nparts / ngroups parts */
+  int ngroups;
+  const int *group_count;
 } rsb_params;
 
 long long rsb_lower_count(long long n, int p);
 long long rsb_balance_expected(long long n, int p, int bin);
 int rsb_ncuts(int nparts);
 int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep);
-/* partitioner 1 (pre = 1, RCB) or 2 (pre = 2, RIB) */
-int rcb_run(rsbgpu *g, int nel, int nparts, int pre);
+/* partitioner 1 (pre = 1, RCB) or 2 (pre = 2, RIB); ngroups / group_count
+   as in rsb_params (ngroups <= 1: one group of nel elements) */
+int rcb_run(rsbgpu *g, int nel, int nparts, int pre, int ngroups, const int *group_count);
 
 #endif

@@ -98,9 +98,22 @@ static void segtab_free(segtab *t) { free(t->cstart), free(t->n), free(t->p), fr
     }                           \
   } while (0)
 
+/* Initial segments: one per tag group with nparts / ngroups virtual ranks
+   each (parrsb.c:423-431, 532-541), or one segment of all elements. */
+static int initial_segments(segtab *t, int nel, int P, int ngroups, const int *count) {
+  if (ngroups <= 1) {
+    t->cstart[0] = 0, t->n[0] = nel, t->p[0] = P, t->r0[0] = 0;
+    return 1;
+  }
+  const int chunk = P / ngroups;
+  for (int i = 0, c = 0; i < ngroups; c += count[i], i++)
+    t->cstart[i] = c, t->n[i] = count[i], t->p[i] = chunk, t->r0[i] = i * chunk;
+  return ngroups;
+}
+
 int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   int rc = 0, P = prm->nparts;
-  int ncuts = rsb_ncuts(P);
+  int ncuts = rsb_ncuts(prm->ngroups > 1 ? P / prm->ngroups : P);
   segtab A, B;
   int ok = segtab_alloc(&A, P) & segtab_alloc(&B, P);
   int *nl = (int *)calloc((size_t)P, sizeof(int)), *snd = (int *)calloc((size_t)P, sizeof(int));
@@ -116,8 +129,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     rc = 2;
     goto done;
   }
-  int nseg = 1;
-  A.cstart[0] = 0, A.n[0] = nel, A.p[0] = P, A.r0[0] = 0;
+  int nseg = initial_segments(&A, nel, P, prm->ngroups, prm->group_count);
   rep->nlevels = ncuts;
   for (int cut = 0; cut < ncuts; cut++) {
     double tl = wall();
@@ -238,20 +250,24 @@ static void rcb_ranges(int *cstart, int *n, int start, int len, int p, int r0) {
   rcb_ranges(cstart, n, start + l, len - l, p / 2, r0 + (p + 1) / 2);
 }
 
-int rcb_run(rsbgpu *g, int nel, int P, int pre) {
-  int rc = 0, one_c = 0, one_p = P, one_r = 0;
+int rcb_run(rsbgpu *g, int nel, int P, int pre, int ngroups, const int *group_count) {
+  int rc = 0;
+  segtab G;
   int *cs = (int *)calloc((size_t)P, sizeof(int)), *n = (int *)calloc((size_t)P, sizeof(int));
   int *p = (int *)calloc((size_t)P, sizeof(int)), *r0 = (int *)calloc((size_t)P, sizeof(int));
-  if (!cs || !n || !p || !r0) {
+  int ok = segtab_alloc(&G, P);
+  if (!cs || !n || !p || !r0 || !ok) {
     rc = 2;
     goto done;
   }
-  TRY(rsbgpu_set_segments(g, 1, &one_c, &nel, &one_p, &one_r));
+  int ng = initial_segments(&G, nel, P, ngroups, group_count);
+  TRY(rsbgpu_set_segments(g, ng, G.cstart, G.n, G.p, G.r0));
   TRY(rsbgpu_prepartition(g, pre));
-  rcb_ranges(cs, n, 0, nel, P, 0);
+  for (int i = 0; i < ng; i++) rcb_ranges(cs, n, G.cstart[i], G.n[i], G.p[i], G.r0[i]);
   for (int r = 0; r < P; r++) p[r] = 1, r0[r] = r;
   TRY(rsbgpu_set_segments(g, P, cs, n, p, r0));
 done:
+  segtab_free(&G);
   free(cs), free(n), free(p), free(r0);
   return rc;
 }

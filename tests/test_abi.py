@@ -111,9 +111,6 @@ def test_input_errors_before_device(monkeypatch):
     assert _call(vtx, xyz, 8, opts) == 3             # no such partitioner
     opts = pa.default_options(verbose_level=0, nparts=2, partitioner=2)
     assert _call(vtx, None, 8, opts) == 3            # RIB needs coordinates
-    opts = pa.default_options(verbose_level=0, nparts=2, tagged=1)
-    tag = np.zeros(1, dtype=np.int32)
-    assert _call(vtx, xyz, 8, opts, tag) == 3        # tagged mode not provided
 
 
 def test_oracle_is_not_linked_by_the_product():
