@@ -95,14 +95,72 @@ int parrsb_part_mesh(int *part, const long long *const vtx,
  * reference, con.c:189-295): coord[nel][nv][nDim] in the preprocessor corner
  * order (nv = 8 for nDim = 3, 4 for nDim = 2) -> vtx[nel][nv], 1-based ids,
  * symmetric corner order.  Corners closer than tol times their element's
- * shortest incident edge share an id.  Returns 0, 1 if the element or face
- * check of the result fails (con-check.c), 2 on a device failure, 3 for
- * periodic faces (nPeriodicFaces > 0, not provided by this build), a
- * communicator of more than one rank, or bad sizes.
+ * shortest incident edge share an id.  periodicInfo holds nPeriodicFaces
+ * records (element, face, partner element, partner face), 1-based,
+ * preprocessor face numbers, as parrsb_read_mesh returns them: matched faces
+ * share their corner ids (con-periodic.c) and the ids are renumbered densely.
+ * Returns 0, 1 if the element or face check of the result fails
+ * (con-check.c) or a periodic face does not match its partner (the
+ * reference exits there), 2 on a device failure, 3 for a communicator of
+ * more than one rank, bad sizes or a periodic record out of range.
  */
 int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
                      long long *periodicInfo, int nPeriodicFaces, double tol,
                      MPI_Comm comm);
+
+/*
+ * Nek5000 mesh files (parRSB.h:71-83 of the reference, io.c), read and
+ * written by the one rank that holds the mesh (a larger communicator
+ * returns 1).
+ *   parrsb_read_mesh  read & 1: <name>.re2 -> *nel, *nv, *coord (malloc'd,
+ *                     [nel][nv][ndim] preprocessor order), *nbcs periodic
+ *                     face records in *bcs (malloc'd, 4 per record as
+ *                     parrsb_conn_mesh takes them); read & 2: <name>.co2 ->
+ *                     *vl (malloc'd vertex ids; nel and nv must agree with
+ *                     the .re2 when both are read).  Returns 0, or 1 with a
+ *                     message (parrsb_amd_last_error) on a missing, truncated
+ *                     or foreign-endian file, where the reference exits.
+ *   parrsb_dump_con   <name>.co2 from vl[nelt][nv].
+ *   parrsb_dump_map   <name>.ma2 from vl[nelt][nv], rank field 0 (the
+ *                     calling rank; see parrsb_amd_dump_map_parts for one
+ *                     rank holding all parts).
+ */
+int parrsb_read_mesh(unsigned *nel, unsigned *nv, long long **vl, double **coord,
+                     unsigned *nbcs, long long **bcs, char *name, MPI_Comm comm, int read);
+int parrsb_dump_con(char *name, unsigned nelt, unsigned nv, long long *vl, MPI_Comm comm);
+int parrsb_dump_map(char *name, unsigned nelt, unsigned nv, long long *vl, MPI_Comm comm);
+
+/* Command-line options of the example drivers (parRSB.h:88-99 of the
+ * reference): --mesh=<name> (required), --tol=<gencon tolerance, 0.2>,
+ * --test, --dump, --nactive=<ranks>, --verbose=<level>.  NULL when --mesh is
+ * missing or an option is unknown. */
+typedef struct {
+  char *mesh;
+  double tol;
+  int test;
+  int dump;
+  int nactive;
+  int verbose;
+} parrsb_cmd_line_opts;
+
+parrsb_cmd_line_opts *parrsb_parse_cmd_opts(int argc, char *argv[]);
+void parrsb_cmd_opts_free(parrsb_cmd_line_opts *opts);
+
+/*
+ * Redistribution after partitioning (helpers.c:35-92).  One rank holds all
+ * parts here, so the elements -- vl, coord (may be NULL) and part itself --
+ * are reordered by part id, ascending, keeping their order within a part:
+ * the concatenation, in rank order, of what each rank receives in the
+ * reference.  *nelt is unchanged.  Returns 1 on a negative part id.
+ */
+int parrsb_dist_mesh(unsigned *nelt, long long **vl, double **coord, int *part, unsigned nv,
+                     MPI_Comm comm);
+
+/* Read <opts->mesh>.re2, find the connectivity, partition with the default
+ * options (PARRSB_NPARTS sets the part count) and redistribute
+ * (helpers.c:94-130). */
+int parrsb_setup_mesh(unsigned *nelt, unsigned *nv, long long **vl, double **coord,
+                      parrsb_cmd_line_opts *opts, MPI_Comm comm);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,17 @@ int parrsb_amd_partition_quality(const int *part, const long long *vtx, int nel,
 int parrsb_amd_partition_quality_dev(const int *d_part, const long long *d_vtx, int nel,
                                      int nv, int nparts, int64_t *out);
 
+/* free() for the arrays parrsb_read_mesh allocates, for callers that cannot
+ * reach the C library's free (bindings from other languages). */
+void parrsb_amd_free(void *p);
+
+/* .ma2 for one rank holding all parts: the rank field of element e is
+ * part[e] (what a P-rank genmap run writes after parrsb_dist_mesh; call
+ * parrsb_dist_mesh first to get the same record order).  No counterpart in
+ * the reference. */
+int parrsb_amd_dump_map_parts(const char *name, unsigned nelt, unsigned nv, const long long *vl,
+                              const int *part);
+
 #ifdef __cplusplus
 }
 #endif

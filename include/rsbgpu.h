@@ -176,10 +176,14 @@ int rsbgpu_neighbors(rsbgpu *g, int *nbrs);
 
 /* Connectivity (parrsb_conn_mesh): vtx_out[nel*nv] 1-based ids in the
  * symmetric corner order from coord[nel*nv*ndim] (preprocessor order), on
- * device `device`; host or device arrays.  0, 1 (element/face check failed),
- * 2 (device failure, message in msg), 3 (bad input). */
+ * device `device`; host or device arrays (vtx_out and coord).  pinfo (host,
+ * may be NULL when npinfo == 0) holds npinfo periodic face records
+ * (element, face, partner element, partner face), 1-based, preprocessor face
+ * numbers (con.c:41-47).  0, 1 (element/face check failed, or periodic faces
+ * that do not match), 2 (device failure, message in msg), 3 (bad input). */
 int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *coord, int nel, int ndim,
-                     double tol, int on_device, char *msg, int msg_len);
+                     double tol, const long long *pinfo, int npinfo, int on_device, char *msg,
+                     int msg_len);
 
 /* Small helpers for callers holding host data: copy n ints (or `bytes`
  * bytes) to a new device buffer (freed with rsbgpu_free). */

@@ -1028,7 +1028,15 @@ static const int CON_NEIGHBOR[8][3] = {{1, 2, 4}, {0, 3, 5}, {0, 3, 6}, {1, 2, 7
 static const int CON_FACES3D[6][4] = {{1, 5, 7, 3}, {2, 4, 8, 6}, {1, 2, 6, 5},
                                       {3, 7, 8, 4}, {1, 3, 4, 2}, {5, 6, 8, 7}};
 
+static int conn_periodic(long long *vtx, const double *coord, int nel, int ndim,
+                         const long long *pinfo, int npinfo);
+
 int oracle_conn_mesh(long long *vtx, const double *coord, int nel, int ndim, double tol) {
+  return oracle_conn_mesh_periodic(vtx, coord, nel, ndim, tol, NULL, 0);
+}
+
+int oracle_conn_mesh_periodic(long long *vtx, const double *coord, int nel, int ndim,
+                              double tol, const long long *pinfo, int npinfo) {
   const int nv = ndim == 3 ? 8 : 4;
   const int64_t np = (int64_t)nel * nv;
   double *x = xmalloc(sizeof(double) * 3 * (size_t)(np ? np : 1));
@@ -1122,5 +1130,115 @@ int oracle_conn_mesh(long long *vtx, const double *coord, int nel, int ndim, dou
     free(cnt), free(els), free(fill);
     if (err) return 1;
   }
-  return 0;
+  return conn_periodic(vtx, coord, nel, ndim, pinfo, npinfo);
+}
+
+/* Periodic faces, con-periodic.c:8-288.  pinfo[4i..4i+3] = (element, face,
+ * element, face), 1-based, faces in the preprocessor numbering (con.c:41-47).
+ * A face whose (element, face) is named by a later record's partner is matched
+ * with that record's face: both are centred, the partner's corners are taken
+ * in reverse order under the cyclic shift with the least sum of squared
+ * distances (first such shift), and the match fails (returns 1; the reference
+ * exits) if the root of that sum exceeds 1e-3 times the largest absolute
+ * coordinate of the two faces.  Every matched corner pair joins two ids; each
+ * id then takes the smallest id of its class, and the ids are renumbered
+ * densely in ascending order (1-based). */
+static const int CON_FACE_PRE_TO_SYM[6] = {2, 1, 3, 0, 4, 5};
+static const int CON_FACES2D[4][2] = {{3, 1}, {2, 4}, {1, 2}, {4, 3}};
+
+static int conn_periodic(long long *vtx, const double *coord, int nel, int ndim,
+                         const long long *pinfo, int npinfo) {
+  if (npinfo <= 0) return 0;
+  const int nv = ndim == 3 ? 8 : 4, nvf = nv / 2;
+  for (int i = 0; i < npinfo; i++) {
+    const long long *r = pinfo + 4 * (int64_t)i;
+    if (r[0] < 1 || r[0] > nel || r[2] < 1 || r[2] > nel || r[1] < 1 || r[1] > 2 * ndim ||
+        r[3] < 1 || r[3] > 2 * ndim)
+      return 3;
+  }
+  /* corners of each record's face: coordinates and current ids */
+  double *fx = xmalloc(sizeof(double) * (size_t)npinfo * 4 * 3);
+  long long *fid = xmalloc(sizeof(long long) * (size_t)npinfo * 4);
+  for (int i = 0; i < npinfo; i++) {
+    const int e = (int)pinfo[4 * (int64_t)i] - 1;
+    const int f = CON_FACE_PRE_TO_SYM[pinfo[4 * (int64_t)i + 1] - 1];
+    for (int j = 0; j < nvf; j++) {
+      const int s = (ndim == 3 ? CON_FACES3D[f][j] : CON_FACES2D[f][j]) - 1;
+      const int pre = CON_PRE_TO_SYM[s];
+      for (int d = 0; d < 3; d++)
+        fx[((int64_t)i * 4 + j) * 3 + d] =
+            d < ndim ? coord[((int64_t)e * nv + pre) * ndim + d] : 0.0;
+      fid[(int64_t)i * 4 + j] = vtx[(int64_t)e * nv + s];
+    }
+  }
+  /* matched id pairs */
+  int64_t npair = 0, cap = 16;
+  long long *pa = xmalloc(sizeof(long long) * (size_t)cap),
+            *pb = xmalloc(sizeof(long long) * (size_t)cap);
+  int rc = 0;
+  for (int i = 0; i < npinfo - 1 && !rc; i++) {
+    const long long ei = pinfo[4 * (int64_t)i], fi = CON_FACE_PRE_TO_SYM[pinfo[4 * (int64_t)i + 1] - 1];
+    for (int j = i + 1; j < npinfo && !rc; j++) {
+      const long long *rj = pinfo + 4 * (int64_t)j;
+      if (rj[2] != ei || CON_FACE_PRE_TO_SYM[rj[3] - 1] != fi) continue;
+      double F[4][3], G[4][3], fmax = 0.0, gmax = 0.0;
+      for (int d = 0; d < ndim; d++) {
+        double mf = 0.0, mg = 0.0;
+        for (int c = 0; c < nvf; c++) {
+          F[c][d] = fx[((int64_t)i * 4 + c) * 3 + d], G[c][d] = fx[((int64_t)j * 4 + c) * 3 + d];
+          fmax = fabs(F[c][d]) > fmax ? fabs(F[c][d]) : fmax;
+          gmax = fabs(G[c][d]) > gmax ? fabs(G[c][d]) : gmax;
+          mf += F[c][d], mg += G[c][d];
+        }
+        for (int c = 0; c < nvf; c++) F[c][d] -= mf / nvf, G[c][d] -= mg / nvf;
+      }
+      int shift = 0;
+      double best = 1e20;
+      for (int sh = 0; sh < nvf; sh++) {
+        double d2 = 0.0;
+        for (int c = 0; c < nvf; c++) {
+          const int k = nvf - 1 - (c + sh) % nvf;
+          for (int d = 0; d < ndim; d++) d2 += (F[c][d] - G[k][d]) * (F[c][d] - G[k][d]);
+        }
+        if (d2 < best) best = d2, shift = sh;
+      }
+      if (sqrt(best) > 1e-3 * (fmax > gmax ? fmax : gmax)) {
+        rc = 1;
+        break;
+      }
+      for (int c = 0; c < nvf; c++) {
+        const int k = nvf - 1 - (c + shift) % nvf;
+        const long long a = fid[(int64_t)i * 4 + c], b = fid[(int64_t)j * 4 + k];
+        if (npair == cap) {
+          cap *= 2;
+          pa = realloc(pa, sizeof(long long) * (size_t)cap);
+          pb = realloc(pb, sizeof(long long) * (size_t)cap);
+        }
+        pa[npair] = a < b ? a : b, pb[npair] = a < b ? b : a, npair++;
+      }
+    }
+  }
+  if (!rc && npair > 0) {
+    /* smallest id of each class: relax the pairs until nothing changes */
+    const int64_t np = (int64_t)nel * nv;
+    long long nid = 0;
+    for (int64_t p = 0; p < np; p++) nid = vtx[p] > nid ? vtx[p] : nid;
+    long long *lab = xmalloc(sizeof(long long) * (size_t)(nid + 1));
+    for (long long v = 0; v <= nid; v++) lab[v] = v;
+    for (int changed = 1; changed;) {
+      changed = 0;
+      for (int64_t q = 0; q < npair; q++) {
+        const long long m = lab[pa[q]] < lab[pb[q]] ? lab[pa[q]] : lab[pb[q]];
+        if (lab[pa[q]] != m || lab[pb[q]] != m) lab[pa[q]] = lab[pb[q]] = m, changed = 1;
+      }
+    }
+    /* dense renumbering in ascending order of the surviving ids */
+    long long *rank = xmalloc(sizeof(long long) * (size_t)(nid + 1));
+    long long r = 0;
+    for (long long v = 1; v <= nid; v++) rank[v] = lab[v] == v ? ++r : 0;
+    for (int64_t p = 0; p < np; p++) vtx[p] = rank[lab[vtx[p]]];
+    free(lab), free(rank);
+  }
+  free(fx), free(fid), free(pa), free(pb);
+  return rc;
 }

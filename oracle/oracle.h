@@ -145,6 +145,13 @@ int oracle_part_mesh_tagged(int *part, const long long *vtx, const double *xyz,
  * fails (con-check.c), 0 otherwise. */
 int oracle_conn_mesh(long long *vtx, const double *coord, int nel, int ndim, double tol);
 
+/* Same with periodic faces (con-periodic.c): pinfo[4*npinfo] = (element,
+ * face, partner element, partner face), 1-based, preprocessor face order.
+ * Returns 1 if a face pair does not match within 1e-3 of the faces' size,
+ * 3 if a record names an element or face out of range. */
+int oracle_conn_mesh_periodic(long long *vtx, const double *coord, int nel, int ndim,
+                              double tol, const long long *pinfo, int npinfo);
+
 /* Partition quality.  out[0] weighted edge cut (sum over undirected edges of
  * shared-vertex weight), out[1] unweighted cut edges, out[2] max part size,
  * out[3] min part size, out[4] total components over parts, out[5] max

@@ -62,6 +62,7 @@ def _declare(L):
     L.oracle_centroids.argtypes = [P, C.c_int, C.c_int, P]
     L.oracle_rib_order.argtypes = [P, C.c_int, P, C.c_int, C.c_int]
     L.oracle_conn_mesh.argtypes = [P, P, C.c_int, C.c_int, C.c_double]
+    L.oracle_conn_mesh_periodic.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, C.c_int]
     L.oracle_rib_axis.argtypes = [P, C.c_int, P, C.c_int, P, P]
     L.oracle_sym_dominant.argtypes = [P, C.c_int, P]
     L.oracle_rank_count.argtypes = [C.c_int64, C.c_int, C.c_int]
@@ -151,13 +152,19 @@ def sym_dominant(A, n):
     return ev[:n]
 
 
-def conn_mesh(xyz, ndim, tol=0.2):
-    """(rc, vtx[nel, nv]) from per-corner coordinates xyz[nel, nv, ndim]."""
+def conn_mesh(xyz, ndim, tol=0.2, periodic=None):
+    """(rc, vtx[nel, nv]) from per-corner coordinates xyz[nel, nv, ndim];
+    periodic = int64 records (element, face, element, face), 1-based."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     nv = 8 if ndim == 3 else 4
     nel = xyz.size // (nv * ndim)
     vtx = np.zeros((nel, nv), dtype=np.int64)
-    rc = lib().oracle_conn_mesh(_p(vtx), _p(xyz), nel, ndim, tol)
+    if periodic is None or len(periodic) == 0:
+        rc = lib().oracle_conn_mesh(_p(vtx), _p(xyz), nel, ndim, tol)
+    else:
+        pi = np.ascontiguousarray(periodic, dtype=np.int64).reshape(-1, 4)
+        rc = lib().oracle_conn_mesh_periodic(_p(vtx), _p(xyz), nel, ndim, tol, _p(pi),
+                                             pi.shape[0])
     return rc, vtx
 
 

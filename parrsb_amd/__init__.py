@@ -80,6 +80,12 @@ class FiedlerOut(C.Structure):
                 ("rnorm", C.c_void_p), ("resid", C.c_void_p), ("fallback", C.c_void_p)]
 
 
+class CmdOpts(C.Structure):
+    """parrsb_cmd_line_opts (parRSB.h)."""
+    _fields_ = [("mesh", C.c_char_p), ("tol", C.c_double), ("test", C.c_int),
+                ("dump", C.c_int), ("nactive", C.c_int), ("verbose", C.c_int)]
+
+
 P = C.c_void_p
 I = C.c_int
 
@@ -100,6 +106,19 @@ SIGNATURES = {
     "parrsb_amd_set_warm_start": (I, [P, I, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
     "parrsb_amd_partition_quality_dev": (I, [P, P, I, I, I, P]),
+    "parrsb_read_mesh": (I, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(P),
+                             C.POINTER(P), C.POINTER(C.c_uint), C.POINTER(P), C.c_char_p, I,
+                             I]),
+    "parrsb_dump_con": (I, [C.c_char_p, C.c_uint, C.c_uint, P, I]),
+    "parrsb_dump_map": (I, [C.c_char_p, C.c_uint, C.c_uint, P, I]),
+    "parrsb_amd_dump_map_parts": (I, [C.c_char_p, C.c_uint, C.c_uint, P, P]),
+    "parrsb_dist_mesh": (I, [C.POINTER(C.c_uint), C.POINTER(P), C.POINTER(P), P, C.c_uint,
+                             I]),
+    "parrsb_setup_mesh": (I, [C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(P),
+                              C.POINTER(P), C.POINTER(CmdOpts), I]),
+    "parrsb_parse_cmd_opts": (C.POINTER(CmdOpts), [I, C.POINTER(C.c_char_p)]),
+    "parrsb_cmd_opts_free": (None, [C.POINTER(CmdOpts)]),
+    "parrsb_amd_free": (None, [P]),
     # rsbgpu.h
     "rsbgpu_open": (I, [C.POINTER(P), I]),
     "rsbgpu_close": (None, [P]),
@@ -131,7 +150,7 @@ SIGNATURES = {
     "rsbgpu_neighbors": (I, [P, P]),
     "rsbgpu_write_seq": (I, [P, P, I]),
     "rsbgpu_group_by_tag": (I, [P, P, I, I, P, P]),
-    "rsbgpu_conn_mesh": (I, [I, P, P, I, I, C.c_double, I, C.c_char_p, I]),
+    "rsbgpu_conn_mesh": (I, [I, P, P, I, I, C.c_double, P, I, I, C.c_char_p, I]),
     "rsbgpu_lws_nparams": (I, []),
     "rsbgpu_lws_set_params": (I, [P, P, C.c_float, I, I]),
     "rsbgpu_lws_predict": (I, [P, P, C.c_float, P]),
@@ -229,17 +248,93 @@ def part_mesh_seq(vtx, xyz, nv, nparts, tag=None, comm=0, options=None, **kw):
     return part, seq, opts
 
 
-def conn_mesh(xyz, ndim, tol=0.2, comm=0):
+def conn_mesh(xyz, ndim, tol=0.2, comm=0, periodic=None):
     """parrsb_conn_mesh: vtx[nel, nv] (1-based ids, symmetric corner order)
-    from per-corner coordinates xyz[nel, nv, ndim] (preprocessor order)."""
+    from per-corner coordinates xyz[nel, nv, ndim] (preprocessor order);
+    periodic = records (element, face, partner element, partner face),
+    1-based, preprocessor face numbers (as parrsb_read_mesh returns them)."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     nv = 8 if ndim == 3 else 4
     nel = xyz.size // (nv * ndim)
     vtx = np.zeros((nel, nv), dtype=np.int64)
-    rc = lib().parrsb_conn_mesh(_p(vtx), _p(xyz), nel, ndim, None, 0, tol, comm)
+    pi, npi = None, 0
+    if periodic is not None and len(periodic):
+        pi = np.ascontiguousarray(periodic, dtype=np.int64).reshape(-1, 4)
+        npi = pi.shape[0]
+    rc = lib().parrsb_conn_mesh(_p(vtx), _p(xyz), nel, ndim, _p(pi), npi, tol, comm)
     if rc:
         raise ParrsbError(rc, last_error())
     return vtx
+
+
+def _take(ptr, ctype, n, shape):
+    """Copy n items of a library-allocated array into numpy and free it."""
+    if not ptr.value:
+        return None
+    a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), (max(n, 1),))[:n].copy()
+    lib().parrsb_amd_free(ptr)
+    return a.reshape(shape)
+
+
+def read_mesh(name, read=1, comm=0):
+    """parrsb_read_mesh: dict with nel, nv, and (read & 1) coord[nel, nv, ndim]
+    and the periodic records bcs[nbcs, 4], (read & 2) vl[nel, nv]."""
+    nel, nv, nb = C.c_uint(), C.c_uint(), C.c_uint()
+    vl, coord, bcs = P(), P(), P()
+    rc = lib().parrsb_read_mesh(C.byref(nel), C.byref(nv), C.byref(vl) if read & 2 else None,
+                                C.byref(coord) if read & 1 else None, C.byref(nb),
+                                C.byref(bcs), os.fsencode(name), comm, read)
+    if rc:
+        for x in (vl, coord, bcs):
+            if x.value:
+                lib().parrsb_amd_free(x)
+        raise ParrsbError(rc, last_error())
+    n, v = nel.value, nv.value
+    ndim = 3 if v == 8 else 2
+    out = {"nel": n, "nv": v}
+    if read & 1:
+        out["coord"] = _take(coord, C.c_double, n * v * ndim, (n, v, ndim))
+        out["bcs"] = _take(bcs, C.c_longlong, 4 * nb.value, (nb.value, 4))
+        if out["bcs"] is None:
+            out["bcs"] = np.zeros((0, 4), dtype=np.int64)
+    if read & 2:
+        out["vl"] = _take(vl, C.c_longlong, n * v, (n, v))
+    return out
+
+
+def dump_con(name, vl, comm=0):
+    vl = np.ascontiguousarray(vl, dtype=np.int64)
+    rc = lib().parrsb_dump_con(os.fsencode(name), vl.shape[0], vl.shape[1], _p(vl), comm)
+    if rc:
+        raise ParrsbError(rc, last_error())
+
+
+def dump_map(name, vl, part=None, comm=0):
+    """.ma2 writer: parrsb_dump_map (rank field 0), or with part the
+    one-rank-holds-all-parts form (rank field = part id)."""
+    vl = np.ascontiguousarray(vl, dtype=np.int64)
+    if part is None:
+        rc = lib().parrsb_dump_map(os.fsencode(name), vl.shape[0], vl.shape[1], _p(vl), comm)
+    else:
+        part = np.ascontiguousarray(part, dtype=np.int32)
+        rc = lib().parrsb_amd_dump_map_parts(os.fsencode(name), vl.shape[0], vl.shape[1],
+                                             _p(vl), _p(part))
+    if rc:
+        raise ParrsbError(rc, last_error())
+
+
+def dist_mesh(vl, coord, part, comm=0):
+    """parrsb_dist_mesh on copies: returns (vl, coord, part) grouped by part."""
+    vl = np.array(vl, dtype=np.int64, order="C", copy=True)
+    cd = None if coord is None else np.array(coord, dtype=np.float64, order="C", copy=True)
+    part = np.array(part, dtype=np.int32, copy=True)
+    n = C.c_uint(vl.shape[0])
+    pv, pc = P(vl.ctypes.data), P(cd.ctypes.data if cd is not None else None)
+    rc = lib().parrsb_dist_mesh(C.byref(n), C.byref(pv), C.byref(pc) if cd is not None else None,
+                                _p(part), vl.shape[1], comm)
+    if rc:
+        raise ParrsbError(rc, last_error())
+    return vl, cd, part
 
 
 def part_mesh_dev(d_part, d_vtx, d_xyz, nel, nv, nparts, options=None, **kw):

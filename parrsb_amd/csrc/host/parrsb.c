@@ -67,11 +67,22 @@ int parrsb_amd_set_profiling(int on) {
   return 0;
 }
 
-static void set_error(const char *fmt, ...) {
+void parrsb_host_error(const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(last_error, sizeof(last_error), fmt, ap);
   va_end(ap);
+}
+#define set_error parrsb_host_error
+
+int parrsb_host_comm_size(MPI_Comm comm) {
+  int np = 1;
+  if (MPI_Initialized && MPI_Comm_size) {
+    int init = 0;
+    MPI_Initialized(&init);
+    if (init) MPI_Comm_size(comm, &np);
+  }
+  return np;
 }
 
 int parrsb_amd_warm_start_nparams(void) { return rsbgpu_lws_nparams(); }
@@ -415,8 +426,8 @@ int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
     set_error("all elements must be passed from one rank (communicator size %d)", np);
     return 3;
   }
-  if (nPeriodicFaces > 0 || (periodicInfo && nPeriodicFaces < 0)) {
-    set_error("periodic faces are not provided by this build");
+  if (nPeriodicFaces < 0 || (nPeriodicFaces > 0 && !periodicInfo)) {
+    set_error("periodicInfo must hold nPeriodicFaces records");
     return 3;
   }
   if (nDim != 2 && nDim != 3) {
@@ -430,10 +441,12 @@ int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
   if (verbose > 0) printf("Running parCon ...\n"), fflush(stdout);
   double t0 = now();
   char msg[256] = {0};
-  int rc = rsbgpu_conn_mesh(device_index(), vtx, coord, (int)nel, (int)nDim, tol, 0, msg,
+  int rc = rsbgpu_conn_mesh(device_index(), vtx, coord, (int)nel, (int)nDim, tol, periodicInfo,
+                            nPeriodicFaces, 0, msg,
                             (int)sizeof(msg));
-  if (rc == 1) set_error("connectivity check failed (repeated vertex in an element, or a face "
-                         "sharing exactly three corners with another element)");
+  if (rc == 1 && msg[0]) set_error("%s", msg);
+  else if (rc == 1) set_error("connectivity check failed (repeated vertex in an element, or a "
+                              "face sharing exactly three corners with another element)");
   else if (rc) set_error("%s", msg[0] ? msg : "connectivity failed");
   if (verbose > 0 && rc == 0)
     printf("parCon (tol = %e) finished in %g s\n", tol, now() - t0), fflush(stdout);
@@ -444,7 +457,7 @@ int parrsb_amd_conn_mesh_dev(long long *d_vtx, const double *d_coord, int nel, i
                              double tol) {
   last_error[0] = 0;
   char msg[256] = {0};
-  int rc = rsbgpu_conn_mesh(device_index(), d_vtx, d_coord, nel, ndim, tol, 1, msg,
+  int rc = rsbgpu_conn_mesh(device_index(), d_vtx, d_coord, nel, ndim, tol, NULL, 0, 1, msg,
                             (int)sizeof(msg));
   if (rc == 1) set_error("connectivity check failed");
   else if (rc) set_error("%s", msg[0] ? msg : "connectivity failed");

@@ -16,6 +16,11 @@ typedef struct {
   const int *group_count;
 } rsb_params;
 
+/* last-error message of the library (parrsb_amd_last_error) */
+void parrsb_host_error(const char *fmt, ...);
+/* size of comm if the process has MPI loaded and initialised, else 1 */
+int parrsb_host_comm_size(MPI_Comm comm);
+
 long long rsb_lower_count(long long n, int p);
 long long rsb_balance_expected(long long n, int p, int bin);
 int rsb_ncuts(int nparts);

@@ -242,3 +242,61 @@ def ncuts(nparts: int) -> int:
     while (1 << c) < nparts:
         c += 1
     return c
+
+
+def periodic_box(nx: int, ny: int, nz: int = 0, periodic=(True, False, False),
+                 warp: float = 0.0, seed: int = 1):
+    """Structured hex box (nz > 0) or quad grid (nz == 0) whose faces on the
+    chosen axes are periodic.
+
+    Returns (vtx, xyz, records): vtx[nel, nv] are the lattice ids with the
+    periodic wrap applied (what a connectivity must find), xyz[nel, nv, ndim]
+    the corner coordinates in preprocessor order (not wrapped), and
+    records[nrec, 4] the periodic face records (element, face, partner
+    element, partner face), 1-based, preprocessor face numbers (hex: 1 y=0,
+    2 x=1, 3 y=1, 4 x=0, 5 z=0, 6 z=1; quad: 1 y=0, 2 x=1, 3 y=1, 4 x=0), one
+    record per face as a .re2 file lists them.  warp moves the interior
+    lattice points smoothly (periodic boundaries stay planar)."""
+    ndim = 3 if nz > 0 else 2
+    n = (nx, ny, nz)[:ndim]
+    corners = _HEX if ndim == 3 else _QUAD
+    grids = np.meshgrid(*[np.arange(m, dtype=np.int64) for m in reversed(n)], indexing="ij")
+    idx = [g.ravel() for g in reversed(grids)]  # idx[0] = i (fastest)
+    nel = idx[0].size
+    h = 1.0 / max(n)
+    lat = np.stack([np.stack([idx[d] + corners[c, d] for d in range(ndim)], -1)
+                    for c in range(len(corners))], 1)  # [nel, nv, ndim] lattice ints
+    x = lat.astype(np.float64)
+    if warp:
+        rng = np.random.default_rng(seed)
+        for d in range(ndim):
+            ph = rng.uniform(0, 2 * np.pi, size=ndim)
+            s = np.ones(lat.shape[:2])
+            for e in range(ndim):
+                s = s * np.sin(np.pi * lat[..., e] / n[e])
+            x[..., d] += warp * s * np.cos(ph[d])
+    xyz = x * h
+    wrap = lat.copy()
+    for d in range(ndim):
+        if periodic[d]:
+            wrap[..., d] %= n[d]
+    m = [n[d] + (0 if periodic[d] else 1) for d in range(ndim)]
+    vid = wrap[..., 0] + m[0] * wrap[..., 1]
+    if ndim == 3:
+        vid = vid + m[0] * m[1] * wrap[..., 2]
+    vtx = 1 + vid
+    # faces: (axis, low face, high face) in preprocessor numbering
+    faces = {0: (4, 2), 1: (1, 3), 2: (5, 6)}
+    recs = []
+    stride = [1, n[0], n[0] * (n[1] if ndim > 1 else 1)]
+    for d in range(ndim):
+        if not periodic[d]:
+            continue
+        lo, hi = faces[d]
+        low = np.flatnonzero(idx[d] == 0)
+        for e0 in low:
+            e1 = e0 + (n[d] - 1) * stride[d]
+            recs.append((e0 + 1, lo, e1 + 1, hi))
+            recs.append((e1 + 1, hi, e0 + 1, lo))
+    rec = np.array(recs, dtype=np.int64).reshape(-1, 4)
+    return vtx.astype(np.int64), xyz, rec

@@ -1,5 +1,10 @@
 """Connectivity (parrsb_conn_mesh): vertex ids from corner coordinates.
 
+Periodic faces (con-periodic.c) are pinned the same way: boxes and quad
+grids periodic along one, two or all axes (corner classes that chain across
+several periodic pairs) must give the wrapped lattice's vertex classes, with
+ids 1..#vertices; faces whose partner does not fit fail.
+
 The oracle is pinned by the reference's own acceptance test (gencon.c:21-63:
 the ids define the same vertex classes as a known connectivity, both ways)
 on generated meshes whose connectivity is known, including a periodic
@@ -101,3 +106,65 @@ def test_device_checks_and_partition_of_generated_ids(oracle):
     p2, _ = pa.part_mesh(vtx, xyz, 8, 16)
     q1, q2 = pa.quality(p1, vtx, 8, 16), pa.quality(p2, vtx, 8, 16)
     assert q1 == q2
+
+
+PERIODIC = {
+    "hex-x": dict(nx=5, ny=4, nz=3, periodic=(True, False, False)),
+    "hex-xyz": dict(nx=5, ny=4, nz=3, periodic=(True, True, True)),
+    "hex-yz-warped": dict(nx=4, ny=6, nz=5, periodic=(False, True, True), warp=0.15),
+    "quad-xy": dict(nx=7, ny=5, nz=0, periodic=(True, True, False), warp=0.1),
+    "quad-y": dict(nx=6, ny=5, nz=0, periodic=(False, True, False)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PERIODIC))
+def test_oracle_periodic_faces_give_the_wrapped_lattice(oracle, name):
+    vtx, xyz, rec = M.periodic_box(**PERIODIC[name])
+    ndim = xyz.shape[2]
+    nv = 8 if ndim == 3 else 4
+    rc, out = oracle.conn_mesh(xyz, ndim, 0.2, rec)
+    assert rc == 0
+    assert same_classes(out, vtx[:, PRE_TO_SYM[:nv]])
+    assert out.min() == 1 and out.max() == len(np.unique(vtx))
+    # without the records the periodic faces stay apart
+    rc0, out0 = oracle.conn_mesh(xyz, ndim, 0.2)
+    assert rc0 == 0 and out0.max() > out.max()
+
+
+def test_oracle_periodic_faces_that_do_not_fit_fail(oracle):
+    vtx, xyz, rec = M.periodic_box(5, 4, 3, periodic=(True, False, False))
+    bad = rec.copy()
+    bad[0, 1], bad[1, 3] = 1, 1  # element face y=0 paired with a face x=1
+    rc, _ = oracle.conn_mesh(xyz, 3, 0.2, bad)
+    assert rc == 1
+    out_of_range = rec.copy()
+    out_of_range[0, 0] = 10 ** 6
+    rc, _ = oracle.conn_mesh(xyz, 3, 0.2, out_of_range)
+    assert rc == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(PERIODIC))
+def test_device_periodic_ids_equal_oracle(oracle, name):
+    import parrsb_amd as pa
+    vtx, xyz, rec = M.periodic_box(**PERIODIC[name])
+    ndim = xyz.shape[2]
+    rc, ref = oracle.conn_mesh(xyz, ndim, 0.2, rec)
+    out = pa.conn_mesh(xyz, ndim, 0.2, periodic=rec)
+    assert rc == 0 and np.array_equal(out, ref)
+
+
+@pytest.mark.gpu
+def test_device_periodic_mismatch_and_range_errors():
+    import parrsb_amd as pa
+    vtx, xyz, rec = M.periodic_box(5, 4, 3, periodic=(True, False, False))
+    bad = rec.copy()
+    bad[0, 1], bad[1, 3] = 1, 1
+    with pytest.raises(pa.ParrsbError) as e:
+        pa.conn_mesh(xyz, 3, 0.2, periodic=bad)
+    assert e.value.rc == 1 and "did not match" in str(e.value)
+    out_of_range = rec.copy()
+    out_of_range[0, 0] = 10 ** 6
+    with pytest.raises(pa.ParrsbError) as e:
+        pa.conn_mesh(xyz, 3, 0.2, periodic=out_of_range)
+    assert e.value.rc == 3
