@@ -94,6 +94,22 @@ int rsbgpu_level_laplacian(rsbgpu *g);
 int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_passes, double tol,
                    int init_mode, rsbgpu_fiedler_out *out);
 int rsbgpu_set_start(rsbgpu *g, const float *d_start_per_element);
+/* Fiedler vectors by MG-preconditioned inverse iteration (rsb_algo = 1,
+ * fiedler.c:94-321 with rsb_mg_grammian = 0; multigrid.c:184-317), FP64,
+ * for every segment with p > 1 and at least 2 elements: each projection is
+ * at most max_iter preconditioned CG iterations to tol, at most max_passes
+ * passes, stopping after a projection of one iteration.  factor (> 1) is the
+ * MG coarsening factor.  out->niter = summed CG iterations, out->npass =
+ * passes, out->lambda / out->resid = Rayleigh quotient and relative
+ * eigen-residual of the result.  Same result layout as rsbgpu_fiedler. */
+int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double tol, int factor,
+                      rsbgpu_fiedler_out *out);
+/* Parity hook: MG level `level` (>= 1) of segment `seg` from the last
+ * rsbgpu_fiedler_mg call, in the segment's local numbering: rowptr[n+1],
+ * col[nnz], w[nnz] (positive off-diagonal weights), diag[n].  With NULL
+ * arrays only *n and *nnz are returned. */
+int rsbgpu_mg_level_csr(rsbgpu *g, int level, int seg, int *n, int64_t *nnz, int64_t *rowptr,
+                        int *col, double *w, double *diag);
 
 /* Stable sort of every segment with p > 1 by its Fiedler values. */
 int rsbgpu_fiedler_sort(rsbgpu *g);

@@ -12,6 +12,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef M_PI
+#define M_PI 3.141592653589793
+#endif
+
 void oracle_free(void *p) { free(p); }
 
 static void *xmalloc(size_t n) {
@@ -640,6 +644,324 @@ int oracle_lanczos_pass(const long long *vtx, int nv, const int *elems, int n, i
 }
 
 /* ------------------------------------------------------------------------ */
+/* MG-preconditioned inverse iteration, rsb_algo = 1.
+ *
+ * Matrices (mat.c:17-61, 704-764): CSR of the off-diagonal weights, stored
+ * positive (the reference stores them negated), and the diagonal, which is
+ * always the row sum of those weights.  Level 0 is the segment's
+ * shared-vertex Laplacian.  Level l + 1 aggregates `factor` consecutive rows
+ * of level l (row i -> i / factor, multigrid.c:147-160): entries between two
+ * aggregates add up, entries inside one aggregate vanish (the diagonal is
+ * reset to the row sum of what remains).  Levels continue until one row is
+ * left (multigrid.c:221-235). */
+typedef struct {
+  int n;
+  int64_t *rp;
+  int *ci;
+  double *w, *diag;
+} mgm_t;
+
+typedef struct {
+  int nl, factor;
+  mgm_t *M;
+} mgh_t;
+
+typedef struct {
+  int r, c;
+  double w;
+} rcw_t;
+
+static int cmp_rcw(const void *a, const void *b) {
+  const rcw_t *x = a, *y = b;
+  if (x->r != y->r) return x->r - y->r;
+  return x->c - y->c;
+}
+
+/* CSR from (row, col, weight) triples: duplicates add, row == col dropped. */
+static void mgm_from_triples(mgm_t *M, int n, rcw_t *t, int64_t nt) {
+  qsort(t, (size_t)nt, sizeof(rcw_t), cmp_rcw);
+  M->n = n;
+  M->rp = xmalloc(sizeof(int64_t) * (size_t)(n + 1));
+  M->ci = xmalloc(sizeof(int) * (size_t)(nt ? nt : 1));
+  M->w = xmalloc(sizeof(double) * (size_t)(nt ? nt : 1));
+  M->diag = xmalloc(sizeof(double) * (size_t)(n ? n : 1));
+  int64_t k = 0, i = 0;
+  for (int r = 0; r < n; r++) {
+    M->rp[r] = k;
+    double d = 0.0;
+    while (i < nt && t[i].r == r) {
+      if (t[i].c == r) {
+        i++;
+        continue;
+      }
+      if (k > M->rp[r] && M->ci[k - 1] == t[i].c) {
+        M->w[k - 1] += t[i].w;
+      } else {
+        M->ci[k] = t[i].c, M->w[k] = t[i].w, k++;
+      }
+      i++;
+    }
+    for (int64_t j = M->rp[r]; j < k; j++) d += M->w[j];
+    M->diag[r] = d;
+  }
+  M->rp[n] = k;
+}
+
+static void mgm_free(mgm_t *M) { free(M->rp), free(M->ci), free(M->w), free(M->diag); }
+
+/* y = M x = diag x - A x (mat.c:829-846) */
+static void mgm_apply(const mgm_t *M, const double *x, double *y) {
+  for (int i = 0; i < M->n; i++) {
+    double s = M->diag[i] * x[i];
+    for (int64_t j = M->rp[i]; j < M->rp[i + 1]; j++) s -= M->w[j] * x[M->ci[j]];
+    y[i] = s;
+  }
+}
+
+static void mg_setup(mgh_t *h, const gs_lap *L, int factor) {
+  /* level 0 from the vertex groups: one unit of weight per shared id */
+  int64_t nt = 0, cap = 1024;
+  rcw_t *t = xmalloc(sizeof(rcw_t) * (size_t)cap);
+  for (int g = 0; g < L->ng; g++)
+    for (int a = L->gstart[g]; a < L->gstart[g + 1]; a++)
+      for (int b = L->gstart[g]; b < L->gstart[g + 1]; b++) {
+        const int ea = L->gslots[a] / L->nv, eb = L->gslots[b] / L->nv;
+        if (ea == eb) continue;
+        if (nt == cap) cap *= 2, t = realloc(t, sizeof(rcw_t) * (size_t)cap);
+        t[nt].r = ea, t[nt].c = eb, t[nt].w = 1.0, nt++;
+      }
+  h->factor = factor;
+  h->nl = 1;
+  h->M = xmalloc(sizeof(mgm_t));
+  mgm_from_triples(&h->M[0], L->n, t, nt);
+  while (h->M[h->nl - 1].n > 1) {
+    const mgm_t *F = &h->M[h->nl - 1];
+    const int nc = (F->n + factor - 1) / factor;
+    nt = 0;
+    for (int i = 0; i < F->n; i++)
+      for (int64_t j = F->rp[i]; j < F->rp[i + 1]; j++) {
+        if (nt == cap) cap *= 2, t = realloc(t, sizeof(rcw_t) * (size_t)cap);
+        t[nt].r = i / factor, t[nt].c = F->ci[j] / factor, t[nt].w = F->w[j], nt++;
+      }
+    h->M = realloc(h->M, sizeof(mgm_t) * (size_t)(h->nl + 1));
+    mgm_from_triples(&h->M[h->nl], nc, t, nt);
+    h->nl++;
+  }
+  free(t);
+}
+
+static void mg_free(mgh_t *h) {
+  for (int l = 0; l < h->nl; l++) mgm_free(&h->M[l]);
+  free(h->M);
+}
+
+/* Chebyshev-spaced Jacobi weights 1/lambda_k on [1, 2] (multigrid.c:24-30) */
+static double sigma_cheb(int k, int n, double lmin, double lmax) {
+  k = (k - 1) % n + 1;
+  const double theta = M_PI * (k - 0.5) / n;
+  return 1.0 / (lmin + 0.5 * (lmax - lmin) * (cos(theta) + 1.0));
+}
+
+/* One V-cycle (multigrid.c:235-303): on every level but the last, three
+ * weighted Jacobi steps from zero (weights sigma_cheb(1..3, 4, 1, 2)) and
+ * the residual summed into the aggregates; on the one-row level the solve
+ * (zero, since that row's diagonal is zero); going up, each level's result
+ * is its smoothed correction plus over times its aggregate's result (over =
+ * 4/3 on level 0 as 1.33333, 1.5 above).  A zero diagonal gives a zero
+ * Jacobi step here (the reference divides by it). */
+static void mg_vcycle_run(const mgh_t *h, const double *rhs, double *out) {
+  double **r = xmalloc(sizeof(double *) * (size_t)h->nl), **u = xmalloc(sizeof(double *) * (size_t)h->nl);
+  for (int l = 0; l < h->nl; l++) {
+    r[l] = calloc((size_t)h->M[l].n + 1, sizeof(double));
+    u[l] = calloc((size_t)h->M[l].n + 1, sizeof(double));
+  }
+  memcpy(r[0], rhs, sizeof(double) * (size_t)h->M[0].n);
+  double *s = xmalloc(sizeof(double) * (size_t)(h->M[0].n + 1));
+  double *gs = xmalloc(sizeof(double) * (size_t)(h->M[0].n + 1));
+  for (int l = 0; l + 1 < h->nl; l++) {
+    const mgm_t *M = &h->M[l];
+    const int n = M->n;
+    for (int step = 0; step < 3; step++) {
+      const double sg = sigma_cheb(step + 1, 4, 1.0, 2.0);
+      for (int i = 0; i < n; i++) s[i] = M->diag[i] != 0.0 ? sg * r[l][i] / M->diag[i] : 0.0;
+      for (int i = 0; i < n; i++) u[l][i] = step == 0 ? s[i] : u[l][i] + s[i];
+      mgm_apply(M, s, gs);
+      for (int i = 0; i < n; i++) r[l][i] -= gs[i];
+    }
+    for (int i = 0; i < n; i++) r[l + 1][i / h->factor] += r[l][i];
+  }
+  {
+    const mgm_t *M = &h->M[h->nl - 1];
+    if (M->n == 1) {
+      const double v = fabs(M->diag[0]) > 1e-6 ? r[h->nl - 1][0] / M->diag[0] : 0.0;
+      r[h->nl - 1][0] = v;
+    }
+  }
+  for (int l = h->nl - 2; l >= 0; l--) {
+    const double over = l == 0 ? 1.33333 : 1.5;
+    for (int i = 0; i < h->M[l].n; i++) r[l][i] = over * r[l + 1][i / h->factor] + u[l][i];
+  }
+  memcpy(out, r[0], sizeof(double) * (size_t)h->M[0].n);
+  for (int l = 0; l < h->nl; l++) free(r[l]), free(u[l]);
+  free(r), free(u), free(s), free(gs);
+}
+
+/* Projection (fiedler.c:94-187): x ~ L^+ b by conjugate gradients with the
+ * V-cycle as preconditioner, its result kept orthogonal to the constant
+ * vector; the new direction uses r.(z - z_old) / (r.z_old) and is made
+ * L-orthogonal to every earlier one.  Stops when |r|^2 < tol^2 |b|^2 or
+ * |r| < tol; returns the iterations taken (miter if it never stopped). */
+static int mg_project(double *x, int n, const double *b, const gs_lap *L, const mgh_t *h,
+                      int miter, double tol) {
+  if (n == 0) return 0;
+  double *z = xmalloc(sizeof(double) * 6 * (size_t)n);
+  double *w = z + n, *r = w + n, *p = r + n, *z0 = p + n, *dz = z0 + n;
+  double *P = xmalloc(sizeof(double) * 2 * (size_t)(miter + 1) * n), *W = P + (size_t)n * (miter + 1);
+  for (int i = 0; i < n; i++) x[i] = 0.0, r[i] = b[i];
+  double rr = dotp(r, r, n);
+  const double rtol = rr * tol * tol;
+  memcpy(z, r, sizeof(double) * (size_t)n);
+  ortho(z, n);
+  double rz1 = dotp(z, z, n), rzt, rz2, alpha, beta;
+  memcpy(p, z, sizeof(double) * (size_t)n);
+  int i;
+  for (i = 0; i < miter; i++) {
+    gs_lap_apply(L, p, w);
+    double pw = dotp(p, w, n);
+    alpha = rz1 / pw;
+    pw = 1.0 / sqrt(pw);
+    for (int j = 0; j < n; j++) W[(size_t)i * n + j] = pw * w[j], P[(size_t)i * n + j] = pw * p[j];
+    for (int j = 0; j < n; j++) x[j] += alpha * p[j], r[j] -= alpha * w[j];
+    rr = dotp(r, r, n);
+    if (rr < rtol || sqrt(rr) < tol) break;
+    memcpy(z0, z, sizeof(double) * (size_t)n);
+    mg_vcycle_run(h, r, z);
+    rzt = rz1;
+    ortho(z, n);
+    rz1 = dotp(r, z, n);
+    for (int j = 0; j < n; j++) dz[j] = z[j] - z0[j];
+    rz2 = dotp(r, dz, n);
+    beta = rz2 / rzt;
+    for (int j = 0; j < n; j++) p[j] = z[j] + beta * p[j];
+    double *acc = P + (size_t)miter * n;
+    for (int k = 0; k < n; k++) acc[k] = 0.0;
+    for (int j = 0; j <= i; j++) {
+      const double c = dotp(W + (size_t)j * n, p, n);
+      for (int k = 0; k < n; k++) acc[k] += c * P[(size_t)j * n + k];
+    }
+    for (int k = 0; k < n; k++) p[k] -= acc[k];
+  }
+  free(z), free(P);
+  return i == miter ? i : i + 1;
+}
+
+/* Inverse iteration (fiedler.c:191-321, rsb_mg_grammian = 0): y = project(z);
+ * orthogonalise y; z = y / |y| orthogonalised; stop after a projection that
+ * took a single iteration (z was already an eigenvector to tol). */
+static int mg_inverse(double *y, double *z, int n, const gs_lap *L, const mgh_t *h, int miter,
+                      int mpass, double tol, int *passes) {
+  int iters = 0, i;
+  for (i = 0; i < mpass; i++) {
+    const int ppfi = mg_project(y, n, z, L, h, miter, tol);
+    iters += ppfi;
+    ortho(y, n);
+    const double nrm = dotp(y, y, n), ni = 1.0 / sqrt(nrm);
+    for (int j = 0; j < n; j++) z[j] = y[j] * ni;
+    ortho(z, n);
+    if (ppfi == 1) {
+      i++;
+      break;
+    }
+  }
+  if (passes) *passes = i;
+  return iters;
+}
+
+int oracle_fiedler_mg(const long long *vtx, int nv, const int *elems, int n, int max_iter,
+                      int max_passes, double tol, int factor, double *fiedler,
+                      int *passes_out) {
+  double *z = xmalloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    z[i] = i + 1.0;
+    if (i < n / 2) z[i] += (double)(1000 * (long long)n);
+  }
+  ortho(z, n);
+  const double rtr = dotp(z, z, n), rni = 1.0 / sqrt(rtr);
+  for (int i = 0; i < n; i++) z[i] *= rni;
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  mgh_t h;
+  mg_setup(&h, &L, factor);
+  const int it = mg_inverse(fiedler, z, n, &L, &h, max_iter, max_passes, tol, passes_out);
+  mg_free(&h);
+  gs_lap_free(&L);
+  double nrm = 0.0;
+  for (int i = 0; i < n; i++) nrm += fiedler[i] * fiedler[i];
+  const double ni = 1.0 / sqrt(nrm);
+  for (int i = 0; i < n; i++) fiedler[i] *= ni;
+  free(z);
+  return it;
+}
+
+int oracle_mg_levels(const long long *vtx, int nv, const int *elems, int n, int factor,
+                     int *sizes, int64_t *nnz) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  mgh_t h;
+  mg_setup(&h, &L, factor);
+  for (int l = 0; l < h.nl; l++) {
+    if (sizes) sizes[l] = h.M[l].n;
+    if (nnz) nnz[l] = h.M[l].rp[h.M[l].n];
+  }
+  const int nl = h.nl;
+  mg_free(&h);
+  gs_lap_free(&L);
+  return nl;
+}
+
+int oracle_mg_level_matrix(const long long *vtx, int nv, const int *elems, int n, int factor,
+                           int level, int64_t *rowptr, int *col, double *w, double *diag) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  mgh_t h;
+  mg_setup(&h, &L, factor);
+  int rc = level < 0 || level >= h.nl;
+  if (!rc) {
+    const mgm_t *M = &h.M[level];
+    memcpy(rowptr, M->rp, sizeof(int64_t) * (size_t)(M->n + 1));
+    memcpy(col, M->ci, sizeof(int) * (size_t)M->rp[M->n]);
+    memcpy(w, M->w, sizeof(double) * (size_t)M->rp[M->n]);
+    memcpy(diag, M->diag, sizeof(double) * (size_t)M->n);
+  }
+  mg_free(&h);
+  gs_lap_free(&L);
+  return rc;
+}
+
+void oracle_mg_vcycle(const long long *vtx, int nv, const int *elems, int n, int factor,
+                      const double *rhs, double *out) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  mgh_t h;
+  mg_setup(&h, &L, factor);
+  mg_vcycle_run(&h, rhs, out);
+  mg_free(&h);
+  gs_lap_free(&L);
+}
+
+int oracle_mg_project(const long long *vtx, int nv, const int *elems, int n, int factor,
+                      int miter, double tol, const double *b, double *x) {
+  gs_lap L;
+  gs_lap_init(&L, vtx, nv, elems, n);
+  mgh_t h;
+  mg_setup(&h, &L, factor);
+  const int it = mg_project(x, n, b, &L, &h, miter, tol);
+  mg_free(&h);
+  gs_lap_free(&L);
+  return it;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Components by union-find over shared vertex ids. */
 static int uf_find(int *par, int x) {
   while (par[x] != x) {
@@ -809,7 +1131,10 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       /* Fiedler (fiedler.c:563-621) */
       int it = 0;
       double lam = 0.0;
-      if (n >= 2) {
+      if (n >= 2 && o->rsb_algo == 1) {
+        it = oracle_fiedler_mg(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol,
+                               o->mg_factor, fv, NULL);
+      } else if (n >= 2) {
         it = oracle_fiedler(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol, fv, &lam,
                             NULL);
       } else {

@@ -38,6 +38,8 @@ typedef struct {
   int max_iter, max_passes;
   double tol;
   int nparts;
+  int rsb_algo;  /* 0 Lanczos, 1 MG-preconditioned inverse iteration */
+  int mg_factor; /* MG coarsening factor (rsb_mg_factor, > 1) */
 } oracle_opts;
 
 /* Per-cut (level) statistics, as the reference's metric stack records them
@@ -124,6 +126,28 @@ int oracle_fiedler_from(const long long *vtx, int nv, const int *elems, int n,
 int oracle_lanczos_pass(const long long *vtx, int nv, const int *elems, int n, int niter,
                         double tol, const double *f, double *diag, double *upper,
                         double *basis);
+
+/* Fiedler vector by MG-preconditioned inverse iteration (rsb_algo = 1,
+ * fiedler.c:94-321 with rsb_mg_grammian = 0, multigrid.c:184-317,
+ * mat.c:17-61, 704-764) of the sub-mesh elems[0..n), from the same start
+ * vector as the Lanczos path.  fiedler[n] normalised; returns the summed PCG
+ * iterations (RSB_FIEDLER_CALC_NITER); passes_out (may be NULL) receives the
+ * number of inverse-iteration passes. */
+int oracle_fiedler_mg(const long long *vtx, int nv, const int *elems, int n, int max_iter,
+                      int max_passes, double tol, int factor, double *fiedler,
+                      int *passes_out);
+
+/* Pieces of it, for the tests: the MG hierarchy's level sizes and matrices
+ * (CSR of positive off-diagonal weights plus the diagonal, level by level,
+ * concatenated), one V-cycle, and one projection (PCG solve of L x = b). */
+int oracle_mg_levels(const long long *vtx, int nv, const int *elems, int n, int factor,
+                     int *sizes, int64_t *nnz);
+int oracle_mg_level_matrix(const long long *vtx, int nv, const int *elems, int n, int factor,
+                           int level, int64_t *rowptr, int *col, double *w, double *diag);
+void oracle_mg_vcycle(const long long *vtx, int nv, const int *elems, int n, int factor,
+                      const double *rhs, double *out);
+int oracle_mg_project(const long long *vtx, int nv, const int *elems, int n, int factor,
+                      int miter, double tol, const double *b, double *x);
 
 /* Connected components of the sub-mesh elems[0..n) under shared-vertex
  * adjacency (components.c:160-328 returns the same count). */

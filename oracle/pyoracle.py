@@ -35,7 +35,7 @@ def lib():
 class Opts(C.Structure):
     _fields_ = [("find_disconnected_comps", C.c_int), ("rsb_pre", C.c_int),
                 ("max_iter", C.c_int), ("max_passes", C.c_int), ("tol", C.c_double),
-                ("nparts", C.c_int)]
+                ("nparts", C.c_int), ("rsb_algo", C.c_int), ("mg_factor", C.c_int)]
 
 
 ML = 32
@@ -89,6 +89,16 @@ def _declare(L):
                                       P, P]
     L.oracle_lanczos_pass.restype = C.c_int
     L.oracle_components.argtypes = [P, C.c_int, P, C.c_int]
+    L.oracle_fiedler_mg.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_int, P, P]
+    L.oracle_fiedler_mg.restype = C.c_int
+    L.oracle_mg_levels.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+    L.oracle_mg_levels.restype = C.c_int
+    L.oracle_mg_level_matrix.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, P, P, P, P]
+    L.oracle_mg_level_matrix.restype = C.c_int
+    L.oracle_mg_vcycle.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
+    L.oracle_mg_project.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double, P, P]
+    L.oracle_mg_project.restype = C.c_int
     L.oracle_components.restype = C.c_int
     L.oracle_part_mesh.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(Opts),
                                    C.POINTER(Report)]
@@ -238,8 +248,62 @@ def components(vtx, nv, elems):
     return lib().oracle_components(_p(vtx), nv, _p(elems), elems.size)
 
 
+def fiedler_mg(vtx, nv, elems, max_iter=50, max_passes=50, tol=1e-5, factor=2):
+    """(fiedler, iterations, passes) of the MG inverse iteration."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    f = np.zeros(elems.size)
+    npass = C.c_int()
+    it = lib().oracle_fiedler_mg(_p(vtx), nv, _p(elems), elems.size, max_iter, max_passes, tol,
+                                 factor, _p(f), C.byref(npass))
+    return f, it, npass.value
+
+
+def mg_levels(vtx, nv, elems, factor=2):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    sizes = np.zeros(64, dtype=np.int32)
+    nnz = np.zeros(64, dtype=np.int64)
+    nl = lib().oracle_mg_levels(_p(vtx), nv, _p(elems), elems.size, factor, _p(sizes), _p(nnz))
+    return sizes[:nl], nnz[:nl]
+
+
+def mg_level_matrix(vtx, nv, elems, level, factor=2):
+    """(rowptr, col, w, diag) of one MG level (positive off-diagonal weights)."""
+    sizes, nnz = mg_levels(vtx, nv, elems, factor)
+    n, m = int(sizes[level]), int(nnz[level])
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    rp = np.zeros(n + 1, dtype=np.int64)
+    col = np.zeros(max(m, 1), dtype=np.int32)
+    w = np.zeros(max(m, 1))
+    d = np.zeros(max(n, 1))
+    lib().oracle_mg_level_matrix(_p(vtx), nv, _p(elems), elems.size, factor, level, _p(rp),
+                                 _p(col), _p(w), _p(d))
+    return rp, col[:m], w[:m], d[:n]
+
+
+def mg_vcycle(vtx, nv, elems, rhs, factor=2):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    out = np.zeros_like(rhs)
+    lib().oracle_mg_vcycle(_p(vtx), nv, _p(elems), elems.size, factor, _p(rhs), _p(out))
+    return out
+
+
+def mg_project(vtx, nv, elems, b, miter=50, tol=1e-5, factor=2):
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    elems = np.ascontiguousarray(elems, dtype=np.int32)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    it = lib().oracle_mg_project(_p(vtx), nv, _p(elems), elems.size, factor, miter, tol, _p(b),
+                                 _p(x))
+    return x, it
+
+
 def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
-              max_passes=50, tol=1e-5):
+              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2):
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     nel = vtx.size // nv
     xp = None
@@ -247,7 +311,7 @@ def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         xp = _p(xyz)
     part = np.full(nel, -1, dtype=np.int32)
-    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts)
+    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts, rsb_algo, mg_factor)
     rep = Report()
     rc = lib().oracle_part_mesh(_p(part), _p(vtx), xp, nel, nv, C.byref(o), C.byref(rep))
     if rc != 0:
@@ -256,7 +320,7 @@ def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
 
 
 def part_mesh_tagged(vtx, xyz, nv, nparts, tag, rsb_pre=1, find_comps=1, max_iter=50,
-                     max_passes=50, tol=1e-5):
+                     max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2):
     """Tagged partitioning; returns part (raises on rc != 0)."""
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     nel = vtx.size // nv
@@ -266,7 +330,7 @@ def part_mesh_tagged(vtx, xyz, nv, nparts, tag, rsb_pre=1, find_comps=1, max_ite
         xp = _p(xyz)
     tag = np.ascontiguousarray(tag, dtype=np.int32)
     part = np.full(nel, -1, dtype=np.int32)
-    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts)
+    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts, rsb_algo, mg_factor)
     rc = lib().oracle_part_mesh_tagged(_p(part), _p(vtx), xp, _p(tag), nel, nv, C.byref(o))
     if rc != 0:
         raise RuntimeError(f"oracle_part_mesh_tagged returned {rc}")

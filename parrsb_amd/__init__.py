@@ -135,6 +135,8 @@ SIGNATURES = {
     "rsbgpu_level_laplacian": (I, [P]),
     "rsbgpu_fiedler": (I, [P, I, I, I, C.c_double, I, C.POINTER(FiedlerOut)]),
     "rsbgpu_set_start": (I, [P, P]),
+    "rsbgpu_fiedler_mg": (I, [P, I, I, C.c_double, I, C.POINTER(FiedlerOut)]),
+    "rsbgpu_mg_level_csr": (I, [P, I, I, C.POINTER(I), C.POINTER(C.c_int64), P, P, P, P]),
     "rsbgpu_fiedler_sort": (I, [P]),
     "rsbgpu_bisect": (I, [P, P, P, P, I, P, P]),
     "rsbgpu_write_part": (I, [P, P, I]),
@@ -451,6 +453,30 @@ class Device:
         self._chk(lib().rsbgpu_fiedler(self.h, precision, max_iter, max_passes, tol, init_mode,
                                        C.byref(fo)))
         return out
+
+    def fiedler_mg(self, nseg, max_iter=50, max_passes=50, tol=1e-5, factor=2):
+        """MG-preconditioned inverse iteration (rsb_algo = 1) on every segment."""
+        out = {k: np.zeros(nseg, dtype=np.int32) for k in ("niter", "npass", "fallback")}
+        out["lambda"] = np.zeros(nseg)
+        out["rnorm"] = np.zeros(3 * nseg)
+        out["resid"] = np.zeros(nseg)
+        fo = FiedlerOut(_p(out["niter"]), _p(out["npass"]), _p(out["lambda"]),
+                        _p(out["rnorm"]), _p(out["resid"]), _p(out["fallback"]))
+        self._chk(lib().rsbgpu_fiedler_mg(self.h, max_iter, max_passes, tol, factor,
+                                          C.byref(fo)))
+        return out
+
+    def mg_level_csr(self, level, seg):
+        n, nnz = C.c_int(), C.c_int64()
+        self._chk(lib().rsbgpu_mg_level_csr(self.h, level, seg, C.byref(n), C.byref(nnz), None,
+                                            None, None, None))
+        rp = np.zeros(n.value + 1, dtype=np.int64)
+        col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        w = np.zeros(max(nnz.value, 1))
+        d = np.zeros(max(n.value, 1))
+        self._chk(lib().rsbgpu_mg_level_csr(self.h, level, seg, C.byref(n), C.byref(nnz),
+                                            _p(rp), _p(col), _p(w), _p(d)))
+        return rp, col[:nnz.value], w[:nnz.value], d[:n.value]
 
     # learned warm start (device parameter arrays given as integer pointers,
     # e.g. torch tensor data_ptr())

@@ -64,6 +64,7 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   if (!g) return;
   hipSetDevice(g->device);
   if (g->stream) hipStreamSynchronize(g->stream);
+  rsb_mg_free(g);
   FREE(g->own_vtx);
   FREE(g->own_xyz);
   FREE(g->d_coord);

@@ -1,0 +1,986 @@
+// Fiedler vectors by MG-preconditioned inverse iteration (rsb_algo = 1;
+// fiedler.c:94-321 with rsb_mg_grammian = 0, multigrid.c:184-317, the
+// matrices of mat.c:17-61 and 704-764), for every segment of a level at once.
+//
+// Level 0 of the hierarchy is the level Laplacian already assembled for the
+// Lanczos path (padded rows, SELL-64 entries with 4-bit weights, diagonal =
+// row sum of the weights).  Level l + 1 aggregates `factor` consecutive rows
+// of level l inside each segment (row i -> i / factor); its matrix is built
+// on the device: every entry of the children rows is keyed (coarse row,
+// coarse column), one radix sort groups equal keys, entries inside an
+// aggregate are dropped and equal keys add up (integer-valued, so the sums
+// are exact in any order).  Coarse rows are compact CSR in FP64.
+//
+// The solve is the reference's, segment by segment in lockstep: every
+// reduction is a per-tile partial sum (fixed tree) finished by a
+// per-segment kernel (tiles in order), so results do not depend on
+// scheduling.  Segments that finished skip every launch.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "state.hpp"
+
+#define MG_NP 64  // partial-sum slots per tile (rsb_max_iter <= 60 dots + spare)
+
+// per-segment FP64 scalars
+enum { MS_RZ1 = 0, MS_ALPHA, MS_PWINV, MS_BETA, MS_RTOL, MS_RR, MS_MEAN, MS_NI, MS_LAMBDA,
+       MS_RES, MS_NSC };
+// per-segment ints
+enum { MI_INV = 0, MI_PCG, MI_IT, MI_PPFI, MI_ITERS, MI_NPASS, MI_NC, MI_NI };
+// what a finishing kernel does with a segment's totals
+enum { FIN_MEAN = 0, FIN_NORM, FIN_PASS0, FIN_RZ1, FIN_ALPHA, FIN_CONV, FIN_BETA,
+       FIN_PASSEND, FIN_RAYLEIGH };
+// which segments a launch works on
+enum { GATE_ELIG = 0, GATE_INV, GATE_PCG };
+
+struct mg_args {
+  int nseg;
+  const int *seg;        // 5*nseg: cstart, n, p, r0, pstart
+  const int *tile;       // 3*ntiles: seg, row0, row1
+  const int *seg_tiles;  // 2*nseg: first tile, count
+  double *sc;            // MS_NSC per segment
+  int *st;               // MI_NI per segment
+  double *partial;       // MG_NP per tile
+  int *active;           // count written by finishing kernels
+  double tol;
+  int miter, mpass;
+};
+
+#define MSC(s, k) a.sc[(size_t)(s) * MS_NSC + (k)]
+#define MST(s, k) a.st[(size_t)(s) * MI_NI + (k)]
+
+__device__ inline bool mg_elig(const mg_args &a, int s) {
+  return a.seg[2 * a.nseg + s] > 1 && a.seg[a.nseg + s] >= 2;
+}
+__device__ inline bool mg_gate(const mg_args &a, int s, int gate) {
+  if (gate == GATE_ELIG) return mg_elig(a, s);
+  if (gate == GATE_INV) return MST(s, MI_INV) != 0;
+  return MST(s, MI_PCG) != 0;
+}
+
+// Tile prologue: rows [row0, row1) of segment s, valid while local < n.
+struct mg_tile {
+  int ti, s, row0, row1, pstart, n;
+};
+__device__ inline bool mg_tile_of(const mg_args &a, int gate, mg_tile &t) {
+  t.ti = blockIdx.x;
+  t.s = a.tile[3 * t.ti];
+  if (!mg_gate(a, t.s, gate)) return false;
+  t.row0 = a.tile[3 * t.ti + 1];
+  t.row1 = a.tile[3 * t.ti + 2];
+  t.n = a.seg[a.nseg + t.s];
+  t.pstart = a.seg[4 * a.nseg + t.s];
+  return true;
+}
+#define MG_ROWS(t, row) \
+  for (int row = (t).row0 + (int)threadIdx.x; row < (t).row1 && row - (t).pstart < (t).n; row += RSB_BLOCK)
+
+template <int NV>
+__device__ inline void mg_publish(const mg_args &a, const mg_tile &t, double (&v)[NV]) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  block_sum<NV>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NV; k++) a.partial[(size_t)t.ti * MG_NP + k] = v[k];
+}
+
+// Level-0 Laplacian row: sum_j w_ij x_j over the SELL-64 entries; the row
+// sum of the weights is the diagonal.
+struct mg_sell {
+  const int64_t *slice_off;
+  const uint32_t *ell;
+  const int *rowlen;
+  const double *diag;
+};
+__device__ inline double sell_offsum(const mg_sell &L, int row, const double *x) {
+  const int64_t base = L.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
+  const int len = L.rowlen[row];
+  double s = 0.0;
+  for (int j = 0; j < len; j++) {
+    const uint32_t e = L.ell[base + (int64_t)j * RSB_WAVE];
+    s += (double)(e >> 28) * x[e & 0x0fffffffu];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Finishing kernel: one wave per segment sums the tile partials (tiles in
+// order, slot k by lane k) and applies the scalar step.
+__global__ void __launch_bounds__(RSB_WAVE) k_mg_fin(mg_args a, int op, int gate, int nslots) {
+  const int s = blockIdx.x;
+  if (!mg_gate(a, s, gate)) return;
+  __shared__ double tot[MG_NP];
+  const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
+  for (int k = threadIdx.x; k < nslots; k += RSB_WAVE) {
+    double v = 0.0;
+    for (int t = 0; t < count; t++) v += a.partial[(size_t)(first + t) * MG_NP + k];
+    tot[k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double n = (double)a.seg[a.nseg + s];
+  switch (op) {
+  case FIN_MEAN: MSC(s, MS_MEAN) = tot[0] / n; break;
+  case FIN_NORM: MSC(s, MS_NI) = 1.0 / sqrt(tot[0]); break;
+  case FIN_PASS0: {  // tot: sum r, r.r
+    MSC(s, MS_RR) = tot[1];
+    MSC(s, MS_RTOL) = tot[1] * a.tol * a.tol;
+    MSC(s, MS_MEAN) = tot[0] / n;
+    MST(s, MI_PCG) = 1, MST(s, MI_IT) = 0;
+    break;
+  }
+  case FIN_RZ1: MSC(s, MS_RZ1) = tot[0]; break;
+  case FIN_ALPHA: {  // tot: p.w
+    const double pw = tot[0];
+    MSC(s, MS_ALPHA) = MSC(s, MS_RZ1) / pw;
+    MSC(s, MS_PWINV) = 1.0 / sqrt(pw);
+    break;
+  }
+  case FIN_CONV: {  // tot: r.r after the update of iteration i
+    const double rr = tot[0];
+    const int i = MST(s, MI_IT);
+    MSC(s, MS_RR) = rr;
+    if (rr < MSC(s, MS_RTOL) || sqrt(rr) < a.tol) {
+      MST(s, MI_PCG) = 0, MST(s, MI_PPFI) = i + 1;
+    } else if (i + 1 == a.miter) {
+      MST(s, MI_PCG) = 0, MST(s, MI_PPFI) = a.miter;
+    } else {
+      atomicAdd(a.active, 1);
+    }
+    break;
+  }
+  case FIN_BETA: {  // tot: r.z, r.(z - z0)
+    const double rzt = MSC(s, MS_RZ1);
+    MSC(s, MS_RZ1) = tot[0];
+    MSC(s, MS_BETA) = tot[1] / rzt;
+    break;
+  }
+  case FIN_PASSEND: {  // tot: sum of the new start vector (after ortho of y)
+    MSC(s, MS_MEAN) = tot[0] / n;
+    const int ppfi = MST(s, MI_PPFI);
+    MST(s, MI_ITERS) += ppfi;
+    MST(s, MI_NPASS) += 1;
+    if (ppfi == 1 || MST(s, MI_NPASS) == a.mpass) MST(s, MI_INV) = 0;
+    else atomicAdd(a.active, 1);
+    break;
+  }
+  case FIN_RAYLEIGH: {  // tot: f.Lf, Lf.Lf, f.f
+    const double lam = tot[0] / tot[2];
+    MSC(s, MS_LAMBDA) = lam;
+    const double r2 = tot[1] - 2.0 * lam * tot[0] + lam * lam * tot[2];
+    MSC(s, MS_RES) = sqrt(r2 > 0.0 ? r2 : 0.0) / sqrt(tot[2]) / (lam != 0.0 ? fabs(lam) : 1.0);
+    break;
+  }
+  }
+}
+
+// Totals of the re-orthogonalisation dots W[j].p (j <= i) into the
+// per-segment coefficient array read by k_mg_pcorr; the iteration ends here.
+__global__ void __launch_bounds__(RSB_WAVE) k_mg_coef(mg_args a, double *coef) {
+  const int s = blockIdx.x;
+  if (!mg_gate(a, s, GATE_PCG)) return;
+  const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
+  const int nc = MST(s, MI_IT) + 1;
+  for (int k = threadIdx.x; k < nc; k += RSB_WAVE) {
+    double v = 0.0;
+    for (int t = 0; t < count; t++) v += a.partial[(size_t)(first + t) * MG_NP + k];
+    coef[(size_t)s * MG_NP + k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) MST(s, MI_NC) = nc, MST(s, MI_IT) = nc;
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels over the tiles of level 0.
+
+// start vector (fiedler.c:572-578) into b; partial sum
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_start(mg_args a, double *b) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_ELIG, t)) return;
+  double v[1] = {0.0};
+  MG_ROWS(t, row) {
+    const int i = row - t.pstart;
+    double x = i + 1.0;
+    if (i < t.n / 2) x += (double)(1000 * (long long)t.n);
+    b[row] = x;
+    v[0] += x;
+  }
+  mg_publish<1>(a, t, v);
+}
+
+// partial sum of v
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_sum(mg_args a, int gate, const double *v) {
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) acc[0] += v[row];
+  mg_publish<1>(a, t, acc);
+}
+
+// v -= mean; partial v.v
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_center_dot(mg_args a, int gate, double *v) {
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  const double m = MSC(t.s, MS_MEAN);
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double x = v[row] - m;
+    v[row] = x;
+    acc[0] += x * x;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// dst = v * ni; partial sum of dst
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_scale_sum(mg_args a, int gate, const double *v,
+                                                           double *dst) {
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  const double ni = MSC(t.s, MS_NI);
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double x = v[row] * ni;
+    dst[row] = x;
+    acc[0] += x;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// v -= mean (no partials)
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_center(mg_args a, int gate, double *v) {
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  const double m = MSC(t.s, MS_MEAN);
+  MG_ROWS(t, row) v[row] -= m;
+}
+
+// projection start (fiedler.c:110-126): x = 0, r = b; partials sum r, r.r
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_pass0(mg_args a, const double *b, double *x,
+                                                       double *r) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_INV, t)) return;
+  double acc[2] = {0.0, 0.0};
+  MG_ROWS(t, row) {
+    const double v = b[row];
+    x[row] = 0.0, r[row] = v;
+    acc[0] += v, acc[1] += v * v;
+  }
+  mg_publish<2>(a, t, acc);
+}
+
+// z = r - mean(r), p = z; partial z.z
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_pass1(mg_args a, const double *r, double *z,
+                                                       double *p) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const double m = MSC(t.s, MS_MEAN);
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double v = r[row] - m;
+    z[row] = v, p[row] = v;
+    acc[0] += v * v;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// w = L p; partial p.w
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_apply(mg_args a, int gate, mg_sell L,
+                                                       const double *p, double *w) {
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double v = L.diag[row] * p[row] - sell_offsum(L, row, p);
+    w[row] = v;
+    acc[0] += p[row] * v;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// W[i] = pwinv w, P[i] = pwinv p, x += alpha p, r -= alpha w; partial r.r
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_update(mg_args a, int64_t mpad,
+                                                        const double *p, const double *w,
+                                                        double *x, double *r, double *Wb,
+                                                        double *Pb) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const int i = MST(t.s, MI_IT);
+  const double alpha = MSC(t.s, MS_ALPHA), pwinv = MSC(t.s, MS_PWINV);
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double pv = p[row], wv = w[row];
+    Wb[(size_t)i * mpad + row] = pwinv * wv;
+    Pb[(size_t)i * mpad + row] = pwinv * pv;
+    x[row] += alpha * pv;
+    const double rv = r[row] - alpha * wv;
+    r[row] = rv;
+    acc[0] += rv * rv;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// z -= mean(z); partials r.z, r.(z - z0)
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_zdots(mg_args a, const double *r, double *z,
+                                                       const double *z0) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const double m = MSC(t.s, MS_MEAN);
+  double acc[2] = {0.0, 0.0};
+  MG_ROWS(t, row) {
+    const double zv = z[row] - m;
+    z[row] = zv;
+    acc[0] += r[row] * zv;
+    acc[1] += r[row] * (zv - z0[row]);
+  }
+  mg_publish<2>(a, t, acc);
+}
+
+// p = z + beta p; partials W[j].p for j <= i
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_pnew(mg_args a, int64_t mpad, const double *z,
+                                                      double *p, const double *Wb) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const double beta = MSC(t.s, MS_BETA);
+  const int i = MST(t.s, MI_IT);
+  double pv[RSB_TILE / RSB_BLOCK];
+  int k = 0;
+  MG_ROWS(t, row) {
+    const double v = z[row] + beta * p[row];
+    p[row] = v;
+    pv[k++] = v;
+  }
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  for (int j = 0; j <= i; j++) {
+    double acc[1] = {0.0};
+    int q = 0;
+    MG_ROWS(t, row) acc[0] += Wb[(size_t)j * mpad + row] * pv[q++];
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) a.partial[(size_t)t.ti * MG_NP + j] = acc[0];
+    __syncthreads();
+  }
+}
+
+// p -= sum_j c_j P[j]
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_pcorr(mg_args a, int64_t mpad, double *p,
+                                                       const double *Pb, const double *coef) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const int nc = MST(t.s, MI_NC);
+  const double *c = coef + (size_t)t.s * MG_NP;
+  MG_ROWS(t, row) {
+    double acc = 0.0;
+    for (int j = 0; j < nc; j++) acc += c[j] * Pb[(size_t)j * mpad + row];
+    p[row] -= acc;
+  }
+}
+
+// fiedler[cstart + i] = x * ni (fiedler.c:606-616)
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_out(mg_args a, const double *x, double *fied) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_ELIG, t)) return;
+  const double ni = MSC(t.s, MS_NI);
+  const int cstart = a.seg[t.s];
+  MG_ROWS(t, row) fied[cstart + (row - t.pstart)] = x[row] * ni;
+}
+
+// f.Lf, Lf.Lf, f.f for the Rayleigh quotient and residual of the result
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_rayleigh(mg_args a, mg_sell L, const double *x) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_ELIG, t)) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  MG_ROWS(t, row) {
+    const double f = x[row];
+    const double lf = L.diag[row] * f - sell_offsum(L, row, x);
+    acc[0] += f * lf, acc[1] += lf * lf, acc[2] += f * f;
+  }
+  mg_publish<3>(a, t, acc);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle.  Weighted Jacobi step k from r_in: s = sigma r / D (0 where D is
+// 0), u (= s on the first step, += s after), r_out = r_in - (D s - A s).
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_smooth0(mg_args a, mg_sell L, double sigma,
+                                                         int first, const double *rin,
+                                                         double *rout, double *u) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  MG_ROWS(t, row) {
+    const int64_t base = L.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
+    const int len = L.rowlen[row];
+    double as = 0.0;
+    for (int j = 0; j < len; j++) {
+      const uint32_t e = L.ell[base + (int64_t)j * RSB_WAVE];
+      const int c = (int)(e & 0x0fffffffu);
+      const double dc = L.diag[c];
+      const double sc = dc != 0.0 ? sigma * rin[c] / dc : 0.0;
+      as += (double)(e >> 28) * sc;
+    }
+    const double d = L.diag[row];
+    const double s = d != 0.0 ? sigma * rin[row] / d : 0.0;
+    u[row] = first ? s : u[row] + s;
+    rout[row] = rin[row] - (d * s - as);
+  }
+}
+
+// coarse level: rows [0, m) with row -> segment rseg; CSR (rp, ci, w), diag
+struct mg_csr {
+  int64_t m;
+  const int *rseg;
+  const int64_t *rp;
+  const int *ci;
+  const double *w, *diag;
+};
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_smooth_c(mg_args a, mg_csr M, double sigma,
+                                                          int first, const double *rin,
+                                                          double *rout, double *u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.m) return;
+  const int s = M.rseg[i];
+  if (!MST(s, MI_PCG)) return;
+  double as = 0.0;
+  for (int64_t k = M.rp[i]; k < M.rp[i + 1]; k++) {
+    const int c = M.ci[k];
+    const double dc = M.diag[c];
+    as += M.w[k] * (dc != 0.0 ? sigma * rin[c] / dc : 0.0);
+  }
+  const double d = M.diag[i];
+  const double sv = d != 0.0 ? sigma * rin[i] / d : 0.0;
+  u[i] = first ? sv : u[i] + sv;
+  rout[i] = rin[i] - (d * sv - as);
+}
+
+// per-level segment layout: off[s], n[s] (n = 0: the segment has no rows on
+// this level); level 0 uses pstart and the segment size
+struct mg_lvl {
+  const int *off, *n;
+};
+
+// restriction: coarse row c = sum of its children (multigrid.c:271-272)
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_restrict(mg_args a, int64_t mc,
+                                                          const int *rseg_c, mg_lvl F,
+                                                          mg_lvl Cl, int factor,
+                                                          const double *rf, double *rc) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= mc) return;
+  const int s = rseg_c[c];
+  if (!MST(s, MI_PCG)) return;
+  const int k = (int)(c - Cl.off[s]);
+  const int nf = F.n[s], f0 = F.off[s];
+  const int i0 = k * factor, i1 = min(i0 + factor, nf);
+  double v = 0.0;
+  for (int i = i0; i < i1; i++) v += rf[f0 + i];
+  rc[c] = v;
+}
+
+// prolongation, coarse levels: r_f = over * r_c (0 on a one-row level) + u_f
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_prolong_c(mg_args a, int64_t mf,
+                                                           const int *rseg_f, mg_lvl F,
+                                                           mg_lvl Cl, int factor, double over,
+                                                           const double *rc, const double *u,
+                                                           double *rf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= mf) return;
+  const int s = rseg_f[i];
+  if (!MST(s, MI_PCG) || F.n[s] <= 1) return;
+  const int k = (int)(i - F.off[s]);
+  const double cv = Cl.n[s] == 1 ? 0.0 : rc[Cl.off[s] + k / factor];
+  rf[i] = over * cv + u[i];
+}
+
+// prolongation onto level 0 (tiles), result into z
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_prolong0(mg_args a, mg_lvl Cl, int factor,
+                                                          double over, const double *rc,
+                                                          const double *u, double *z) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const int coff = Cl.off[t.s], cn = Cl.n[t.s];
+  MG_ROWS(t, row) {
+    const int k = row - t.pstart;
+    const double cv = cn == 1 ? 0.0 : rc[coff + k / factor];
+    z[row] = over * cv + u[row];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchy build.
+
+__global__ void k_mg_diag0(int64_t mpad, mg_sell L, double *diag) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= mpad) return;
+  const int64_t base = L.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
+  int s = 0;
+  for (int j = 0; j < L.rowlen[row]; j++) s += (int)(L.ell[base + (int64_t)j * RSB_WAVE] >> 28);
+  diag[row] = (double)s;
+}
+
+__global__ void k_mg_rseg(int nseg, const int *off, const int *n, int *rseg) {
+  const int s = blockIdx.x;
+  for (int i = threadIdx.x; i < n[s]; i += blockDim.x) rseg[off[s] + i] = s;
+}
+
+// raw entry count of every coarse row (sum of its children's row lengths)
+__global__ void k_mg_rawcnt(int64_t mc, const int *rseg_c, mg_lvl F, mg_lvl Cl, int factor,
+                            int level, const int *rowlen0, const int64_t *rp_f,
+                            int64_t *cnt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > mc) return;
+  if (c == mc) {
+    cnt[c] = 0;
+    return;
+  }
+  const int s = rseg_c[c];
+  const int k = (int)(c - Cl.off[s]);
+  const int i0 = k * factor, i1 = min(i0 + factor, F.n[s]);
+  int64_t v = 0;
+  for (int i = i0; i < i1; i++) {
+    const int64_t row = (int64_t)F.off[s] + i;
+    v += level == 0 ? rowlen0[row] : rp_f[row + 1] - rp_f[row];
+  }
+  cnt[c] = v;
+}
+
+// raw entries: key (coarse row << 32 | coarse column), weight, index
+__global__ void k_mg_rawfill(int64_t mc, const int *rseg_c, mg_lvl F, mg_lvl Cl, int factor,
+                             int level, mg_sell L0, const int64_t *rp_f, const int *ci_f,
+                             const double *w_f, const int64_t *off, uint64_t *key, int *idx,
+                             double *wr) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= mc) return;
+  const int s = rseg_c[c];
+  const int k = (int)(c - Cl.off[s]);
+  const int i0 = k * factor, i1 = min(i0 + factor, F.n[s]);
+  const int foff = F.off[s], coff = Cl.off[s];
+  int64_t o = off[c];
+  for (int i = i0; i < i1; i++) {
+    const int64_t row = (int64_t)foff + i;
+    if (level == 0) {
+      const int64_t base = L0.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
+      for (int j = 0; j < L0.rowlen[row]; j++) {
+        const uint32_t e = L0.ell[base + (int64_t)j * RSB_WAVE];
+        const int cc = coff + ((int)(e & 0x0fffffffu) - foff) / factor;
+        key[o] = ((uint64_t)c << 32) | (uint32_t)cc, idx[o] = (int)o, wr[o] = (double)(e >> 28);
+        o++;
+      }
+    } else {
+      for (int64_t j = rp_f[row]; j < rp_f[row + 1]; j++) {
+        const int cc = coff + (ci_f[j] - foff) / factor;
+        key[o] = ((uint64_t)c << 32) | (uint32_t)cc, idx[o] = (int)o, wr[o] = w_f[j];
+        o++;
+      }
+    }
+  }
+}
+
+// keep the first of each run of equal keys that is off the diagonal; its
+// weight is the run's sum
+__global__ void k_mg_heads(int64_t nr, const uint64_t *key, const int *idx, const double *wr,
+                           int *flag, double *wsum, int *rowcnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const uint64_t k = key[i];
+  const bool head = i == 0 || key[i - 1] != k;
+  const bool diag = (uint32_t)(k >> 32) == (uint32_t)k;
+  const int keep = head && !diag;
+  flag[i] = keep;
+  if (keep) {
+    double v = 0.0;
+    for (int64_t j = i; j < nr && key[j] == k; j++) v += wr[idx[j]];
+    wsum[i] = v;
+    atomicAdd(&rowcnt[k >> 32], 1);
+  }
+}
+
+__global__ void k_mg_compact(int64_t nr, const uint64_t *key, const int *flag, const int64_t *pos,
+                             const double *wsum, int *ci, double *w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr || !flag[i]) return;
+  ci[pos[i]] = (int)(uint32_t)key[i];
+  w[pos[i]] = wsum[i];
+}
+
+__global__ void k_mg_widen(int64_t n, const int *in, int64_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+__global__ void k_mg_rowdiag(int64_t m, const int64_t *rp, const double *w, double *diag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int64_t k = rp[i]; k < rp[i + 1]; k++) s += w[k];
+  diag[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+namespace {
+
+struct mg_level_dev {
+  int64_t m = 0;
+  int *off = nullptr, *n = nullptr, *rseg = nullptr;
+  int64_t *rp = nullptr;
+  int *ci = nullptr;
+  double *w = nullptr, *diag = nullptr;
+  double *r0 = nullptr, *r1 = nullptr, *u = nullptr;  // V-cycle vectors
+};
+
+struct mg_hier {
+  std::vector<mg_level_dev> lv;
+  std::vector<std::vector<int>> h_off, h_n;
+  ~mg_hier() {
+    for (auto &l : lv) {
+      hipFree(l.off), hipFree(l.n), hipFree(l.rseg), hipFree(l.rp), hipFree(l.ci);
+      hipFree(l.w), hipFree(l.diag), hipFree(l.r0), hipFree(l.r1), hipFree(l.u);
+    }
+  }
+};
+
+double sigma_cheb(int k, int n, double lmin, double lmax) {
+  k = (k - 1) % n + 1;
+  const double theta = M_PI * (k - 0.5) / n;
+  return 1.0 / (lmin + 0.5 * (lmax - lmin) * (cos(theta) + 1.0));
+}
+
+}  // namespace
+
+// The hierarchy of the last rsbgpu_fiedler_mg call stays with the handle
+// (for the parity hook rsbgpu_mg_level_csr) until the next call or close.
+void rsb_mg_free(rsbgpu *g) {
+  delete (mg_hier *)g->mg;
+  g->mg = nullptr;
+}
+
+#define MG_TRY(x)             \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_) return rc_;      \
+  } while (0)
+
+static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
+  hipStream_t st = g->stream;
+  const int nseg = g->nseg;
+  // level 0 layout: pstart / n of the eligible segments
+  std::vector<int> off0(nseg), n0(nseg);
+  for (int s = 0; s < nseg; s++) {
+    const bool el = g->h_p[s] > 1 && g->h_n[s] >= 2;
+    off0[s] = el ? g->h_pstart[s] : 0;
+    n0[s] = el ? g->h_n[s] : 0;
+  }
+  H.h_off.push_back(off0), H.h_n.push_back(n0);
+  H.lv.emplace_back();
+  {
+    mg_level_dev &L0 = H.lv[0];
+    L0.m = g->mpad;
+    RSB_TRY(hipMalloc((void **)&L0.off, sizeof(int) * nseg));
+    RSB_TRY(hipMalloc((void **)&L0.n, sizeof(int) * nseg));
+    RSB_TRY(hipMemcpyAsync(L0.off, off0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    RSB_TRY(hipMemcpyAsync(L0.n, n0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    L0.diag = diag0;  // freed with the hierarchy
+    RSB_TRY(hipMalloc((void **)&L0.r0, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(hipMalloc((void **)&L0.r1, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(hipMalloc((void **)&L0.u, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+  }
+  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
+  for (int l = 0;; l++) {
+    const std::vector<int> nf = H.h_n[l];
+    std::vector<int> nc(nseg, 0), oc(nseg, 0);
+    int64_t mc = 0;
+    for (int s = 0; s < nseg; s++) {
+      nc[s] = nf[s] > 1 ? (nf[s] + factor - 1) / factor : 0;
+      oc[s] = (int)mc;
+      mc += nc[s];
+    }
+    if (mc == 0) break;
+    if (mc >= (int64_t)1 << 31) RSB_FAIL(2, "MG level too large");
+    H.h_off.push_back(oc), H.h_n.push_back(nc);
+    H.lv.emplace_back();
+    mg_level_dev &F = H.lv[l], &C = H.lv[l + 1];
+    C.m = mc;
+    RSB_TRY(hipMalloc((void **)&C.off, sizeof(int) * nseg));
+    RSB_TRY(hipMalloc((void **)&C.n, sizeof(int) * nseg));
+    RSB_TRY(hipMemcpyAsync(C.off, oc.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    RSB_TRY(hipMemcpyAsync(C.n, nc.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    RSB_TRY(hipMalloc((void **)&C.rseg, sizeof(int) * (size_t)mc));
+    k_mg_rseg<<<nseg, 256, 0, st>>>(nseg, C.off, C.n, C.rseg);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipMalloc((void **)&C.r0, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.r1, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.u, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.diag, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.rp, sizeof(int64_t) * (size_t)(mc + 1)));
+    // raw entries
+    int64_t *cnt = nullptr, *roff = nullptr;
+    RSB_TRY(hipMalloc((void **)&cnt, sizeof(int64_t) * (size_t)(mc + 1)));
+    RSB_TRY(hipMalloc((void **)&roff, sizeof(int64_t) * (size_t)(mc + 1)));
+    mg_lvl FL{F.off, F.n}, CL{C.off, C.n};
+    k_mg_rawcnt<<<nblocks(mc + 1, 256), 256, 0, st>>>(mc, C.rseg, FL, CL, factor, l,
+                                                       g->d_rowlen, F.rp, cnt);
+    RSB_LAUNCH_CHECK();
+    MG_TRY(rsb_exclusive_scan_i64(g, cnt, roff, mc + 1));
+    int64_t nr = 0;
+    RSB_TRY(hipMemcpyAsync(&nr, roff + mc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    if (nr >= (int64_t)1 << 31) RSB_FAIL(2, "MG level has too many entries");
+    uint64_t *key = nullptr;
+    int *idx = nullptr, *flag = nullptr, *rowcnt = nullptr;
+    double *wr = nullptr, *wsum = nullptr;
+    int64_t *pos = nullptr, *flag64 = nullptr, *rc64 = nullptr;
+    const size_t nra = (size_t)std::max<int64_t>(nr, 1);
+    RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * nra));
+    RSB_TRY(hipMalloc((void **)&idx, sizeof(int) * nra));
+    RSB_TRY(hipMalloc((void **)&wr, sizeof(double) * nra));
+    k_mg_rawfill<<<nblocks(mc, 256), 256, 0, st>>>(mc, C.rseg, FL, CL, factor, l, S, F.rp, F.ci,
+                                                    F.w, roff, key, idx, wr);
+    RSB_LAUNCH_CHECK();
+    int cbits = 1;
+    while (((int64_t)1 << cbits) < mc) cbits++;
+    RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * nra));
+    RSB_TRY(hipMalloc((void **)&wsum, sizeof(double) * nra));
+    RSB_TRY(hipMalloc((void **)&rowcnt, sizeof(int) * (size_t)(mc + 1)));
+    RSB_TRY(hipMemsetAsync(rowcnt, 0, sizeof(int) * (size_t)(mc + 1), st));
+    RSB_TRY(hipMalloc((void **)&flag64, sizeof(int64_t) * nra));
+    RSB_TRY(hipMalloc((void **)&pos, sizeof(int64_t) * nra));
+    if (nr > 0) {
+      MG_TRY(rsb_sort_pairs_u64(g, key, idx, nr, 0, 32 + cbits));
+      k_mg_heads<<<nblocks(nr, 256), 256, 0, st>>>(nr, key, idx, wr, flag, wsum, rowcnt);
+      RSB_LAUNCH_CHECK();
+      k_mg_widen<<<nblocks(nr, 256), 256, 0, st>>>(nr, flag, flag64);
+      MG_TRY(rsb_exclusive_scan_i64(g, flag64, pos, nr));
+    }
+    RSB_TRY(hipMalloc((void **)&rc64, sizeof(int64_t) * (size_t)(mc + 1)));
+    k_mg_widen<<<nblocks(mc + 1, 256), 256, 0, st>>>(mc + 1, rowcnt, rc64);
+    MG_TRY(rsb_exclusive_scan_i64(g, rc64, C.rp, mc + 1));
+    int64_t nnz = 0;
+    RSB_TRY(hipMemcpyAsync(&nnz, C.rp + mc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    RSB_TRY(hipMalloc((void **)&C.ci, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)));
+    RSB_TRY(hipMalloc((void **)&C.w, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
+    if (nr > 0) k_mg_compact<<<nblocks(nr, 256), 256, 0, st>>>(nr, key, flag, pos, wsum, C.ci, C.w);
+    k_mg_rowdiag<<<nblocks(mc, 256), 256, 0, st>>>(mc, C.rp, C.w, C.diag);
+    RSB_LAUNCH_CHECK();
+    RSB_TRY(hipStreamSynchronize(st));
+    hipFree(cnt), hipFree(roff), hipFree(key), hipFree(idx), hipFree(wr), hipFree(flag);
+    hipFree(wsum), hipFree(rowcnt), hipFree(flag64), hipFree(pos), hipFree(rc64);
+  }
+  return 0;
+}
+
+extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double tol,
+                                 int factor, rsbgpu_fiedler_out *out) {
+  RSB_TRY(hipSetDevice(g->device));
+  if (factor < 2) RSB_FAIL(3, "rsb_mg_factor must be > 1");
+  if (max_iter < 1 || max_iter > MG_NP - 4) RSB_FAIL(3, "rsb_max_iter must be in 1..%d", MG_NP - 4);
+  const int nseg = g->nseg;
+  rsb_phase_begin(g);
+  hipStream_t st = g->stream;
+  const int64_t mpad = g->mpad;
+  const int ntiles = g->ntiles;
+  for (int s = 0; s < nseg; s++) {
+    if (out->niter) out->niter[s] = 0;
+    if (out->npass) out->npass[s] = 0;
+    if (out->lambda) out->lambda[s] = 0.0;
+    if (out->rnorm) out->rnorm[3 * s] = out->rnorm[3 * s + 1] = out->rnorm[3 * s + 2] = 0.0;
+    if (out->resid) out->resid[s] = 0.0;
+    if (out->fallback) out->fallback[s] = 0;
+  }
+  // segments that cannot be bisected get Fiedler value 0 (as the Lanczos path)
+  RSB_TRY(dev_alloc(g->d_fied, (size_t)std::max(g->nel, 1)));
+  RSB_TRY(hipMemsetAsync(g->d_fied, 0, sizeof(double) * (size_t)std::max(g->nel, 1), st));
+  bool any = false;
+  for (int s = 0; s < nseg; s++) any |= g->h_p[s] > 1 && g->h_n[s] >= 2;
+  if (!any || mpad == 0) {
+    rsb_phase_end(g, PH_LANCZOS);
+    return 0;
+  }
+  // vectors and state
+  const size_t vb = sizeof(double) * (size_t)mpad;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *z0 = nullptr, *p = nullptr,
+         *w = nullptr, *diag0 = nullptr, *Wb = nullptr, *Pb = nullptr, *sc = nullptr,
+         *partial = nullptr, *coef = nullptr;
+  int *stt = nullptr, *active = nullptr;
+  RSB_TRY(hipMalloc((void **)&b, vb));
+  RSB_TRY(hipMalloc((void **)&x, vb));
+  RSB_TRY(hipMalloc((void **)&r, vb));
+  RSB_TRY(hipMalloc((void **)&z, vb));
+  RSB_TRY(hipMalloc((void **)&z0, vb));
+  RSB_TRY(hipMalloc((void **)&p, vb));
+  RSB_TRY(hipMalloc((void **)&w, vb));
+  RSB_TRY(hipMalloc((void **)&diag0, vb));
+  RSB_TRY(hipMalloc((void **)&Wb, vb * (size_t)max_iter));
+  RSB_TRY(hipMalloc((void **)&Pb, vb * (size_t)max_iter));
+  RSB_TRY(hipMalloc((void **)&sc, sizeof(double) * MS_NSC * (size_t)nseg));
+  RSB_TRY(hipMalloc((void **)&stt, sizeof(int) * MI_NI * (size_t)nseg));
+  RSB_TRY(hipMalloc((void **)&partial, sizeof(double) * MG_NP * (size_t)std::max(ntiles, 1)));
+  RSB_TRY(hipMalloc((void **)&coef, sizeof(double) * MG_NP * (size_t)nseg));
+  RSB_TRY(hipMalloc((void **)&active, sizeof(int)));
+  RSB_TRY(hipMemsetAsync(sc, 0, sizeof(double) * MS_NSC * (size_t)nseg, st));
+  // every eligible segment starts in the inverse iteration
+  {
+    std::vector<int> h((size_t)MI_NI * nseg, 0);
+    for (int s = 0; s < nseg; s++) h[(size_t)s * MI_NI + MI_INV] = g->h_p[s] > 1 && g->h_n[s] >= 2;
+    RSB_TRY(hipMemcpyAsync(stt, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
+  }
+  mg_args a{nseg, g->d_seg, g->d_tile, g->d_seg_tiles, sc, stt, partial, active, tol,
+            max_iter, max_passes};
+  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
+  k_mg_diag0<<<nblocks(mpad, 256), 256, 0, st>>>(mpad, S, diag0);
+  RSB_LAUNCH_CHECK();
+  rsb_mg_free(g);
+  mg_hier *hp = new mg_hier();
+  g->mg = hp;
+  mg_hier &H = *hp;
+  MG_TRY(mg_build(g, factor, H, diag0));
+  const int nlev = (int)H.lv.size();
+  auto fin = [&](int op, int gate, int nslots) {
+    k_mg_fin<<<nseg, RSB_WAVE, 0, st>>>(a, op, gate, nslots);
+  };
+  auto count_active = [&](int &n) -> int {
+    RSB_TRY(hipMemcpyAsync(&n, active, sizeof(int), hipMemcpyDeviceToHost, st));
+    RSB_TRY(hipStreamSynchronize(st));
+    return 0;
+  };
+  // start vector, orthogonalised and normalised (fiedler.c:572-585)
+  k_mg_start<<<ntiles, RSB_BLOCK, 0, st>>>(a, b);
+  fin(FIN_MEAN, GATE_ELIG, 1);
+  k_mg_center_dot<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_ELIG, b);
+  fin(FIN_NORM, GATE_ELIG, 1);
+  k_mg_scale_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_ELIG, b, b);
+  RSB_LAUNCH_CHECK();
+  const double s1 = sigma_cheb(1, 4, 1.0, 2.0), s2 = sigma_cheb(2, 4, 1.0, 2.0),
+               s3 = sigma_cheb(3, 4, 1.0, 2.0);
+  const double sig[3] = {s1, s2, s3};
+  double *zc = z, *zo = z0;
+  for (int pass = 0; pass < max_passes; pass++) {
+    // projection (fiedler.c:94-187)
+    k_mg_pass0<<<ntiles, RSB_BLOCK, 0, st>>>(a, b, x, r);
+    fin(FIN_PASS0, GATE_INV, 2);
+    k_mg_pass1<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, zc, p);
+    fin(FIN_RZ1, GATE_PCG, 1);
+    RSB_LAUNCH_CHECK();
+    for (int it = 0; it < max_iter; it++) {
+      k_mg_apply<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, S, p, w);
+      fin(FIN_ALPHA, GATE_PCG, 1);
+      k_mg_update<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, w, x, r, Wb, Pb);
+      RSB_TRY(hipMemsetAsync(active, 0, sizeof(int), st));
+      fin(FIN_CONV, GATE_PCG, 1);
+      RSB_LAUNCH_CHECK();
+      int nact = 0;
+      MG_TRY(count_active(nact));
+      if (nact == 0) break;
+      // z = V-cycle(r) into the other buffer; zo keeps the previous z
+      std::swap(zc, zo);
+      {
+        // down
+        for (int l = 0; l + 1 < nlev; l++) {
+          mg_level_dev &F = H.lv[l];
+          const double *rin = l == 0 ? r : F.r0;
+          double *bufs[2] = {F.r1, F.r0};
+          for (int k = 0; k < 3; k++) {
+            double *rout = bufs[k & 1];
+            if (l == 0)
+              k_mg_smooth0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, sig[k], k == 0, rin, rout, F.u);
+            else
+              k_mg_smooth_c<<<nblocks(F.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
+                  a, mg_csr{F.m, F.rseg, F.rp, F.ci, F.w, F.diag}, sig[k], k == 0, rin, rout,
+                  F.u);
+            rin = rout;
+          }
+          // after three steps the residual is in r1 (steps write r1, r0, r1)
+          mg_level_dev &C = H.lv[l + 1];
+          k_mg_restrict<<<nblocks(C.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
+              a, C.m, C.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, F.r1, C.r0);
+        }
+        // up: each coarse level's result goes to its r0
+        for (int l = nlev - 2; l >= 1; l--) {
+          mg_level_dev &F = H.lv[l], &C = H.lv[l + 1];
+          k_mg_prolong_c<<<nblocks(F.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
+              a, F.m, F.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, 1.5, C.r0, F.u,
+              F.r0);
+        }
+        mg_level_dev &F0 = H.lv[0], &C1 = H.lv[1];
+        k_mg_prolong0<<<ntiles, RSB_BLOCK, 0, st>>>(a, mg_lvl{C1.off, C1.n}, factor, 1.33333,
+                                                     C1.r0, F0.u, zc);
+        RSB_LAUNCH_CHECK();
+      }
+      k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, zc);
+      fin(FIN_MEAN, GATE_PCG, 1);
+      k_mg_zdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, zc, zo);
+      fin(FIN_BETA, GATE_PCG, 2);
+      k_mg_pnew<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, zc, p, Wb);
+      k_mg_coef<<<nseg, RSB_WAVE, 0, st>>>(a, coef);
+      k_mg_pcorr<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, Pb, coef);
+      RSB_LAUNCH_CHECK();
+    }
+    // end of pass (fiedler.c:261-275): y orthogonalised, z = y / |y|, z orthogonalised
+    k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
+    fin(FIN_MEAN, GATE_INV, 1);
+    k_mg_center_dot<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
+    fin(FIN_NORM, GATE_INV, 1);
+    k_mg_scale_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x, b);
+    RSB_TRY(hipMemsetAsync(active, 0, sizeof(int), st));
+    fin(FIN_PASSEND, GATE_INV, 1);
+    k_mg_center<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, b);
+    RSB_LAUNCH_CHECK();
+    int nact = 0;
+    MG_TRY(count_active(nact));
+    if (nact == 0) break;
+  }
+  // result: the last y, normalised; Rayleigh quotient for the report
+  k_mg_out<<<ntiles, RSB_BLOCK, 0, st>>>(a, x, g->d_fied);
+  k_mg_rayleigh<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, x);
+  fin(FIN_RAYLEIGH, GATE_ELIG, 3);
+  RSB_LAUNCH_CHECK();
+  std::vector<int> hs((size_t)MI_NI * nseg);
+  std::vector<double> hsc((size_t)MS_NSC * nseg);
+  RSB_TRY(hipMemcpyAsync(hs.data(), stt, sizeof(int) * hs.size(), hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipMemcpyAsync(hsc.data(), sc, sizeof(double) * hsc.size(), hipMemcpyDeviceToHost, st));
+  RSB_TRY(hipStreamSynchronize(st));
+  for (int s = 0; s < nseg; s++) {
+    if (!(g->h_p[s] > 1 && g->h_n[s] >= 2)) continue;
+    if (out->niter) out->niter[s] = hs[(size_t)s * MI_NI + MI_ITERS];
+    if (out->npass) out->npass[s] = hs[(size_t)s * MI_NI + MI_NPASS];
+    if (out->lambda) out->lambda[s] = hsc[(size_t)s * MS_NSC + MS_LAMBDA];
+    if (out->resid) out->resid[s] = hsc[(size_t)s * MS_NSC + MS_RES];
+  }
+  hipFree(b), hipFree(x), hipFree(r), hipFree(z), hipFree(z0), hipFree(p), hipFree(w);
+  hipFree(Wb), hipFree(Pb), hipFree(sc), hipFree(stt), hipFree(partial), hipFree(coef);
+  hipFree(active);
+  // diag0 stays alive with the kept hierarchy (level 0's diagonal)
+  rsb_phase_end(g, PH_LANCZOS);
+  return 0;
+}
+
+// Parity hook: level `level` of the hierarchy built by the last
+// rsbgpu_fiedler_mg call, rows of segment `seg` only, in that segment's local
+// numbering: rowptr[n+1], col[], w[], diag[n].  Level 0 is the level
+// Laplacian itself (not exported here).  Returns the row count through *n
+// (and nnz through *nnz) when the arrays are NULL.
+extern "C" int rsbgpu_mg_level_csr(rsbgpu *g, int level, int seg, int *n, int64_t *nnz,
+                                   int64_t *rowptr, int *col, double *w, double *diag) {
+  RSB_TRY(hipSetDevice(g->device));
+  mg_hier *hp = (mg_hier *)g->mg;
+  if (!hp || level < 1 || level >= (int)hp->lv.size() || seg < 0 ||
+      seg >= (int)hp->h_n[level].size())
+    RSB_FAIL(3, "no such MG level");
+  mg_hier &H = *hp;
+  const mg_level_dev &L = H.lv[level];
+  const int rn = H.h_n[level][seg], ro = H.h_off[level][seg];
+  std::vector<int64_t> rp((size_t)rn + 1);
+  if (rn > 0)
+    RSB_TRY(hipMemcpy(rp.data(), L.rp + ro, sizeof(int64_t) * (rn + 1), hipMemcpyDeviceToHost));
+  *n = rn;
+  *nnz = rn > 0 ? rp[rn] - rp[0] : 0;
+  if (!rowptr) return 0;
+  for (int i = 0; i <= rn; i++) rowptr[i] = rp[i] - rp[0];
+  if (*nnz > 0) {
+    RSB_TRY(hipMemcpy(col, L.ci + rp[0], sizeof(int) * (size_t)*nnz, hipMemcpyDeviceToHost));
+    RSB_TRY(hipMemcpy(w, L.w + rp[0], sizeof(double) * (size_t)*nnz, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < *nnz; k++) col[k] -= ro;
+  }
+  if (rn > 0)
+    RSB_TRY(hipMemcpy(diag, L.diag + ro, sizeof(double) * rn, hipMemcpyDeviceToHost));
+  return 0;
+}

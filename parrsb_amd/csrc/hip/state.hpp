@@ -101,6 +101,7 @@ struct rsbgpu {
   void *d_lws_ws = nullptr;
   size_t lws_ws_bytes = 0;
   lz_state lz{};
+  void *mg = nullptr;  // MG hierarchy of the last rsbgpu_fiedler_mg (mg.hip)
   int lz_nseg_cap = 0, lz_tile_cap = 0, lz_miter_cap = 0;
 
   // sort scratch
@@ -174,6 +175,7 @@ inline hipError_t dev_alloc(T *&p, size_t n) {
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 int rsb_build_tiles(rsbgpu *g);  // tiles.hip
+void rsb_mg_free(rsbgpu *g);     // mg.hip
 int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows);  // lws.hip
 
 // sort helpers (sort.hip)

@@ -206,7 +206,11 @@ static int precision_bits(void) {
 /* Warnings of check_rsb_partition (rsb.c:61-111). */
 static void check_partition(const parrsb_amd_report *r, const parrsb_options *o) {
   for (int l = 0; l < r->nlevels && l < PARRSB_AMD_MAX_LEVELS; l++) {
-    if (r->nconverged_fail[l] > 0)
+    if (r->nconverged_fail[l] > 0 && o->rsb_algo == 1)
+      fprintf(stderr, "Warning: Inverse iteration didn't converge after %d iterations in "
+                      "Level = %d\n",
+              o->rsb_max_passes, l);
+    else if (r->nconverged_fail[l] > 0)
       fprintf(stderr,
               "Warning: Lanczos did not reach the target residual after %d x %d iterations "
               "in Level=%d (%d segments)!\n",
@@ -256,8 +260,13 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, con
     set_error("RCB and RIB need coordinates");
     return 3;
   }
-  if (options->partitioner == 0 && options->rsb_algo != 0) {
-    set_error("only the Lanczos solver (rsb_algo = 0) is provided");
+  if (options->partitioner == 0 && options->rsb_algo != 0 && options->rsb_algo != 1) {
+    set_error("rsb_algo must be 0 (Lanczos) or 1 (MG inverse iteration)");
+    return 3;
+  }
+  if (options->partitioner == 0 && options->rsb_algo == 1 &&
+      (options->rsb_mg_factor < 2 || options->rsb_mg_grammian != 0)) {
+    set_error("MG inverse iteration needs rsb_mg_factor > 1 and rsb_mg_grammian = 0");
     return 3;
   }
   if (vtx == NULL && options->partitioner == 0) {
@@ -320,6 +329,7 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, con
       prm.residuals = 1;
       prm.warm = NULL, prm.warm_passes = cfg_warm_passes, prm.warm_max_n = cfg_warm_max_n;
       prm.ngroups = ngroups, prm.group_count = gcount;
+      prm.rsb_algo = options->rsb_algo, prm.mg_factor = options->rsb_mg_factor;
       void *dw = NULL;
       if (cfg_warm) {
         rc = rsbgpu_upload(g, cfg_warm, sizeof(float) * (size_t)rsbgpu_lws_nparams(), &dw);

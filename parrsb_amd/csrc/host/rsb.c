@@ -140,12 +140,16 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     TRY(rsbgpu_level_laplacian(g));
     rsbgpu_fiedler_out fo = {niter, npass, lam, rn, prm->residuals ? res : NULL, fb};
     int mode = 0;
-    if (prm->warm) {
+    if (prm->warm && prm->rsb_algo == 0) {
       TRY(rsbgpu_lws_set_params(g, prm->warm, (float)cut, prm->warm_passes, prm->warm_max_n));
       mode = 2;
     }
     memset(fb, 0, sizeof(int) * (size_t)nseg);
-    TRY(rsbgpu_fiedler(g, prm->precision, prm->max_iter, prm->max_passes, prm->tol, mode, &fo));
+    if (prm->rsb_algo == 1)
+      TRY(rsbgpu_fiedler_mg(g, prm->max_iter, prm->max_passes, prm->tol, prm->mg_factor, &fo));
+    else
+      TRY(rsbgpu_fiedler(g, prm->precision, prm->max_iter, prm->max_passes, prm->tol, mode,
+                         &fo));
     TRY(rsbgpu_fiedler_sort(g));
     for (int s = 0; s < nseg; s++) {
       snd[s] = -1, scnt[s] = 0, nl[s] = A.n[s];
@@ -181,7 +185,11 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
         int mi = prm->max_iter < A.n[s] ? prm->max_iter : A.n[s];
         /* iterations of the reference-start solve (a fallback counts the
            learned attempt's fb[s] - 1 iterations in front of them) */
-        if (niter[s] - (fb[s] ? fb[s] - 1 : 0) == mi * prm->max_passes) nfail++;
+        if (prm->rsb_algo == 1) {
+          if (niter[s] == prm->max_passes) nfail++; /* the test of rsb.c:70-71 */
+        } else if (niter[s] - (fb[s] ? fb[s] - 1 : 0) == mi * prm->max_passes) {
+          nfail++;
+        }
       }
       if (prm->find_comps) {
         for (int h = 0; h < 2; h++) {
