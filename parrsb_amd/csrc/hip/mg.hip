@@ -104,19 +104,22 @@ __device__ inline double sell_offsum(const mg_sell &L, int row, const double *x)
 }
 
 // ---------------------------------------------------------------------------
-// Finishing kernel: one wave per segment sums the tile partials (tiles in
-// order, slot k by lane k) and applies the scalar step.
+// Segment total of partial slot k: lane j sums tiles j, j + 64, ..., then a
+// fixed shuffle tree (the same order every time).
+__device__ inline double seg_total(const mg_args &a, int first, int count, int k) {
+  double v = 0.0;
+  for (int t = threadIdx.x; t < count; t += RSB_WAVE) v += a.partial[(size_t)(first + t) * MG_NP + k];
+  return wave_sum(v);
+}
+
+// Finishing kernel: one wave per segment forms the totals of the tile
+// partials and applies the scalar step.
 __global__ void __launch_bounds__(RSB_WAVE) k_mg_fin(mg_args a, int op, int gate, int nslots) {
   const int s = blockIdx.x;
   if (!mg_gate(a, s, gate)) return;
-  __shared__ double tot[MG_NP];
+  double tot[4];
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
-  for (int k = threadIdx.x; k < nslots; k += RSB_WAVE) {
-    double v = 0.0;
-    for (int t = 0; t < count; t++) v += a.partial[(size_t)(first + t) * MG_NP + k];
-    tot[k] = v;
-  }
-  __syncthreads();
+  for (int k = 0; k < nslots; k++) tot[k] = seg_total(a, first, count, k);
   if (threadIdx.x != 0) return;
   const double n = (double)a.seg[a.nseg + s];
   switch (op) {
@@ -181,10 +184,9 @@ __global__ void __launch_bounds__(RSB_WAVE) k_mg_coef(mg_args a, double *coef) {
   if (!mg_gate(a, s, GATE_PCG)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
   const int nc = MST(s, MI_IT) + 1;
-  for (int k = threadIdx.x; k < nc; k += RSB_WAVE) {
-    double v = 0.0;
-    for (int t = 0; t < count; t++) v += a.partial[(size_t)(first + t) * MG_NP + k];
-    coef[(size_t)s * MG_NP + k] = v;
+  for (int k = 0; k < nc; k++) {
+    const double v = seg_total(a, first, count, k);
+    if (threadIdx.x == 0) coef[(size_t)s * MG_NP + k] = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) MST(s, MI_NC) = nc, MST(s, MI_IT) = nc;
@@ -374,6 +376,15 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_pcorr(mg_args a, int64_t mpad,
   }
 }
 
+__global__ void k_mg_zero(int *c) { *c = 0; }
+
+// z0 = z (the previous preconditioned residual)
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_copy(mg_args a, const double *z, double *z0) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  MG_ROWS(t, row) z0[row] = z[row];
+}
+
 // fiedler[cstart + i] = x * ni (fiedler.c:606-616)
 __global__ void __launch_bounds__(RSB_BLOCK) k_mg_out(mg_args a, const double *x, double *fied) {
   mg_tile t;
@@ -397,29 +408,39 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_rayleigh(mg_args a, mg_sell L,
 }
 
 // ---------------------------------------------------------------------------
-// V-cycle.  Weighted Jacobi step k from r_in: s = sigma r / D (0 where D is
-// 0), u (= s on the first step, += s after), r_out = r_in - (D s - A s).
+// V-cycle.  Weighted Jacobi step k (multigrid.c:249-268): s = sigma_k r / D
+// (0 where D is 0), u = s on the first step and u += s after, r -= D s - A s.
+// The step's s is formed once per row by the kernel before (k_mg_jac* for
+// the first, the residual kernel of step k - 1 for the others), so the
+// residual kernel reads its neighbours' s and writes the next step's s
+// into the other buffer.
 
-__global__ void __launch_bounds__(RSB_BLOCK) k_mg_smooth0(mg_args a, mg_sell L, double sigma,
-                                                         int first, const double *rin,
-                                                         double *rout, double *u) {
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_jac0(mg_args a, mg_sell L, double sigma,
+                                                      const double *r, double *sv, double *u) {
   mg_tile t;
   if (!mg_tile_of(a, GATE_PCG, t)) return;
   MG_ROWS(t, row) {
-    const int64_t base = L.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
-    const int len = L.rowlen[row];
-    double as = 0.0;
-    for (int j = 0; j < len; j++) {
-      const uint32_t e = L.ell[base + (int64_t)j * RSB_WAVE];
-      const int c = (int)(e & 0x0fffffffu);
-      const double dc = L.diag[c];
-      const double sc = dc != 0.0 ? sigma * rin[c] / dc : 0.0;
-      as += (double)(e >> 28) * sc;
-    }
     const double d = L.diag[row];
-    const double s = d != 0.0 ? sigma * rin[row] / d : 0.0;
-    u[row] = first ? s : u[row] + s;
-    rout[row] = rin[row] - (d * s - as);
+    const double v = d != 0.0 ? sigma * r[row] / d : 0.0;
+    sv[row] = v, u[row] = v;
+  }
+}
+
+// rout = rin - (D s - A s); then, if sigma_next > 0, the next step's s
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_res0(mg_args a, mg_sell L, const double *rin,
+                                                      double *rout, const double *sv,
+                                                      double sigma_next, double *snext,
+                                                      double *u) {
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_PCG, t)) return;
+  MG_ROWS(t, row) {
+    const double d = L.diag[row];
+    const double r = rin[row] - (d * sv[row] - sell_offsum(L, row, sv));
+    rout[row] = r;
+    if (sigma_next > 0.0) {
+      const double v = d != 0.0 ? sigma_next * r / d : 0.0;
+      snext[row] = v, u[row] += v;
+    }
   }
 }
 
@@ -427,28 +448,39 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_smooth0(mg_args a, mg_sell L, 
 struct mg_csr {
   int64_t m;
   const int *rseg;
-  const int64_t *rp;
-  const int *ci;
-  const double *w, *diag;
+  const int64_t *soff;  // SELL-64 slices
+  const int *rlen, *sci, *sw;
+  const double *diag;
 };
 
-__global__ void __launch_bounds__(RSB_BLOCK) k_mg_smooth_c(mg_args a, mg_csr M, double sigma,
-                                                          int first, const double *rin,
-                                                          double *rout, double *u) {
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_jac_c(mg_args a, mg_csr M, double sigma,
+                                                       const double *r, double *sv, double *u) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M.m) return;
-  const int s = M.rseg[i];
-  if (!MST(s, MI_PCG)) return;
+  if (i >= M.m || !MST(M.rseg[i], MI_PCG)) return;
+  const double d = M.diag[i];
+  const double v = d != 0.0 ? sigma * r[i] / d : 0.0;
+  sv[i] = v, u[i] = v;
+}
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_res_c(mg_args a, mg_csr M, double *r,
+                                                       const double *sv, double sigma_next,
+                                                       double *snext, double *u) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M.m || !MST(M.rseg[i], MI_PCG)) return;
   double as = 0.0;
-  for (int64_t k = M.rp[i]; k < M.rp[i + 1]; k++) {
-    const int c = M.ci[k];
-    const double dc = M.diag[c];
-    as += M.w[k] * (dc != 0.0 ? sigma * rin[c] / dc : 0.0);
+  const int64_t base = M.soff[i / RSB_WAVE] + (i % RSB_WAVE);
+  const int len = M.rlen[i];
+  for (int j = 0; j < len; j++) {
+    const int64_t k = base + (int64_t)j * RSB_WAVE;
+    as += (double)M.sw[k] * sv[M.sci[k]];
   }
   const double d = M.diag[i];
-  const double sv = d != 0.0 ? sigma * rin[i] / d : 0.0;
-  u[i] = first ? sv : u[i] + sv;
-  rout[i] = rin[i] - (d * sv - as);
+  const double rv = r[i] - (d * sv[i] - as);
+  r[i] = rv;
+  if (sigma_next > 0.0) {
+    const double v = d != 0.0 ? sigma_next * rv / d : 0.0;
+    snext[i] = v, u[i] += v;
+  }
 }
 
 // per-level segment layout: off[s], n[s] (n = 0: the segment has no rows on
@@ -605,6 +637,37 @@ __global__ void k_mg_widen(int64_t n, const int *in, int64_t *out) {
   if (i < n) out[i] = in[i];
 }
 
+// SELL-64 layout of a coarse level: rlen per row (0 past the last row),
+// bytes[slice] = 64 * longest row of the slice (entries per slice)
+__global__ void k_mg_slice_len(int64_t m, int64_t nsl, const int64_t *rp, int *rlen,
+                               int64_t *ents) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsl * RSB_WAVE) {
+    if (i == nsl * RSB_WAVE) ents[nsl] = 0;
+    return;
+  }
+  const int len = i < m ? (int)(rp[i + 1] - rp[i]) : 0;
+  rlen[i] = len;
+  int mx = len;
+  for (int o = RSB_WAVE / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, RSB_WAVE));
+  if ((threadIdx.x & (RSB_WAVE - 1)) == 0) ents[i / RSB_WAVE] = (int64_t)mx * RSB_WAVE;
+}
+
+__global__ void k_mg_sell_fill(int64_t m, int64_t nsl, const int64_t *rp, const int *ci,
+                               const double *w, const int64_t *soff, int *sci, int *sw) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nsl * RSB_WAVE) return;
+  const int64_t sl = i / RSB_WAVE, lane = i % RSB_WAVE;
+  const int width = (int)((soff[sl + 1] - soff[sl]) / RSB_WAVE);
+  const int64_t base = soff[sl] + lane;
+  int j = 0;
+  if (i < m)
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++, j++)
+      sci[base + (int64_t)j * RSB_WAVE] = ci[k], sw[base + (int64_t)j * RSB_WAVE] = (int)w[k];
+  const int self = i < m ? (int)i : 0;
+  for (; j < width; j++) sci[base + (int64_t)j * RSB_WAVE] = self, sw[base + (int64_t)j * RSB_WAVE] = 0;
+}
+
 __global__ void k_mg_rowdiag(int64_t m, const int64_t *rp, const double *w, double *diag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -624,7 +687,12 @@ struct mg_level_dev {
   int64_t *rp = nullptr;
   int *ci = nullptr;
   double *w = nullptr, *diag = nullptr;
-  double *r0 = nullptr, *r1 = nullptr, *u = nullptr;  // V-cycle vectors
+  // the same matrix as SELL-64 for the smoother: slices of 64 rows, entries
+  // column-major within a slice (a wave's loads are contiguous), padded with
+  // weight-0 entries to the slice's longest row; weights are integers
+  int64_t *soff = nullptr;
+  int *rlen = nullptr, *sci = nullptr, *sw = nullptr;
+  double *r0 = nullptr, *sa = nullptr, *sb = nullptr, *u = nullptr;  // V-cycle vectors
 };
 
 struct mg_hier {
@@ -633,7 +701,8 @@ struct mg_hier {
   ~mg_hier() {
     for (auto &l : lv) {
       hipFree(l.off), hipFree(l.n), hipFree(l.rseg), hipFree(l.rp), hipFree(l.ci);
-      hipFree(l.w), hipFree(l.diag), hipFree(l.r0), hipFree(l.r1), hipFree(l.u);
+      hipFree(l.w), hipFree(l.diag), hipFree(l.r0), hipFree(l.sa), hipFree(l.sb), hipFree(l.u);
+      hipFree(l.soff), hipFree(l.rlen), hipFree(l.sci), hipFree(l.sw);
     }
   }
 };
@@ -680,7 +749,8 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     RSB_TRY(hipMemcpyAsync(L0.n, n0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     L0.diag = diag0;  // freed with the hierarchy
     RSB_TRY(hipMalloc((void **)&L0.r0, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
-    RSB_TRY(hipMalloc((void **)&L0.r1, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(hipMalloc((void **)&L0.sa, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(hipMalloc((void **)&L0.sb, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
     RSB_TRY(hipMalloc((void **)&L0.u, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
   }
   mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
@@ -707,7 +777,8 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     k_mg_rseg<<<nseg, 256, 0, st>>>(nseg, C.off, C.n, C.rseg);
     RSB_LAUNCH_CHECK();
     RSB_TRY(hipMalloc((void **)&C.r0, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.r1, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.sa, sizeof(double) * (size_t)mc));
+    RSB_TRY(hipMalloc((void **)&C.sb, sizeof(double) * (size_t)mc));
     RSB_TRY(hipMalloc((void **)&C.u, sizeof(double) * (size_t)mc));
     RSB_TRY(hipMalloc((void **)&C.diag, sizeof(double) * (size_t)mc));
     RSB_TRY(hipMalloc((void **)&C.rp, sizeof(int64_t) * (size_t)(mc + 1)));
@@ -761,7 +832,27 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     if (nr > 0) k_mg_compact<<<nblocks(nr, 256), 256, 0, st>>>(nr, key, flag, pos, wsum, C.ci, C.w);
     k_mg_rowdiag<<<nblocks(mc, 256), 256, 0, st>>>(mc, C.rp, C.w, C.diag);
     RSB_LAUNCH_CHECK();
-    RSB_TRY(hipStreamSynchronize(st));
+    // SELL-64 copy for the smoother
+    {
+      const int64_t nsl = (mc + RSB_WAVE - 1) / RSB_WAVE;
+      int64_t *sbytes = nullptr;
+      RSB_TRY(hipMalloc((void **)&C.rlen, sizeof(int) * (size_t)(nsl * RSB_WAVE)));
+      RSB_TRY(hipMalloc((void **)&C.soff, sizeof(int64_t) * (size_t)(nsl + 1)));
+      RSB_TRY(hipMalloc((void **)&sbytes, sizeof(int64_t) * (size_t)(nsl + 1)));
+      k_mg_slice_len<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.rlen, sbytes);
+      RSB_LAUNCH_CHECK();
+      MG_TRY(rsb_exclusive_scan_i64(g, sbytes, C.soff, nsl + 1));
+      int64_t ns = 0;
+      RSB_TRY(hipMemcpyAsync(&ns, C.soff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+      RSB_TRY(hipStreamSynchronize(st));
+      RSB_TRY(hipMalloc((void **)&C.sci, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+      RSB_TRY(hipMalloc((void **)&C.sw, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+      k_mg_sell_fill<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.ci, C.w,
+                                                                    C.soff, C.sci, C.sw);
+      RSB_LAUNCH_CHECK();
+      RSB_TRY(hipStreamSynchronize(st));
+      hipFree(sbytes);
+    }
     hipFree(cnt), hipFree(roff), hipFree(key), hipFree(idx), hipFree(wr), hipFree(flag);
     hipFree(wsum), hipFree(rowcnt), hipFree(flag64), hipFree(pos), hipFree(rc64);
   }
@@ -852,67 +943,92 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   const double s1 = sigma_cheb(1, 4, 1.0, 2.0), s2 = sigma_cheb(2, 4, 1.0, 2.0),
                s3 = sigma_cheb(3, 4, 1.0, 2.0);
   const double sig[3] = {s1, s2, s3};
-  double *zc = z, *zo = z0;
+  // One CG iteration is part_a (w = L p, alpha, the update of x and r, the
+  // stop test) and part_b (z = V-cycle(r), beta, the new direction made
+  // L-orthogonal to the earlier ones).  The host reads the count of running
+  // segments after each part_a; part_b + part_a of the next iteration run as
+  // one captured graph (no memset nodes: counters are cleared by a kernel).
+  auto part_a = [&]() -> int {
+    k_mg_apply<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, S, p, w);
+    fin(FIN_ALPHA, GATE_PCG, 1);
+    k_mg_update<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, w, x, r, Wb, Pb);
+    k_mg_zero<<<1, 1, 0, st>>>(active);
+    fin(FIN_CONV, GATE_PCG, 1);
+    RSB_LAUNCH_CHECK();
+    return 0;
+  };
+  auto part_b = [&]() -> int {
+    k_mg_copy<<<ntiles, RSB_BLOCK, 0, st>>>(a, z, z0);
+    // down
+    for (int l = 0; l + 1 < nlev; l++) {
+      mg_level_dev &F = H.lv[l];
+      // the residual ends in F.r0 (level 0 starts from the CG residual r)
+      if (l == 0) {
+        k_mg_jac0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, sig[0], r, F.sa, F.u);
+        k_mg_res0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, r, F.r0, F.sa, sig[1], F.sb, F.u);
+        k_mg_res0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, F.r0, F.r0, F.sb, sig[2], F.sa, F.u);
+        k_mg_res0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, F.r0, F.r0, F.sa, 0.0, nullptr, F.u);
+      } else {
+        const mg_csr M{F.m, F.rseg, F.soff, F.rlen, F.sci, F.sw, F.diag};
+        const unsigned nb = nblocks(F.m, RSB_BLOCK);
+        k_mg_jac_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, sig[0], F.r0, F.sa, F.u);
+        k_mg_res_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, F.r0, F.sa, sig[1], F.sb, F.u);
+        k_mg_res_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, F.r0, F.sb, sig[2], F.sa, F.u);
+        k_mg_res_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, F.r0, F.sa, 0.0, nullptr, F.u);
+      }
+      mg_level_dev &C = H.lv[l + 1];
+      k_mg_restrict<<<nblocks(C.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
+          a, C.m, C.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, F.r0, C.r0);
+    }
+    // up: each coarse level's result goes to its r0
+    for (int l = nlev - 2; l >= 1; l--) {
+      mg_level_dev &F = H.lv[l], &C = H.lv[l + 1];
+      k_mg_prolong_c<<<nblocks(F.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
+          a, F.m, F.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, 1.5, C.r0, F.u, F.r0);
+    }
+    mg_level_dev &F0 = H.lv[0], &C1 = H.lv[1];
+    k_mg_prolong0<<<ntiles, RSB_BLOCK, 0, st>>>(a, mg_lvl{C1.off, C1.n}, factor, 1.33333, C1.r0,
+                                                 F0.u, z);
+    k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, z);
+    fin(FIN_MEAN, GATE_PCG, 1);
+    k_mg_zdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, z, z0);
+    fin(FIN_BETA, GATE_PCG, 2);
+    k_mg_pnew<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, z, p, Wb);
+    k_mg_coef<<<nseg, RSB_WAVE, 0, st>>>(a, coef);
+    k_mg_pcorr<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, Pb, coef);
+    RSB_LAUNCH_CHECK();
+    return 0;
+  };
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  {
+    const char *ev = getenv("PARRSB_AMD_MG_GRAPH");
+    if (!(ev && atoi(ev) == 0)) {
+      RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rb = part_b(), ra = part_a();
+      hipError_t ec = hipStreamEndCapture(st, &graph);
+      if (rb || ra || ec != hipSuccess) RSB_FAIL(2, "MG graph capture failed");
+      RSB_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+  }
   for (int pass = 0; pass < max_passes; pass++) {
     // projection (fiedler.c:94-187)
     k_mg_pass0<<<ntiles, RSB_BLOCK, 0, st>>>(a, b, x, r);
     fin(FIN_PASS0, GATE_INV, 2);
-    k_mg_pass1<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, zc, p);
+    k_mg_pass1<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, z, p);
     fin(FIN_RZ1, GATE_PCG, 1);
     RSB_LAUNCH_CHECK();
-    for (int it = 0; it < max_iter; it++) {
-      k_mg_apply<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, S, p, w);
-      fin(FIN_ALPHA, GATE_PCG, 1);
-      k_mg_update<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, w, x, r, Wb, Pb);
-      RSB_TRY(hipMemsetAsync(active, 0, sizeof(int), st));
-      fin(FIN_CONV, GATE_PCG, 1);
-      RSB_LAUNCH_CHECK();
-      int nact = 0;
-      MG_TRY(count_active(nact));
-      if (nact == 0) break;
-      // z = V-cycle(r) into the other buffer; zo keeps the previous z
-      std::swap(zc, zo);
-      {
-        // down
-        for (int l = 0; l + 1 < nlev; l++) {
-          mg_level_dev &F = H.lv[l];
-          const double *rin = l == 0 ? r : F.r0;
-          double *bufs[2] = {F.r1, F.r0};
-          for (int k = 0; k < 3; k++) {
-            double *rout = bufs[k & 1];
-            if (l == 0)
-              k_mg_smooth0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, sig[k], k == 0, rin, rout, F.u);
-            else
-              k_mg_smooth_c<<<nblocks(F.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
-                  a, mg_csr{F.m, F.rseg, F.rp, F.ci, F.w, F.diag}, sig[k], k == 0, rin, rout,
-                  F.u);
-            rin = rout;
-          }
-          // after three steps the residual is in r1 (steps write r1, r0, r1)
-          mg_level_dev &C = H.lv[l + 1];
-          k_mg_restrict<<<nblocks(C.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
-              a, C.m, C.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, F.r1, C.r0);
-        }
-        // up: each coarse level's result goes to its r0
-        for (int l = nlev - 2; l >= 1; l--) {
-          mg_level_dev &F = H.lv[l], &C = H.lv[l + 1];
-          k_mg_prolong_c<<<nblocks(F.m, RSB_BLOCK), RSB_BLOCK, 0, st>>>(
-              a, F.m, F.rseg, mg_lvl{F.off, F.n}, mg_lvl{C.off, C.n}, factor, 1.5, C.r0, F.u,
-              F.r0);
-        }
-        mg_level_dev &F0 = H.lv[0], &C1 = H.lv[1];
-        k_mg_prolong0<<<ntiles, RSB_BLOCK, 0, st>>>(a, mg_lvl{C1.off, C1.n}, factor, 1.33333,
-                                                     C1.r0, F0.u, zc);
-        RSB_LAUNCH_CHECK();
+    MG_TRY(part_a());
+    int nact = 0;
+    MG_TRY(count_active(nact));
+    for (int it = 1; it < max_iter && nact > 0; it++) {
+      if (exec) {
+        RSB_TRY(hipGraphLaunch(exec, st));
+      } else {
+        MG_TRY(part_b());
+        MG_TRY(part_a());
       }
-      k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_PCG, zc);
-      fin(FIN_MEAN, GATE_PCG, 1);
-      k_mg_zdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, zc, zo);
-      fin(FIN_BETA, GATE_PCG, 2);
-      k_mg_pnew<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, zc, p, Wb);
-      k_mg_coef<<<nseg, RSB_WAVE, 0, st>>>(a, coef);
-      k_mg_pcorr<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, Pb, coef);
-      RSB_LAUNCH_CHECK();
+      MG_TRY(count_active(nact));
     }
     // end of pass (fiedler.c:261-275): y orthogonalised, z = y / |y|, z orthogonalised
     k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
@@ -920,13 +1036,13 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     k_mg_center_dot<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
     fin(FIN_NORM, GATE_INV, 1);
     k_mg_scale_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x, b);
-    RSB_TRY(hipMemsetAsync(active, 0, sizeof(int), st));
+    k_mg_zero<<<1, 1, 0, st>>>(active);
     fin(FIN_PASSEND, GATE_INV, 1);
     k_mg_center<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, b);
     RSB_LAUNCH_CHECK();
-    int nact = 0;
-    MG_TRY(count_active(nact));
-    if (nact == 0) break;
+    int nrun = 0;
+    MG_TRY(count_active(nrun));
+    if (nrun == 0) break;
   }
   // result: the last y, normalised; Rayleigh quotient for the report
   k_mg_out<<<ntiles, RSB_BLOCK, 0, st>>>(a, x, g->d_fied);
@@ -948,6 +1064,8 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   hipFree(b), hipFree(x), hipFree(r), hipFree(z), hipFree(z0), hipFree(p), hipFree(w);
   hipFree(Wb), hipFree(Pb), hipFree(sc), hipFree(stt), hipFree(partial), hipFree(coef);
   hipFree(active);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
   // diag0 stays alive with the kept hierarchy (level 0's diagonal)
   rsb_phase_end(g, PH_LANCZOS);
   return 0;

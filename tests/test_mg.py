@@ -228,3 +228,22 @@ def test_device_mg_option_errors():
         with pytest.raises(pa.ParrsbError) as e:
             pa.part_mesh(vtx, xyz, nv, 2, **kw)
         assert e.value.rc == 3
+
+
+@pytest.mark.gpu
+def test_device_mg_graph_replays_equal_eager_launches(monkeypatch):
+    """Each CG iteration after the first runs as a replay of one captured
+    graph; with the graph off (PARRSB_AMD_MG_GRAPH=0) the same kernels run
+    eagerly.  Every replay feeds the next, so equal results over a whole
+    partition (eager solves in between) check all of them."""
+    flush("mg graph vs eager")
+    vtx, xyz, nv = M.box_hex(12, 10, 9, jitter=0.1, seed=7)
+    a, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
+    ra = pa.last_report()
+    monkeypatch.setenv("PARRSB_AMD_MG_GRAPH", "0")
+    b, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
+    rb = pa.last_report()
+    monkeypatch.delenv("PARRSB_AMD_MG_GRAPH")
+    c, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert ra["niter_max"] == rb["niter_max"] and ra["lambda_min"] == rb["lambda_min"]
