@@ -9,7 +9,7 @@ batched FP32 path from the reference start and the plain FP64 GPU path
 (reference arithmetic and start), and the CPU oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5]
-                    [--warm-start PATH|none]
+                    [--warm-start PATH|none] [--mg]
 """
 import argparse
 import json
@@ -109,6 +109,20 @@ def cpu_baseline(cfg):
             "sample": sample, "seconds": dt}
 
 
+def cpu_baseline_mg():
+    """The oracle's MG inverse iteration on a bounded sample: 36^3 warped hex
+    box into 182 parts (256 elements per part, as the 256^3 workload)."""
+    from oracle import pyoracle as O
+    from parrsb_amd import meshgen as M
+    vtx, xyz, nv = M.warped_box_hex(36, seed=5)
+    t0 = time.perf_counter()
+    O.part_mesh(vtx, xyz, nv, 182, rsb_algo=1, mg_factor=2)
+    dt = time.perf_counter() - t0
+    return {"value": vtx.shape[0] / dt / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+            "sample": "warped hex 36^3 = 46,656 elements into 182 parts, rsb_algo=1",
+            "seconds": dt}
+
+
 def summarize_traffic(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, desc,
                       commands):
     """HBM bytes per launch of k_lz_apply (and k_lz_rupdate) from two
@@ -184,6 +198,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--warm-start", default=os.path.join(ROOT, "parrsb_amd", "lws_weights.bin"),
                     help="network parameter file of the learned start, or 'none'")
+    ap.add_argument("--mg", action="store_true",
+                    help="also time the MG inverse-iteration solver (rsb_algo = 1) on the "
+                         "workload, with its own CPU-oracle sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build partitions on one GPU; only --gpus 1 is supported",
@@ -260,6 +277,24 @@ def main():
         out["plain_fp64"] = {"value": nel / dt64 / 1e6, "ms": dt64 * 1e3, "quality": q64}
         out["speedup_vs_plain_fp64"] = (nel / dt64 / 1e6 and value / (nel / dt64 / 1e6))
         out["edge_cut_vs_plain_fp64"] = q["edge_cut"] / max(q64["edge_cut"], 1)
+    if args.mg:
+        pa.set_precision(64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pa.part_mesh_dev(d_part.data_ptr(), d_vtx.data_ptr(), d_xyz.data_ptr(), nel, nv, P,
+                         rsb_algo=1, rsb_mg_factor=2)
+        torch.cuda.synchronize()
+        dtm = time.perf_counter() - t0
+        repm = pa.last_report()
+        qm = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
+        log(f"MG inverse iteration: {dtm:.3f} s")
+        out["mg"] = {"value": nel / dtm / 1e6, "ms": dtm * 1e3, "quality": qm,
+                     "solver": "MG-preconditioned inverse iteration (rsb_algo=1, factor 2), "
+                               "FP64",
+                     "niter_max": repm["niter_max"], "phase_ms": repm["phase_ms"],
+                     "edge_cut_vs_lanczos": qm["edge_cut"] / max(q["edge_cut"], 1)}
+        if not args.no_cpu:
+            out["mg"]["cpu_baseline"] = cpu_baseline_mg()
     if not args.no_cpu:
         log("CPU baseline (oracle) on a bounded sample")
         cb = cpu_baseline(args.config)
