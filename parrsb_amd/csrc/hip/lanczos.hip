@@ -299,7 +299,7 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
   }
   pnew[row] = qi;
   w[row] = sum;
-  b[row] = rl[li] * rni;
+  __builtin_nontemporal_store(rl[li] * rni, &b[row]);
   if (li < nreal) {
     sq.add(qi);
     swq.dot(sum, qi);
@@ -373,7 +373,7 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
       acc_entry(tl[k].y >> 16, qi, ql, sum);
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
-      b[rows[k]] = rl[li] * rni;
+      __builtin_nontemporal_store(rl[li] * rni, &b[rows[k]]);
       if (li < nreal) {
         sq.add(qi);
         swq.dot(sum, qi);
@@ -392,15 +392,34 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
   const int nrows = t.row1 - t.row0;
   const int64_t ho = ta.halo_off[t.ti];
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
-  for (int idx = threadIdx.x; idx < nrows + nh; idx += blockDim.x) {
-    const int gr = idx < nrows ? t.row0 + idx : ta.halo[ho + idx - nrows];
-    const T rg = r[gr];
-    T q = FIRST ? rg : beta * (pold[gr] - m) + rg;
-    if (idx < nrows) {
-      rl[idx] = rg;
-      if (idx >= nreal) q = (T)0;
+  const int tot = nrows + nh;
+  // staging: SU indices per thread and step, all their loads issued before
+  // any value is used (the halo rows are gathers)
+  constexpr int SU = 4;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += SU * blockDim.x) {
+    int gr[SU];
+    T rg[SU], po[SU];
+#pragma unroll
+    for (int k = 0; k < SU; k++) {
+      const int idx = i0 + k * blockDim.x;
+      gr[k] = idx < nrows ? t.row0 + idx : (idx < tot ? ta.halo[ho + idx - nrows] : t.row0);
     }
-    ql[idx] = q;
+#pragma unroll
+    for (int k = 0; k < SU; k++) {
+      rg[k] = r[gr[k]];
+      po[k] = FIRST ? (T)0 : pold[gr[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < SU; k++) {
+      const int idx = i0 + k * blockDim.x;
+      if (idx >= tot) break;
+      T q = FIRST ? rg[k] : beta * (po[k] - m) + rg[k];
+      if (idx < nrows) {
+        rl[idx] = rg[k];
+        if (idx >= nreal) q = (T)0;
+      }
+      ql[idx] = q;
+    }
   }
   __syncthreads();
   // two rows per thread and step (measured 15% faster than one at a time
