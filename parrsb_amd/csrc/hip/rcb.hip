@@ -236,7 +236,7 @@ __global__ void k_child_count(int K, const nodes_t nd, int *__restrict__ cnt) {
 }
 
 __global__ void k_children(int K, const nodes_t nd, const int *__restrict__ first, nodes_t ch,
-                           int *__restrict__ split) {
+                           int *__restrict__ split, int *__restrict__ stat) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   int c = first[k];
@@ -253,6 +253,18 @@ __global__ void k_children(int K, const nodes_t nd, const int *__restrict__ firs
   ch.start[c + 1] = nd.start[k] + nl, ch.len[c + 1] = n - nl, ch.p[c + 1] = pu;
   ch.frozen[c + 1] = (n - nl) <= 1;
   split[k] = nd.start[k] + nl;
+  // stat[0]: longest child that still splits; stat[1]: some such child has
+  // more ranks than elements (then empty nodes can appear below it)
+  if (stat) {
+    if (nl > 1) {
+      atomicMax(&stat[0], nl);
+      if (pl > nl) atomicOr(&stat[1], 1);
+    }
+    if (n - nl > 1) {
+      atomicMax(&stat[0], n - nl);
+      if (pu > n - nl) atomicOr(&stat[1], 1);
+    }
+  }
 }
 
 __global__ void k_nodeof_next(int nel, const int *__restrict__ first,
@@ -261,6 +273,154 @@ __global__ void k_nodeof_next(int nel, const int *__restrict__ first,
   if (i >= nel) return;
   int k = nodeof[i];
   nodeof[i] = first[k] + (i >= split[k] ? 1 : 0);
+}
+
+// Tail of the RCB tree: once no node holds more than RCB_TAIL elements, one
+// workgroup finishes each node in LDS, depth by depth, with the same rules as
+// the global rounds: per sub-node extents (coordinate min/max), axis of the
+// largest extent (ties to the lower axis), a stable sort along it (bitonic
+// sort of (sub-node, dense rank, position) keys, so ties keep their order),
+// and the split after the lower ranks' share (p > 1) or at n / 2 (p == 1).
+#define RCB_TAIL 1024
+#define RCB_TAIL_T 256
+
+__global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int *__restrict__ order,
+                                                        const double *__restrict__ coord,
+                                                        const uint32_t *__restrict__ rank,
+                                                        int nel, int ndim) {
+  const int k = blockIdx.x;
+  if (k >= K || nd.frozen[k]) return;
+  const int start = nd.start[k], n = nd.len[k];
+  __shared__ int el[RCB_TAIL], el2[RCB_TAIL];
+  __shared__ uint64_t key[RCB_TAIL];
+  __shared__ short sub[RCB_TAIL];
+  // sub-node tables (at most n / 2 non-frozen sub-nodes, up to n in all)
+  __shared__ int s_start[RCB_TAIL], s_len[RCB_TAIL], s_p[RCB_TAIL], s_first[RCB_TAIL + 1];
+  __shared__ unsigned long long s_ext[6][RCB_TAIL / 2];
+  __shared__ int s_axis[RCB_TAIL], s_ci[RCB_TAIL];
+  __shared__ int nsub, nsub2, busy, nsplit;
+  for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el[i] = order[start + i], sub[i] = 0;
+  if (threadIdx.x == 0) {
+    s_start[0] = 0, s_len[0] = n, s_p[0] = nd.p[k];
+    nsub = 1;
+  }
+  __syncthreads();
+  int npow = 1;
+  while (npow < n) npow <<= 1;
+  for (int depth = 0; depth < 64; depth++) {
+    const int ns = nsub;
+    // sub-nodes that still split (len >= 2) get a compact index into the
+    // extent table; they hold >= 2 elements each, so there are <= n / 2
+    if (threadIdx.x == 0) {
+      int c = 0;
+      for (int t = 0; t < ns; t++) s_ci[t] = s_len[t] >= 2 ? c++ : -1;
+      nsplit = c;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nsplit; j += RCB_TAIL_T)
+      for (int d = 0; d < 3; d++) s_ext[2 * d][j] = ~0ull, s_ext[2 * d + 1][j] = 0ull;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) {
+      const int j = s_ci[sub[i]];
+      if (j < 0) continue;
+      const int e = el[i];
+      for (int d = 0; d < ndim; d++) {
+        const unsigned long long v = dkey(coord[(size_t)d * nel + e]);
+        atomicMin(&s_ext[2 * d][j], v);
+        atomicMax(&s_ext[2 * d + 1][j], v);
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) {
+      int a = -1;
+      const int j = s_ci[t];
+      if (j >= 0) {
+        double len[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < ndim; d++) len[d] = dkey_inv(s_ext[2 * d + 1][j]) - dkey_inv(s_ext[2 * d][j]);
+        a = 0;
+        if (len[1] > len[0]) a = 1;
+        if (ndim == 3 && len[2] > len[a]) a = 2;
+      }
+      s_axis[t] = a;
+    }
+    __syncthreads();
+    // keys: sub-node (11 bits) | dense rank (32 bits) | position (10 bits)
+    for (int i = threadIdx.x; i < npow; i += RCB_TAIL_T) {
+      if (i < n) {
+        const int t = sub[i], a = s_axis[t];
+        const uint64_t r = a < 0 ? 0ull : (uint64_t)rank[(size_t)a * nel + el[i]];
+        key[i] = ((uint64_t)t << 42) | (r << 10) | (uint64_t)i;
+      } else {
+        key[i] = ~0ull;
+      }
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= npow; kk <<= 1)
+      for (int j = kk >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < npow; i += RCB_TAIL_T) {
+          const int l = i ^ j;
+          if (l > i) {
+            const uint64_t a = key[i], b = key[l];
+            const bool up = (i & kk) == 0;
+            if ((a > b) == up) key[i] = b, key[l] = a;
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el2[i] = el[key[i] & 1023u];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el[i] = el2[i];
+    // children: 2 per sub-node that splits, 1 otherwise (prefix by thread 0:
+    // at most RCB_TAIL sub-nodes)
+    if (threadIdx.x == 0) {
+      int c = 0, any = 0;
+      for (int t = 0; t < ns; t++) {
+        s_first[t] = c;
+        const int sp = s_len[t] >= 2;
+        c += sp ? 2 : 1;
+        any |= sp;
+      }
+      s_first[ns] = c;
+      nsub2 = c;
+      busy = any;
+    }
+    __syncthreads();
+    if (!busy) break;
+    // new tables written to the upper half of the arrays would overlap: form
+    // them in registers of the owning thread, then write after a barrier
+    int cs[RCB_TAIL / RCB_TAIL_T][2], cl[RCB_TAIL / RCB_TAIL_T][2], cp[RCB_TAIL / RCB_TAIL_T][2];
+    int q = 0;
+    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T, q++) {
+      const int st = s_start[t], ln = s_len[t], pp = s_p[t];
+      if (ln >= 2) {
+        const int nl = pp > 1 ? (int)lower_count_dev(ln, pp) : ln / 2;
+        cs[q][0] = st, cl[q][0] = nl, cp[q][0] = pp > 1 ? (pp + 1) / 2 : 1;
+        cs[q][1] = st + nl, cl[q][1] = ln - nl, cp[q][1] = pp > 1 ? pp / 2 : 1;
+      } else {
+        cs[q][0] = st, cl[q][0] = ln, cp[q][0] = pp;
+      }
+    }
+    // element -> child sub-node (reads the old tables)
+    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) {
+      const int t = sub[i];
+      int c = s_first[t];
+      if (s_len[t] >= 2) {
+        const int pp = s_p[t], ln = s_len[t];
+        const int nl = pp > 1 ? (int)lower_count_dev(ln, pp) : ln / 2;
+        if (i >= s_start[t] + nl) c++;
+      }
+      sub[i] = (short)c;
+    }
+    __syncthreads();
+    q = 0;
+    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T, q++) {
+      const int c = s_first[t], two = s_first[t + 1] - c;
+      for (int h = 0; h < two; h++) s_start[c + h] = cs[q][h], s_len[c + h] = cl[q][h], s_p[c + h] = cp[q][h];
+    }
+    if (threadIdx.x == 0) nsub = nsub2;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) order[start + i] = el[i];
 }
 
 // Depth of the RCB tree of (n, p): the larger child is never shallower.
@@ -297,11 +457,29 @@ static int run_rcb(rsbgpu *g) {
   RSB_TRY(hipMalloc((void **)&ext, sizeof(uint64_t) * 6 * cap));
   RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ((size_t)nel + 1)));
   int K = nseg;
+  int *stat = nullptr;
+  RSB_TRY(hipMalloc((void **)&stat, sizeof(int) * 2));
+  // the tail kernel takes over once every splitting node is small and has
+  // no more ranks than elements
+  const char *tv = getenv("PARRSB_AMD_RCB_TAIL");
+  bool tail_ok = tv && atoi(tv) == 1;
+  int maxlen = 0, pbig = 0;
+  for (int s = 0; s < nseg; s++)
+    if (g->h_p[s] > 1 && g->h_n[s] > 1) {
+      maxlen = std::max(maxlen, g->h_n[s]);
+      pbig |= g->h_p[s] > g->h_n[s];
+    }
   k_nodes_from_segments<<<nblocks(nseg, 256), 256, 0, g->stream>>>(g->d_seg, nseg, A);
   RSB_LAUNCH_CHECK();
   k_nodeof_init<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_seg, nseg, nel, nodeof);
   RSB_LAUNCH_CHECK();
   for (int r = 0; r < rounds; r++) {
+    if (tail_ok && maxlen <= RCB_TAIL && !pbig) {
+      k_rcb_tail<<<K, RCB_TAIL_T, 0, g->stream>>>(A, K, g->d_order, g->d_coord, g->d_rank, nel,
+                                                  g->ndim);
+      RSB_LAUNCH_CHECK();
+      break;
+    }
     k_node_extent_init<<<nblocks(K, 256), 256, 0, g->stream>>>(K, ext);
     RSB_LAUNCH_CHECK();
     k_node_extent<<<nblocks(nel, 256), 256, 0, g->stream>>>(
@@ -330,14 +508,19 @@ static int run_rcb(rsbgpu *g) {
     RSB_TRY(hipStreamSynchronize(g->stream));
     int K2 = last[0] + last[1];
     if ((size_t)K2 > cap) RSB_FAIL(2, "RCB node table overflow (%d > %zu)", K2, cap);
-    k_children<<<nblocks(K, 256), 256, 0, g->stream>>>(K, A, first, B, split);
+    RSB_TRY(hipMemsetAsync(stat, 0, sizeof(int) * 2, g->stream));
+    k_children<<<nblocks(K, 256), 256, 0, g->stream>>>(K, A, first, B, split, stat);
     RSB_LAUNCH_CHECK();
+    int hstat[2];
+    RSB_TRY(hipMemcpyAsync(hstat, stat, sizeof(int) * 2, hipMemcpyDeviceToHost, g->stream));
     k_nodeof_next<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, first, split, nodeof);
     RSB_LAUNCH_CHECK();
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    maxlen = hstat[0], pbig = hstat[1];
     K = K2;
     std::swap(A, B);
   }
-  hipFree(buf), hipFree(ext), hipFree(key);
+  hipFree(buf), hipFree(ext), hipFree(key), hipFree(stat);
   return 0;
 }
 
@@ -657,7 +840,7 @@ static int run_rib(rsbgpu *g) {
     RSB_TRY(hipStreamSynchronize(st));
     const int K2 = tail[0] + tail[1];
     if ((size_t)K2 > cap) RSB_FAIL(2, "RIB node table overflow (%d > %zu)", K2, cap);
-    k_children<<<nblocks(K, 256), 256, 0, st>>>(K, A, first, B, split);
+    k_children<<<nblocks(K, 256), 256, 0, st>>>(K, A, first, B, split, nullptr);
     RSB_LAUNCH_CHECK();
     k_nodeof_next<<<nblocks(nel, 256), 256, 0, st>>>(nel, first, split, nodeof);
     RSB_LAUNCH_CHECK();

@@ -65,6 +65,33 @@ def test_rcb_order_bit_exact(oracle, name, p):
     assert np.array_equal(got, ref)
 
 
+def test_rcb_tail_kernel_equals_global_rounds(oracle, monkeypatch):
+    """With PARRSB_AMD_RCB_TAIL=1, nodes of <= 1024 elements are finished by
+    one LDS kernel; the global rounds alone and the oracle give the same
+    orders, also next to a segment with more ranks than elements (which keeps
+    the global rounds)."""
+    flush("rcb tail")
+    vtx, xyz, nv = M.box_hex(36, 30, 24, jitter=0.1, seed=12)
+    n = vtx.shape[0]
+    c = oracle.centroids(xyz, nv)
+    for sizes, ps in (([n - 3000, 3000], [7, 1]), ([n - 10, 10], [13, 16])):
+        starts = [0, sizes[0]]
+        orders = []
+        for env in ("1", "0"):
+            monkeypatch.setenv("PARRSB_AMD_RCB_TAIL", env)
+            with pa.Device() as d:
+                d.set_mesh(vtx, xyz, nv)
+                d.set_segments(starts, sizes, ps, [0, ps[0]])
+                d.prepartition(1)
+                orders.append(d.order())
+            monkeypatch.delenv("PARRSB_AMD_RCB_TAIL", raising=False)
+        assert np.array_equal(orders[0], orders[1])
+        for s0, sz, p in zip(starts, sizes, ps):
+            seg = np.arange(s0, s0 + sz, dtype=np.int32)
+            ref = oracle.rcb_order(c, 3, seg, p) if p > 1 else seg
+            assert np.array_equal(orders[0][s0:s0 + sz], ref)
+
+
 def test_rcb_many_segments_bit_exact(oracle):
     vtx, xyz, nv = M.quad_grid(40, 31)  # lattice centroids: ties everywhere
     n = vtx.shape[0]
