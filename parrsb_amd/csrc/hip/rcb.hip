@@ -284,64 +284,135 @@ __global__ void k_nodeof_next(int nel, const int *__restrict__ first,
 #define RCB_TAIL 1024
 #define RCB_TAIL_T 256
 
+// Exclusive prefix sum of v[0..n) (n <= RCB_TAIL) in place by the block;
+// returns the total.  Each thread owns RCB_TAIL / RCB_TAIL_T consecutive
+// items.
+__device__ inline int tail_scan(int *v, int n, int *wsum) {
+  constexpr int PER = RCB_TAIL / RCB_TAIL_T;
+  const int t = threadIdx.x, lane = t & (RSB_WAVE - 1), wid = t / RSB_WAVE;
+  int loc[PER], run = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = t * PER + k;
+    loc[k] = i < n ? v[i] : 0;
+    run += loc[k];
+  }
+  // inclusive scan of the thread totals inside the wave
+  int x = run;
+  for (int o = 1; o < RSB_WAVE; o <<= 1) {
+    const int y = __shfl_up(x, o, RSB_WAVE);
+    if (lane >= o) x += y;
+  }
+  if (lane == RSB_WAVE - 1) wsum[wid] = x;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int w = 0; w < RCB_TAIL_T / RSB_WAVE; w++) {
+    if (w < wid) before += wsum[w];
+    total += wsum[w];
+  }
+  int acc = before + x - run;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int i = t * PER + k;
+    if (i < n) v[i] = acc;
+    acc += loc[k];
+  }
+  __syncthreads();
+  return total;
+}
+
 __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int *__restrict__ order,
                                                         const double *__restrict__ coord,
                                                         const uint32_t *__restrict__ rank,
                                                         int nel, int ndim) {
+  constexpr int PER = RCB_TAIL / RCB_TAIL_T;
   const int k = blockIdx.x;
   if (k >= K || nd.frozen[k]) return;
   const int start = nd.start[k], n = nd.len[k];
-  __shared__ int el[RCB_TAIL], el2[RCB_TAIL];
+  __shared__ int el[RCB_TAIL];
   __shared__ uint64_t key[RCB_TAIL];
   __shared__ short sub[RCB_TAIL];
-  // sub-node tables (at most n / 2 non-frozen sub-nodes, up to n in all)
-  __shared__ int s_start[RCB_TAIL], s_len[RCB_TAIL], s_p[RCB_TAIL], s_first[RCB_TAIL + 1];
+  // sub-node tables: start, length, ranks; scratch for the prefix sums
+  __shared__ int s_start[RCB_TAIL], s_len[RCB_TAIL], s_p[RCB_TAIL], s_sc[RCB_TAIL];
   __shared__ unsigned long long s_ext[6][RCB_TAIL / 2];
-  __shared__ int s_axis[RCB_TAIL], s_ci[RCB_TAIL];
-  __shared__ int nsub, nsub2, busy, nsplit;
+  __shared__ signed char s_axis[RCB_TAIL];
+  __shared__ int wsum[RCB_TAIL_T / RSB_WAVE];
+  __shared__ int nsub;
   for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el[i] = order[start + i], sub[i] = 0;
-  if (threadIdx.x == 0) {
-    s_start[0] = 0, s_len[0] = n, s_p[0] = nd.p[k];
-    nsub = 1;
-  }
+  if (threadIdx.x == 0) s_start[0] = 0, s_len[0] = n, s_p[0] = nd.p[k], nsub = 1;
   __syncthreads();
   int npow = 1;
   while (npow < n) npow <<= 1;
   for (int depth = 0; depth < 64; depth++) {
     const int ns = nsub;
-    // sub-nodes that still split (len >= 2) get a compact index into the
-    // extent table; they hold >= 2 elements each, so there are <= n / 2
-    if (threadIdx.x == 0) {
-      int c = 0;
-      for (int t = 0; t < ns; t++) s_ci[t] = s_len[t] >= 2 ? c++ : -1;
-      nsplit = c;
-    }
+    // compact index of the sub-nodes that still split (>= 2 elements: at
+    // most n / 2 of them)
+    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) s_sc[t] = s_len[t] >= 2;
     __syncthreads();
+    const int nsplit = tail_scan(s_sc, ns, wsum);
+    if (nsplit == 0) break;
     for (int j = threadIdx.x; j < nsplit; j += RCB_TAIL_T)
       for (int d = 0; d < 3; d++) s_ext[2 * d][j] = ~0ull, s_ext[2 * d + 1][j] = 0ull;
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) {
-      const int j = s_ci[sub[i]];
-      if (j < 0) continue;
-      const int e = el[i];
+    {
+      // each thread takes PER consecutive elements; runs of one sub-node are
+      // reduced in registers, a wave whose elements all lie in one sub-node
+      // reduces by shuffles, so the LDS atomics stay few (no contention on
+      // the shallow depths, where a sub-node spans many waves)
+      const int i0 = threadIdx.x * PER, lane = threadIdx.x & (RSB_WAVE - 1);
+      const int tf = i0 < n ? sub[i0] : -1;
+      const int tl = i0 + PER - 1 < n ? sub[i0 + PER - 1] : -2;
+      const int wfirst = __shfl(tf, 0, RSB_WAVE), wlast = __shfl(tl, RSB_WAVE - 1, RSB_WAVE);
+      const bool whole = wfirst >= 0 && wfirst == wlast && s_len[wfirst] >= 2;
       for (int d = 0; d < ndim; d++) {
-        const unsigned long long v = dkey(coord[(size_t)d * nel + e]);
-        atomicMin(&s_ext[2 * d][j], v);
-        atomicMax(&s_ext[2 * d + 1][j], v);
+        unsigned long long mn = ~0ull, mx = 0ull;
+        int cur = -1;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+          const int i = i0 + q;
+          const int t = i < n ? sub[i] : -1;
+          if (t != cur) {
+            if (!whole && cur >= 0 && s_len[cur] >= 2) {
+              atomicMin(&s_ext[2 * d][s_sc[cur]], mn);
+              atomicMax(&s_ext[2 * d + 1][s_sc[cur]], mx);
+            }
+            cur = t, mn = ~0ull, mx = 0ull;
+          }
+          if (t >= 0) {
+            const unsigned long long v = dkey(coord[(size_t)d * nel + el[i]]);
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+          }
+        }
+        if (whole) {
+          for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
+            const unsigned long long a = __shfl_xor(mn, o, RSB_WAVE), b = __shfl_xor(mx, o, RSB_WAVE);
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+          }
+          if (lane == 0) {
+            atomicMin(&s_ext[2 * d][s_sc[wfirst]], mn);
+            atomicMax(&s_ext[2 * d + 1][s_sc[wfirst]], mx);
+          }
+        } else if (cur >= 0 && s_len[cur] >= 2) {
+          atomicMin(&s_ext[2 * d][s_sc[cur]], mn);
+          atomicMax(&s_ext[2 * d + 1][s_sc[cur]], mx);
+        }
       }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) {
       int a = -1;
-      const int j = s_ci[t];
-      if (j >= 0) {
+      if (s_len[t] >= 2) {
+        const int j = s_sc[t];
         double len[3] = {0.0, 0.0, 0.0};
-        for (int d = 0; d < ndim; d++) len[d] = dkey_inv(s_ext[2 * d + 1][j]) - dkey_inv(s_ext[2 * d][j]);
+        for (int d = 0; d < ndim; d++)
+          len[d] = dkey_inv(s_ext[2 * d + 1][j]) - dkey_inv(s_ext[2 * d][j]);
         a = 0;
         if (len[1] > len[0]) a = 1;
         if (ndim == 3 && len[2] > len[a]) a = 2;
       }
-      s_axis[t] = a;
+      s_axis[t] = (signed char)a;
     }
     __syncthreads();
     // keys: sub-node (11 bits) | dense rank (32 bits) | position (10 bits)
@@ -355,69 +426,71 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
       }
     }
     __syncthreads();
+    // bitonic network: thread work is indexed by compare pair (npow / 2 of
+    // them per stage), so no lane idles on the l < i half
     for (int kk = 2; kk <= npow; kk <<= 1)
       for (int j = kk >> 1; j > 0; j >>= 1) {
-        for (int i = threadIdx.x; i < npow; i += RCB_TAIL_T) {
-          const int l = i ^ j;
-          if (l > i) {
-            const uint64_t a = key[i], b = key[l];
-            const bool up = (i & kk) == 0;
-            if ((a > b) == up) key[i] = b, key[l] = a;
-          }
+        for (int pr = threadIdx.x; pr < (npow >> 1); pr += RCB_TAIL_T) {
+          const int i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1)), l = i + j;
+          const uint64_t a = key[i], b = key[l];
+          const bool up = (i & kk) == 0;
+          if ((a > b) == up) key[i] = b, key[l] = a;
         }
         __syncthreads();
       }
-    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el2[i] = el[key[i] & 1023u];
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el[i] = el2[i];
-    // children: 2 per sub-node that splits, 1 otherwise (prefix by thread 0:
-    // at most RCB_TAIL sub-nodes)
-    if (threadIdx.x == 0) {
-      int c = 0, any = 0;
-      for (int t = 0; t < ns; t++) {
-        s_first[t] = c;
-        const int sp = s_len[t] >= 2;
-        c += sp ? 2 : 1;
-        any |= sp;
+    {
+      int mv[PER];
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int i = threadIdx.x + q * RCB_TAIL_T;
+        mv[q] = i < n ? el[key[i] & 1023u] : 0;
       }
-      s_first[ns] = c;
-      nsub2 = c;
-      busy = any;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int i = threadIdx.x + q * RCB_TAIL_T;
+        if (i < n) el[i] = mv[q];
+      }
     }
+    // children: 2 per sub-node that splits, 1 otherwise
+    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) s_sc[t] = s_len[t] >= 2 ? 2 : 1;
     __syncthreads();
-    if (!busy) break;
-    // new tables written to the upper half of the arrays would overlap: form
-    // them in registers of the owning thread, then write after a barrier
-    int cs[RCB_TAIL / RCB_TAIL_T][2], cl[RCB_TAIL / RCB_TAIL_T][2], cp[RCB_TAIL / RCB_TAIL_T][2];
-    int q = 0;
-    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T, q++) {
+    const int ns2 = tail_scan(s_sc, ns, wsum);  // s_sc[t] = first child of t
+    int cs[PER][2], cl[PER][2], cp[PER][2], cf[PER], two[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int t = threadIdx.x + q * RCB_TAIL_T;
+      two[q] = 0;
+      if (t >= ns) continue;
       const int st = s_start[t], ln = s_len[t], pp = s_p[t];
+      cf[q] = s_sc[t];
       if (ln >= 2) {
         const int nl = pp > 1 ? (int)lower_count_dev(ln, pp) : ln / 2;
         cs[q][0] = st, cl[q][0] = nl, cp[q][0] = pp > 1 ? (pp + 1) / 2 : 1;
         cs[q][1] = st + nl, cl[q][1] = ln - nl, cp[q][1] = pp > 1 ? pp / 2 : 1;
+        two[q] = 2;
       } else {
         cs[q][0] = st, cl[q][0] = ln, cp[q][0] = pp;
+        two[q] = 1;
       }
     }
     // element -> child sub-node (reads the old tables)
     for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) {
-      const int t = sub[i];
-      int c = s_first[t];
-      if (s_len[t] >= 2) {
-        const int pp = s_p[t], ln = s_len[t];
+      const int t = sub[i], ln = s_len[t];
+      int c = s_sc[t];
+      if (ln >= 2) {
+        const int pp = s_p[t];
         const int nl = pp > 1 ? (int)lower_count_dev(ln, pp) : ln / 2;
         if (i >= s_start[t] + nl) c++;
       }
       sub[i] = (short)c;
     }
     __syncthreads();
-    q = 0;
-    for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T, q++) {
-      const int c = s_first[t], two = s_first[t + 1] - c;
-      for (int h = 0; h < two; h++) s_start[c + h] = cs[q][h], s_len[c + h] = cl[q][h], s_p[c + h] = cp[q][h];
-    }
-    if (threadIdx.x == 0) nsub = nsub2;
+#pragma unroll
+    for (int q = 0; q < PER; q++)
+      for (int h = 0; h < two[q]; h++)
+        s_start[cf[q] + h] = cs[q][h], s_len[cf[q] + h] = cl[q][h], s_p[cf[q] + h] = cp[q][h];
+    if (threadIdx.x == 0) nsub = ns2;
     __syncthreads();
   }
   for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) order[start + i] = el[i];
@@ -462,7 +535,7 @@ static int run_rcb(rsbgpu *g) {
   // the tail kernel takes over once every splitting node is small and has
   // no more ranks than elements
   const char *tv = getenv("PARRSB_AMD_RCB_TAIL");
-  bool tail_ok = tv && atoi(tv) == 1;
+  bool tail_ok = !(tv && atoi(tv) == 0);
   int maxlen = 0, pbig = 0;
   for (int s = 0; s < nseg; s++)
     if (g->h_p[s] > 1 && g->h_n[s] > 1) {

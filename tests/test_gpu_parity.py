@@ -66,10 +66,10 @@ def test_rcb_order_bit_exact(oracle, name, p):
 
 
 def test_rcb_tail_kernel_equals_global_rounds(oracle, monkeypatch):
-    """With PARRSB_AMD_RCB_TAIL=1, nodes of <= 1024 elements are finished by
-    one LDS kernel; the global rounds alone and the oracle give the same
-    orders, also next to a segment with more ranks than elements (which keeps
-    the global rounds)."""
+    """Nodes of <= 1024 elements are finished by one LDS kernel; the global
+    rounds alone (PARRSB_AMD_RCB_TAIL=0) and the oracle give the same orders,
+    also next to a segment with more ranks than elements (which keeps the
+    global rounds)."""
     flush("rcb tail")
     vtx, xyz, nv = M.box_hex(36, 30, 24, jitter=0.1, seed=12)
     n = vtx.shape[0]
