@@ -274,7 +274,17 @@ def main():
     if not args.no_compare and args.precision == 32:
         dt64, reps64 = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 64, 1, 0, False)
         q64 = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
-        out["plain_fp64"] = {"value": nel / dt64 / 1e6, "ms": dt64 * 1e3, "quality": q64}
+        r64 = reps64[-1]
+        lam_err = []
+        for l in range(len(rep["lambda_min"])):
+            e = 0.0
+            for k in ("lambda_min", "lambda_max"):
+                a, b = rep[k][l], r64[k][l]
+                if b:
+                    e = max(e, abs(a - b) / abs(b))
+            lam_err.append(e)
+        out["plain_fp64"] = {"value": nel / dt64 / 1e6, "ms": dt64 * 1e3, "quality": q64,
+                             "lambda_rel_err_per_level": lam_err}
         out["speedup_vs_plain_fp64"] = (nel / dt64 / 1e6 and value / (nel / dt64 / 1e6))
         out["edge_cut_vs_plain_fp64"] = q["edge_cut"] / max(q64["edge_cut"], 1)
     if args.mg:

@@ -1187,6 +1187,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       if (compact(LZ_RUNNING)) return 2;
       RSB_TRY(hipEventRecord(g->lz_ev0, st));
       int it, nrec = 0;
+      const char *gv = getenv("PARRSB_AMD_LZ_GRAPH");
+      const bool use_graph = !g->prof_spmv && gv && atoi(gv) == 1;
+      hipGraphExec_t gexec = nullptr;
+      int gn[2] = {-1, -1};
       if (g->prof_spmv && (int)g->spmv_ev.size() < 2 * miter_max) {
         size_t old = g->spmv_ev.size();
         g->spmv_ev.resize(2 * miter_max);
@@ -1195,6 +1199,36 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       for (it = 0; it < miter_max; it++) {
         T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
         if (nact[0] == 0) break;
+        if (use_graph && (it & 7) == 0 && it + 8 < miter_max) {
+          // eight iterations as one graph replay (same kernels and grids as
+          // the launches below; the grids follow the compacted lists, so
+          // the chunk is captured again when they changed)
+          if (!gexec || gn[0] != nact[0] || gn[1] != nact[1]) {
+            hipGraph_t gr = nullptr;
+            RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < 8; q++) {
+              T *po = (q & 1) ? p2 : p, *pn = (q & 1) ? p : p2;
+              k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis, mpad, ta);
+              k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
+              k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
+              k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+            }
+            RSB_TRY(hipStreamEndCapture(st, &gr));
+            if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
+            gexec = nullptr;
+            RSB_TRY(hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0));
+            RSB_TRY(hipGraphDestroy(gr));
+            gn[0] = nact[0], gn[1] = nact[1];
+          }
+          RSB_TRY(hipGraphLaunch(gexec, st));
+          g->lz_launches += 8;
+          it += 7;
+          if (it + 1 < miter_max) {
+            if (compact(LZ_RUNNING)) return 2;
+            if (nact[1] == 0) break;
+          }
+          continue;
+        }
         if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
         k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta);
         if (g->prof_spmv) {
@@ -1214,6 +1248,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       }
       RSB_TRY(hipEventRecord(g->lz_ev1, st));
       RSB_TRY(hipEventSynchronize(g->lz_ev1));
+      if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
       float ms = 0.f;
       RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
       g->lz_ms += ms;

@@ -439,3 +439,24 @@ def test_part_mesh_seq_is_order_within_part(oracle):
     assert np.array_equal(part2, part3)
     for q in range(P):
         assert np.array_equal(np.sort(seq2[part2 == q]), np.arange(np.count_nonzero(part2 == q)))
+
+
+def test_lanczos_graph_chunks_equal_eager_launches(monkeypatch):
+    """PARRSB_AMD_LZ_GRAPH=1 replays eight CG-Lanczos iterations as one
+    captured graph (captured again when the running lists change); every
+    replay feeds the next and eager launches run between them (restarts,
+    Ritz, compaction), so equal partitions and iteration counts over a whole
+    call check all of them."""
+    vtx, xyz, nv = M.box_hex(24, 20, 16, jitter=0.1, seed=5)
+    pa.set_precision(32)
+    try:
+        monkeypatch.setenv("PARRSB_AMD_LZ_GRAPH", "1")
+        a, _ = pa.part_mesh(vtx, xyz, nv, 16)
+        ra = pa.last_report()
+        monkeypatch.setenv("PARRSB_AMD_LZ_GRAPH", "0")
+        b, _ = pa.part_mesh(vtx, xyz, nv, 16)
+        rb = pa.last_report()
+    finally:
+        pa.set_precision(64)
+    assert np.array_equal(a, b)
+    assert ra["niter_max"] == rb["niter_max"] and ra["lambda_min"] == rb["lambda_min"]
