@@ -121,22 +121,40 @@ __global__ void k_uf_init(int64_t mpad, int *__restrict__ par) {
   if (i < mpad) par[i] = (int)i;
 }
 
-__global__ void k_uf_edges(int64_t mpad, const int *__restrict__ row_elem,
-                           const int *__restrict__ hseg, const int64_t *__restrict__ slice_off,
-                           const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
-                           int *__restrict__ par) {
+// Half of every row (-1 for padding rows), so the edge pass gathers one
+// array instead of two in a chain.
+__global__ void k_uf_hrow(int64_t mpad, const int *__restrict__ row_elem,
+                          const int *__restrict__ hseg, int *__restrict__ hrow) {
   int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= mpad) return;
-  int e = row_elem[row];
-  if (e < 0) return;
-  int h = hseg[e];
+  const int e = row_elem[row];
+  hrow[row] = e >= 0 ? hseg[e] : -1;
+}
+
+__global__ void k_uf_edges(int64_t mpad, const int *__restrict__ hrow,
+                           const int64_t *__restrict__ slice_off, const int *__restrict__ slice_w,
+                           const uint32_t *__restrict__ ell, int *__restrict__ par) {
+  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= mpad) return;
+  const int h = hrow[row];
+  if (h < 0) return;
   int64_t sl = row >> 6;
   const uint32_t *q = ell + slice_off[sl] + (row & 63);
-  for (int j = 0; j < slice_w[sl]; j++) {
-    uint32_t a = q[(int64_t)j * RSB_WAVE];
-    int c = (int)(a & 0x0fffffffu);
-    if ((a >> 28) == 0 || c >= row) continue;
-    if (hseg[row_elem[c]] == h) uf_union(par, (int)row, c);
+  const int w = slice_w[sl];
+  // eight entries and their halves in flight, then the unions in entry order
+  for (int j0 = 0; j0 < w; j0 += 8) {
+    uint32_t a[8];
+    int hc[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = j0 + u < w ? q[(int64_t)(j0 + u) * RSB_WAVE] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = (int)(a[u] & 0x0fffffffu);
+      hc[u] = ((a[u] >> 28) != 0 && c < row) ? hrow[c] : -2;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (hc[u] == h) uf_union(par, (int)row, (int)(a[u] & 0x0fffffffu));
   }
 }
 
@@ -260,10 +278,12 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
   // components of each half before balancing (rsb.c:320-327)
   if (find_comps && g->mpad > 0) {
     rsb_phase_begin(g);
-    RSB_TRY(hipMalloc((void **)&par, sizeof(int) * g->mpad));
+    RSB_TRY(hipMalloc((void **)&par, sizeof(int) * 2 * g->mpad));
+    int *hrow = par + g->mpad;
     k_uf_init<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, par);
-    k_uf_edges<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(
-        g->mpad, g->d_row_elem, hseg, g->d_slice_off, g->d_slice_w, g->d_ell, par);
+    k_uf_hrow<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, hrow);
+    k_uf_edges<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, hrow, g->d_slice_off,
+                                                             g->d_slice_w, g->d_ell, par);
     k_uf_count<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, par,
                                                              d_ncomp);
     RSB_LAUNCH_CHECK();

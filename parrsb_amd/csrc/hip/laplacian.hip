@@ -26,6 +26,11 @@ __global__ void k_row_maps(const int *__restrict__ order, const int *__restrict_
   elem_row[e] = row;
 }
 
+// Graph entries per batch in the level kernels: all their loads are issued
+// before any is used (one dependent gather at a time left the 16.7M-row
+// levels latency-bound).
+#define LV_U 8
+
 __global__ void k_fill_i32(int *__restrict__ a, int64_t n, int v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -41,10 +46,16 @@ __global__ void k_level_rowlen(int64_t mpad, const int *__restrict__ row_elem,
   if (row < mpad) {
     int e = row_elem[row];
     if (e >= 0) {
-      int s = segof[e];
-      for (int64_t k = growptr[e]; k < growptr[e + 1]; k++) {
-        int c = (int)(gadj[k] & 0x0fffffffu);
-        len += (segof[c] == s);
+      const int s = segof[e];
+      const int64_t k1 = growptr[e + 1];
+      // LV_U graph entries, then their segments, in flight at once
+      for (int64_t k0 = growptr[e]; k0 < k1; k0 += LV_U) {
+        int c[LV_U];
+#pragma unroll
+        for (int u = 0; u < LV_U; u++)
+          c[u] = k0 + u < k1 ? (int)(gadj[k0 + u] & 0x0fffffffu) : e;
+#pragma unroll
+        for (int u = 0; u < LV_U; u++) len += (k0 + u < k1) && segof[c[u]] == s;
       }
     }
     rowlen[row] = len;
@@ -85,14 +96,25 @@ __global__ void k_level_fill(int64_t mpad, const int *__restrict__ row_elem,
   int w = slice_w[sl], j = 0;
   int e = row_elem[row];
   if (e >= 0) {
-    int s = segof[e];
-    for (int64_t k = growptr[e]; k < growptr[e + 1]; k++) {
-      uint32_t a = gadj[k];
-      int c = (int)(a & 0x0fffffffu);
-      if (segof[c] == s) {
-        out[(int64_t)j * RSB_WAVE] = (uint32_t)elem_row[c] | (a & 0xf0000000u);
-        j++;
+    const int s = segof[e];
+    const int64_t k1 = growptr[e + 1];
+    for (int64_t k0 = growptr[e]; k0 < k1; k0 += LV_U) {
+      uint32_t a[LV_U];
+      int t[LV_U], r[LV_U];
+#pragma unroll
+      for (int u = 0; u < LV_U; u++) a[u] = k0 + u < k1 ? gadj[k0 + u] : (uint32_t)e;
+#pragma unroll
+      for (int u = 0; u < LV_U; u++) {
+        const int c = (int)(a[u] & 0x0fffffffu);
+        t[u] = segof[c];
+        r[u] = elem_row[c];
       }
+#pragma unroll
+      for (int u = 0; u < LV_U; u++)
+        if (k0 + u < k1 && t[u] == s) {
+          out[(int64_t)j * RSB_WAVE] = (uint32_t)r[u] | (a[u] & 0xf0000000u);
+          j++;
+        }
     }
   }
   for (; j < w; j++) out[(int64_t)j * RSB_WAVE] = (uint32_t)row;  // weight 0

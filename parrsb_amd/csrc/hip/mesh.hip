@@ -164,18 +164,27 @@ __global__ void k_scatter_rank(const int *__restrict__ excl, const int *__restri
 __global__ void k_slot_keys(const long long *__restrict__ vtx, int64_t ns,
                             uint64_t *__restrict__ key, int *__restrict__ val,
                             unsigned long long *__restrict__ vmax) {
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // grid-stride over the slots; one atomic per workgroup (one per wave on
+  // a single address serialised ~2M atomics at 16.7M hexes: 24 ms)
+  __shared__ unsigned long long wm[RSB_BLOCK / RSB_WAVE];
   unsigned long long m = 0;
-  if (s < ns) {
-    key[s] = (uint64_t)vtx[s];
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long v = (unsigned long long)vtx[s];
+    key[s] = (uint64_t)v;
     val[s] = (int)s;
-    m = (unsigned long long)vtx[s];
+    m = v > m ? v : m;
   }
   for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
     unsigned long long t = __shfl_xor(m, o, RSB_WAVE);
     m = t > m ? t : m;
   }
-  if ((threadIdx.x & (RSB_WAVE - 1)) == 0) atomicMax(vmax, m);
+  if ((threadIdx.x & (RSB_WAVE - 1)) == 0) wm[threadIdx.x / RSB_WAVE] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x / RSB_WAVE); w++) m = wm[w] > m ? wm[w] : m;
+    atomicMax(vmax, m);
+  }
 }
 
 __global__ void k_group_flags(const uint64_t *__restrict__ key, int64_t n, int *__restrict__ flag) {
@@ -304,7 +313,8 @@ static int build_graph(rsbgpu *g) {
   RSB_TRY(hipMalloc((void **)&excl, sizeof(int) * (ns + 1)));
   RSB_TRY(hipMalloc((void **)&vmax, sizeof(unsigned long long) * 2));
   RSB_TRY(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 2, g->stream));
-  k_slot_keys<<<nblocks(ns, 256), 256, 0, g->stream>>>(g->d_vtx, ns, key, val, vmax);
+  k_slot_keys<<<std::min<int64_t>(nblocks(ns, RSB_BLOCK), 4096), RSB_BLOCK, 0, g->stream>>>(
+      g->d_vtx, ns, key, val, vmax);
   RSB_LAUNCH_CHECK();
   unsigned long long hmax = 0;
   RSB_TRY(hipMemcpyAsync(&hmax, vmax, sizeof(hmax), hipMemcpyDeviceToHost, g->stream));

@@ -26,21 +26,29 @@ __global__ void k_nb_pairs(int nel, const int64_t *__restrict__ growptr,
   int found[NB_FOUND];
   int nf = 0;
   bool many = false;
-  for (int64_t k = r0; k < r1; k++) {
-    const int t = segof[gadj[k] & 0x0fffffffu];
-    if (t == s) continue;
-    bool seen = false;
+  // eight entries and their segments in flight, then examined in order
+  for (int64_t k0 = r0; k0 < r1 && !many; k0 += 8) {
+    int tt[8];
 #pragma unroll
-    for (int j = 0; j < NB_FOUND; j++) seen |= j < nf && found[j] == t;
-    if (seen) continue;
-    if (nf < NB_FOUND) {
+    for (int u = 0; u < 8; u++) tt[u] = k0 + u < r1 ? (int)(gadj[k0 + u] & 0x0fffffffu) : e;
 #pragma unroll
-      for (int j = 0; j < NB_FOUND; j++)
-        if (j == nf) found[j] = t;
-      nf++;
-    } else {
-      many = true;
-      break;
+    for (int u = 0; u < 8; u++) tt[u] = segof[tt[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int t = tt[u];
+      if (t == s || many) continue;  // padding entries are e itself: t == s
+      bool seen = false;
+#pragma unroll
+      for (int j = 0; j < NB_FOUND; j++) seen |= j < nf && found[j] == t;
+      if (seen) continue;
+      if (nf < NB_FOUND) {
+#pragma unroll
+        for (int j = 0; j < NB_FOUND; j++)
+          if (j == nf) found[j] = t;
+        nf++;
+      } else {
+        many = true;
+      }
     }
   }
   if (nf == 0) return;
