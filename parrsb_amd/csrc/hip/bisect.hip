@@ -131,30 +131,107 @@ __global__ void k_uf_hrow(int64_t mpad, const int *__restrict__ row_elem,
   hrow[row] = e >= 0 ? hseg[e] : -1;
 }
 
-__global__ void k_uf_edges(int64_t mpad, const int *__restrict__ hrow,
-                           const int64_t *__restrict__ slice_off, const int *__restrict__ slice_w,
-                           const uint32_t *__restrict__ ell, int *__restrict__ par) {
-  int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= mpad) return;
-  const int h = hrow[row];
-  if (h < 0) return;
-  int64_t sl = row >> 6;
-  const uint32_t *q = ell + slice_off[sl] + (row & 63);
-  const int w = slice_w[sl];
-  // eight entries and their halves in flight, then the unions in entry order
-  for (int j0 = 0; j0 < w; j0 += 8) {
-    uint32_t a[8];
-    int hc[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) a[u] = j0 + u < w ? q[(int64_t)(j0 + u) * RSB_WAVE] : 0u;
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int c = (int)(a[u] & 0x0fffffffu);
-      hc[u] = ((a[u] >> 28) != 0 && c < row) ? hrow[c] : -2;
+// Union-find in two steps.  Edges inside a tile (RSB_TILE consecutive rows
+// of one segment, most edges of a level) are united in LDS by the tile's
+// workgroup, which then writes every row's parent as the tile-local root (a
+// set's root is its smallest row, so parents stay below their rows); only
+// edges that leave the tile go through the global atomics.  One edge pass
+// over the global forest took ~9 ms a level at 16.7M rows, most of it in
+// agent-scope atomic loads of the parent chains.
+__device__ inline int luf_rep(int *lp, int x) {
+  int cur = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (cur != x) {
+    const int nx = __hip_atomic_load(&lp[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (nx != cur)  // path halving: x's parent becomes its grandparent
+      __hip_atomic_store(&lp[x], nx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    x = cur;
+    cur = nx;
+  }
+  return cur;
+}
+
+__device__ inline void luf_union(int *lp, int u, int v) {
+  int ru = luf_rep(lp, u), rv = luf_rep(lp, v);
+  while (ru != rv) {
+    if (ru < rv) {
+      const int t = ru;
+      ru = rv;
+      rv = t;
     }
+    int old = ru;
+    __hip_atomic_compare_exchange_strong(&lp[ru], &old, rv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == ru) break;
+    ru = luf_rep(lp, old);
+    rv = luf_rep(lp, rv);
+  }
+}
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_uf_tile(const int *__restrict__ tile,
+                                                       const int *__restrict__ hrow,
+                                                       const int64_t *__restrict__ slice_off,
+                                                       const int *__restrict__ slice_w,
+                                                       const uint32_t *__restrict__ ell,
+                                                       int *__restrict__ par) {
+  __shared__ int lp[RSB_TILE], lh[RSB_TILE];
+  const int row0 = tile[3 * blockIdx.x + 1], row1 = tile[3 * blockIdx.x + 2];
+  const int n = row1 - row0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lp[i] = i;
+    lh[i] = hrow[row0 + i];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int h = lh[i];
+    if (h < 0) continue;
+    const int row = row0 + i;
+    const int64_t sl = row >> 6;
+    const uint32_t *q = ell + slice_off[sl] + (row & 63);
+    const int w = slice_w[sl];
+    for (int j0 = 0; j0 < w; j0 += 8) {
+      uint32_t a[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++)
-      if (hc[u] == h) uf_union(par, (int)row, (int)(a[u] & 0x0fffffffu));
+      for (int u = 0; u < 8; u++) a[u] = j0 + u < w ? q[(int64_t)(j0 + u) * RSB_WAVE] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = (int)(a[u] & 0x0fffffffu);
+        if ((a[u] >> 28) != 0 && c >= row0 && c < row && lh[c - row0] == h)
+          luf_union(lp, i, c - row0);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) par[row0 + i] = row0 + luf_rep(lp, i);
+}
+
+// The edges that leave their tile (to a lower row of the same half).
+__global__ void __launch_bounds__(RSB_BLOCK) k_uf_cross(const int *__restrict__ tile,
+                                                        const int *__restrict__ hrow,
+                                                        const int64_t *__restrict__ slice_off,
+                                                        const int *__restrict__ slice_w,
+                                                        const uint32_t *__restrict__ ell,
+                                                        int *__restrict__ par) {
+  const int row0 = tile[3 * blockIdx.x + 1], row1 = tile[3 * blockIdx.x + 2];
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int h = hrow[row];
+    if (h < 0) continue;
+    const int64_t sl = row >> 6;
+    const uint32_t *q = ell + slice_off[sl] + (row & 63);
+    const int w = slice_w[sl];
+    for (int j0 = 0; j0 < w; j0 += 8) {
+      uint32_t a[8];
+      int hc[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] = j0 + u < w ? q[(int64_t)(j0 + u) * RSB_WAVE] : 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = (int)(a[u] & 0x0fffffffu);
+        hc[u] = ((a[u] >> 28) != 0 && c < row0) ? hrow[c] : -2;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (hc[u] == h) uf_union(par, row, (int)(a[u] & 0x0fffffffu));
+    }
   }
 }
 
@@ -280,10 +357,11 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
     rsb_phase_begin(g);
     RSB_TRY(hipMalloc((void **)&par, sizeof(int) * 2 * g->mpad));
     int *hrow = par + g->mpad;
-    k_uf_init<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, par);
     k_uf_hrow<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, hrow);
-    k_uf_edges<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, hrow, g->d_slice_off,
-                                                             g->d_slice_w, g->d_ell, par);
+    k_uf_tile<<<g->ntiles, RSB_BLOCK, 0, g->stream>>>(g->d_tile, hrow, g->d_slice_off,
+                                                      g->d_slice_w, g->d_ell, par);
+    k_uf_cross<<<g->ntiles, RSB_BLOCK, 0, g->stream>>>(g->d_tile, hrow, g->d_slice_off,
+                                                       g->d_slice_w, g->d_ell, par);
     k_uf_count<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, par,
                                                              d_ncomp);
     RSB_LAUNCH_CHECK();
