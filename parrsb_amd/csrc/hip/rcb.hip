@@ -283,6 +283,9 @@ __global__ void k_nodeof_next(int nel, const int *__restrict__ first,
 // and the split after the lower ranks' share (p > 1) or at n / 2 (p == 1).
 #define RCB_TAIL 1024
 #define RCB_TAIL_T 256
+// depths where no splitting sub-node is larger than this rank by counting
+// instead of the 55-stage bitonic network over the whole node
+#define RCB_RANKMAX 64
 
 // Exclusive prefix sum of v[0..n) (n <= RCB_TAIL) in place by the block;
 // returns the total.  Each thread owns RCB_TAIL / RCB_TAIL_T consecutive
@@ -337,7 +340,7 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
   __shared__ unsigned long long s_ext[6][RCB_TAIL / 2];
   __shared__ signed char s_axis[RCB_TAIL];
   __shared__ int wsum[RCB_TAIL_T / RSB_WAVE];
-  __shared__ int nsub;
+  __shared__ int nsub, s_max;
   for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) el[i] = order[start + i], sub[i] = 0;
   if (threadIdx.x == 0) s_start[0] = 0, s_len[0] = n, s_p[0] = nd.p[k], nsub = 1;
   __syncthreads();
@@ -351,6 +354,15 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
     __syncthreads();
     const int nsplit = tail_scan(s_sc, ns, wsum);
     if (nsplit == 0) break;
+    {
+      // the widest sub-node that splits (picks the sort below)
+      int m = 0;
+      for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) m = max(m, s_len[t]);
+      for (int o = RSB_WAVE / 2; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, RSB_WAVE));
+      if (threadIdx.x == 0) s_max = 0;
+      __syncthreads();
+      if ((threadIdx.x & (RSB_WAVE - 1)) == 0) atomicMax(&s_max, m);
+    }
     for (int j = threadIdx.x; j < nsplit; j += RCB_TAIL_T)
       for (int d = 0; d < 3; d++) s_ext[2 * d][j] = ~0ull, s_ext[2 * d + 1][j] = 0ull;
     __syncthreads();
@@ -415,6 +427,36 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
       s_axis[t] = (signed char)a;
     }
     __syncthreads();
+    if (s_max <= RCB_RANKMAX) {
+      // every sub-node that splits is small: an element's place in its
+      // sub-node is the number of (dense rank, position) keys below its own
+      // there, which is the order the stable sort gives
+      for (int i = threadIdx.x; i < n; i += RCB_TAIL_T) {
+        const int a = s_axis[sub[i]];
+        const uint64_t r = a < 0 ? 0ull : (uint64_t)rank[(size_t)a * nel + el[i]];
+        key[i] = (r << 10) | (uint64_t)i;
+      }
+      __syncthreads();
+      int dst[PER], mv[PER];
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int i = threadIdx.x + q * RCB_TAIL_T;
+        dst[q] = i, mv[q] = 0;
+        if (i >= n) continue;
+        mv[q] = el[i];
+        const int t = sub[i], ln = s_len[t];
+        if (ln < 2) continue;
+        const int st = s_start[t];
+        const uint64_t ki = key[i];
+        int c = 0;
+        for (int j = st; j < st + ln; j++) c += key[j] < ki;
+        dst[q] = st + c;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < PER; q++)
+        if (threadIdx.x + q * RCB_TAIL_T < n) el[dst[q]] = mv[q];
+    } else {
     // keys: sub-node (11 bits) | dense rank (32 bits) | position (10 bits)
     for (int i = threadIdx.x; i < npow; i += RCB_TAIL_T) {
       if (i < n) {
@@ -451,6 +493,7 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
         const int i = threadIdx.x + q * RCB_TAIL_T;
         if (i < n) el[i] = mv[q];
       }
+    }
     }
     // children: 2 per sub-node that splits, 1 otherwise
     for (int t = threadIdx.x; t < ns; t += RCB_TAIL_T) s_sc[t] = s_len[t] >= 2 ? 2 : 1;
