@@ -136,6 +136,13 @@ __global__ void k_node_extent_init(int K, uint64_t *__restrict__ ext) {
   }
 }
 
+// Each workgroup covers NE_PER * 256 consecutive positions.  A workgroup
+// inside one node (large nodes) reduces in registers, shuffles and LDS and
+// issues one atomic per extent: on the first rounds every workgroup hits the
+// same node, and one atomic per 256 positions serialised ~400k atomics on
+// six addresses (0.57 ms a round at 16.7M).  Otherwise runs of equal node in
+// a wave issue one atomic each.
+#define NE_PER 8
 __global__ void __launch_bounds__(256) k_node_extent(const int *__restrict__ order,
                                                      const int *__restrict__ nodeof,
                                                      const int *__restrict__ frozen,
@@ -144,59 +151,77 @@ __global__ void __launch_bounds__(256) k_node_extent(const int *__restrict__ ord
                                                      unsigned long long *__restrict__ ext) {
   __shared__ uint64_t smin[4][3], smax[4][3];
   __shared__ int block_node;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool act = i < nel;
-  int node = act ? nodeof[i] : -1;
-  if (act && frozen[node]) act = false;
-  int e = act ? order[i] : 0;
-  // The whole block inside one node (large nodes): reduce in the block and
-  // issue one atomic per extent.
+  const int64_t base = (int64_t)blockIdx.x * blockDim.x * NE_PER;
   if (threadIdx.x == 0) {
-    int last = min((int)(blockIdx.x * blockDim.x + blockDim.x), nel) - 1;
-    block_node = (act && nodeof[last] == node) ? node : -1;
+    const int64_t last = min(base + (int64_t)blockDim.x * NE_PER, (int64_t)nel) - 1;
+    const int nf = nodeof[base], nl = nodeof[last];
+    block_node = (nf == nl && !frozen[nf]) ? nf : -1;
   }
   __syncthreads();
   const int lane = threadIdx.x & (RSB_WAVE - 1), wid = threadIdx.x / RSB_WAVE;
-  const bool whole = block_node >= 0;
-  for (int d = 0; d < ndim; d++) {
-    uint64_t kv = act ? dkey(coord[(size_t)d * nel + e]) : 0;
-    uint64_t mn = act ? kv : ~0ull, mx = act ? kv : 0ull;
-    if (whole) {
+  const int bn = block_node;
+  if (bn >= 0) {
+    // every position of the workgroup lies in node bn (nodes are ranges)
+    int e[NE_PER];
+#pragma unroll
+    for (int u = 0; u < NE_PER; u++) {
+      const int64_t i = base + threadIdx.x + (int64_t)u * blockDim.x;
+      e[u] = i < nel ? order[i] : -1;
+    }
+    for (int d = 0; d < ndim; d++) {
+      uint64_t mn = ~0ull, mx = 0ull;
+#pragma unroll
+      for (int u = 0; u < NE_PER; u++)
+        if (e[u] >= 0) {
+          const uint64_t kv = dkey(coord[(size_t)d * nel + e[u]]);
+          mn = kv < mn ? kv : mn;
+          mx = kv > mx ? kv : mx;
+        }
       for (int o = RSB_WAVE / 2; o > 0; o >>= 1) {
         uint64_t a = __shfl_xor(mn, o, RSB_WAVE), b = __shfl_xor(mx, o, RSB_WAVE);
         mn = a < mn ? a : mn;
         mx = b > mx ? b : mx;
       }
       if (lane == 0) smin[wid][d] = mn, smax[wid][d] = mx;
-      continue;
     }
-    // runs of equal node within the wave, then one atomic per run
-    for (int o = 1; o < RSB_WAVE; o <<= 1) {
-      int nb = __shfl_down(node, o, RSB_WAVE);
-      uint64_t a = __shfl_down(mn, o, RSB_WAVE), b = __shfl_down(mx, o, RSB_WAVE);
-      if (lane + o < RSB_WAVE && nb == node) {
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
+    __syncthreads();
+    if (threadIdx.x < ndim) {
+      const int d = threadIdx.x;
+      uint64_t mn = smin[0][d], mx = smax[0][d];
+      for (int w = 1; w < (int)blockDim.x / RSB_WAVE; w++) {
+        mn = smin[w][d] < mn ? smin[w][d] : mn;
+        mx = smax[w][d] > mx ? smax[w][d] : mx;
+      }
+      atomicMin(&ext[(size_t)(2 * d) * K + bn], (unsigned long long)mn);
+      atomicMax(&ext[(size_t)(2 * d + 1) * K + bn], (unsigned long long)mx);
+    }
+    return;
+  }
+  for (int u = 0; u < NE_PER; u++) {
+    const int64_t i = base + threadIdx.x + (int64_t)u * blockDim.x;
+    bool act = i < nel;
+    const int node = act ? nodeof[i] : -1;
+    if (act && frozen[node]) act = false;
+    const int e = act ? order[i] : 0;
+    for (int d = 0; d < ndim; d++) {
+      const uint64_t kv = act ? dkey(coord[(size_t)d * nel + e]) : 0;
+      uint64_t mn = act ? kv : ~0ull, mx = act ? kv : 0ull;
+      // runs of equal node within the wave, then one atomic per run
+      for (int o = 1; o < RSB_WAVE; o <<= 1) {
+        int nb = __shfl_down(node, o, RSB_WAVE);
+        uint64_t a = __shfl_down(mn, o, RSB_WAVE), b = __shfl_down(mx, o, RSB_WAVE);
+        if (lane + o < RSB_WAVE && nb == node) {
+          mn = a < mn ? a : mn;
+          mx = b > mx ? b : mx;
+        }
+      }
+      int prev = __shfl_up(node, 1, RSB_WAVE);
+      bool head = act && (lane == 0 || prev != node);
+      if (head) {
+        atomicMin(&ext[(size_t)(2 * d) * K + node], (unsigned long long)mn);
+        atomicMax(&ext[(size_t)(2 * d + 1) * K + node], (unsigned long long)mx);
       }
     }
-    int prev = __shfl_up(node, 1, RSB_WAVE);
-    bool head = act && (lane == 0 || prev != node);
-    if (head) {
-      atomicMin(&ext[(size_t)(2 * d) * K + node], (unsigned long long)mn);
-      atomicMax(&ext[(size_t)(2 * d + 1) * K + node], (unsigned long long)mx);
-    }
-  }
-  if (!whole) return;
-  __syncthreads();
-  if (threadIdx.x < ndim) {
-    int d = threadIdx.x;
-    uint64_t mn = smin[0][d], mx = smax[0][d];
-    for (int w = 1; w < (int)blockDim.x / RSB_WAVE; w++) {
-      mn = smin[w][d] < mn ? smin[w][d] : mn;
-      mx = smax[w][d] > mx ? smax[w][d] : mx;
-    }
-    atomicMin(&ext[(size_t)(2 * d) * K + block_node], (unsigned long long)mn);
-    atomicMax(&ext[(size_t)(2 * d + 1) * K + block_node], (unsigned long long)mx);
   }
 }
 
@@ -284,7 +309,8 @@ __global__ void k_nodeof_next(int nel, const int *__restrict__ first,
 #define RCB_TAIL 1024
 #define RCB_TAIL_T 256
 // depths where no splitting sub-node is larger than this rank by counting
-// instead of the 55-stage bitonic network over the whole node
+// instead of the 55-stage bitonic network over the whole node (environment
+// PARRSB_AMD_RCB_RANKMAX overrides; 0 always sorts)
 #define RCB_RANKMAX 64
 
 // Exclusive prefix sum of v[0..n) (n <= RCB_TAIL) in place by the block;
@@ -327,7 +353,7 @@ __device__ inline int tail_scan(int *v, int n, int *wsum) {
 __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int *__restrict__ order,
                                                         const double *__restrict__ coord,
                                                         const uint32_t *__restrict__ rank,
-                                                        int nel, int ndim) {
+                                                        int nel, int ndim, int rankmax) {
   constexpr int PER = RCB_TAIL / RCB_TAIL_T;
   const int k = blockIdx.x;
   if (k >= K || nd.frozen[k]) return;
@@ -427,7 +453,7 @@ __global__ void __launch_bounds__(RCB_TAIL_T) k_rcb_tail(nodes_t nd, int K, int 
       s_axis[t] = (signed char)a;
     }
     __syncthreads();
-    if (s_max <= RCB_RANKMAX) {
+    if (s_max <= rankmax) {
       // every sub-node that splits is small: an element's place in its
       // sub-node is the number of (dense rank, position) keys below its own
       // there, which is the order the stable sort gives
@@ -591,14 +617,15 @@ static int run_rcb(rsbgpu *g) {
   RSB_LAUNCH_CHECK();
   for (int r = 0; r < rounds; r++) {
     if (tail_ok && maxlen <= RCB_TAIL && !pbig) {
+      const char *rm = getenv("PARRSB_AMD_RCB_RANKMAX");
       k_rcb_tail<<<K, RCB_TAIL_T, 0, g->stream>>>(A, K, g->d_order, g->d_coord, g->d_rank, nel,
-                                                  g->ndim);
+                                                  g->ndim, rm ? atoi(rm) : RCB_RANKMAX);
       RSB_LAUNCH_CHECK();
       break;
     }
     k_node_extent_init<<<nblocks(K, 256), 256, 0, g->stream>>>(K, ext);
     RSB_LAUNCH_CHECK();
-    k_node_extent<<<nblocks(nel, 256), 256, 0, g->stream>>>(
+    k_node_extent<<<nblocks(nel, 256 * NE_PER), 256, 0, g->stream>>>(
         g->d_order, nodeof, A.frozen, g->d_coord, nel, g->ndim, K,
         (unsigned long long *)ext);
     RSB_LAUNCH_CHECK();
