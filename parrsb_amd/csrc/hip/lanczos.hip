@@ -83,8 +83,22 @@ struct acc_t<float> {
 struct tile_t {
   int ti, s, row0, row1, pstart, n, first, count;
 };
-__device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t) {
-  const int ti = a.act_tiles ? a.act_tiles[blockIdx.x] : blockIdx.x;
+// Workgroups are handed to the XCDs round robin (workgroup b to XCD b % 8),
+// so consecutive tiles, which are neighbours in space, would land in eight
+// different L2 caches.  With nlist > 0 the list of nlist tiles is split in
+// eight contiguous parts, one per XCD, so a tile's halo rows are mostly rows
+// its XCD has just read (grid: nlist rounded up to a multiple of 8).
+#define LZ_XCDS 8
+__device__ inline int list_index(int nlist) {
+  if (nlist <= 0) return blockIdx.x;
+  const int per = (int)(gridDim.x / LZ_XCDS);
+  const int i = (int)(blockIdx.x % LZ_XCDS) * per + (int)(blockIdx.x / LZ_XCDS);
+  return i < nlist ? i : -1;
+}
+__device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t, int nlist = 0) {
+  const int li = list_index(nlist);
+  if (li < 0) return false;
+  const int ti = a.act_tiles ? a.act_tiles[li] : li;
   t.ti = ti;
   t.s = a.tile[3 * ti];
   if (a.status[t.s] != want_status) return false;
@@ -432,11 +446,11 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
                                                         const T *__restrict__ pold,
                                                         T *__restrict__ pnew, T *__restrict__ w,
                                                         T *__restrict__ basis, int64_t mpad,
-                                                        tiled_args ta) {
+                                                        tiled_args ta, int nlist) {
   __shared__ T ql[TB_LOCAL];
   __shared__ T rl[RSB_TILE];
   tile_t t;
-  if (!tile_of(a, LZ_RUNNING, t)) return;
+  if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
   const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
   acc_t<T> sq, swq, sw;
@@ -1148,6 +1162,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     return 0;
   };
   lz_args ai = a;  // iteration launches over the running lists
+  auto xgrid = [](int n) { return (unsigned)((n + LZ_XCDS - 1) / LZ_XCDS * LZ_XCDS); };
   ai.act_tiles = act_t, ai.act_segs = act_s;
   tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell, g->d_tnhalo, g->d_thalo_off,
                 g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
@@ -1208,7 +1223,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
             RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (int q = 0; q < 8; q++) {
               T *po = (q & 1) ? p2 : p, *pn = (q & 1) ? p : p2;
-              k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis, mpad, ta);
+              k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis, mpad,
+                                                                  ta, nact[0]);
               k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
               k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
               k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
@@ -1230,7 +1246,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
           continue;
         }
         if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-        k_lz_apply<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta);
+        k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta,
+                                                            nact[0]);
         if (g->prof_spmv) {
           RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
           nrec++;
