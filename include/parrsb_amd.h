@@ -28,8 +28,9 @@ typedef struct {
   double lanczos_ms;          /* time of the iteration loops */
   double lanczos_iters;       /* iterations launched (3 kernels each) */
   double spmv_bytes;          /* algorithmic bytes of the SpMV kernel */
-  double spmv_ms;             /* time inside SpMV launches (profiling on) */
-  double spmv_launches;
+  double spmv_ms;             /* time inside SpMV launches (profiling on: every
+                                 11th launch is timed, the mean times all) */
+  double spmv_launches;       /* SpMV launches while profiling */
   int64_t graph_nnz, graph_wsum; /* dual graph: off-diagonals, sum of weights */
   int nsegs[PARRSB_AMD_MAX_LEVELS];      /* segments bisected at the level */
   int niter_min[PARRSB_AMD_MAX_LEVELS];  /* RSB_FIEDLER_CALC_NITER */

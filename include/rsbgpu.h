@@ -49,9 +49,11 @@ int rsbgpu_phase_times(rsbgpu *g, double *t8);
  *   v6[0] algorithmic bytes of the CG-Lanczos iterations, v6[1] ms of the
  *   iteration loops, v6[2] iterations launched;
  *   v6[3] algorithmic bytes of the SpMV kernel alone, v6[4] ms inside SpMV
- *   launches (only with profiling on), v6[5] SpMV launches timed. */
+ *   launches (only with profiling on: the mean of the timed launches times
+ *   v6[5]), v6[5] SpMV launches while profiling. */
 int rsbgpu_lanczos_traffic(rsbgpu *g, double *v6);
-/* 1 = bracket every SpMV launch with events (for the roofline figure). */
+/* 1 = bracket every 11th SpMV launch with events (for the roofline figure;
+ * timing every launch costs ~5% of the call). */
 int rsbgpu_set_profiling(rsbgpu *g, int spmv_events);
 /* Parity hook: tridiagonal eigen-solve of one (d, e) on the device; y gets
  * the coefficients of the eigenvector of smallest |eigenvalue|. */

@@ -89,6 +89,9 @@ struct tile_t {
 // eight contiguous parts, one per XCD, so a tile's halo rows are mostly rows
 // its XCD has just read (grid: nlist rounded up to a multiple of 8).
 #define LZ_XCDS 8
+// profiling samples one SpMV launch in this many (coprime with the 8
+// iterations between list compactions and the 50 of a pass)
+#define LZ_PROF_EVERY 11
 __device__ inline int list_index(int nlist) {
   if (nlist <= 0) return blockIdx.x;
   const int per = (int)(gridDim.x / LZ_XCDS);
@@ -1245,10 +1248,14 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
           }
           continue;
         }
-        if (g->prof_spmv) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
+        // profiling: every LZ_PROF_EVERY-th launch is bracketed by events (each
+        // pair of records costs several microseconds of stream time, so
+        // timing every launch slowed the whole call by ~5%)
+        const bool samp = g->prof_spmv && (g->spmv_total++ % LZ_PROF_EVERY) == 0;
+        if (samp) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
         k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta,
                                                             nact[0]);
-        if (g->prof_spmv) {
+        if (samp) {
           RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
           nrec++;
         }

@@ -24,7 +24,11 @@ extern "C" int rsbgpu_phase_times(rsbgpu *g, double *t) {
 
 extern "C" int rsbgpu_lanczos_traffic(rsbgpu *g, double *v6) {
   v6[0] = g->lz_bytes, v6[1] = g->lz_ms, v6[2] = (double)g->lz_launches;
-  v6[3] = g->spmv_bytes, v6[4] = g->spmv_ms, v6[5] = (double)g->spmv_launches;
+  // the timed launches are a sample (every LZ_PROF_EVERY-th): their mean
+  // duration, times all launches
+  v6[3] = g->spmv_bytes;
+  v6[4] = g->spmv_launches ? g->spmv_ms * (double)g->spmv_total / (double)g->spmv_launches : 0.0;
+  v6[5] = (double)g->spmv_total;
   return 0;
 }
 

@@ -121,8 +121,9 @@ struct rsbgpu {
   double lz_bytes = 0, lz_ms = 0;      // iteration kernels: model bytes, loop time
   long long lz_launches = 0;           // iterations launched (3 kernels each)
   double spmv_bytes = 0, spmv_ms = 0;  // SpMV kernel alone: model bytes, kernel time
-  long long spmv_launches = 0;
-  int prof_spmv = 0;                   // bracket every SpMV launch with events
+  long long spmv_launches = 0;         // launches timed
+  long long spmv_total = 0;            // launches while profiling (timed or not)
+  int prof_spmv = 0;                   // bracket every LZ_PROF_EVERY-th SpMV launch with events
   std::vector<hipEvent_t> spmv_ev;
   hipEvent_t lz_ev0 = nullptr, lz_ev1 = nullptr;
 };
