@@ -295,6 +295,22 @@ def main():
                              "level_ms": r64["level_ms"]}
         out["speedup_vs_plain_fp64"] = (nel / dt64 / 1e6 and value / (nel / dt64 / 1e6))
         out["edge_cut_vs_plain_fp64"] = q["edge_cut"] / max(q64["edge_cut"], 1)
+    if not args.no_compare:
+        # parrsb_part_mesh on host arrays (the library copies vtx and xyz in and
+        # part out over PCIe): reported beside `value`, never as it
+        xyz_h = d_xyz.cpu().numpy()
+        pa.set_precision(args.precision)
+        pa.set_warm_start(warm, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pa.part_mesh(vtx, xyz_h, nv, P)
+        dth = time.perf_counter() - t0
+        pa.set_warm_start(None)
+        del xyz_h
+        log(f"host arrays (PCIe included): {dth:.3f} s")
+        out["host_arrays"] = {"value": nel / dth / 1e6, "ms": dth * 1e3,
+                              "what": "parrsb_part_mesh with host vtx, xyz and part (copies "
+                                      "over PCIe included), same solver as `value`, one call"}
     if args.mg:
         pa.set_precision(64)
         torch.cuda.synchronize()
