@@ -92,13 +92,24 @@ struct mg_sell {
   const int *rowlen;
   const double *diag;
 };
+// Eight entries and the eight values they point at are loaded before any is
+// used (one entry, then its gather, at a time left the smoother
+// latency-bound); the sum still runs over j in order.
 __device__ inline double sell_offsum(const mg_sell &L, int row, const double *x) {
   const int64_t base = L.slice_off[row / RSB_WAVE] + (row % RSB_WAVE);
   const int len = L.rowlen[row];
   double s = 0.0;
-  for (int j = 0; j < len; j++) {
-    const uint32_t e = L.ell[base + (int64_t)j * RSB_WAVE];
-    s += (double)(e >> 28) * x[e & 0x0fffffffu];
+  for (int j0 = 0; j0 < len; j0 += 8) {
+    uint32_t e[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      e[u] = j0 + u < len ? L.ell[base + (int64_t)(j0 + u) * RSB_WAVE] : (uint32_t)row;
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[e[u] & 0x0fffffffu];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < len) s += (double)(e[u] >> 28) * v[u];
   }
   return s;
 }
@@ -470,9 +481,22 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_res_c(mg_args a, mg_csr M, dou
   double as = 0.0;
   const int64_t base = M.soff[i / RSB_WAVE] + (i % RSB_WAVE);
   const int len = M.rlen[i];
-  for (int j = 0; j < len; j++) {
-    const int64_t k = base + (int64_t)j * RSB_WAVE;
-    as += (double)M.sw[k] * sv[M.sci[k]];
+  // eight entries and their gathers in flight, summed in order
+  for (int j0 = 0; j0 < len; j0 += 8) {
+    int c[8], wt[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int64_t k = base + (int64_t)(j0 + u) * RSB_WAVE;
+      const bool in = j0 + u < len;
+      c[u] = in ? M.sci[k] : (int)i;
+      wt[u] = in ? M.sw[k] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = sv[c[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (j0 + u < len) as += (double)wt[u] * v[u];
   }
   const double d = M.diag[i];
   const double rv = r[i] - (d * sv[i] - as);
