@@ -251,19 +251,32 @@ __global__ void k_lws_agg(int64_t mpad, const int64_t *__restrict__ slice_off,
   const uint32_t *e = ell + slice_off[sl] + (row & 63);
   float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   float dsum = 0.f;
-  for (int j = 0; j < slice_w[sl]; j++) {
-    const uint32_t q = e[(size_t)j * RSB_WAVE];
-    const float w = (float)(q >> 28);
-    if (w == 0.f) continue;
-    const uint32_t c = q & 0x0fffffffu;
-    const float sc = pre ? w * dinv[c] : w;
-    const uint4 v = *(const uint4 *)(in + (size_t)c * ldin + cg);
-    const unsigned vv[4] = {v.x, v.y, v.z, v.w};
-    for (int k = 0; k < 4; k++) {
-      acc[2 * k] += sc * bf2f((unsigned short)(vv[k] & 0xffff));
-      acc[2 * k + 1] += sc * bf2f((unsigned short)(vv[k] >> 16));
+  const int wd = slice_w[sl];
+  // four entries and their 16-byte gathers in flight, then summed in order
+  for (int j0 = 0; j0 < wd; j0 += 4) {
+    uint32_t q[4];
+    uint4 v[4];
+    float dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) q[u] = j0 + u < wd ? e[(size_t)(j0 + u) * RSB_WAVE] : 0u;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t c = (q[u] >> 28) ? (q[u] & 0x0fffffffu) : (uint32_t)row;
+      v[u] = *(const uint4 *)(in + (size_t)c * ldin + cg);
+      dv[u] = pre ? dinv[c] : 1.f;
     }
-    dsum += w;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const float w = (float)(q[u] >> 28);
+      if (w == 0.f) continue;
+      const float sc = pre ? w * dv[u] : w;
+      const unsigned vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+      for (int k = 0; k < 4; k++) {
+        acc[2 * k] += sc * bf2f((unsigned short)(vv[k] & 0xffff));
+        acc[2 * k + 1] += sc * bf2f((unsigned short)(vv[k] >> 16));
+      }
+      dsum += w;
+    }
   }
   const float s = norm ? (dsum > 0.f ? 1.f / dsum : 0.f) : 1.f;
   for (int k = 0; k < 8; k++) {
