@@ -65,12 +65,16 @@ def test_rcb_order_bit_exact(oracle, name, p):
     assert np.array_equal(got, ref)
 
 
-def test_rcb_tail_kernel_equals_global_rounds(oracle, monkeypatch):
+@pytest.mark.parametrize("rankmax", ["0", "64", "1024"])
+def test_rcb_tail_kernel_equals_global_rounds(oracle, monkeypatch, rankmax):
     """Nodes of <= 1024 elements are finished by one LDS kernel; the global
     rounds alone (PARRSB_AMD_RCB_TAIL=0) and the oracle give the same orders,
     also next to a segment with more ranks than elements (which keeps the
-    global rounds)."""
-    flush("rcb tail")
+    global rounds).  rankmax: the sub-node size up to which the kernel ranks
+    by counting instead of sorting (0: always the bitonic sort, 1024: always
+    counting)."""
+    flush(f"rcb tail, rankmax {rankmax}")
+    monkeypatch.setenv("PARRSB_AMD_RCB_RANKMAX", rankmax)
     vtx, xyz, nv = M.box_hex(36, 30, 24, jitter=0.1, seed=12)
     n = vtx.shape[0]
     c = oracle.centroids(xyz, nv)
@@ -285,6 +289,40 @@ def test_bisect_components_and_balance(oracle):
     assert ncomp[1] == oracle.components(vtx, nv, order[nl:])
     if sc and moved[0]:
         assert nl2[0] == e0
+
+
+def test_components_of_many_blocks_match_oracle(oracle):
+    """Union-find inside tiles (LDS) and across them: six segments of a grid
+    of 8 x 8 quad blocks separated by removed rows and columns (no shared
+    vertices across a gap), so each half holds many components, some split
+    between tiles; every count equals the oracle's on the same half."""
+    flush("components of many blocks")
+    vtx, xyz, nv = M.quad_grid(134, 107)
+    j, i = np.divmod(np.arange(vtx.shape[0]), 134)
+    keep = (i % 9 != 8) & (j % 9 != 8)
+    vtx, xyz = np.ascontiguousarray(vtx[keep]), np.ascontiguousarray(xyz[keep])
+    n = vtx.shape[0]
+    sizes = [n // 6] * 5 + [n - 5 * (n // 6)]
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    ps = [2, 3, 2, 5, 2, 4]
+    r0 = np.concatenate([[0], np.cumsum(ps)[:-1]]).astype(np.int32)
+    with pa.Device() as d:
+        d.set_mesh(vtx, xyz, nv)
+        d.set_segments(starts, sizes, ps, r0)
+        d.prepartition(1)
+        d.level_laplacian()
+        d.fiedler(6, precision=32)
+        d.fiedler_sort()
+        order = d.order()
+        nl = [oracle.lower_count(sz, p) for sz, p in zip(sizes, ps)]
+        _, ncomp, _ = d.bisect(nl, [-1] * 6, [0] * 6, 1)
+    total = 0
+    for s in range(6):
+        a, b, c = starts[s], starts[s] + nl[s], starts[s] + sizes[s]
+        want = (oracle.components(vtx, nv, order[a:b]), oracle.components(vtx, nv, order[b:c]))
+        assert (ncomp[2 * s], ncomp[2 * s + 1]) == want, s
+        total += sum(want)
+    assert total >= 100
 
 
 def test_full_partition_16cube_matches_oracle(oracle):
