@@ -462,6 +462,7 @@ struct mg_csr {
   const int64_t *soff;  // SELL-64 slices
   const int *rlen, *sci, *sw;
   const double *diag;
+  const uint32_t *spk;  // if set: packed entries col | weight << 24 (sci, sw unused)
 };
 
 __global__ void __launch_bounds__(RSB_BLOCK) k_mg_jac_c(mg_args a, mg_csr M, double sigma,
@@ -489,8 +490,13 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_res_c(mg_args a, mg_csr M, dou
     for (int u = 0; u < 8; u++) {
       const int64_t k = base + (int64_t)(j0 + u) * RSB_WAVE;
       const bool in = j0 + u < len;
-      c[u] = in ? M.sci[k] : (int)i;
-      wt[u] = in ? M.sw[k] : 0;
+      if (M.spk) {
+        const uint32_t e = in ? M.spk[k] : (uint32_t)i;
+        c[u] = (int)(e & 0xffffffu), wt[u] = (int)(e >> 24);
+      } else {
+        c[u] = in ? M.sci[k] : (int)i;
+        wt[u] = in ? M.sw[k] : 0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) v[u] = sv[c[u]];
@@ -677,8 +683,11 @@ __global__ void k_mg_slice_len(int64_t m, int64_t nsl, const int64_t *rp, int *r
   if ((threadIdx.x & (RSB_WAVE - 1)) == 0) ents[i / RSB_WAVE] = (int64_t)mx * RSB_WAVE;
 }
 
+// spk != nullptr: one packed word per entry (col | weight << 24; the caller
+// has checked m <= 2^24 and every weight < 256), else two int arrays
 __global__ void k_mg_sell_fill(int64_t m, int64_t nsl, const int64_t *rp, const int *ci,
-                               const double *w, const int64_t *soff, int *sci, int *sw) {
+                               const double *w, const int64_t *soff, int *sci, int *sw,
+                               uint32_t *spk) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nsl * RSB_WAVE) return;
   const int64_t sl = i / RSB_WAVE, lane = i % RSB_WAVE;
@@ -686,10 +695,27 @@ __global__ void k_mg_sell_fill(int64_t m, int64_t nsl, const int64_t *rp, const 
   const int64_t base = soff[sl] + lane;
   int j = 0;
   if (i < m)
-    for (int64_t k = rp[i]; k < rp[i + 1]; k++, j++)
-      sci[base + (int64_t)j * RSB_WAVE] = ci[k], sw[base + (int64_t)j * RSB_WAVE] = (int)w[k];
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++, j++) {
+      const int64_t o = base + (int64_t)j * RSB_WAVE;
+      if (spk) spk[o] = (uint32_t)ci[k] | ((uint32_t)(int)w[k] << 24);
+      else sci[o] = ci[k], sw[o] = (int)w[k];
+    }
   const int self = i < m ? (int)i : 0;
-  for (; j < width; j++) sci[base + (int64_t)j * RSB_WAVE] = self, sw[base + (int64_t)j * RSB_WAVE] = 0;
+  for (; j < width; j++) {
+    const int64_t o = base + (int64_t)j * RSB_WAVE;
+    if (spk) spk[o] = (uint32_t)self;
+    else sci[o] = self, sw[o] = 0;
+  }
+}
+
+// largest weight of a coarse matrix (non-negative integers)
+__global__ void k_mg_wmax(int64_t nnz, const double *w, int *out) {
+  int mx = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * blockDim.x)
+    mx = max(mx, w[k] < 2147483647.0 ? (int)w[k] : 2147483647);
+  for (int o = RSB_WAVE / 2; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, RSB_WAVE));
+  if ((threadIdx.x & (RSB_WAVE - 1)) == 0) atomicMax(out, mx);
 }
 
 __global__ void k_mg_rowdiag(int64_t m, const int64_t *rp, const double *w, double *diag) {
@@ -716,6 +742,7 @@ struct mg_level_dev {
   // weight-0 entries to the slice's longest row; weights are integers
   int64_t *soff = nullptr;
   int *rlen = nullptr, *sci = nullptr, *sw = nullptr;
+  uint32_t *spk = nullptr;  // packed col | weight << 24 where it fits (half the bytes)
   double *r0 = nullptr, *sa = nullptr, *sb = nullptr, *u = nullptr;  // V-cycle vectors
 };
 
@@ -726,7 +753,7 @@ struct mg_hier {
     for (auto &l : lv) {
       hipFree(l.off), hipFree(l.n), hipFree(l.rseg), hipFree(l.rp), hipFree(l.ci);
       hipFree(l.w), hipFree(l.diag), hipFree(l.r0), hipFree(l.sa), hipFree(l.sb), hipFree(l.u);
-      hipFree(l.soff), hipFree(l.rlen), hipFree(l.sci), hipFree(l.sw);
+      hipFree(l.soff), hipFree(l.rlen), hipFree(l.sci), hipFree(l.sw), hipFree(l.spk);
     }
   }
 };
@@ -869,10 +896,26 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       int64_t ns = 0;
       RSB_TRY(hipMemcpyAsync(&ns, C.soff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
       RSB_TRY(hipStreamSynchronize(st));
-      RSB_TRY(hipMalloc((void **)&C.sci, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
-      RSB_TRY(hipMalloc((void **)&C.sw, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+      // one packed word per entry where the row index fits in 24 bits and
+      // the weights in 8 (the large coarse levels), two ints otherwise
+      int *dmax = nullptr, hmax = 0;
+      RSB_TRY(hipMalloc((void **)&dmax, sizeof(int)));
+      RSB_TRY(hipMemsetAsync(dmax, 0, sizeof(int), st));
+      if (nnz > 0) {
+        k_mg_wmax<<<std::min<int64_t>(nblocks(nnz, 256), 2048), 256, 0, st>>>(nnz, C.w, dmax);
+        RSB_LAUNCH_CHECK();
+      }
+      RSB_TRY(hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, st));
+      RSB_TRY(hipStreamSynchronize(st));
+      hipFree(dmax);
+      if (mc <= (1 << 24) && hmax < 256) {
+        RSB_TRY(hipMalloc((void **)&C.spk, sizeof(uint32_t) * (size_t)std::max<int64_t>(ns, 1)));
+      } else {
+        RSB_TRY(hipMalloc((void **)&C.sci, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+        RSB_TRY(hipMalloc((void **)&C.sw, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+      }
       k_mg_sell_fill<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.ci, C.w,
-                                                                    C.soff, C.sci, C.sw);
+                                                                    C.soff, C.sci, C.sw, C.spk);
       RSB_LAUNCH_CHECK();
       RSB_TRY(hipStreamSynchronize(st));
       hipFree(sbytes);
@@ -993,7 +1036,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
         k_mg_res0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, F.r0, F.r0, F.sb, sig[2], F.sa, F.u);
         k_mg_res0<<<ntiles, RSB_BLOCK, 0, st>>>(a, S, F.r0, F.r0, F.sa, 0.0, nullptr, F.u);
       } else {
-        const mg_csr M{F.m, F.rseg, F.soff, F.rlen, F.sci, F.sw, F.diag};
+        const mg_csr M{F.m, F.rseg, F.soff, F.rlen, F.sci, F.sw, F.diag, F.spk};
         const unsigned nb = nblocks(F.m, RSB_BLOCK);
         k_mg_jac_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, sig[0], F.r0, F.sa, F.u);
         k_mg_res_c<<<nb, RSB_BLOCK, 0, st>>>(a, M, F.r0, F.sa, sig[1], F.sb, F.u);
