@@ -45,6 +45,7 @@ struct mg_args {
   int *active;           // count written by finishing kernels
   double tol;
   int miter, mpass;
+  int ntiles;            // partial is slot-major: slot k of tile t at k * ntiles + t
 };
 
 #define MSC(s, k) a.sc[(size_t)(s) * MS_NSC + (k)]
@@ -81,7 +82,7 @@ __device__ inline void mg_publish(const mg_args &a, const mg_tile &t, double (&v
   __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   block_sum<NV>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < NV; k++) a.partial[(size_t)t.ti * MG_NP + k] = v[k];
+    for (int k = 0; k < NV; k++) a.partial[(size_t)k * a.ntiles + t.ti] = v[k];
 }
 
 // Level-0 Laplacian row: sum_j w_ij x_j over the SELL-64 entries; the row
@@ -115,22 +116,55 @@ __device__ inline double sell_offsum(const mg_sell &L, int row, const double *x)
 }
 
 // ---------------------------------------------------------------------------
-// Segment total of partial slot k: lane j sums tiles j, j + 64, ..., then a
-// fixed shuffle tree (the same order every time).
-__device__ inline double seg_total(const mg_args &a, int first, int count, int k) {
-  double v = 0.0;
-  for (int t = threadIdx.x; t < count; t += RSB_WAVE) v += a.partial[(size_t)(first + t) * MG_NP + k];
-  return wave_sum(v);
+// Totals of partial slots k0 .. k0 + ns - 1 (ns <= 4) of a segment, formed
+// by the whole workgroup: thread t sums tiles t, t + blockDim, ... in that
+// order (four tiles of every slot in flight; a slot's partials are
+// contiguous), then wave shuffles and the waves in order.  Every thread
+// returns the same totals.  (One wave walking the tile-major records of
+// 16k tiles took ~0.1 ms per slot on a 16.7M-row segment.)
+__device__ inline void seg_totals(const mg_args &a, int first, int count, int k0, int ns,
+                                  double (&tot)[4], double *red) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t0 = threadIdx.x; t0 < count; t0 += 4 * (int)blockDim.x) {
+    double x[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int t = t0 + u * (int)blockDim.x;
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        x[u][k] = (k < ns && t < count) ? a.partial[(size_t)(k0 + k) * a.ntiles + first + t] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) v[k] += x[u][k];
+  }
+  const int lane = threadIdx.x & (RSB_WAVE - 1), wid = threadIdx.x / RSB_WAVE;
+  const int nw = blockDim.x / RSB_WAVE;
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = wave_sum(v[k]);
+  __syncthreads();  // red may still be read from the previous call
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[wid * 4 + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double t = red[k];
+    for (int w = 1; w < nw; w++) t += red[w * 4 + k];
+    tot[k] = t;
+  }
 }
 
-// Finishing kernel: one wave per segment forms the totals of the tile
+// Finishing kernel: one workgroup per segment forms the totals of the tile
 // partials and applies the scalar step.
-__global__ void __launch_bounds__(RSB_WAVE) k_mg_fin(mg_args a, int op, int gate, int nslots) {
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_fin(mg_args a, int op, int gate, int nslots) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   const int s = blockIdx.x;
   if (!mg_gate(a, s, gate)) return;
   double tot[4];
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
-  for (int k = 0; k < nslots; k++) tot[k] = seg_total(a, first, count, k);
+  seg_totals(a, first, count, 0, nslots, tot, red);
   if (threadIdx.x != 0) return;
   const double n = (double)a.seg[a.nseg + s];
   switch (op) {
@@ -190,14 +224,18 @@ __global__ void __launch_bounds__(RSB_WAVE) k_mg_fin(mg_args a, int op, int gate
 
 // Totals of the re-orthogonalisation dots W[j].p (j <= i) into the
 // per-segment coefficient array read by k_mg_pcorr; the iteration ends here.
-__global__ void __launch_bounds__(RSB_WAVE) k_mg_coef(mg_args a, double *coef) {
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_coef(mg_args a, double *coef) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   const int s = blockIdx.x;
   if (!mg_gate(a, s, GATE_PCG)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
   const int nc = MST(s, MI_IT) + 1;
-  for (int k = 0; k < nc; k++) {
-    const double v = seg_total(a, first, count, k);
-    if (threadIdx.x == 0) coef[(size_t)s * MG_NP + k] = v;
+  for (int k0 = 0; k0 < nc; k0 += 4) {
+    double tot[4];
+    const int ns = min(4, nc - k0);
+    seg_totals(a, first, count, k0, ns, tot, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < ns; k++) coef[(size_t)s * MG_NP + k0 + k] = tot[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) MST(s, MI_NC) = nc, MST(s, MI_IT) = nc;
@@ -368,7 +406,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_pnew(mg_args a, int64_t mpad, 
     int q = 0;
     MG_ROWS(t, row) acc[0] += Wb[(size_t)j * mpad + row] * pv[q++];
     block_sum<1>(acc, red);
-    if (threadIdx.x == 0) a.partial[(size_t)t.ti * MG_NP + j] = acc[0];
+    if (threadIdx.x == 0) a.partial[(size_t)j * a.ntiles + t.ti] = acc[0];
     __syncthreads();
   }
 }
@@ -982,7 +1020,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     RSB_TRY(hipMemcpyAsync(stt, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, st));
   }
   mg_args a{nseg, g->d_seg, g->d_tile, g->d_seg_tiles, sc, stt, partial, active, tol,
-            max_iter, max_passes};
+            max_iter, max_passes, ntiles};
   mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
   k_mg_diag0<<<nblocks(mpad, 256), 256, 0, st>>>(mpad, S, diag0);
   RSB_LAUNCH_CHECK();
@@ -993,7 +1031,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   MG_TRY(mg_build(g, factor, H, diag0));
   const int nlev = (int)H.lv.size();
   auto fin = [&](int op, int gate, int nslots) {
-    k_mg_fin<<<nseg, RSB_WAVE, 0, st>>>(a, op, gate, nslots);
+    k_mg_fin<<<nseg, RSB_BLOCK, 0, st>>>(a, op, gate, nslots);
   };
   auto count_active = [&](int &n) -> int {
     RSB_TRY(hipMemcpyAsync(&n, active, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1061,7 +1099,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     k_mg_zdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, r, z, z0);
     fin(FIN_BETA, GATE_PCG, 2);
     k_mg_pnew<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, z, p, Wb);
-    k_mg_coef<<<nseg, RSB_WAVE, 0, st>>>(a, coef);
+    k_mg_coef<<<nseg, RSB_BLOCK, 0, st>>>(a, coef);
     k_mg_pcorr<<<ntiles, RSB_BLOCK, 0, st>>>(a, mpad, p, Pb, coef);
     RSB_LAUNCH_CHECK();
     return 0;
