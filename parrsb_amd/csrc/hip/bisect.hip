@@ -142,7 +142,7 @@ __device__ inline int luf_rep(int *lp, int x) {
   int cur = __hip_atomic_load(&lp[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   while (cur != x) {
     const int nx = __hip_atomic_load(&lp[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (nx != cur)  // path halving: x's parent becomes its grandparent
+    if (nx != cur)  // path splitting: each visited row's parent becomes its grandparent
       __hip_atomic_store(&lp[x], nx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     x = cur;
     cur = nx;
