@@ -92,7 +92,57 @@ struct mg_sell {
   const uint32_t *ell;
   const int *rowlen;
   const double *diag;
+  // the level's LDS tiles (16-bit entries: local index | weight << 12 over
+  // the tile's rows and then its halo rows), or nhalo == nullptr
+  const int *nhalo;       // per tile, -1 if its halo does not fit
+  const int64_t *halo_off;
+  const int *halo;
+  const int64_t *off16;
+  const int *w16;
+  const uint16_t *e16;
 };
+
+typedef unsigned mg_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned mg_u32x2 __attribute__((ext_vector_type(2)));
+#define MG_TB_LOCAL 4096
+
+// Stage x over the tile's rows and halo rows into LDS; returns false (and
+// stages nothing) if the tile has no LDS form.
+__device__ inline bool mg_stage(const mg_sell &L, const mg_tile &t, const double *x, double *xl) {
+  if (!L.nhalo) return false;
+  const int nh = L.nhalo[t.ti];
+  if (nh < 0) return false;
+  const int nrows = t.row1 - t.row0, tot = nrows + nh;
+  const int64_t ho = L.halo_off[t.ti];
+  for (int i = threadIdx.x; i < tot; i += blockDim.x)
+    xl[i] = x[i < nrows ? t.row0 + i : L.halo[ho + i - nrows]];
+  __syncthreads();
+  return true;
+}
+
+// sum_j w_ij x_j over the row's 16-bit entries (the same entries in the same
+// order as the 32-bit form, then weight-0 padding, which adds exact zeros)
+__device__ inline double tile_offsum(const mg_sell &L, int row, const double *xl) {
+  const int sl = row >> 6;
+  const uint16_t *sbase = L.e16 + L.off16[sl];
+  const mg_u32x4 *ep = (const mg_u32x4 *)(sbase + (row & 63) * 8);
+  const int w16 = L.w16[sl], nb = w16 >> 3;
+  double s = 0.0;
+  for (int jb = 0; jb < nb; jb++) {
+    const mg_u32x4 v = ep[(size_t)jb * RSB_WAVE];
+    const unsigned e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
+                           v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += (double)(e[k] >> 12) * xl[e[k] & 0xfffu];
+  }
+  if (w16 & 4) {
+    const mg_u32x2 v = *(const mg_u32x2 *)(sbase + nb * 512 + (row & 63) * 4);
+    const unsigned e[4] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16};
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += (double)(e[k] >> 12) * xl[e[k] & 0xfffu];
+  }
+  return s;
+}
 // Eight entries and the eight values they point at are loaded before any is
 // used (one entry, then its gather, at a time left the smoother
 // latency-bound); the sum still runs over j in order.
@@ -337,13 +387,23 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_pass1(mg_args a, const double 
 // w = L p; partial p.w
 __global__ void __launch_bounds__(RSB_BLOCK) k_mg_apply(mg_args a, int gate, mg_sell L,
                                                        const double *p, double *w) {
+  __shared__ double xl[MG_TB_LOCAL];
   mg_tile t;
   if (!mg_tile_of(a, gate, t)) return;
   double acc[1] = {0.0};
-  MG_ROWS(t, row) {
-    const double v = L.diag[row] * p[row] - sell_offsum(L, row, p);
-    w[row] = v;
-    acc[0] += p[row] * v;
+  if (mg_stage(L, t, p, xl)) {
+    MG_ROWS(t, row) {
+      const double pi = xl[row - t.row0];
+      const double v = L.diag[row] * pi - tile_offsum(L, row, xl);
+      w[row] = v;
+      acc[0] += pi * v;
+    }
+  } else {
+    MG_ROWS(t, row) {
+      const double v = L.diag[row] * p[row] - sell_offsum(L, row, p);
+      w[row] = v;
+      acc[0] += p[row] * v;
+    }
   }
   mg_publish<1>(a, t, acc);
 }
@@ -480,11 +540,14 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_res0(mg_args a, mg_sell L, con
                                                       double *rout, const double *sv,
                                                       double sigma_next, double *snext,
                                                       double *u) {
+  __shared__ double xl[MG_TB_LOCAL];
   mg_tile t;
   if (!mg_tile_of(a, GATE_PCG, t)) return;
+  const bool tiled = mg_stage(L, t, sv, xl);
   MG_ROWS(t, row) {
     const double d = L.diag[row];
-    const double r = rin[row] - (d * sv[row] - sell_offsum(L, row, sv));
+    const double r = tiled ? rin[row] - (d * xl[row - t.row0] - tile_offsum(L, row, xl))
+                           : rin[row] - (d * sv[row] - sell_offsum(L, row, sv));
     rout[row] = r;
     if (sigma_next > 0.0) {
       const double v = d != 0.0 ? sigma_next * r / d : 0.0;
@@ -842,7 +905,8 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     RSB_TRY(hipMalloc((void **)&L0.sb, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
     RSB_TRY(hipMalloc((void **)&L0.u, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
   }
-  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
+  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0, nullptr, nullptr, nullptr, nullptr,
+            nullptr,          nullptr};
   for (int l = 0;; l++) {
     const std::vector<int> nf = H.h_n[l];
     std::vector<int> nc(nseg, 0), oc(nseg, 0);
@@ -1021,7 +1085,8 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   }
   mg_args a{nseg, g->d_seg, g->d_tile, g->d_seg_tiles, sc, stt, partial, active, tol,
             max_iter, max_passes, ntiles};
-  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0};
+  mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0,   g->d_tnhalo, g->d_thalo_off,
+            g->d_halo,      g->d_off16, g->d_w16,     g->d_e16};
   k_mg_diag0<<<nblocks(mpad, 256), 256, 0, st>>>(mpad, S, diag0);
   RSB_LAUNCH_CHECK();
   rsb_mg_free(g);

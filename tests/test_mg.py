@@ -247,3 +247,21 @@ def test_device_mg_graph_replays_equal_eager_launches(monkeypatch):
     c, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
     assert np.array_equal(a, b) and np.array_equal(a, c)
     assert ra["niter_max"] == rb["niter_max"] and ra["lambda_min"] == rb["lambda_min"]
+
+
+@pytest.mark.gpu
+def test_device_mg_lds_tiles_equal_global_form(monkeypatch):
+    """The level-0 smoother and SpMV read the 16-bit LDS tiles; with the
+    tiles off (PARRSB_AMD_NO_TILES=1) they read the 32-bit global form.  The
+    entries and their order are the same, so the solves are bit-identical
+    (a mesh big enough for several tiles per segment and halos)."""
+    flush("mg lds tiles vs global form")
+    vtx, xyz, nv = M.box_hex(24, 20, 18, jitter=0.1, seed=8)
+    a, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
+    ra = pa.last_report()
+    monkeypatch.setenv("PARRSB_AMD_NO_TILES", "1")
+    b, _ = pa.part_mesh(vtx, xyz, nv, 6, rsb_algo=1)
+    rb = pa.last_report()
+    monkeypatch.delenv("PARRSB_AMD_NO_TILES")
+    assert np.array_equal(a, b)
+    assert ra["niter_max"] == rb["niter_max"] and ra["lambda_min"] == rb["lambda_min"]
