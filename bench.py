@@ -264,6 +264,13 @@ def main():
                                        "nfallback", "nbrs_mean", "resid_med", "resid_max",
                                        "level_ms")},
         "phase_ms": rep["phase_ms"],
+        # the same times under the reference's timer names (metrics.h:10-27)
+        "rsb_metrics_ms": {"RSB_PRE": rep["phase_ms"]["prepartition"],
+                           "RSB_FIEDLER_SETUP": rep["phase_ms"]["laplacian"],
+                           "RSB_FIEDLER_CALC": rep["phase_ms"]["lanczos"],
+                           "RSB_SORT": rep["phase_ms"]["sort"],
+                           "RSB_COMPONENTS": rep["phase_ms"]["components"],
+                           "RSB_BALANCE": rep["phase_ms"]["balance"]},
     }
     if not args.no_compare and warm is not None:
         # same protocol as the main leg (warmup, then the timed steps)

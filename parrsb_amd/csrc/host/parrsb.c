@@ -225,8 +225,10 @@ static void check_partition(const parrsb_amd_report *r, const parrsb_options *o)
 }
 
 static void print_profile(const parrsb_amd_report *r) {
-  printf("phase ms: setup %.3f pre %.3f laplacian %.3f lanczos %.3f sort %.3f components "
-         "%.3f balance %.3f\n",
+  /* the reference's timer names (metrics.h:10-27): the level Laplacian is the
+     Fiedler setup, the solve its calculation */
+  printf("phase ms: setup %.3f RSB_PRE %.3f RSB_FIEDLER_SETUP %.3f RSB_FIEDLER_CALC %.3f "
+         "RSB_SORT %.3f RSB_COMPONENTS %.3f RSB_BALANCE %.3f\n",
          r->phase_ms[0], r->phase_ms[1], r->phase_ms[2], r->phase_ms[3], r->phase_ms[5],
          r->phase_ms[6], r->phase_ms[7]);
   for (int l = 0; l < r->nlevels && l < PARRSB_AMD_MAX_LEVELS; l++)
