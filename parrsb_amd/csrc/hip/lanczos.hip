@@ -46,6 +46,7 @@ struct lz_args {
   int miter_max;
   const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
   const int *act_segs;   // if set: blockIdx.x -> segment, for k_lz_fin
+  int pstride;           // partial is slot-major: slot k of tile t at k * pstride + t
 };
 
 #define SC(s, j) a.sc[(size_t)(s) * LZ_NSC + (j)]
@@ -122,7 +123,7 @@ __device__ inline void publish(const lz_args &a, const tile_t &t, double (&v)[NV
   __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
   block_sum<NV>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 4; k++) a.partial[(size_t)t.ti * 4 + k] = k < NV ? v[k] : 0.0;
+    for (int k = 0; k < NV; k++) a.partial[(size_t)k * a.pstride + t.ti] = v[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -955,19 +956,21 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
   // 8 tiles per thread in flight (large segments have up to 16k tiles), a
-  // fixed order of summation
+  // fixed order of summation; only the slots the phase wrote are read (each
+  // slot's partials are contiguous)
+  const int nv = op == FIN_APPLY ? 3 : 1;
   double tot[3] = {0.0, 0.0, 0.0};
-  const double2 *pp = (const double2 *)a.partial;
   for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
-    double2 v0[8], v1[8];
+    double v[3][8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int i = i0 + u * blockDim.x;
-      v0[u] = i < count ? pp[2 * (size_t)(first + i)] : make_double2(0.0, 0.0);
-      v1[u] = i < count ? pp[2 * (size_t)(first + i) + 1] : make_double2(0.0, 0.0);
-    }
+    for (int k = 0; k < 3; k++)
 #pragma unroll
-    for (int u = 0; u < 8; u++) tot[0] += v0[u].x, tot[1] += v0[u].y, tot[2] += v1[u].x;
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u * blockDim.x;
+        v[k][u] = (k < nv && i < count) ? a.partial[(size_t)k * a.pstride + first + i] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < 8; u++) tot[0] += v[0][u], tot[1] += v[1][u], tot[2] += v[2][u];
   }
   block_sum<3>(tot, red);
   if (threadIdx.x != 0) return;
@@ -1169,6 +1172,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
   a.running = L.running;
   a.mpass = mpass, a.mode = init_mode, a.tol = tol, a.miter_max = miter_max;
+  a.pstride = std::max(g->lz_tile_cap, 1);
   a.max_n = g->lws_max_n;
   T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
