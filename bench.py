@@ -31,6 +31,8 @@ CONFIGS = {
            "hex_box_128x128x64_jitter_1024_parts"),
     "c3": ("holed_quads", dict(nx=2048, ny=2048, seed=3), 4096,
            "holed_quads_2048x2048_4096_parts"),
+    "c4": ("pipe_hex", dict(nr=8, nt=256, nz=1024, seed=7), 2048,
+           "warped_pipe_hex_8x256x1024_2048_parts"),
     "c5": ("warped_box_hex", dict(n=256, seed=5), 65536, "warped_hex_256^3_65536_parts"),
 }
 
@@ -89,6 +91,10 @@ def cpu_baseline(cfg):
         n, P = 64, 1024
         vtx, xyz, nv = M.warped_box_hex(n, seed=5)
         sample = f"warped hex 64^3 = 262,144 elements into 1,024 parts (256 per part, as the workload)"
+    elif cfg == "c4":
+        vtx, xyz, nv = M.pipe_hex(8, 128, 256, seed=7)
+        P = 256
+        sample = "warped pipe 8x128x256 = 262,144 elements into 256 parts (1,024 per part, as the workload)"
     elif cfg == "c3":
         vtx, xyz, nv = M.holed_quads(512, 512, seed=3)
         P = 256
