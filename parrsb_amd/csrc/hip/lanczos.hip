@@ -1412,7 +1412,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
 
 extern "C" int rsbgpu_set_start(rsbgpu *g, const float *d_start) {
   RSB_TRY(hipSetDevice(g->device));
-  RSB_TRY(dev_alloc(g->d_start, (size_t)g->nel));
+  RSB_TRY(dev_alloc(g, g->d_start, (size_t)g->nel));
   RSB_TRY(hipMemcpyAsync(g->d_start, d_start, sizeof(float) * (size_t)g->nel,
                          hipMemcpyDeviceToDevice, g->stream));
   return 0;

@@ -126,11 +126,11 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
   const int64_t mpad = g->mpad;
   const int nel = g->nel;
   g->nslices = mpad / RSB_WAVE;
-  RSB_TRY(dev_alloc(g->d_row_elem, (size_t)mpad));
-  RSB_TRY(dev_alloc(g->d_elem_row, (size_t)nel));
-  RSB_TRY(dev_alloc(g->d_rowlen, (size_t)mpad));
-  RSB_TRY(dev_alloc(g->d_slice_w, (size_t)g->nslices + 1));
-  RSB_TRY(dev_alloc(g->d_slice_off, (size_t)g->nslices + 1));
+  RSB_TRY(dev_alloc(g, g->d_row_elem, (size_t)mpad));
+  RSB_TRY(dev_alloc(g, g->d_elem_row, (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_rowlen, (size_t)mpad));
+  RSB_TRY(dev_alloc(g, g->d_slice_w, (size_t)g->nslices + 1));
+  RSB_TRY(dev_alloc(g, g->d_slice_off, (size_t)g->nslices + 1));
   if (mpad == 0) {
     g->lnnz = g->lnnz_pad = 0;
     g->h_seg_nnz.assign(g->nseg, 0);
@@ -145,7 +145,7 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
   unsigned long long *dnnz = nullptr;
   RSB_TRY(hipMalloc((void **)&dnnz, sizeof(unsigned long long)));
   RSB_TRY(hipMemsetAsync(dnnz, 0, sizeof(unsigned long long), g->stream));
-  RSB_TRY(dev_alloc(g->d_seg_nnz, (size_t)g->nseg));
+  RSB_TRY(dev_alloc(g, g->d_seg_nnz, (size_t)g->nseg));
   RSB_TRY(hipMemsetAsync(g->d_seg_nnz, 0, sizeof(int64_t) * g->nseg, g->stream));
   k_level_rowlen<<<nblocks(mpad, 256), 256, 0, g->stream>>>(
       mpad, g->d_row_elem, g->d_segof, g->d_growptr, g->d_gadj, g->d_rowlen, g->d_slice_w, dnnz,
@@ -167,7 +167,7 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
                          hipMemcpyDeviceToHost, g->stream));
   RSB_TRY(hipStreamSynchronize(g->stream));
   g->lnnz = (int64_t)hn;
-  RSB_TRY(dev_alloc(g->d_ell, (size_t)g->lnnz_pad));
+  RSB_TRY(dev_alloc(g, g->d_ell, (size_t)g->lnnz_pad));
   k_level_fill<<<nblocks(mpad, 256), 256, 0, g->stream>>>(mpad, g->d_row_elem, g->d_segof,
                                                           g->d_elem_row, g->d_growptr,
                                                           g->d_gadj, g->d_slice_off,

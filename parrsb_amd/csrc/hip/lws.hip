@@ -683,7 +683,7 @@ extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const
     sc[SC_LAMBDA] = lam;
     std::vector<int> segof(M, 0);
 #define UP(dst, src, n, T)                                                               \
-  if (!rc && dev_alloc(dst, (size_t)(n)) != hipSuccess) rc = 2;                         \
+  if (!rc && dev_alloc(g, dst, (size_t)(n)) != hipSuccess) rc = 2;                         \
   if (!rc && hipMemcpy(dst, (src), sizeof(T) * (size_t)(n), hipMemcpyHostToDevice) != hipSuccess) rc = 2;
     UP(g->d_slice_w, sw.data(), ns + 1, int);
     UP(g->d_slice_off, so.data(), ns + 1, int64_t);

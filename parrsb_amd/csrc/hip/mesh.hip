@@ -349,7 +349,7 @@ static int build_graph(rsbgpu *g) {
   key = nullptr;
   // count, scan, fill
   RSB_TRY(hipMalloc((void **)&rowlen, sizeof(int64_t) * (n + 1)));
-  RSB_TRY(dev_alloc(g->d_growptr, (size_t)n + 1));
+  RSB_TRY(dev_alloc(g, g->d_growptr, (size_t)n + 1));
   RSB_TRY(hipMalloc((void **)&ovf, sizeof(int)));
   RSB_TRY(hipMemsetAsync(ovf, 0, sizeof(int), g->stream));
   RSB_TRY(hipMemsetAsync(rowlen + n, 0, sizeof(int64_t), g->stream));
@@ -361,7 +361,7 @@ static int build_graph(rsbgpu *g) {
   RSB_TRY(hipMemcpyAsync(&g->gnnz, g->d_growptr + n, sizeof(int64_t), hipMemcpyDeviceToHost,
                          g->stream));
   RSB_TRY(hipStreamSynchronize(g->stream));
-  RSB_TRY(dev_alloc(g->d_gadj, (size_t)g->gnnz));
+  RSB_TRY(dev_alloc(g, g->d_gadj, (size_t)g->gnnz));
   k_dual_rows<true><<<nblocks(n, 128), 128, 0, g->stream>>>(
       n, g->nv, slot_group, gstart, val, nullptr, g->d_growptr, g->d_gadj, ovf);
   RSB_LAUNCH_CHECK();
@@ -394,7 +394,7 @@ extern "C" int rsbgpu_set_mesh(rsbgpu *g, const long long *vtx, const double *xy
     g->d_vtx = vtx, g->d_xyz = xyz;
   } else {
     if (vtx) {
-      RSB_TRY(dev_alloc(g->own_vtx, ns));
+      RSB_TRY(dev_alloc(g, g->own_vtx, ns));
       RSB_TRY(hipMemcpyAsync(g->own_vtx, vtx, ns * sizeof(long long), hipMemcpyHostToDevice,
                              g->stream));
       g->d_vtx = g->own_vtx;
@@ -402,7 +402,7 @@ extern "C" int rsbgpu_set_mesh(rsbgpu *g, const long long *vtx, const double *xy
       g->d_vtx = nullptr;
     }
     if (xyz) {
-      RSB_TRY(dev_alloc(g->own_xyz, ns * g->ndim));
+      RSB_TRY(dev_alloc(g, g->own_xyz, ns * g->ndim));
       RSB_TRY(hipMemcpyAsync(g->own_xyz, xyz, ns * g->ndim * sizeof(double),
                              hipMemcpyHostToDevice, g->stream));
       g->d_xyz = g->own_xyz;
@@ -411,8 +411,8 @@ extern "C" int rsbgpu_set_mesh(rsbgpu *g, const long long *vtx, const double *xy
     }
   }
   g->has_coord = xyz != nullptr;
-  RSB_TRY(dev_alloc(g->d_coord, 3 * (size_t)nel));
-  RSB_TRY(dev_alloc(g->d_rank, 3 * (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_coord, 3 * (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_rank, 3 * (size_t)nel));
   if (g->has_coord && nel > 0) {
     k_centroids<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_xyz, nel, nv, g->ndim,
                                                           g->d_coord);
@@ -427,10 +427,10 @@ extern "C" int rsbgpu_set_mesh(rsbgpu *g, const long long *vtx, const double *xy
     int rc = build_graph(g);
     if (rc) return rc;
   }
-  RSB_TRY(dev_alloc(g->d_order, (size_t)nel));
-  RSB_TRY(dev_alloc(g->d_order2, (size_t)nel));
-  RSB_TRY(dev_alloc(g->d_segof, (size_t)nel));
-  RSB_TRY(dev_alloc(g->d_fied, (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_order, (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_order2, (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_segof, (size_t)nel));
+  RSB_TRY(dev_alloc(g, g->d_fied, (size_t)nel));
   if (nel > 0) {
     k_iota<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, nel);
     RSB_LAUNCH_CHECK();

@@ -1047,7 +1047,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     if (out->fallback) out->fallback[s] = 0;
   }
   // segments that cannot be bisected get Fiedler value 0 (as the Lanczos path)
-  RSB_TRY(dev_alloc(g->d_fied, (size_t)std::max(g->nel, 1)));
+  RSB_TRY(dev_alloc(g, g->d_fied, (size_t)std::max(g->nel, 1)));
   RSB_TRY(hipMemsetAsync(g->d_fied, 0, sizeof(double) * (size_t)std::max(g->nel, 1), st));
   bool any = false;
   for (int s = 0; s < nseg; s++) any |= g->h_p[s] > 1 && g->h_n[s] >= 2;

@@ -76,14 +76,14 @@ extern "C" int rsbgpu_set_segments(rsbgpu *g, int nseg, const int *cstart, const
     h[3 * nseg + s] = r0[s];
     h[4 * nseg + s] = g->h_pstart[s];
   }
-  RSB_TRY(dev_alloc(g->d_seg, h.size()));
+  RSB_TRY(dev_alloc(g, g->d_seg, h.size()));
   RSB_TRY(hipMemcpyAsync(g->d_seg, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice,
                          g->stream));
-  RSB_TRY(dev_alloc(g->d_tile, g->h_tile.size()));
+  RSB_TRY(dev_alloc(g, g->d_tile, g->h_tile.size()));
   if (!g->h_tile.empty())
     RSB_TRY(hipMemcpyAsync(g->d_tile, g->h_tile.data(), sizeof(int) * g->h_tile.size(),
                            hipMemcpyHostToDevice, g->stream));
-  RSB_TRY(dev_alloc(g->d_seg_tiles, g->h_seg_tiles.size()));
+  RSB_TRY(dev_alloc(g, g->d_seg_tiles, g->h_seg_tiles.size()));
   RSB_TRY(hipMemcpyAsync(g->d_seg_tiles, g->h_seg_tiles.data(),
                          sizeof(int) * g->h_seg_tiles.size(), hipMemcpyHostToDevice, g->stream));
   if (g->nel > 0) {

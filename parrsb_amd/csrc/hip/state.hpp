@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <unordered_map>
 #include <vector>
 
 #include "rsbgpu.h"
@@ -126,6 +127,7 @@ struct rsbgpu {
   int prof_spmv = 0;                   // bracket every LZ_PROF_EVERY-th SpMV launch with events
   std::vector<hipEvent_t> spmv_ev;
   hipEvent_t lz_ev0 = nullptr, lz_ev1 = nullptr;
+  std::unordered_map<const void *, size_t> cap;  // bytes held by each dev_alloc slot
 };
 
 // ---------------------------------------------------------------------------
@@ -164,13 +166,22 @@ inline hipError_t dev_ensure(T *&p, size_t &cap_elems, size_t need) {
 }
 
 template <typename T>
-inline hipError_t dev_alloc(T *&p, size_t n) {
+inline hipError_t dev_alloc(rsbgpu *g, T *&p, size_t n) {
+  // a buffer slot of the handle keeps its allocation while it is large
+  // enough: the level loop re-sizes the same slots every level, and freeing
+  // and mapping the multi-GB level matrices again each time cost ~10 ms a
+  // level at 16.7M elements (contents are never assumed to be fresh)
+  const size_t bytes = (n ? n : 1) * sizeof(T);
+  auto it = g->cap.find((const void *)&p);
+  if (p && it != g->cap.end() && it->second >= bytes) return hipSuccess;
   if (p) {
     hipError_t e = hipFree(p);
     if (e != hipSuccess) return e;
     p = nullptr;
   }
-  return hipMalloc((void **)&p, (n ? n : 1) * sizeof(T));
+  hipError_t e = hipMalloc((void **)&p, bytes);
+  if (e == hipSuccess) g->cap[(const void *)&p] = bytes;
+  return e;
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }

@@ -188,10 +188,10 @@ int rsb_build_tiles(rsbgpu *g) {
   const int nt = g->ntiles;
   const int64_t ns = g->nslices;
   if (nt == 0) return 0;
-  RSB_TRY(dev_alloc(g->d_tnhalo, (size_t)nt + 1));
-  RSB_TRY(dev_alloc(g->d_thalo_off, (size_t)nt + 1));
-  RSB_TRY(dev_alloc(g->d_w16, (size_t)ns + 1));
-  RSB_TRY(dev_alloc(g->d_off16, (size_t)ns + 1));
+  RSB_TRY(dev_alloc(g, g->d_tnhalo, (size_t)nt + 1));
+  RSB_TRY(dev_alloc(g, g->d_thalo_off, (size_t)nt + 1));
+  RSB_TRY(dev_alloc(g, g->d_w16, (size_t)ns + 1));
+  RSB_TRY(dev_alloc(g, g->d_off16, (size_t)ns + 1));
   int64_t *tmp = nullptr;
   RSB_TRY(hipMalloc((void **)&tmp, sizeof(int64_t) * (std::max<int64_t>(ns, nt) + 1)));
   k_tile_count<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w,
@@ -223,8 +223,8 @@ int rsb_build_tiles(rsbgpu *g) {
   g->e16_count = n16;
   g->nwide = 0;
   for (int v : nhv) g->nwide += v < 0;
-  RSB_TRY(dev_alloc(g->d_halo, (size_t)nh + 1));
-  RSB_TRY(dev_alloc(g->d_e16, (size_t)n16 + 8));
+  RSB_TRY(dev_alloc(g, g->d_halo, (size_t)nh + 1));
+  RSB_TRY(dev_alloc(g, g->d_e16, (size_t)n16 + 8));
   k_tile_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w, g->d_ell,
                                                 g->d_tnhalo, g->d_thalo_off, g->d_off16,
                                                 g->d_w16, g->d_halo, g->d_e16);
