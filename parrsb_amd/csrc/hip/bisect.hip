@@ -45,9 +45,9 @@ extern "C" int rsbgpu_fiedler_sort(rsbgpu *g) {
   uint64_t *key = nullptr;
   int *perm = nullptr;
   double *fied2 = nullptr;
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * (nel + 1)));
-  RSB_TRY(hipMalloc((void **)&perm, sizeof(int) * (nel + 1)));
-  RSB_TRY(hipMalloc((void **)&fied2, sizeof(double) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(key, sizeof(uint64_t) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(perm, sizeof(int) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(fied2, sizeof(double) * (nel + 1)));
   k_fkeys<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_fied, g->d_order, g->d_segof, g->d_seg,
                                                     g->nseg, nel, key, perm);
   RSB_LAUNCH_CHECK();
@@ -68,7 +68,6 @@ extern "C" int rsbgpu_fiedler_sort(rsbgpu *g) {
   RSB_TRY(hipMemcpyAsync(g->d_fied, fied2, sizeof(double) * nel, hipMemcpyDeviceToDevice,
                          g->stream));
   RSB_TRY(hipStreamSynchronize(g->stream));
-  hipFree(key), hipFree(perm), hipFree(fied2);
   rsb_phase_end(g, PH_SORT);
   return 0;
 }
@@ -340,10 +339,10 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
   int *d_ncomp = nullptr, *par = nullptr, *flag = nullptr, *C = nullptr, *low = nullptr;
   int *d_nlo = nullptr;
   double *fied2 = nullptr;
-  RSB_TRY(hipMalloc((void **)&d_nl, sizeof(int) * nseg * 6));
+  RSB_TRY(RSB_SCRATCH(d_nl, sizeof(int) * nseg * 6));
   d_snd = d_nl + nseg, d_sc = d_nl + 2 * nseg, d_nif = d_nl + 3 * nseg, d_nlo = d_nl + 4 * nseg;
-  RSB_TRY(hipMalloc((void **)&d_ncomp, sizeof(int) * 2 * nseg));
-  RSB_TRY(hipMalloc((void **)&hseg, sizeof(int) * nel));
+  RSB_TRY(RSB_SCRATCH(d_ncomp, sizeof(int) * 2 * nseg));
+  RSB_TRY(RSB_SCRATCH(hseg, sizeof(int) * nel));
   RSB_TRY(hipMemcpyAsync(d_nl, nlower, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
   RSB_TRY(hipMemcpyAsync(d_snd, sender, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
   RSB_TRY(hipMemcpyAsync(d_sc, scount, sizeof(int) * nseg, hipMemcpyHostToDevice, g->stream));
@@ -355,7 +354,7 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
   // components of each half before balancing (rsb.c:320-327)
   if (find_comps && g->mpad > 0) {
     rsb_phase_begin(g);
-    RSB_TRY(hipMalloc((void **)&par, sizeof(int) * 2 * g->mpad));
+    RSB_TRY(RSB_SCRATCH(par, sizeof(int) * 2 * g->mpad));
     int *hrow = par + g->mpad;
     k_uf_hrow<<<nblocks(g->mpad, 256), 256, 0, g->stream>>>(g->mpad, g->d_row_elem, hseg, hrow);
     k_uf_tile<<<g->ntiles, RSB_BLOCK, 0, g->stream>>>(g->d_tile, hrow, g->d_slice_off,
@@ -368,10 +367,10 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
     rsb_phase_end(g, PH_COMP);
   }
   rsb_phase_begin(g);
-  RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * (nel + 1)));
-  RSB_TRY(hipMalloc((void **)&C, sizeof(int) * (nel + 1)));
-  RSB_TRY(hipMalloc((void **)&low, sizeof(int) * (nel + 1)));
-  RSB_TRY(hipMalloc((void **)&fied2, sizeof(double) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(flag, sizeof(int) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(C, sizeof(int) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(low, sizeof(int) * (nel + 1)));
+  RSB_TRY(RSB_SCRATCH(fied2, sizeof(double) * (nel + 1)));
   RSB_TRY(hipMemsetAsync(flag + nel, 0, sizeof(int), g->stream));
   RSB_TRY(hipMemsetAsync(low + nel, 0, sizeof(int), g->stream));
   k_iface<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nseg, d_nl,
@@ -413,9 +412,6 @@ extern "C" int rsbgpu_bisect(rsbgpu *g, int *nlower, const int *sender, const in
     }
     if (moved) moved[s] = mv;
   }
-  hipFree(d_nl), hipFree(d_ncomp), hipFree(hseg), hipFree(flag), hipFree(C), hipFree(low),
-      hipFree(fied2);
-  if (par) hipFree(par);
   return 0;
 }
 

@@ -1141,7 +1141,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     g->basis_bytes = bb;
   }
   double *xrow = nullptr;
-  RSB_TRY(hipMalloc((void **)&xrow, sizeof(double) * (mpad + 1)));
+  RSB_TRY(RSB_SCRATCH(xrow, sizeof(double) * (mpad + 1)));
   lz_state &L = g->lz;
   if (nseg > g->lz_nseg_cap || miter_max > g->lz_miter_cap) {
     // iter, k, npass, niter and miter are carved from the status block
@@ -1184,7 +1184,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   } while (0)
   // compacted lists of running tiles / segments for the iteration launches
   int *act = nullptr;
-  RSB_TRY(hipMalloc((void **)&act, sizeof(int) * ((size_t)ntiles + nseg + 2)));
+  RSB_TRY(RSB_SCRATCH(act, sizeof(int) * ((size_t)ntiles + nseg + 2)));
   int *act_t = act, *act_s = act + ntiles, *act_n = act + ntiles + nseg;
   int nact[2] = {ntiles, nseg};
   auto compact = [&](int want) -> int {
@@ -1208,8 +1208,6 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   k_lz_zero_small<<<nblocks(nseg, 256), 256, 0, st>>>(a, g->d_fied);
   RSB_LAUNCH_CHECK();
   if (ntiles == 0) {
-    (void)hipFree(act);
-    (void)hipFree(xrow);
     return 0;
   }
   // start vector of the segments in LZ_SETUP, normalised into f
@@ -1366,7 +1364,6 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     RSB_LAUNCH_CHECK();
     FIN(FIN_RES1);
   }
-  (void)hipFree(act);
   // read back per-segment results
   std::vector<int> ni(nseg), np(nseg);
   std::vector<double> sc((size_t)nseg * LZ_NSC);
@@ -1375,7 +1372,6 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   RSB_TRY(hipMemcpyAsync(sc.data(), a.sc, sizeof(double) * nseg * LZ_NSC,
                          hipMemcpyDeviceToHost, st));
   RSB_TRY(hipStreamSynchronize(st));
-  (void)hipFree(xrow);
   // Algorithmic bytes, per iteration and segment of n rows and nnz
   // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 9 s n vector
   // streams for the whole iteration; for k_lz_apply alone 4(n+1) + 4 nnz +

@@ -143,7 +143,7 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
                                                        nel, g->d_row_elem, g->d_elem_row);
   RSB_LAUNCH_CHECK();
   unsigned long long *dnnz = nullptr;
-  RSB_TRY(hipMalloc((void **)&dnnz, sizeof(unsigned long long)));
+  RSB_TRY(RSB_SCRATCH(dnnz, sizeof(unsigned long long)));
   RSB_TRY(hipMemsetAsync(dnnz, 0, sizeof(unsigned long long), g->stream));
   RSB_TRY(dev_alloc(g, g->d_seg_nnz, (size_t)g->nseg));
   RSB_TRY(hipMemsetAsync(g->d_seg_nnz, 0, sizeof(int64_t) * g->nseg, g->stream));
@@ -152,7 +152,7 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
       g->d_seg_nnz);
   RSB_LAUNCH_CHECK();
   int64_t *bytes = nullptr;
-  RSB_TRY(hipMalloc((void **)&bytes, sizeof(int64_t) * (g->nslices + 1)));
+  RSB_TRY(RSB_SCRATCH(bytes, sizeof(int64_t) * (g->nslices + 1)));
   k_slice_bytes<<<nblocks(g->nslices + 1, 256), 256, 0, g->stream>>>(g->nslices, g->d_slice_w,
                                                                      bytes);
   RSB_LAUNCH_CHECK();
@@ -173,7 +173,6 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
                                                           g->d_gadj, g->d_slice_off,
                                                           g->d_slice_w, g->d_ell);
   RSB_LAUNCH_CHECK();
-  hipFree(dnnz), hipFree(bytes);
   rc = rsb_build_tiles(g);
   if (rc) return rc;
   rsb_phase_end(g, PH_LAP);

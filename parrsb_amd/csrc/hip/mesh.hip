@@ -116,6 +116,9 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_k1);
   FREE(g->d_v1);
   FREE(g->d_iscratch);
+  for (auto &kv : g->scratch)
+    if (kv.second.first) hipFree(kv.second.first);
+  g->scratch.clear();
   if (g->ev0) hipEventDestroy(g->ev0);
   if (g->ev1) hipEventDestroy(g->ev1);
   if (g->lz_ev0) hipEventDestroy(g->lz_ev0);

@@ -590,17 +590,17 @@ static int run_rcb(rsbgpu *g) {
   // growing by at most that bound.
   size_t cap = (size_t)nseg + (size_t)nel * (size_t)std::min(rounds, 4) / 2 + (size_t)nel + 16;
   int *buf = nullptr;
-  RSB_TRY(hipMalloc((void **)&buf, sizeof(int) * cap * 13 + sizeof(int) * (size_t)nel));
+  RSB_TRY(RSB_SCRATCH(buf, sizeof(int) * cap * 13 + sizeof(int) * (size_t)nel));
   nodes_t A{buf, buf + cap, buf + 2 * cap, buf + 3 * cap};
   nodes_t B{buf + 4 * cap, buf + 5 * cap, buf + 6 * cap, buf + 7 * cap};
   int *cnt = buf + 8 * cap, *first = buf + 9 * cap, *split = buf + 10 * cap;
   int *axis = buf + 11 * cap, *nodeof = buf + 13 * cap;
   uint64_t *ext = nullptr, *key = nullptr;
-  RSB_TRY(hipMalloc((void **)&ext, sizeof(uint64_t) * 6 * cap));
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ((size_t)nel + 1)));
+  RSB_TRY(RSB_SCRATCH(ext, sizeof(uint64_t) * 6 * cap));
+  RSB_TRY(RSB_SCRATCH(key, sizeof(uint64_t) * ((size_t)nel + 1)));
   int K = nseg;
   int *stat = nullptr;
-  RSB_TRY(hipMalloc((void **)&stat, sizeof(int) * 2));
+  RSB_TRY(RSB_SCRATCH(stat, sizeof(int) * 2));
   // the tail kernel takes over once every splitting node is small and has
   // no more ranks than elements
   const char *tv = getenv("PARRSB_AMD_RCB_TAIL");
@@ -663,7 +663,6 @@ static int run_rcb(rsbgpu *g) {
     K = K2;
     std::swap(A, B);
   }
-  hipFree(buf), hipFree(ext), hipFree(key), hipFree(stat);
   return 0;
 }
 
@@ -919,7 +918,7 @@ static int run_rib(rsbgpu *g) {
   // node tables (two generations) and per-node counts, per-position arrays,
   // and the chunk tables of the large nodes
   int *buf = nullptr;
-  RSB_TRY(hipMalloc((void **)&buf, sizeof(int) * (cap * 13 + ccap + 3 * (size_t)nel)));
+  RSB_TRY(RSB_SCRATCH(buf, sizeof(int) * (cap * 13 + ccap + 3 * (size_t)nel)));
   nodes_t A{buf, buf + cap, buf + 2 * cap, buf + 3 * cap};
   nodes_t B{buf + 4 * cap, buf + 5 * cap, buf + 6 * cap, buf + 7 * cap};
   int *cnt = buf + 8 * cap, *first = buf + 9 * cap, *split = buf + 10 * cap;
@@ -927,9 +926,9 @@ static int run_rib(rsbgpu *g) {
   int *nodeof = cnode + ccap, *pos = nodeof + nel, *val2 = pos + nel;
   double *ax = nullptr, *part = nullptr;
   uint64_t *key = nullptr;
-  RSB_TRY(hipMalloc((void **)&ax, sizeof(double) * 6 * cap));
-  RSB_TRY(hipMalloc((void **)&part, sizeof(double) * 12 * ccap));
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ((size_t)nel + 1)));
+  RSB_TRY(RSB_SCRATCH(ax, sizeof(double) * 6 * cap));
+  RSB_TRY(RSB_SCRATCH(part, sizeof(double) * 12 * ccap));
+  RSB_TRY(RSB_SCRATCH(key, sizeof(uint64_t) * ((size_t)nel + 1)));
   uint32_t *key2 = (uint32_t *)key;  // the second sort reuses the key array
   hipStream_t st = g->stream;
   int K = nseg;
@@ -991,7 +990,6 @@ static int run_rib(rsbgpu *g) {
     std::swap(A, B);
   }
   RSB_TRY(hipStreamSynchronize(st));
-  hipFree(buf), hipFree(ax), hipFree(part), hipFree(key);
   return 0;
 }
 
@@ -1010,15 +1008,13 @@ __global__ void k_gid_keys(const int *__restrict__ order, const int *__restrict_
 static int run_gid(rsbgpu *g) {
   const int nel = g->nel;
   uint64_t *key = nullptr;
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ((size_t)nel + 1)));
+  RSB_TRY(RSB_SCRATCH(key, sizeof(uint64_t) * ((size_t)nel + 1)));
   k_gid_keys<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg,
                                                        g->nseg, nel, key);
   RSB_LAUNCH_CHECK();
   int sb = 1;
   while ((1ll << sb) < g->nseg) sb++;
-  int rc = rsb_sort_pairs_u64(g, key, g->d_order, nel, 0, 32 + sb);
-  hipFree(key);
-  return rc;
+  return rsb_sort_pairs_u64(g, key, g->d_order, nel, 0, 32 + sb);
 }
 
 extern "C" int rsbgpu_prepartition(rsbgpu *g, int rsb_pre) {

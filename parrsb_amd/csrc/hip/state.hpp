@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -128,6 +129,9 @@ struct rsbgpu {
   std::vector<hipEvent_t> spmv_ev;
   hipEvent_t lz_ev0 = nullptr, lz_ev1 = nullptr;
   std::unordered_map<const void *, size_t> cap;  // bytes held by each dev_alloc slot
+  // per-call scratch buffers, one per call site (file, line), kept across
+  // calls while large enough and freed with the handle (rsb_scratch)
+  std::map<std::pair<const void *, int>, std::pair<void *, size_t>> scratch;
 };
 
 // ---------------------------------------------------------------------------
@@ -183,6 +187,34 @@ inline hipError_t dev_alloc(rsbgpu *g, T *&p, size_t n) {
   if (e == hipSuccess) g->cap[(const void *)&p] = bytes;
   return e;
 }
+
+// Scratch buffer of one call site: the level loop used to hipMalloc and
+// hipFree ~25 buffers a level, ~75 us each (about a third of a 1M-element
+// call).  The buffer of a site lives until the handle is closed and is
+// reused while large enough; its contents are never assumed.  Work queued
+// on the handle's stream is ordered with the next use of the same site.
+inline hipError_t rsb_scratch(rsbgpu *g, const void *file, int line, size_t bytes, void **out) {
+  auto &s = g->scratch[std::make_pair(file, line)];
+  if (s.first && s.second >= bytes) {
+    *out = s.first;
+    return hipSuccess;
+  }
+  if (s.first) {
+    hipError_t e = hipFree(s.first);
+    s.first = nullptr, s.second = 0;
+    if (e != hipSuccess) return e;
+  }
+  hipError_t e = hipMalloc(&s.first, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    s.first = nullptr;
+    return e;
+  }
+  s.second = bytes;
+  *out = s.first;
+  return hipSuccess;
+}
+#define RSB_SCRATCH(ptr, bytes) \
+  rsb_scratch(g, (const void *)__FILE__, __LINE__, (size_t)(bytes), (void **)&(ptr))
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

@@ -96,11 +96,11 @@ extern "C" int rsbgpu_neighbors(rsbgpu *g, int *nbrs) {
   unsigned long long *total = nullptr;
   uint64_t *keys = nullptr;
   int *vals = nullptr, *nb = nullptr;
-  RSB_TRY(hipMalloc((void **)&total, sizeof(unsigned long long)));
-  RSB_TRY(hipMalloc((void **)&nb, sizeof(int) * (size_t)nseg));
+  RSB_TRY(RSB_SCRATCH(total, sizeof(unsigned long long)));
+  RSB_TRY(RSB_SCRATCH(nb, sizeof(int) * (size_t)nseg));
   unsigned long long npairs = 0;
   for (int attempt = 0; attempt < 2; attempt++) {
-    RSB_TRY(hipMalloc((void **)&keys, sizeof(uint64_t) * (size_t)cap));
+    RSB_TRY(RSB_SCRATCH(keys, sizeof(uint64_t) * (size_t)cap));
     RSB_TRY(hipMemsetAsync(total, 0, sizeof(unsigned long long), st));
     k_nb_pairs<<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_growptr, g->d_gadj,
                                                         g->d_segof, total, cap, keys);
@@ -108,15 +108,13 @@ extern "C" int rsbgpu_neighbors(rsbgpu *g, int *nbrs) {
     RSB_TRY(hipMemcpyAsync(&npairs, total, sizeof(npairs), hipMemcpyDeviceToHost, st));
     RSB_TRY(hipStreamSynchronize(st));
     if ((int64_t)npairs <= cap) break;
-    hipFree(keys);
-    keys = nullptr;
     if (attempt == 1 || npairs >= (1ull << 31))
       RSB_FAIL(2, "too many neighbour pairs (%llu)", (unsigned long long)npairs);
     cap = (int64_t)npairs;
   }
   RSB_TRY(hipMemsetAsync(nb, 0, sizeof(int) * (size_t)nseg, st));
   if (npairs > 0) {
-    RSB_TRY(hipMalloc((void **)&vals, sizeof(int) * (size_t)npairs));
+    RSB_TRY(RSB_SCRATCH(vals, sizeof(int) * (size_t)npairs));
     int sb = 1;
     while ((1ll << sb) < nseg) sb++;
     int rc = rsb_sort_pairs_u64(g, keys, vals, (int64_t)npairs, 0, 32 + sb);
@@ -126,6 +124,5 @@ extern "C" int rsbgpu_neighbors(rsbgpu *g, int *nbrs) {
   }
   RSB_TRY(hipMemcpyAsync(nbrs, nb, sizeof(int) * (size_t)nseg, hipMemcpyDeviceToHost, st));
   RSB_TRY(hipStreamSynchronize(st));
-  hipFree(total), hipFree(keys), hipFree(vals), hipFree(nb);
   return 0;
 }

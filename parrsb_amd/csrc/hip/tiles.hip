@@ -193,7 +193,7 @@ int rsb_build_tiles(rsbgpu *g) {
   RSB_TRY(dev_alloc(g, g->d_w16, (size_t)ns + 1));
   RSB_TRY(dev_alloc(g, g->d_off16, (size_t)ns + 1));
   int64_t *tmp = nullptr;
-  RSB_TRY(hipMalloc((void **)&tmp, sizeof(int64_t) * (std::max<int64_t>(ns, nt) + 1)));
+  RSB_TRY(RSB_SCRATCH(tmp, sizeof(int64_t) * (std::max<int64_t>(ns, nt) + 1)));
   k_tile_count<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w,
                                                  g->d_ell, g->d_tnhalo);
   RSB_LAUNCH_CHECK();
@@ -230,6 +230,5 @@ int rsb_build_tiles(rsbgpu *g) {
                                                 g->d_w16, g->d_halo, g->d_e16);
   RSB_LAUNCH_CHECK();
   RSB_TRY(hipStreamSynchronize(g->stream));
-  hipFree(tmp);
   return 0;
 }
