@@ -8,7 +8,7 @@ parameters, parrsb_amd/lws_weights.bin); beside it, on the same mesh, the
 batched FP32 path from the reference start and the plain FP64 GPU path
 (reference arithmetic and start), and the CPU oracle on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config warped256]
                     [--warm-start PATH|none] [--mg]
 """
 import argparse
@@ -25,15 +25,15 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: (generator, kwargs, nparts, description)
-    "c1": ("box_hex", dict(nx=16, ny=16, nz=16, jitter=0.1, seed=1), 8,
+    "hex16": ("box_hex", dict(nx=16, ny=16, nz=16, jitter=0.1, seed=1), 8,
            "hex_box_16x16x16_jitter_8_parts"),
-    "c2": ("box_hex", dict(nx=128, ny=128, nz=64, jitter=0.1, seed=2), 1024,
+    "hex128": ("box_hex", dict(nx=128, ny=128, nz=64, jitter=0.1, seed=2), 1024,
            "hex_box_128x128x64_jitter_1024_parts"),
-    "c3": ("holed_quads", dict(nx=2048, ny=2048, seed=3), 4096,
+    "holed2048": ("holed_quads", dict(nx=2048, ny=2048, seed=3), 4096,
            "holed_quads_2048x2048_4096_parts"),
-    "c4": ("pipe_hex", dict(nr=8, nt=256, nz=1024, seed=7), 2048,
+    "pipe2m": ("pipe_hex", dict(nr=8, nt=256, nz=1024, seed=7), 2048,
            "warped_pipe_hex_8x256x1024_2048_parts"),
-    "c5": ("warped_box_hex", dict(n=256, seed=5), 65536, "warped_hex_256^3_65536_parts"),
+    "warped256": ("warped_box_hex", dict(n=256, seed=5), 65536, "warped_hex_256^3_65536_parts"),
 }
 
 
@@ -87,15 +87,15 @@ def cpu_baseline(cfg):
     a sub-mesh of the same family, partitioned to the same elements per part."""
     from oracle import pyoracle as O
     from parrsb_amd import meshgen as M
-    if cfg == "c5":
+    if cfg == "warped256":
         n, P = 64, 1024
         vtx, xyz, nv = M.warped_box_hex(n, seed=5)
         sample = f"warped hex 64^3 = 262,144 elements into 1,024 parts (256 per part, as the workload)"
-    elif cfg == "c4":
+    elif cfg == "pipe2m":
         vtx, xyz, nv = M.pipe_hex(8, 128, 256, seed=7)
         P = 256
         sample = "warped pipe 8x128x256 = 262,144 elements into 256 parts (1,024 per part, as the workload)"
-    elif cfg == "c3":
+    elif cfg == "holed2048":
         vtx, xyz, nv = M.holed_quads(512, 512, seed=3)
         P = 256
         sample = "holed quads 512^2 into 256 parts (~976 per part, as the workload)"
@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c5", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="warped256", choices=sorted(CONFIGS))
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")

@@ -30,8 +30,8 @@ def _balance_expected(n, p, b):
     return q * nh + rem // 2 + (rem % 2) * (1 - b)
 
 
-def test_c2_fp32_matches_fp64():
-    flush("C2 128x128x64 into 1024 parts, fp32 / fp64")
+def test_hexbox_128x128x64_fp32_matches_fp64():
+    flush("hex box 128x128x64 into 1024 parts, fp32 / fp64")
     vtx, xyz, nv = M.box_hex(128, 128, 64, jitter=0.1, seed=2)
     P = 1024
     pa.set_precision(32)
@@ -65,8 +65,8 @@ def _expected_sizes(n, P):
     return sorted(out)
 
 
-def test_c3_holed_quads_sizes_and_precisions():
-    flush("C3 holed quads 2048^2 into 4096 parts")
+def test_holed_quads_2048_sizes_and_precisions():
+    flush("holed quads 2048^2 into 4096 parts")
     vtx, xyz, nv = M.holed_quads(2048, 2048, seed=3)
     n, P = vtx.shape[0], 4096
     pa.set_precision(32)
@@ -83,8 +83,8 @@ def test_c3_holed_quads_sizes_and_precisions():
     assert len(ra["ncomp_max"]) == 12 and max(ra["ncomp_max"]) >= 1
 
 
-def test_c4_pipe_learned_start_parity():
-    flush("C4 2M pipe into 2048 parts, learned start / reference start")
+def test_pipe_2m_learned_start_parity():
+    flush("2M-element pipe into 2048 parts, learned start / reference start")
     import os
     from parrsb_amd import lws_train
     vtx, xyz, nv = M.pipe_hex(8, 256, 1024, seed=7)
