@@ -27,6 +27,8 @@
 #if defined(PARRSB_NO_MPI)
 /* Callers without MPI: MPICH's ABI (MPI_Comm is an int handle). */
 typedef int MPI_Comm;
+typedef int MPI_Fint;
+#define MPI_Comm_f2c(comm) ((MPI_Comm)(comm))
 #ifndef MPI_COMM_WORLD
 #define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
 #define MPI_COMM_SELF ((MPI_Comm)0x44000001)
@@ -49,15 +51,15 @@ typedef struct {
   int verbose_level; /* default 1 */
   int profile_level; /* default 0 */
   /* RSB options (parRSB.h:32-37) */
-  int rsb_algo;       /* 0 - Lanczos (1 - MG is not provided) */
+  int rsb_algo;       /* 0 - Lanczos, 1 - MG-preconditioned inverse iteration */
   int rsb_pre;        /* 0 - none, 1 - RCB, 2 - RIB (default 1) */
   int rsb_max_iter;   /* Lanczos iterations per pass (default 50) */
   int rsb_max_passes; /* Lanczos restarts (default 50) */
   double rsb_tol;     /* CG residual reduction factor (default 1e-5) */
   int rsb_dump_stats; /* parsed, unused (default 0) */
-  /* RSB MG options (parRSB.h:39-40), accepted and ignored */
-  int rsb_mg_grammian;
-  int rsb_mg_factor;
+  /* RSB MG options (parRSB.h:39-40), used with rsb_algo = 1 */
+  int rsb_mg_grammian; /* Gram-matrix refinement of the iterates (default 0) */
+  int rsb_mg_factor;   /* aggregation factor of the MG hierarchy, >= 2 (default 2) */
   /* Appended: number of parts; 0 => size of comm (reference behaviour). */
   int nparts;
 } parrsb_options;
@@ -107,6 +109,20 @@ int parrsb_part_mesh(int *part, const long long *const vtx,
 int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
                      long long *periodicInfo, int nPeriodicFaces, double tol,
                      MPI_Comm comm);
+
+/*
+ * Fortran binding of parrsb_conn_mesh (parRSB.h:66-70, con.c:297-307 of the
+ * reference), the entry Nek5000 calls: every argument by reference, the
+ * communicator as a Fortran handle, the return code in *err.  Exported under
+ * the usual Fortran manglings: fparrsb_conn_mesh_ (gfortran and most
+ * compilers, gslib's default) and fparrsb_conn_mesh.
+ */
+void fparrsb_conn_mesh_(long long *vtx, double *coord, int *nel, int *nDim,
+                        long long *periodicInfo, int *nPeriodicFaces, double *tol,
+                        MPI_Fint *fcomm, int *err);
+void fparrsb_conn_mesh(long long *vtx, double *coord, int *nel, int *nDim,
+                       long long *periodicInfo, int *nPeriodicFaces, double *tol,
+                       MPI_Fint *fcomm, int *err);
 
 /*
  * Nek5000 mesh files (parRSB.h:71-83 of the reference, io.c), read and

@@ -51,6 +51,12 @@ typedef struct {
      cut, min / max over segments, mean over ranks */
   int nbrs_min[PARRSB_AMD_MAX_LEVELS], nbrs_max[PARRSB_AMD_MAX_LEVELS];
   double nbrs_mean[PARRSB_AMD_MAX_LEVELS];
+  /* TOL_INIT, TOL_TGT, TOL_FNL (fiedler.c:450-451, 506): the CG residual at
+     the start of the last pass, its target and the final residual, each the
+     minimum over the level's segments that stopped at rsb_max_iter *
+     rsb_max_passes (rsb.c:79-90); 0 where every segment converged */
+  double tol_init[PARRSB_AMD_MAX_LEVELS], tol_target[PARRSB_AMD_MAX_LEVELS];
+  double tol_final[PARRSB_AMD_MAX_LEVELS];
 } parrsb_amd_report;
 
 /* Same contract as parrsb_part_mesh, with every array in GPU memory of the

@@ -146,6 +146,10 @@ int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device, int max_gr
  * host; used by the parity tests. */
 int rsbgpu_get_order(rsbgpu *g, int *order_host);
 int rsbgpu_get_fiedler(rsbgpu *g, double *fiedler_host);
+/* Overwrite the Fiedler value of every compact position (host array of nel
+ * values): lets the parity tests drive the balance step with chosen values
+ * (ties, either sender) through rsbgpu_bisect. */
+int rsbgpu_set_fiedler(rsbgpu *g, const double *fiedler_host);
 /* Copy the assembled level Laplacian in padded-row CSR form (host arrays,
  * row-major, columns ascending): rowptr[mpad+1], col[], wgt[].  Returns the
  * padded row count and nnz through the pointers when the arrays are NULL. */

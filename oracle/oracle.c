@@ -1081,6 +1081,52 @@ static void neighbor_stats(const long long *vtx, int nv, const int *segof, int n
 }
 
 /* ------------------------------------------------------------------------ */
+/* One balance step (rsb.c:118-229).  el[0..n) are a segment's elements in
+ * ascending Fiedler order (values fv), el[0..nl) its lower half.  If the
+ * sending side (lower when sender_lower = 1) has at least scnt interface
+ * elements (rsb.c:145-165, 194), it hands over scnt of them: ordered by
+ * mul * fiedler with ties by position (rsb.c:168-187), the upper side offers
+ * its smallest values first, the lower side its largest.  Each half then
+ * keeps its members in the current order (the reference re-sorts each half
+ * by Fiedler value, rsb.c:219-220, which gives the same order up to ties).
+ * el and fv are permuted in place; returns the new size of the lower half
+ * (nl itself when nothing moves). */
+int oracle_balance_step(const long long *vtx, int nv, int *el, double *fv, int n, int nl,
+                        int sender_lower, int64_t scnt) {
+  if (scnt <= 0 || n <= 0) return nl;
+  int *flag = xmalloc(sizeof(int) * (size_t)n);
+  interface_flags(vtx, nv, el, n, nl, sender_lower, flag);
+  int lo = sender_lower ? 0 : nl, hi = sender_lower ? nl : n;
+  int nif = 0;
+  for (int i = lo; i < hi; i++) nif += flag[i];
+  if (nif < scnt) {
+    free(flag);
+    return nl;
+  }
+  int *cand = xmalloc(sizeof(int) * (size_t)nif);
+  double *ck = xmalloc(sizeof(double) * (size_t)nif);
+  int k = 0;
+  for (int i = lo; i < hi; i++)
+    if (flag[i]) cand[k] = i, ck[k] = sender_lower ? -fv[i] : fv[i], k++;
+  stable_sort_by_pos_key(cand, ck, nif);
+  /* membership after the move (1 = lower), reusing flag */
+  for (int i = 0; i < n; i++) flag[i] = i < nl;
+  for (int j = 0; j < scnt; j++) flag[cand[j]] = !sender_lower;
+  int *o2 = xmalloc(sizeof(int) * (size_t)n);
+  double *f2 = xmalloc(sizeof(double) * (size_t)n);
+  int a = 0;
+  for (int i = 0; i < n; i++)
+    if (flag[i]) o2[a] = el[i], f2[a] = fv[i], a++;
+  const int nl2 = a;
+  for (int i = 0; i < n; i++)
+    if (!flag[i]) o2[a] = el[i], f2[a] = fv[i], a++;
+  memcpy(el, o2, sizeof(int) * (size_t)n);
+  memcpy(fv, f2, sizeof(double) * (size_t)n);
+  free(o2), free(f2), free(cand), free(ck), free(flag);
+  return nl2;
+}
+
+/* ------------------------------------------------------------------------ */
 typedef struct {
   int start, n, p, r0;
 } seg_t;
@@ -1103,8 +1149,6 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   while ((1 << ncuts) < P) ncuts++;
   if (rep) memset(rep, 0, sizeof(*rep)), rep->nlevels = ncuts;
   double *fv = xmalloc(sizeof(double) * (size_t)(nel ? nel : 1));
-  int *flag = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
-  int *tmp = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
   int *segof = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
 
   for (int cut = 0; cut < ncuts; cut++) {
@@ -1159,36 +1203,9 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       if (nl > expl) sender_lower = 1, scnt = nl - expl;
       else if (n - nl > expu) sender_lower = 0, scnt = (n - nl) - expu;
       if (scnt > 0) {
-        interface_flags(vtx, nv, el, n, nl, sender_lower, flag);
-        int lo = sender_lower ? 0 : nl, hi = sender_lower ? nl : n;
-        int nif = 0;
-        for (int i = lo; i < hi; i++) nif += flag[i];
-        if (nif >= scnt) {
-          /* Candidates ordered by mul*fiedler, ties by position (rsb.c:168-187):
-           * the upper side offers its smallest values, the lower side its
-           * largest. */
-          int *cand = xmalloc(sizeof(int) * (size_t)nif);
-          double *ck = xmalloc(sizeof(double) * (size_t)nif);
-          int k = 0;
-          for (int i = lo; i < hi; i++)
-            if (flag[i]) cand[k] = i, ck[k] = sender_lower ? -fv[i] : fv[i], k++;
-          stable_sort_by_pos_key(cand, ck, nif);
-          /* moving: mark membership (1 = lower) */
-          for (int i = 0; i < n; i++) tmp[i] = i < nl;
-          for (int j = 0; j < scnt; j++) tmp[cand[j]] = !sender_lower;
-          /* stable partition in the current (sorted) order */
-          int *o2 = xmalloc(sizeof(int) * (size_t)n);
-          int a = 0;
-          for (int i = 0; i < n; i++)
-            if (tmp[i]) o2[a++] = el[i];
-          int nl2 = a;
-          for (int i = 0; i < n; i++)
-            if (!tmp[i]) o2[a++] = el[i];
-          memcpy(el, o2, sizeof(int) * (size_t)n);
-          free(o2), free(cand), free(ck);
-          nl = nl2;
-          nbal++;
-        }
+        int nl2 = oracle_balance_step(vtx, nv, el, fv, n, nl, sender_lower, scnt);
+        if (nl2 != nl) nbal++;
+        nl = nl2;
       }
       nsegs[nn].start = sg.start, nsegs[nn].n = nl, nsegs[nn].p = (p + 1) / 2;
       nsegs[nn].r0 = sg.r0, nn++;
@@ -1216,7 +1233,7 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   }
   for (int s = 0; s < nseg; s++)
     for (int i = 0; i < segs[s].n; i++) part[order[segs[s].start + i]] = segs[s].r0;
-  free(coord), free(order), free(segs), free(nsegs), free(fv), free(flag), free(tmp),
+  free(coord), free(order), free(segs), free(nsegs), free(fv),
       free(segof);
   return 0;
 }
@@ -1228,7 +1245,8 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
  * chunk ranks, 532-541) and tag number t adds t * chunk to its part ids
  * (543-547).  Returns 3 when nparts is not a multiple of ntags. */
 typedef struct {
-  int tag, idx;
+  unsigned tag;
+  int idx;
 } ti_t;
 
 static int cmp_ti(const void *a, const void *b) {
@@ -1241,7 +1259,7 @@ int oracle_part_mesh_tagged(int *part, const long long *vtx, const double *xyz,
                             const int *tag, int nel, int nv, const oracle_opts *o) {
   if (nel <= 0) return 0;
   ti_t *t = xmalloc(sizeof(ti_t) * (size_t)nel);
-  for (int i = 0; i < nel; i++) t[i].tag = tag[i], t[i].idx = i;
+  for (int i = 0; i < nel; i++) t[i].tag = (unsigned)tag[i], t[i].idx = i;
   qsort(t, (size_t)nel, sizeof(ti_t), cmp_ti);
   int ntags = 1;
   for (int i = 1; i < nel; i++) ntags += t[i].tag != t[i - 1].tag;

@@ -153,6 +153,14 @@ int oracle_mg_project(const long long *vtx, int nv, const int *elems, int n, int
  * adjacency (components.c:160-328 returns the same count). */
 int oracle_components(const long long *vtx, int nv, const int *elems, int n);
 
+/* One balance step (rsb.c:118-229) on a segment in ascending Fiedler order
+ * (elements el, values fv, lower half el[0..nl)): the sending side hands
+ * scnt of its interface elements nearest the cut to the other side if it has
+ * that many; both arrays are permuted in place, each half keeping its
+ * members' order.  Returns the new lower-half size. */
+int oracle_balance_step(const long long *vtx, int nv, int *el, double *fv, int n, int nl,
+                        int sender_lower, int64_t scnt);
+
 /* Full partitioner: part[nel] in [0, nparts).  Returns 0 on success. */
 int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel,
                      int nv, const oracle_opts *opts, oracle_report *rep);

@@ -100,6 +100,8 @@ def _declare(L):
     L.oracle_mg_project.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double, P, P]
     L.oracle_mg_project.restype = C.c_int
     L.oracle_components.restype = C.c_int
+    L.oracle_balance_step.argtypes = [P, C.c_int, P, P, C.c_int, C.c_int, C.c_int, C.c_int64]
+    L.oracle_balance_step.restype = C.c_int
     L.oracle_part_mesh.argtypes = [P, P, P, C.c_int, C.c_int, C.POINTER(Opts),
                                    C.POINTER(Report)]
     L.oracle_part_mesh.restype = C.c_int
@@ -246,6 +248,18 @@ def components(vtx, nv, elems):
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     elems = np.ascontiguousarray(elems, dtype=np.int32)
     return lib().oracle_components(_p(vtx), nv, _p(elems), elems.size)
+
+
+def balance_step(vtx, nv, elems, fied, nl, sender_lower, scount):
+    """One balance step (rsb.c:118-229) on a segment in ascending Fiedler
+    order: returns (new element order, permuted Fiedler values, new lower
+    size)."""
+    vtx = np.ascontiguousarray(vtx, dtype=np.int64)
+    el = np.ascontiguousarray(elems, dtype=np.int32).copy()
+    fv = np.ascontiguousarray(fied, dtype=np.float64).copy()
+    nl2 = lib().oracle_balance_step(_p(vtx), nv, _p(el), _p(fv), el.size, int(nl),
+                                    int(sender_lower), int(scount))
+    return el, fv, nl2
 
 
 def fiedler_mg(vtx, nv, elems, max_iter=50, max_passes=50, tol=1e-5, factor=2):

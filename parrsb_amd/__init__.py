@@ -58,7 +58,9 @@ class Report(C.Structure):
                 ("resid_med", C.c_double * ML), ("niter_mean", C.c_double * ML),
                 ("level_ms", C.c_double * ML), ("nfallback", C.c_int * ML),
                 ("warm_start", C.c_int), ("nbrs_min", C.c_int * ML),
-                ("nbrs_max", C.c_int * ML), ("nbrs_mean", C.c_double * ML)]
+                ("nbrs_max", C.c_int * ML), ("nbrs_mean", C.c_double * ML),
+                ("tol_init", C.c_double * ML), ("tol_target", C.c_double * ML),
+                ("tol_final", C.c_double * ML)]
 
     def as_dict(self):
         d = {}
@@ -102,6 +104,8 @@ SIGNATURES = {
     "parrsb_amd_warm_start_nparams": (I, []),
     "parrsb_part_mesh_seq": (I, [P, P, P, P, P, I, I, C.POINTER(Options), I]),
     "parrsb_conn_mesh": (I, [P, P, C.c_uint, C.c_uint, P, I, C.c_double, I]),
+    "fparrsb_conn_mesh_": (None, [P, P, P, P, P, P, P, P, P]),
+    "fparrsb_conn_mesh": (None, [P, P, P, P, P, P, P, P, P]),
     "parrsb_amd_conn_mesh_dev": (I, [P, P, I, I, C.c_double]),
     "parrsb_amd_set_warm_start": (I, [P, I, I]),
     "parrsb_amd_partition_quality": (I, [P, P, I, I, I, P]),
@@ -142,6 +146,7 @@ SIGNATURES = {
     "rsbgpu_write_part": (I, [P, P, I]),
     "rsbgpu_get_order": (I, [P, P]),
     "rsbgpu_get_fiedler": (I, [P, P]),
+    "rsbgpu_set_fiedler": (I, [P, P]),
     "rsbgpu_get_level_csr": (I, [P, P, P, P, P, P, P]),
     "rsbgpu_get_graph": (I, [P, P, P, P]),
     "rsbgpu_get_centroids": (I, [P, P]),
@@ -547,6 +552,11 @@ class Device:
         f = np.zeros(self.nel)
         self._chk(lib().rsbgpu_get_fiedler(self.h, _p(f)))
         return f
+
+    def set_fiedler_values(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == self.nel
+        self._chk(lib().rsbgpu_set_fiedler(self.h, _p(f)))
 
     def write_part(self):
         part = np.zeros(self.nel, dtype=np.int32)

@@ -471,14 +471,16 @@ extern "C" int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device) {
 
 // ---------------------------------------------------------------------------
 // Tagged partitioning (parrsb.c:379-561): the elements of each distinct tag
-// become one group, groups in ascending tag order, elements of a group in
-// their current order.  The (order-preserving) unsigned image of the tag is
-// the key of a stable radix sort of the element order; group heads are the
-// positions where the sorted key changes.
+// become one group, groups in ascending order of the tag read as an unsigned
+// 32-bit value (the reference keeps it in a uint field and sorts that,
+// parrsb.c:388-402, so negative tags come after every non-negative one),
+// elements of a group in their current order.  The tag's bits are the key of
+// a stable radix sort of the element order; group heads are the positions
+// where the sorted key changes.
 __global__ void k_tag_keys(int nel, const int *__restrict__ order, const int *__restrict__ tag,
                            uint32_t *__restrict__ key) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nel) key[i] = (uint32_t)tag[order[i]] ^ 0x80000000u;
+  if (i < nel) key[i] = (uint32_t)tag[order[i]];
 }
 
 __global__ void k_tag_heads(int nel, const uint32_t *__restrict__ key, int cap,

@@ -498,6 +498,15 @@ extern "C" int rsbgpu_get_fiedler(rsbgpu *g, double *f) {
   return 0;
 }
 
+extern "C" int rsbgpu_set_fiedler(rsbgpu *g, const double *f) {
+  RSB_TRY(hipSetDevice(g->device));
+  if (g->nel > 0 && !g->d_fied) RSB_FAIL(3, "no mesh set");
+  RSB_TRY(hipMemcpyAsync(g->d_fied, f, sizeof(double) * (size_t)g->nel, hipMemcpyHostToDevice,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
 extern "C" int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out) {
   RSB_TRY(hipSetDevice(g->device));
   int *d = nullptr;

@@ -212,9 +212,9 @@ static void check_partition(const parrsb_amd_report *r, const parrsb_options *o)
               o->rsb_max_passes, l);
     else if (r->nconverged_fail[l] > 0)
       fprintf(stderr,
-              "Warning: Lanczos did not reach the target residual after %d x %d iterations "
-              "in Level=%d (%d segments)!\n",
-              o->rsb_max_passes, o->rsb_max_iter, l, r->nconverged_fail[l]);
+              "Warning: Lanczos reached a residual of %lf (target: %lf) after %d x %d "
+              "iterations in Level=%d!\n",
+              r->tol_final[l], r->tol_target[l], o->rsb_max_passes, o->rsb_max_iter, l);
     if (o->find_disconnected_comps == 1 && r->ncomp_max[l] > 1)
       fprintf(stderr,
               "Warning: Partition created %d/%d (min/max) disconnected components in "
@@ -280,6 +280,14 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, con
   if (nparts < 1) {
     set_error("number of parts must be >= 1");
     return 3;
+  }
+  if (nel == 0) {
+    /* nothing to partition: no device work and no output (in tagged mode the
+       reference would count 0 tags and divide by that, parrsb.c:436) */
+    if (tag) options->verbose_level = 0, options->profile_level = 0;
+    last_report.nparts = nparts, last_report.precision = precision_bits();
+    last_report.seconds = now() - t0;
+    return 0;
   }
   const char *wf = getenv("PARRSB_AMD_WARM_START");
   if (wf && *wf) {
@@ -463,6 +471,25 @@ int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
   if (verbose > 0 && rc == 0)
     printf("parCon (tol = %e) finished in %g s\n", tol, now() - t0), fflush(stdout);
   return rc;
+}
+
+void fparrsb_conn_mesh_(long long *vtx, double *coord, int *nel, int *nDim,
+                        long long *periodicInfo, int *nPeriodicFaces, double *tol,
+                        MPI_Fint *fcomm, int *err) {
+  *err = 1;
+  if (*nel < 0) {
+    set_error("nel must be >= 0");
+    *err = 3;
+    return;
+  }
+  *err = parrsb_conn_mesh(vtx, coord, (unsigned)*nel, (unsigned)*nDim, periodicInfo,
+                          *nPeriodicFaces, *tol, MPI_Comm_f2c(*fcomm));
+}
+
+void fparrsb_conn_mesh(long long *vtx, double *coord, int *nel, int *nDim,
+                       long long *periodicInfo, int *nPeriodicFaces, double *tol,
+                       MPI_Fint *fcomm, int *err) {
+  fparrsb_conn_mesh_(vtx, coord, nel, nDim, periodicInfo, nPeriodicFaces, tol, fcomm, err);
 }
 
 int parrsb_amd_conn_mesh_dev(long long *d_vtx, const double *d_coord, int nel, int ndim,

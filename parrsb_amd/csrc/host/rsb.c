@@ -123,9 +123,10 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
   double *lam = (double *)calloc((size_t)P, sizeof(double));
   double *rn = (double *)calloc(3 * (size_t)P, sizeof(double));
   double *res = (double *)calloc((size_t)P, sizeof(double));
+  double *rsort = (double *)calloc((size_t)P, sizeof(double));
   int *fb = (int *)calloc((size_t)P, sizeof(int)), *nbr = (int *)calloc((size_t)P, sizeof(int));
   if (!ok || !nl || !snd || !scnt || !moved || !ncomp || !niter || !npass || !lam || !rn || !res ||
-      !fb || !nbr) {
+      !rsort || !fb || !nbr) {
     rc = 2;
     goto done;
   }
@@ -166,6 +167,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     int lv = cut < PARRSB_AMD_MAX_LEVELS ? cut : PARRSB_AMD_MAX_LEVELS - 1;
     int itmin = 1 << 30, itmax = 0, pmax = 0, cmin = 1 << 30, cmax = 0, nb = 0, ncut = 0, nfail = 0;
     double lmin = DBL_MAX, lmax = -DBL_MAX, rmax = 0.0, work = 0.0, nwork = 0.0;
+    double t_init = DBL_MAX, t_tgt = DBL_MAX, t_fnl = DBL_MAX;
     int nres = 0;
     for (int s = 0; s < nseg; s++) {
       if (A.p[s] <= 1) continue;
@@ -178,17 +180,22 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
         lmax = lam[s] > lmax ? lam[s] : lmax;
         if (prm->residuals) {
           if (res[s] > rmax) rmax = res[s];
-          rn[nres++] = res[s]; /* rn is free after the Lanczos call */
+          rsort[nres++] = res[s];
         }
         work += (double)niter[s] * A.n[s];
         nwork += A.n[s];
-        int mi = prm->max_iter < A.n[s] ? prm->max_iter : A.n[s];
-        /* iterations of the reference-start solve (a fallback counts the
-           learned attempt's fb[s] - 1 iterations in front of them) */
-        if (prm->rsb_algo == 1) {
-          if (niter[s] == prm->max_passes) nfail++; /* the test of rsb.c:70-71 */
-        } else if (niter[s] - (fb[s] ? fb[s] - 1 : 0) == mi * prm->max_passes) {
+        /* the test of rsb.c:66-71 on the iterations of the reference-start
+           solve (a fallback counts the learned attempt's fb[s] - 1
+           iterations in front of them): rsb_max_iter * rsb_max_passes for
+           Lanczos, rsb_max_passes for the inverse iteration */
+        int fail;
+        if (prm->rsb_algo == 1) fail = niter[s] == prm->max_passes;
+        else fail = niter[s] - (fb[s] > 0 ? fb[s] - 1 : 0) == prm->max_iter * prm->max_passes;
+        if (fail) {
           nfail++;
+          if (rn[3 * s] < t_init) t_init = rn[3 * s];
+          if (rn[3 * s + 1] < t_tgt) t_tgt = rn[3 * s + 1];
+          if (rn[3 * s + 2] < t_fnl) t_fnl = rn[3 * s + 2];
         }
       }
       if (prm->find_comps) {
@@ -211,9 +218,11 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     rep->lambda_min[lv] = ncut ? lmin : 0.0, rep->lambda_max[lv] = ncut ? lmax : 0.0;
     rep->resid_max[lv] = rmax;
     if (nres > 0) {
-      qsort(rn, (size_t)nres, sizeof(double), cmp_dbl);
-      rep->resid_med[lv] = rn[nres / 2];
+      qsort(rsort, (size_t)nres, sizeof(double), cmp_dbl);
+      rep->resid_med[lv] = rsort[nres / 2];
     }
+    if (nfail > 0 && prm->rsb_algo == 0)
+      rep->tol_init[lv] = t_init, rep->tol_target[lv] = t_tgt, rep->tol_final[lv] = t_fnl;
     rep->niter_mean[lv] = nwork > 0 ? work / nwork : 0.0;
     if (prm->verbose > 1)
       printf("rsb: cut %d/%d segments=%d niter=%d..%d ncomp=%d..%d balanced=%d\n", cut + 1,
@@ -241,7 +250,7 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
 done:
   segtab_free(&A), segtab_free(&B);
   free(nl), free(snd), free(scnt), free(moved), free(ncomp), free(niter), free(npass), free(lam),
-      free(rn), free(res), free(fb), free(nbr);
+      free(rn), free(res), free(rsort), free(fb), free(nbr);
   return rc;
 }
 

@@ -26,7 +26,7 @@ def header_functions():
         txt = re.sub(r"#.*", "", txt)
         for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", txt):
             name = m.group(1)
-            if name.startswith(("parrsb_", "rsbgpu_")):
+            if name.startswith(("parrsb_", "rsbgpu_", "fparrsb_")):
                 names.add(name)
     return names
 
@@ -118,3 +118,39 @@ def test_oracle_is_not_linked_by_the_product():
     assert "oracle_" not in nm
     ldd = subprocess.run(["ldd", pa.LIB_PATH], capture_output=True, text=True).stdout
     assert "liboracle" not in ldd
+
+
+def test_empty_tagged_input_returns_without_device(monkeypatch):
+    """nel = 0 in tagged mode: the reference counts 0 tags and divides by
+    that (parrsb.c:436); here the call returns 0 with nothing to write, and
+    still zeroes verbose/profile on the caller's struct (parrsb.c:535-536)."""
+    for k in list(os.environ):
+        if k.startswith("PARRSB_"):
+            monkeypatch.delenv(k)
+    opts = pa.default_options(verbose_level=1, profile_level=1, nparts=4, tagged=1)
+    part = np.zeros(1, dtype=np.int32)
+    vtx = np.zeros(8, dtype=np.int64)
+    xyz = np.zeros(24)
+    tag = np.zeros(1, dtype=np.int32)
+    rc = pa.lib().parrsb_part_mesh(pa._p(part), pa._p(vtx), pa._p(xyz), pa._p(tag), 0, 8,
+                                   C.byref(opts), 0)
+    assert rc == 0
+    assert opts.verbose_level == 0 and opts.profile_level == 0
+
+
+def test_fortran_conn_mesh_binding_checks_arguments(monkeypatch):
+    """fparrsb_conn_mesh (con.c:297-307): arguments by reference, the return
+    code in *err; a bad dimension is rejected before any device work."""
+    monkeypatch.setenv("PARRSB_VERBOSE_LEVEL", "0")
+    vtx = np.zeros(8, dtype=np.int64)
+    xyz = np.zeros(24)
+    ints = {k: C.c_int(v) for k, v in dict(nel=1, ndim=5, npf=0, comm=0, err=-7).items()}
+    tol = C.c_double(0.2)
+    for name in ("fparrsb_conn_mesh_", "fparrsb_conn_mesh"):
+        fn = getattr(pa.lib(), name)
+        fn.restype = None
+        ints["err"].value = -7
+        fn(pa._p(vtx), pa._p(xyz), C.byref(ints["nel"]), C.byref(ints["ndim"]), None,
+           C.byref(ints["npf"]), C.byref(tol), C.byref(ints["comm"]), C.byref(ints["err"]))
+        assert ints["err"].value == 3
+        assert "nDim" in pa.last_error()

@@ -89,6 +89,25 @@ def test_device_ids_equal_oracle(oracle, name):
 
 
 @pytest.mark.gpu
+def test_fortran_binding_equals_c_entry(oracle):
+    """fparrsb_conn_mesh (con.c:297-307), the entry Nek5000 calls with every
+    argument by reference, gives the C entry's ids and return code."""
+    import ctypes as C
+    import parrsb_amd as pa
+    vtx, xyz, ndim = known_mesh("pipe")
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nel = xyz.shape[0]
+    out = np.zeros((nel, 8), dtype=np.int64)
+    a = [C.c_int(nel), C.c_int(ndim), C.c_int(0), C.c_int(0), C.c_int(-1)]
+    tol = C.c_double(0.2)
+    pa.lib().fparrsb_conn_mesh_(pa._p(out), pa._p(xyz), C.byref(a[0]), C.byref(a[1]), None,
+                                C.byref(a[2]), C.byref(tol), C.byref(a[3]), C.byref(a[4]))
+    assert a[4].value == 0
+    rc, ref = oracle.conn_mesh(xyz, ndim, 0.2)
+    assert np.array_equal(out, ref) and np.array_equal(out, pa.conn_mesh(xyz, ndim, 0.2))
+
+
+@pytest.mark.gpu
 def test_device_checks_and_partition_of_generated_ids(oracle):
     import parrsb_amd as pa
     vtx, xyz, nv = M.box_hex(4, 3, 3, jitter=0.0)

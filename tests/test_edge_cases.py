@@ -1,6 +1,7 @@
 """Edge cases of parrsb_part_mesh on the device against the oracle: an empty
 mesh, one part, more parts than elements, no coordinates (rsb_pre forced to
-0, parrsb.c:250), and the device-pointer entry point against the host one."""
+0, parrsb.c:250), the device-pointer entry point against the host one, and a
+Lanczos forced not to converge (the warning of rsb.c:61-99)."""
 import numpy as np
 import pytest
 
@@ -62,3 +63,33 @@ def test_device_entry_point_equals_host_entry_point():
             for p in (dv, dx, dp):
                 pa.Device.free(p)
     assert np.array_equal(dev, host)
+
+
+def test_non_convergence_warning_and_residuals(oracle, monkeypatch, capfd):
+    """PARRSB_RSB_MAX_ITER=2, PARRSB_RSB_MAX_PASSES=1: every segment stops at
+    2 x 1 iterations without meeting rsb_tol, so every level reports the
+    reference's warning with the minimum final residual and target over its
+    segments (rsb.c:79-95; TOL_TGT / TOL_FNL of fiedler.c:451, 506); on the
+    first cut (one segment, FP64) both equal the oracle's."""
+    flush("forced non-convergence")
+    monkeypatch.setenv("PARRSB_RSB_MAX_ITER", "2")
+    monkeypatch.setenv("PARRSB_RSB_MAX_PASSES", "1")
+    vtx, xyz, nv = M.box_hex(12, 10, 8, jitter=0.1, seed=6)
+    n = vtx.shape[0]
+    part, opts = pa.part_mesh(vtx, xyz, nv, 4)
+    r = pa.last_report()
+    assert (opts.rsb_max_iter, opts.rsb_max_passes) == (2, 1)
+    assert r["niter_min"] == r["niter_max"] == [2, 2]
+    assert r["nconverged_fail"] == [1, 2]
+    err = capfd.readouterr().err
+    for lv in range(2):
+        assert 0 < r["tol_target"][lv] < r["tol_final"][lv]
+        msg = ("Warning: Lanczos reached a residual of %f (target: %f) after 1 x 2 iterations "
+               "in Level=%d!" % (r["tol_final"][lv], r["tol_target"][lv], lv))
+        assert msg in err, (msg, err)
+    order = oracle.rcb_order(oracle.centroids(xyz, nv), 3, np.arange(n, dtype=np.int32), 4)
+    _, it, _, rn = oracle.fiedler(vtx, nv, order, max_iter=2, max_passes=1)
+    assert it == 2
+    assert abs(r["tol_init"][0] - rn[0]) <= 1e-9 * rn[0]
+    assert abs(r["tol_target"][0] - rn[1]) <= 1e-9 * rn[1]
+    assert abs(r["tol_final"][0] - rn[2]) <= 1e-9 * rn[2]

@@ -403,6 +403,115 @@ def test_part_mesh_uneven_parts_balance(oracle):
     assert rep["nbalanced"] == [1, 1]
 
 
+def brute_balance(vtx, elems, fied, nl, sender_lower, scount):
+    """The balance step of rsb.c:118-229 by sets: the sender's interface
+    elements are those sharing a vertex id with any element of the other
+    half; if there are at least scount of them, the scount nearest the cut
+    change side (upper side: smallest Fiedler values, lower side: largest;
+    equal values in position order), and each half keeps the positions of
+    its members in order."""
+    n = len(elems)
+    V = [set(int(v) for v in vtx[e]) for e in elems]
+    snd = range(0, nl) if sender_lower else range(nl, n)
+    rcv = range(nl, n) if sender_lower else range(0, nl)
+    touched = set()
+    for i in rcv:
+        touched |= V[i]
+    iface = [i for i in snd if V[i] & touched]
+    if scount <= 0 or len(iface) < scount:
+        return np.asarray(elems), np.asarray(fied), nl, iface
+    key = (lambda i: (-fied[i], i)) if sender_lower else (lambda i: (fied[i], i))
+    move = set(sorted(iface, key=key)[:scount])
+    low = [(i < nl) != (i in move) for i in range(n)]
+    new = [i for i in range(n) if low[i]] + [i for i in range(n) if not low[i]]
+    return np.asarray(elems)[new], np.asarray(fied)[new], sum(low), iface
+
+
+def _sorted_segment(oracle, vtx, xyz, nv, n, p_rcb=4):
+    """A connected-ish segment: the first n elements of the mesh's RCB order,
+    in ascending order of their Fiedler values (the state rsb.c:308 leaves)."""
+    ndim = 3 if nv == 8 else 2
+    c = oracle.centroids(xyz, nv)
+    order = oracle.rcb_order(c, ndim, np.arange(vtx.shape[0], dtype=np.int32), p_rcb)[:n]
+    order = np.sort(order).astype(np.int32)
+    order = oracle.rcb_order(c, ndim, order, 2)
+    f, _, _, _ = oracle.fiedler(vtx, nv, order)
+    s = np.argsort(f, kind="stable")
+    return order[s], f[s]
+
+
+@pytest.mark.parametrize("case", ["holed_p64", "hex_p13", "holed_lower_sends", "ties"])
+def test_balance_step_matches_brute_force(oracle, case):
+    """oracle_balance_step against the set-based restatement above, on
+    segments where many elements must move (n = 64q + 32 over 64 ranks: the
+    upper half is 16 over; 13 ranks with remainder 6: 3 over), on a lower
+    sender (which the rank arithmetic never produces, but the step takes),
+    and on Fiedler values with many exact ties (position order decides)."""
+    if case.startswith("holed"):
+        vtx, xyz, nv = M.holed_quads(72, 60, seed=9, frac=0.15, rmin=2, rmax=6)
+    else:
+        vtx, xyz, nv = M.box_hex(14, 12, 10, jitter=0.1, seed=3)
+    nel = vtx.shape[0]
+    if case == "holed_p64":
+        p = 64
+        n = (nel - 32) // 64 * 64 + 32
+    elif case == "hex_p13":
+        p = 13
+        n = (nel - 6) // 13 * 13 + 6
+    else:
+        p = 2
+        n = nel - nel % 7
+    el, f = _sorted_segment(oracle, vtx, xyz, nv, n)
+    if case == "ties":
+        f = np.round(f * 8) / 8  # a handful of distinct values, runs of ties
+        s = np.argsort(f, kind="stable")
+        el, f = el[s], f[s]
+    nl = oracle.lower_count(n, p)
+    e0, e1 = oracle.balance_expected(n, p, 0), oracle.balance_expected(n, p, 1)
+    if case in ("holed_p64", "hex_p13"):
+        assert nl < e0 and n - nl > e1  # the upper half sends
+        snd, sc = 0, (n - nl) - e1
+        assert sc == {"holed_p64": 16, "hex_p13": 3}[case]
+    elif case == "holed_lower_sends":
+        nl, snd, sc = n // 2 + 40, 1, 40
+    else:
+        nl, snd, sc = n // 2 - 25, 0, 25
+    got_el, got_f, got_nl = oracle.balance_step(vtx, nv, el, f, nl, snd, sc)
+    ref_el, ref_f, ref_nl, iface = brute_balance(vtx, el, f, nl, snd, sc)
+    assert len(iface) >= sc  # something moves
+    assert got_nl == ref_nl == (nl - sc if snd else nl + sc)
+    assert np.array_equal(got_el, ref_el) and np.array_equal(got_f, ref_f)
+    # the movers are interface elements of the sender and nothing else moved
+    moved = set(got_el[:got_nl]) ^ set(el[:nl])
+    assert len(moved) == sc and moved <= set(el[iface])
+    # too few interface elements: nothing moves (rsb.c:194, 221-224)
+    big = len(iface) + 1
+    o2, f2, nl2 = oracle.balance_step(vtx, nv, el, f, nl, snd, big)
+    assert nl2 == nl and np.array_equal(o2, el) and np.array_equal(f2, f)
+
+
+def test_part_mesh_moves_many_elements(oracle):
+    """A whole partition whose first cut must move 16 elements (n = 64q + 32
+    into 64 parts): every cut ends at the balanced sizes and the first level
+    reports a move."""
+    vtx, xyz, nv = M.holed_quads(72, 60, seed=9, frac=0.15, rmin=2, rmax=6)
+    n = (vtx.shape[0] - 32) // 64 * 64 + 32
+    vtx, xyz = np.ascontiguousarray(vtx[:n]), np.ascontiguousarray(xyz[:n])
+    part, rep = oracle.part_mesh(vtx, xyz, nv, 64)
+    assert rep["nbalanced"][0] == 1
+
+    def rec(n, p, out):
+        if p == 1:
+            out.append(n)
+            return
+        nl = oracle.balance_expected(n, p, 0)
+        rec(nl, (p + 1) // 2, out)
+        rec(n - nl, p // 2, out)
+    sizes = []
+    rec(n, 64, sizes)
+    assert np.bincount(part, minlength=64).tolist() == sizes
+
+
 def test_quality_matches_incidence(oracle):
     rng = np.random.default_rng(2)
     vtx, xyz, nv = M.box_hex(6, 5, 4)

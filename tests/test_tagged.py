@@ -1,6 +1,7 @@
 """Tagged partitioning (parrsb.c:379-561): every distinct tag is partitioned
 on its own into nparts / ntags parts, the i-th smallest tag taking parts
-[i * chunk, (i + 1) * chunk).
+[i * chunk, (i + 1) * chunk).  Tags are ordered as unsigned 32-bit values (the
+reference sorts a uint field, parrsb.c:388-402): -3 comes after 100.
 
 The oracle's restatement is pinned on meshes made of disjoint copies of one
 box: each copy's tagged parts must be the untagged partition of the box
@@ -41,8 +42,9 @@ def layered(nx, ny, nz, per_layer, seed=5):
 
 def test_oracle_tagged_copies_are_shifted_untagged_partitions(oracle):
     V, X, nv, copy, (vtx, xyz) = copies(3, 6, 5, 4)
-    tags = np.array([7, -3, 100], dtype=np.int32)  # ranks: -3 -> 0, 7 -> 1, 100 -> 2
-    rank = {0: 1, 1: 0, 2: 2}
+    # unsigned order: 7 -> 0, 100 -> 1, -3 (0xfffffffd) -> 2
+    tags = np.array([7, -3, 100], dtype=np.int32)
+    rank = {0: 0, 1: 2, 2: 1}
     chunk = 4
     part = oracle.part_mesh_tagged(V, X, nv, 3 * chunk, tags[copy])
     alone, _ = oracle.part_mesh(vtx, xyz, nv, chunk)
@@ -86,7 +88,7 @@ def test_tagged_copies_match_oracle(oracle):
     part, _ = pa.part_mesh(V, X, nv, P, tag=tag, tagged=1)
     ref = oracle.part_mesh_tagged(V, X, nv, P, tag)
     assert np.array_equal(np.bincount(part, minlength=P), np.bincount(ref, minlength=P))
-    for c, r in ((0, 1), (1, 0), (2, 2)):
+    for c, r in ((0, 0), (1, 2), (2, 1)):  # tags 7, -3, 100 in unsigned order
         sel = copy == c
         assert set(np.unique(part[sel])) == set(range(4 * r, 4 * r + 4))
         q = pa.quality(part[sel] - 4 * r, V[sel], nv, 4)
@@ -153,6 +155,6 @@ def test_group_by_tag_is_a_stable_sort():
         order = d.order()
         with pytest.raises(ValueError):
             d.group_by_tag(tag, 4)
-    ref = np.argsort(tag, kind="stable")
+    ref = np.argsort(tag.view(np.uint32), kind="stable")
     assert np.array_equal(order, ref)
-    assert np.array_equal(counts, np.unique(tag, return_counts=True)[1])
+    assert np.array_equal(counts, np.unique(tag.view(np.uint32), return_counts=True)[1])
