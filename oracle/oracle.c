@@ -16,7 +16,32 @@
 #define M_PI 3.141592653589793
 #endif
 
+/* The same source built with -fopenmp -DORACLE_THREADS (liboracle_mt.so)
+ * times the CPU baseline on all host cores: while a level has fewer
+ * segments than threads, the vector loops and the Laplacian of each segment
+ * are split over the threads (the reference's ranks sharing a segment); once
+ * it has at least as many, whole segments run on their own threads.  Sums
+ * then depend on the thread count, so the parity checks use the serial
+ * build, where OMP() expands to nothing. */
+#ifdef ORACLE_THREADS
+#include <omp.h>
+#define OMP(x) _Pragma(#x)
+#else
+#define OMP(x)
+#endif
+#define ORACLE_PAR_MIN 65536
+
 void oracle_free(void *p) { free(p); }
+
+int oracle_threads(int n) {
+#ifdef ORACLE_THREADS
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 
 static void *xmalloc(size_t n) {
   void *p = malloc(n ? n : 1);
@@ -307,6 +332,7 @@ typedef struct {
   int n, nv, ng;
   int *group;   /* per slot */
   int *gstart, *gslots;
+  int *gel;     /* element of each slot, in group order (gslots / nv) */
   double *diag; /* per element */
   double *gsum; /* per group scratch */
 } gs_lap;
@@ -315,6 +341,8 @@ static void gs_lap_init(gs_lap *L, const long long *vtx, int nv, const int *elem
   L->n = n, L->nv = nv;
   L->group = xmalloc(sizeof(int) * (size_t)n * nv);
   L->ng = group_slots(vtx, nv, elems, n, L->group, &L->gstart, &L->gslots);
+  L->gel = xmalloc(sizeof(int) * (size_t)n * nv);
+  for (int k = 0; k < n * nv; k++) L->gel[k] = L->gslots[k] / nv;
   L->diag = xmalloc(sizeof(double) * (size_t)n);
   L->gsum = xmalloc(sizeof(double) * (size_t)L->ng);
   for (int i = 0; i < n; i++) {
@@ -328,12 +356,14 @@ static void gs_lap_init(gs_lap *L, const long long *vtx, int nv, const int *elem
 }
 
 static void gs_lap_apply(const gs_lap *L, const double *u, double *v) {
-  int nv = L->nv;
+  const int nv = L->nv;
+  OMP(omp parallel for if (L->n > ORACLE_PAR_MIN))
   for (int g = 0; g < L->ng; g++) {
     double s = 0.0;
-    for (int k = L->gstart[g]; k < L->gstart[g + 1]; k++) s += u[L->gslots[k] / nv];
+    for (int k = L->gstart[g]; k < L->gstart[g + 1]; k++) s += u[L->gel[k]];
     L->gsum[g] = s;
   }
+  OMP(omp parallel for if (L->n > ORACLE_PAR_MIN))
   for (int i = 0; i < L->n; i++) {
     double r = L->diag[i] * u[i];
     for (int j = 0; j < nv; j++) r -= L->gsum[L->group[i * nv + j]];
@@ -342,7 +372,7 @@ static void gs_lap_apply(const gs_lap *L, const double *u, double *v) {
 }
 
 static void gs_lap_free(gs_lap *L) {
-  free(L->group), free(L->gstart), free(L->gslots), free(L->diag), free(L->gsum);
+  free(L->group), free(L->gstart), free(L->gslots), free(L->gel), free(L->diag), free(L->gsum);
 }
 
 void oracle_laplacian_apply(const long long *vtx, int nv, const int *elems, int n,
@@ -504,20 +534,24 @@ int oracle_tqli(const double *diag, const double *upper, int n, double *z, doubl
 /* CG-Lanczos, fiedler.c:13-30 (dot, ortho) and 434-561. */
 static double dotp(const double *y, const double *x, int n) {
   double s = 0.0;
+  OMP(omp parallel for reduction(+ : s) if (n > ORACLE_PAR_MIN))
   for (int i = 0; i < n; i++) s += x[i] * y[i];
   return s;
 }
 
 static void ortho(double *q, int n) {
   double s = 0.0;
+  OMP(omp parallel for reduction(+ : s) if (n > ORACLE_PAR_MIN))
   for (int i = 0; i < n; i++) s += q[i];
   s /= n;
+  OMP(omp parallel for if (n > ORACLE_PAR_MIN))
   for (int i = 0; i < n; i++) q[i] -= s;
 }
 
 static int lanczos_pass(double *diag, double *upper, double *rr, int n, int niter,
                         double tol, const double *f, const gs_lap *L, double *rn3) {
   double *r = xmalloc(sizeof(double) * 3 * (size_t)n), *p = r + n, *w = p + n;
+  OMP(omp parallel for if (n > ORACLE_PAR_MIN))
   for (int i = 0; i < n; i++) r[i] = f[i], p[i] = 0.0;
   ortho(r, n);
   double rtz1 = 1.0, pap = 0.0, alpha, beta, rtz2, pap_old;
@@ -525,20 +559,24 @@ static int lanczos_pass(double *diag, double *upper, double *rr, int n, int nite
   double rnorm = sqrt(rtr), rtol = rnorm * tol;
   rn3[0] = rnorm, rn3[1] = rtol;
   double rni = 1.0 / rnorm;
+  OMP(omp parallel for if (n > ORACLE_PAR_MIN))
   for (int i = 0; i < n; i++) rr[i] = r[i] * rni;
   int iter;
   for (iter = 0; iter < niter; iter++) {
     rtz2 = rtz1, rtz1 = rtr;
     beta = rtz1 / rtz2;
     if (iter == 0) beta = 0.0;
+    OMP(omp parallel for if (n > ORACLE_PAR_MIN))
     for (int i = 0; i < n; i++) p[i] = beta * p[i] + r[i];
     ortho(p, n);
     gs_lap_apply(L, p, w);
     pap_old = pap, pap = dotp(w, p, n);
     alpha = rtz1 / pap;
+    OMP(omp parallel for if (n > ORACLE_PAR_MIN))
     for (int i = 0; i < n; i++) r[i] = r[i] - alpha * w[i];
     rtr = dotp(r, r, n);
     rnorm = sqrt(rtr), rni = 1.0 / rnorm;
+    OMP(omp parallel for if (n > ORACLE_PAR_MIN))
     for (int i = 0; i < n; i++) rr[(size_t)(iter + 1) * n + i] = r[i] * rni;
     if (iter == 0) {
       diag[iter] = pap / rtz1;
@@ -573,12 +611,14 @@ static int lanczos_run(double *fied, const gs_lap *L, int n, double *initv, int 
     for (int i = 1; i < iter; i++)
       if (fabs(el[i]) < emin) emin = fabs(el[i]), imin = i;
     lam = el[imin];
+    OMP(omp parallel for if (n > ORACLE_PAR_MIN))
     for (int i = 0; i < n; i++) {
       double s = 0.0;
       for (int j = 0; j < iter; j++) s += rr[(size_t)j * n + i] * ev[imin * iter + j];
       fied[i] = s;
     }
     ortho(fied, n);
+    OMP(omp parallel for if (n > ORACLE_PAR_MIN))
     for (int i = 0; i < n; i++) initv[i] = fied[i];
   }
   free(alpha), free(rr), free(ev), free(el);
@@ -1151,43 +1191,55 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   double *fv = xmalloc(sizeof(double) * (size_t)(nel ? nel : 1));
   int *segof = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
 
+  int *coff = xmalloc(sizeof(int) * (size_t)P);
   for (int cut = 0; cut < ncuts; cut++) {
     int nn = 0;
     int itmin = 1 << 30, itmax = 0, cmin = 1 << 30, cmax = 0, nbal = 0, ncut = 0;
     double lmin = DBL_MAX, lmax = -DBL_MAX;
+    /* children slots: a final segment keeps one, a bisected one takes two */
+    for (int s = 0; s < nseg; s++) coff[s] = nn, nn += segs[s].p == 1 ? 1 : 2;
+    int nthr = 1;
+#ifdef ORACLE_THREADS
+    nthr = omp_get_max_threads();
+#endif
+    (void)nthr;
+    OMP(omp parallel for schedule(dynamic, 1) if (nseg >= nthr && nthr > 1)
+        reduction(min : itmin, cmin, lmin) reduction(max : itmax, cmax, lmax)
+        reduction(+ : nbal, ncut))
     for (int s = 0; s < nseg; s++) {
       seg_t sg = segs[s];
       if (sg.p == 1) {
-        nsegs[nn++] = sg;
+        nsegs[coff[s]] = sg;
         continue;
       }
       ncut++;
       int *el = order + sg.start, n = sg.n, p = sg.p;
+      double *fs = fv + sg.start; /* this segment's share of the scratch */
       /* pre-partition (rsb.c:282-291) */
       if (rsb_pre == 1) {
         oracle_rcb_order(coord, ndim, el, n, p);
       } else if (rsb_pre == 2) {
         oracle_rib_order(coord, ndim, el, n, p);
       } else {
-        for (int i = 0; i < n; i++) fv[i] = (double)el[i];
-        stable_sort_by_pos_key(el, fv, n);
+        for (int i = 0; i < n; i++) fs[i] = (double)el[i];
+        stable_sort_by_pos_key(el, fs, n);
       }
       /* Fiedler (fiedler.c:563-621) */
       int it = 0;
       double lam = 0.0;
       if (n >= 2 && o->rsb_algo == 1) {
         it = oracle_fiedler_mg(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol,
-                               o->mg_factor, fv, NULL);
+                               o->mg_factor, fs, NULL);
       } else if (n >= 2) {
-        it = oracle_fiedler(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol, fv, &lam,
+        it = oracle_fiedler(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol, fs, &lam,
                             NULL);
       } else {
-        for (int i = 0; i < n; i++) fv[i] = 0.0;
+        for (int i = 0; i < n; i++) fs[i] = 0.0;
       }
       itmin = it < itmin ? it : itmin, itmax = it > itmax ? it : itmax;
       lmin = lam < lmin ? lam : lmin, lmax = lam > lmax ? lam : lmax;
       /* sort by Fiedler value (rsb.c:308) */
-      stable_sort_by_pos_key(el, fv, n);
+      stable_sort_by_pos_key(el, fs, n);
       int nl = (int)oracle_lower_count(n, p);
       /* components of each half before balancing (rsb.c:320-327) */
       if (o->find_disconnected_comps) {
@@ -1203,14 +1255,13 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       if (nl > expl) sender_lower = 1, scnt = nl - expl;
       else if (n - nl > expu) sender_lower = 0, scnt = (n - nl) - expu;
       if (scnt > 0) {
-        int nl2 = oracle_balance_step(vtx, nv, el, fv, n, nl, sender_lower, scnt);
+        int nl2 = oracle_balance_step(vtx, nv, el, fs, n, nl, sender_lower, scnt);
         if (nl2 != nl) nbal++;
         nl = nl2;
       }
-      nsegs[nn].start = sg.start, nsegs[nn].n = nl, nsegs[nn].p = (p + 1) / 2;
-      nsegs[nn].r0 = sg.r0, nn++;
-      nsegs[nn].start = sg.start + nl, nsegs[nn].n = n - nl, nsegs[nn].p = p / 2;
-      nsegs[nn].r0 = sg.r0 + (p + 1) / 2, nn++;
+      seg_t *c = nsegs + coff[s];
+      c[0].start = sg.start, c[0].n = nl, c[0].p = (p + 1) / 2, c[0].r0 = sg.r0;
+      c[1].start = sg.start + nl, c[1].n = n - nl, c[1].p = p / 2, c[1].r0 = sg.r0 + (p + 1) / 2;
     }
     seg_t *t = segs;
     segs = nsegs, nsegs = t, nseg = nn;
@@ -1233,8 +1284,7 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   }
   for (int s = 0; s < nseg; s++)
     for (int i = 0; i < segs[s].n; i++) part[order[segs[s].start + i]] = segs[s].r0;
-  free(coord), free(order), free(segs), free(nsegs), free(fv),
-      free(segof);
+  free(coord), free(order), free(segs), free(nsegs), free(fv), free(segof), free(coff);
   return 0;
 }
 

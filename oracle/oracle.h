@@ -92,6 +92,10 @@ int64_t oracle_dual_graph(const long long *vtx, int nel, int nv, int64_t **rowpt
                           int **col, int **wgt);
 void oracle_free(void *p);
 
+/* Threads of the OpenMP build (liboracle_mt.so) for later calls; n <= 0
+ * leaves the count.  Returns the count in use (1 in the serial build). */
+int oracle_threads(int n);
+
 /* Apply the shared-vertex Laplacian of the sub-mesh `elems[0..n)` (the
  * gather-scatter form of laplacian.c:159-205) to u, writing v. */
 void oracle_laplacian_apply(const long long *vtx, int nv, const int *elems, int n,

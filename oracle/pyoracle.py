@@ -13,8 +13,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the same source with OpenMP: only the timed CPU baseline loads it
+LIB_MT = os.path.join(HERE, "liboracle_mt.so")
 
-_lib = None
+_libs = {}
 
 
 def build() -> str:
@@ -22,14 +24,15 @@ def build() -> str:
     return LIB
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(threaded=False):
+    path = LIB_MT if threaded else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB)
-        _declare(_lib)
-    return _lib
+        L = C.CDLL(path)
+        _declare(L)
+        _libs[path] = L
+    return _libs[path]
 
 
 class Opts(C.Structure):
@@ -76,6 +79,8 @@ def _declare(L):
                                     C.POINTER(P)]
     L.oracle_dual_graph.restype = C.c_int64
     L.oracle_free.argtypes = [P]
+    L.oracle_threads.argtypes = [C.c_int]
+    L.oracle_threads.restype = C.c_int
     L.oracle_laplacian_apply.argtypes = [P, C.c_int, P, C.c_int, P, P]
     L.oracle_tqli.argtypes = [P, P, C.c_int, P, P]
     L.oracle_tqli.restype = C.c_int
@@ -317,7 +322,13 @@ def mg_project(vtx, nv, elems, b, miter=50, tol=1e-5, factor=2):
 
 
 def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
-              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2):
+              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2, threads=0):
+    """The reference algorithm over virtual ranks.  threads > 0 runs the
+    OpenMP build on that many threads (the CPU baseline); its sums depend on
+    the thread count, so parity checks keep threads = 0 (serial build)."""
+    L = lib(threaded=threads > 0)
+    if threads > 0:
+        L.oracle_threads(int(threads))
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     nel = vtx.size // nv
     xp = None
@@ -327,7 +338,7 @@ def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
     part = np.full(nel, -1, dtype=np.int32)
     o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts, rsb_algo, mg_factor)
     rep = Report()
-    rc = lib().oracle_part_mesh(_p(part), _p(vtx), xp, nel, nv, C.byref(o), C.byref(rep))
+    rc = L.oracle_part_mesh(_p(part), _p(vtx), xp, nel, nv, C.byref(o), C.byref(rep))
     if rc != 0:
         raise RuntimeError(f"oracle_part_mesh returned {rc}")
     return part, rep.as_dict()

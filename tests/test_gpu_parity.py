@@ -290,7 +290,8 @@ def test_bisect_components_and_balance(oracle):
         order, f = d.order(), d.fiedler_values()
         nl, sc = _upper_sends(oracle, n, p)
         assert sc == 16
-        nl2, ncomp, moved = d.bisect([nl], [0], [sc], 1)
+        # rsbgpu_bisect takes the sending half's index (1 = upper)
+        nl2, ncomp, moved = d.bisect([nl], [1], [sc], 1)
         got = d.order()
     assert np.all(np.diff(f) >= 0)
     assert ncomp[0] == oracle.components(vtx, nv, order[:nl])
@@ -326,7 +327,9 @@ def test_balance_pick_with_chosen_values(oracle, case):
         d.set_segments(starts, sizes, [2, 2, 2], [0, 2, 4])
         d.level_laplacian()
         d.set_fiedler_values(f)
-        nl2, _, moved = d.bisect(nl, snd, sc, 0)
+        # the oracle's flag is "the lower half sends"; rsbgpu_bisect takes the
+        # sending half's index (0 = lower, 1 = upper)
+        nl2, _, moved = d.bisect(nl, [1 - x for x in snd], sc, 0)
         got = d.order()
     for s in range(3):
         a, m = starts[s], sizes[s]
