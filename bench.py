@@ -9,7 +9,10 @@ batched FP32 path from the reference start and the plain FP64 GPU path
 (reference arithmetic and start), and the CPU oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config warped256]
-                    [--warm-start PATH|none] [--mg]
+                    [--warm-start PATH|none] [--mg] [--compare-steps 3]
+
+Every leg times its calls without events; the roofline's kernel time comes
+from one more call with events, after the timed ones.
 """
 import argparse
 import json
@@ -49,13 +52,15 @@ def make_mesh(name):
 
 
 def run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, steps, warmup,
-              profile, warm=None):
+              warm=None, label=""):
+    """warmup untimed calls, then `steps` timed calls, event profiling off in
+    both (every leg is timed the same way)."""
     pa.set_precision(precision)
-    pa.set_profiling(profile)
+    pa.set_profiling(False)
     pa.set_warm_start(warm, 1)
     for i in range(warmup):
         pa.part_mesh_dev(d_part.data_ptr(), d_vtx.data_ptr(), d_xyz.data_ptr(), nel, nv, P)
-        log(f"warmup {i + 1}/{warmup} fp{precision}: {pa.last_report()['seconds']:.3f} s")
+        log(f"warmup {i + 1}/{warmup} {label}: {pa.last_report()['seconds']:.3f} s")
     torch.cuda.synchronize()
     reps = []
     t0 = time.perf_counter()
@@ -64,11 +69,32 @@ def run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, steps, war
         reps.append(pa.last_report())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    log(f"timed {steps} step(s) fp{precision}{' learned' if warm is not None else ''}: "
-        f"{dt:.3f} s")
-    pa.set_profiling(False)
+    log(f"timed {steps} step(s) {label}: {dt:.3f} s")
     pa.set_warm_start(None)
     return dt, reps
+
+
+def profiled_call(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, precision, warm=None,
+                  label=""):
+    """One untimed call with HIP events around every 11th k_lz_apply launch
+    on the solver's stream: the kernel's mean launch time for the roofline."""
+    pa.set_precision(precision)
+    pa.set_profiling(True)
+    pa.set_warm_start(warm, 1)
+    pa.part_mesh_dev(d_part.data_ptr(), d_vtx.data_ptr(), d_xyz.data_ptr(), nel, nv, P)
+    torch.cuda.synchronize()
+    rep = pa.last_report()
+    pa.set_profiling(False)
+    pa.set_warm_start(None)
+    log(f"profiled call {label}: {rep['seconds']:.3f} s")
+    return rep
+
+
+def apply_roofline(rep):
+    launches = max(rep["spmv_launches"], 1)
+    gbs = rep["spmv_bytes"] / (rep["spmv_ms"] * 1e-3) / 1e9 if rep["spmv_ms"] else 0.0
+    it_gbs = rep["lanczos_bytes"] / (rep["lanczos_ms"] * 1e-3) / 1e9 if rep["lanczos_ms"] else 0.0
+    return gbs, it_gbs, launches
 
 
 def measured_traffic(desc, precision):
@@ -82,23 +108,49 @@ def measured_traffic(desc, precision):
         return json.load(f).get("bytes_per_launch")
 
 
+def host_cores():
+    """Cores this process may use: the affinity mask, capped at the 16 a
+    one-GPU box grants a job (whatever the machine's count)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    cap = int(os.environ.get("PARRSB_AMD_CPU_CORES", "16"))
+    return max(1, min(n, cap))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg):
-    """Oracle (single-threaded C, FP64) on a bounded sample of the workload:
-    a sub-mesh of the same family, partitioned to the same elements per part."""
+    """The oracle (the reference algorithm restated in C, FP64) on every host
+    core: its OpenMP build splits each segment's Lanczos vectors and
+    Laplacian over the threads while a level has fewer segments than threads,
+    then runs whole segments per thread.  A bounded sample of the workload:
+    a smaller mesh of the same family into the same elements per part."""
     from oracle import pyoracle as O
     from parrsb_amd import meshgen as M
     if cfg == "warped256":
-        n, P = 64, 1024
-        vtx, xyz, nv = M.warped_box_hex(n, seed=5)
-        sample = f"warped hex 64^3 = 262,144 elements into 1,024 parts (256 per part, as the workload)"
+        vtx, xyz, nv = M.warped_box_hex(128, seed=5)
+        P = 8192
+        sample = ("warped hex 128^3 = 2,097,152 elements into 8,192 parts (256 per part, as "
+                  "the workload: its last 13 of 16 cuts at 1/8 the size)")
     elif cfg == "pipe2m":
         vtx, xyz, nv = M.pipe_hex(8, 128, 256, seed=7)
         P = 256
         sample = "warped pipe 8x128x256 = 262,144 elements into 256 parts (1,024 per part, as the workload)"
     elif cfg == "holed2048":
-        vtx, xyz, nv = M.holed_quads(512, 512, seed=3)
-        P = 256
-        sample = "holed quads 512^2 into 256 parts (~976 per part, as the workload)"
+        vtx, xyz, nv = M.holed_quads(1024, 1024, seed=3)
+        P = 1024
+        sample = "holed quads 1024^2 into 1,024 parts (~976 per part, as the workload)"
     else:
         vtx, xyz, nv, P, _ = make_mesh(cfg)
         if vtx.shape[0] > 300000:
@@ -107,12 +159,14 @@ def cpu_baseline(cfg):
             sample = "hex box 64x64x32 into 128 parts (1,024 per part, as the workload)"
         else:
             sample = "the full workload"
+    cores = host_cores()
     t0 = time.perf_counter()
-    part, rep = O.part_mesh(vtx, xyz, nv, P)
+    part, rep = O.part_mesh(vtx, xyz, nv, P, threads=cores)
     dt = time.perf_counter() - t0
     nel = vtx.shape[0]
-    return {"value": nel / dt / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
-            "sample": sample, "seconds": dt}
+    return {"value": nel / dt / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+            "sample": sample, "seconds": dt, "threads": "OpenMP (oracle/liboracle_mt.so)",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def cpu_baseline_mg():
@@ -201,6 +255,9 @@ def main():
     ap.add_argument("--config", default="warped256", choices=sorted(CONFIGS))
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 run")
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="timed calls of each comparison leg (reference start, plain FP64)")
+    ap.add_argument("--compare-warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--warm-start", default=os.path.join(ROOT, "parrsb_amd", "lws_weights.bin"),
                     help="network parameter file of the learned start, or 'none'")
@@ -233,15 +290,16 @@ def main():
         warm = lws_train.load_params(args.warm_start)
         log(f"learned warm start: {args.warm_start}")
 
+    label = f"fp{args.precision}" + (" learned" if warm is not None else "")
     dt, reps = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision,
-                         args.steps, args.warmup, True, warm)
+                         args.steps, args.warmup, warm, label)
     q = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
     rep = reps[-1]
     ms = dt / args.steps * 1e3
     value = nel / (dt / args.steps) / 1e6
-    spmv_gbs = rep["spmv_bytes"] / (rep["spmv_ms"] * 1e-3) / 1e9 if rep["spmv_ms"] else 0.0
-    it_gbs = rep["lanczos_bytes"] / (rep["lanczos_ms"] * 1e-3) / 1e9 if rep["lanczos_ms"] else 0.0
-    launches = max(rep["spmv_launches"], 1)
+    # the kernel's launch time from one more call with events (not timed)
+    prep = profiled_call(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision, warm, label)
+    spmv_gbs, it_gbs, launches = apply_roofline(prep)
     out = {
         "metric": METRIC, "value": value, "unit": "Melem/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -259,11 +317,12 @@ def main():
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(desc, args.precision),
-                     "bytes_per_launch": rep["spmv_bytes"] / launches,
-                     "avg_launch_ms": rep["spmv_ms"] / launches,
-                     "launches": rep["spmv_launches"],
+                     "bytes_per_launch": prep["spmv_bytes"] / launches,
+                     "avg_launch_ms": prep["spmv_ms"] / launches,
+                     "launches": prep["spmv_launches"],
                      "timing": "HIP events on the solver's stream around every 11th k_lz_apply "
-                               "launch of the timed steps (mean launch time)",
+                               "launch of one extra call after the timed steps (mean launch "
+                               "time); the timed steps run without events",
                      "iteration_achieved_gbs": it_gbs},
         "quality": q,
         "levels": {k: rep[k] for k in ("niter_max", "niter_mean", "npass_max", "ncomp_max",
@@ -278,23 +337,27 @@ def main():
                            "RSB_COMPONENTS": rep["phase_ms"]["components"],
                            "RSB_BALANCE": rep["phase_ms"]["balance"]},
     }
+    cw, cs = args.compare_warmup, args.compare_steps
     if not args.no_compare and warm is not None:
-        # same protocol as the main leg (warmup, then the timed steps)
+        # the same solver from the reference start, timed the same way
         dtr, repsr = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision,
-                               args.steps, args.warmup, False)
-        dtr /= args.steps
+                               cs, cw, None, f"fp{args.precision} reference start")
+        dtr /= cs
         qr = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
         out["reference_start"] = {"value": nel / dtr / 1e6, "ms": dtr * 1e3, "quality": qr,
+                                  "steps": cs, "warmup": cw,
                                   "niter_mean": repsr[-1]["niter_mean"],
                                   "level_ms": repsr[-1]["level_ms"]}
         out["speedup_vs_reference_start"] = value / (nel / dtr / 1e6)
         out["edge_cut_vs_reference_start"] = q["edge_cut"] / max(qr["edge_cut"], 1)
     if not args.no_compare and args.precision == 32:
-        dt64, reps64 = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 64, args.steps,
-                                 args.warmup, True)
-        dt64 /= args.steps
+        dt64, reps64 = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 64, cs, cw, None,
+                                 "fp64 plain")
+        dt64 /= cs
         q64 = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
         r64 = reps64[-1]
+        p64 = profiled_call(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 64, None, "fp64 plain")
+        g64, _, l64 = apply_roofline(p64)
         lam_err = []
         for l in range(len(rep["lambda_min"])):
             e = 0.0
@@ -303,16 +366,19 @@ def main():
                 if b:
                     e = max(e, abs(a - b) / abs(b))
             lam_err.append(e)
-        g64 = r64["spmv_bytes"] / (r64["spmv_ms"] * 1e-3) / 1e9 if r64["spmv_ms"] else 0.0
         out["plain_fp64"] = {"value": nel / dt64 / 1e6, "ms": dt64 * 1e3, "quality": q64,
+                             "steps": cs, "warmup": cw,
                              "lambda_rel_err_per_level": lam_err,
                              "level_ms": r64["level_ms"],
+                             "niter_mean": r64["niter_mean"],
+                             # eigen-residuals |Lf - lambda f| / lambda of the FP64 solve,
+                             # beside the FP32 ones under "levels"
+                             "resid_med": r64["resid_med"], "resid_max": r64["resid_max"],
                              # k_lz_apply<double>, timed as in the main leg
                              "roofline": {"kernel": "k_lz_apply<double>", "achieved": g64,
                                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                           "frac": g64 / HBM_PEAK_GBS,
-                                          "avg_launch_ms": r64["spmv_ms"] /
-                                          max(r64["spmv_launches"], 1)}}
+                                          "avg_launch_ms": p64["spmv_ms"] / l64}}
         out["speedup_vs_plain_fp64"] = (nel / dt64 / 1e6 and value / (nel / dt64 / 1e6))
         out["edge_cut_vs_plain_fp64"] = q["edge_cut"] / max(q64["edge_cut"], 1)
     if not args.no_compare:

@@ -54,9 +54,24 @@ def test_traffic_summary_scales_by_the_copy_calibration(tmp_path):
 
 def test_cpu_baseline_record():
     r = bench.cpu_baseline("hex16")
-    assert set(r) == {"value", "unit", "cores", "kind", "sample", "seconds"}
-    assert r["kind"] == "port" and r["cores"] == 1 and r["unit"] == "Melem/s"
+    assert set(r) == {"value", "unit", "cores", "kind", "sample", "seconds", "threads",
+                      "cpu_model", "nproc"}
+    assert r["kind"] == "port" and r["unit"] == "Melem/s"
+    # every core the process may use, capped at a one-GPU box's 16
+    assert r["cores"] == min(len(os.sched_getaffinity(0)), 16)
     assert r["value"] > 0 and r["sample"] == "the full workload"
+
+
+def test_threaded_oracle_partitions_like_the_serial_one():
+    """The CPU baseline's OpenMP build: on a mesh below its threading
+    threshold only whole segments run in parallel, so the partition equals
+    the serial oracle's element for element."""
+    from oracle import pyoracle as O
+    from parrsb_amd import meshgen as M
+    vtx, xyz, nv = M.box_hex(24, 20, 16, jitter=0.1, seed=4)
+    a, ra = O.part_mesh(vtx, xyz, nv, 24)
+    b, rb = O.part_mesh(vtx, xyz, nv, 24, threads=4)
+    assert (a == b).all() and ra == rb
 
 
 def test_default_workload_is_the_256_cubed_box():
