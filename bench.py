@@ -139,18 +139,18 @@ def cpu_baseline(cfg):
     from oracle import pyoracle as O
     from parrsb_amd import meshgen as M
     if cfg == "warped256":
-        vtx, xyz, nv = M.warped_box_hex(128, seed=5)
-        P = 8192
-        sample = ("warped hex 128^3 = 2,097,152 elements into 8,192 parts (256 per part, as "
-                  "the workload: its last 13 of 16 cuts at 1/8 the size)")
+        vtx, xyz, nv = M.warped_box_hex(64, seed=5)
+        P = 1024
+        sample = ("warped hex 64^3 = 262,144 elements into 1,024 parts (256 per part, as the "
+                  "workload)")
     elif cfg == "pipe2m":
         vtx, xyz, nv = M.pipe_hex(8, 128, 256, seed=7)
         P = 256
         sample = "warped pipe 8x128x256 = 262,144 elements into 256 parts (1,024 per part, as the workload)"
     elif cfg == "holed2048":
-        vtx, xyz, nv = M.holed_quads(1024, 1024, seed=3)
-        P = 1024
-        sample = "holed quads 1024^2 into 1,024 parts (~976 per part, as the workload)"
+        vtx, xyz, nv = M.holed_quads(512, 512, seed=3)
+        P = 256
+        sample = "holed quads 512^2 into 256 parts (~976 per part, as the workload)"
     else:
         vtx, xyz, nv, P, _ = make_mesh(cfg)
         if vtx.shape[0] > 300000:
@@ -161,7 +161,8 @@ def cpu_baseline(cfg):
             sample = "the full workload"
     cores = host_cores()
     t0 = time.perf_counter()
-    part, rep = O.part_mesh(vtx, xyz, nv, P, threads=cores)
+    log(f"CPU baseline: {vtx.shape[0]} elements into {P} parts on {cores} threads")
+    part, rep = O.part_mesh(vtx, xyz, nv, P, threads=cores, verbose=1)
     dt = time.perf_counter() - t0
     nel = vtx.shape[0]
     return {"value": nel / dt / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",

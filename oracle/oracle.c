@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifndef M_PI
 #define M_PI 3.141592653589793
@@ -32,6 +33,11 @@
 #define ORACLE_PAR_MIN 65536
 
 void oracle_free(void *p) { free(p); }
+
+/* 1: oracle_part_mesh prints one line per cut to stderr (the long CPU
+ * baseline shows its progress) */
+static int oracle_verbose = 0;
+void oracle_set_verbose(int v) { oracle_verbose = v; }
 
 int oracle_threads(int n) {
 #ifdef ORACLE_THREADS
@@ -1192,6 +1198,12 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
   int *segof = xmalloc(sizeof(int) * (size_t)(nel ? nel : 1));
 
   int *coff = xmalloc(sizeof(int) * (size_t)P);
+  double t_start = 0.0;
+  if (oracle_verbose) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    t_start = ts.tv_sec + 1e-9 * ts.tv_nsec;
+  }
   for (int cut = 0; cut < ncuts; cut++) {
     int nn = 0;
     int itmin = 1 << 30, itmax = 0, cmin = 1 << 30, cmax = 0, nbal = 0, ncut = 0;
@@ -1265,6 +1277,13 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
     }
     seg_t *t = segs;
     segs = nsegs, nsegs = t, nseg = nn;
+    if (oracle_verbose) {
+      struct timespec ts;
+      clock_gettime(CLOCK_MONOTONIC, &ts);
+      fprintf(stderr, "oracle: cut %d/%d done (%d segments) at %.1f s\n", cut + 1, ncuts, ncut,
+              ts.tv_sec + 1e-9 * ts.tv_nsec - t_start);
+      fflush(stderr);
+    }
     if (rep && cut < ORACLE_MAX_LEVELS) {
       rep->niter_min[cut] = ncut ? itmin : 0, rep->niter_max[cut] = itmax;
       rep->ncomp_min[cut] = o->find_disconnected_comps ? cmin : 0;

@@ -95,6 +95,8 @@ void oracle_free(void *p);
 /* Threads of the OpenMP build (liboracle_mt.so) for later calls; n <= 0
  * leaves the count.  Returns the count in use (1 in the serial build). */
 int oracle_threads(int n);
+/* 1: oracle_part_mesh reports each finished cut on stderr. */
+void oracle_set_verbose(int v);
 
 /* Apply the shared-vertex Laplacian of the sub-mesh `elems[0..n)` (the
  * gather-scatter form of laplacian.c:159-205) to u, writing v. */

@@ -79,6 +79,7 @@ def _declare(L):
                                     C.POINTER(P)]
     L.oracle_dual_graph.restype = C.c_int64
     L.oracle_free.argtypes = [P]
+    L.oracle_set_verbose.argtypes = [C.c_int]
     L.oracle_threads.argtypes = [C.c_int]
     L.oracle_threads.restype = C.c_int
     L.oracle_laplacian_apply.argtypes = [P, C.c_int, P, C.c_int, P, P]
@@ -322,13 +323,14 @@ def mg_project(vtx, nv, elems, b, miter=50, tol=1e-5, factor=2):
 
 
 def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
-              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2, threads=0):
+              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2, threads=0, verbose=0):
     """The reference algorithm over virtual ranks.  threads > 0 runs the
     OpenMP build on that many threads (the CPU baseline); its sums depend on
     the thread count, so parity checks keep threads = 0 (serial build)."""
     L = lib(threaded=threads > 0)
     if threads > 0:
         L.oracle_threads(int(threads))
+    L.oracle_set_verbose(int(verbose))
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     nel = vtx.size // nv
     xp = None

@@ -1265,7 +1265,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
             RSB_TRY(hipStreamEndCapture(st, &gr));
             if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
             gexec = nullptr;
-            RSB_TRY(hipGraphInstantiate(&gexec, gr, nullptr, nullptr, 0));
+            RSB_TRY(hipGraphInstantiateWithFlags(&gexec, gr, 0));
             RSB_TRY(hipGraphDestroy(gr));
             gn[0] = nact[0], gn[1] = nact[1];
           }

@@ -880,6 +880,38 @@ void rsb_mg_free(rsbgpu *g) {
     if (rc_) return rc_;      \
   } while (0)
 
+// Device buffers of one scope (a level of mg_build, or one rsbgpu_fiedler_mg
+// call with its CG-iteration graph): freed on every return path.  A capture still open when the scope ends (an
+// error between begin and end) is ended and its graph dropped, so the
+// handle's stream is never left capturing.  The buffers are freed after the
+// stream has drained (an early return may leave launches queued).
+struct mg_scope {
+  hipStream_t st;
+  std::vector<void *> bufs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool capturing = false;
+  explicit mg_scope(hipStream_t s) : st(s) {}
+  template <typename T>
+  hipError_t alloc(T *&ptr, size_t bytes) {
+    hipError_t e = hipMalloc((void **)&ptr, bytes);
+    if (e == hipSuccess) bufs.push_back((void *)ptr);
+    else ptr = nullptr;
+    return e;
+  }
+  ~mg_scope() {
+    if (capturing) {
+      hipGraph_t left = nullptr;
+      (void)hipStreamEndCapture(st, &left);
+      if (left) (void)hipGraphDestroy(left);
+    }
+    (void)hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *q : bufs) (void)hipFree(q);
+  }
+};
+
 static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
   hipStream_t st = g->stream;
   const int nseg = g->nseg;
@@ -891,7 +923,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     n0[s] = el ? g->h_n[s] : 0;
   }
   H.h_off.push_back(off0), H.h_n.push_back(n0);
-  H.lv.emplace_back();
+  if (H.lv.empty()) H.lv.emplace_back();
   {
     mg_level_dev &L0 = H.lv[0];
     L0.m = g->mpad;
@@ -899,7 +931,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     RSB_TRY(hipMalloc((void **)&L0.n, sizeof(int) * nseg));
     RSB_TRY(hipMemcpyAsync(L0.off, off0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     RSB_TRY(hipMemcpyAsync(L0.n, n0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    L0.diag = diag0;  // freed with the hierarchy
+    L0.diag = diag0;  // owned by the hierarchy (the caller put it there)
     RSB_TRY(hipMalloc((void **)&L0.r0, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
     RSB_TRY(hipMalloc((void **)&L0.sa, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
     RSB_TRY(hipMalloc((void **)&L0.sb, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
@@ -936,9 +968,10 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     RSB_TRY(hipMalloc((void **)&C.diag, sizeof(double) * (size_t)mc));
     RSB_TRY(hipMalloc((void **)&C.rp, sizeof(int64_t) * (size_t)(mc + 1)));
     // raw entries
+    mg_scope tmp(st);  // this level's temporaries
     int64_t *cnt = nullptr, *roff = nullptr;
-    RSB_TRY(hipMalloc((void **)&cnt, sizeof(int64_t) * (size_t)(mc + 1)));
-    RSB_TRY(hipMalloc((void **)&roff, sizeof(int64_t) * (size_t)(mc + 1)));
+    RSB_TRY(tmp.alloc(cnt, sizeof(int64_t) * (size_t)(mc + 1)));
+    RSB_TRY(tmp.alloc(roff, sizeof(int64_t) * (size_t)(mc + 1)));
     mg_lvl FL{F.off, F.n}, CL{C.off, C.n};
     k_mg_rawcnt<<<nblocks(mc + 1, 256), 256, 0, st>>>(mc, C.rseg, FL, CL, factor, l,
                                                        g->d_rowlen, F.rp, cnt);
@@ -953,20 +986,20 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     double *wr = nullptr, *wsum = nullptr;
     int64_t *pos = nullptr, *flag64 = nullptr, *rc64 = nullptr;
     const size_t nra = (size_t)std::max<int64_t>(nr, 1);
-    RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * nra));
-    RSB_TRY(hipMalloc((void **)&idx, sizeof(int) * nra));
-    RSB_TRY(hipMalloc((void **)&wr, sizeof(double) * nra));
+    RSB_TRY(tmp.alloc(key, sizeof(uint64_t) * nra));
+    RSB_TRY(tmp.alloc(idx, sizeof(int) * nra));
+    RSB_TRY(tmp.alloc(wr, sizeof(double) * nra));
     k_mg_rawfill<<<nblocks(mc, 256), 256, 0, st>>>(mc, C.rseg, FL, CL, factor, l, S, F.rp, F.ci,
                                                     F.w, roff, key, idx, wr);
     RSB_LAUNCH_CHECK();
     int cbits = 1;
     while (((int64_t)1 << cbits) < mc) cbits++;
-    RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * nra));
-    RSB_TRY(hipMalloc((void **)&wsum, sizeof(double) * nra));
-    RSB_TRY(hipMalloc((void **)&rowcnt, sizeof(int) * (size_t)(mc + 1)));
+    RSB_TRY(tmp.alloc(flag, sizeof(int) * nra));
+    RSB_TRY(tmp.alloc(wsum, sizeof(double) * nra));
+    RSB_TRY(tmp.alloc(rowcnt, sizeof(int) * (size_t)(mc + 1)));
     RSB_TRY(hipMemsetAsync(rowcnt, 0, sizeof(int) * (size_t)(mc + 1), st));
-    RSB_TRY(hipMalloc((void **)&flag64, sizeof(int64_t) * nra));
-    RSB_TRY(hipMalloc((void **)&pos, sizeof(int64_t) * nra));
+    RSB_TRY(tmp.alloc(flag64, sizeof(int64_t) * nra));
+    RSB_TRY(tmp.alloc(pos, sizeof(int64_t) * nra));
     if (nr > 0) {
       MG_TRY(rsb_sort_pairs_u64(g, key, idx, nr, 0, 32 + cbits));
       k_mg_heads<<<nblocks(nr, 256), 256, 0, st>>>(nr, key, idx, wr, flag, wsum, rowcnt);
@@ -974,7 +1007,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       k_mg_widen<<<nblocks(nr, 256), 256, 0, st>>>(nr, flag, flag64);
       MG_TRY(rsb_exclusive_scan_i64(g, flag64, pos, nr));
     }
-    RSB_TRY(hipMalloc((void **)&rc64, sizeof(int64_t) * (size_t)(mc + 1)));
+    RSB_TRY(tmp.alloc(rc64, sizeof(int64_t) * (size_t)(mc + 1)));
     k_mg_widen<<<nblocks(mc + 1, 256), 256, 0, st>>>(mc + 1, rowcnt, rc64);
     MG_TRY(rsb_exclusive_scan_i64(g, rc64, C.rp, mc + 1));
     int64_t nnz = 0;
@@ -991,7 +1024,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       int64_t *sbytes = nullptr;
       RSB_TRY(hipMalloc((void **)&C.rlen, sizeof(int) * (size_t)(nsl * RSB_WAVE)));
       RSB_TRY(hipMalloc((void **)&C.soff, sizeof(int64_t) * (size_t)(nsl + 1)));
-      RSB_TRY(hipMalloc((void **)&sbytes, sizeof(int64_t) * (size_t)(nsl + 1)));
+      RSB_TRY(tmp.alloc(sbytes, sizeof(int64_t) * (size_t)(nsl + 1)));
       k_mg_slice_len<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.rlen, sbytes);
       RSB_LAUNCH_CHECK();
       MG_TRY(rsb_exclusive_scan_i64(g, sbytes, C.soff, nsl + 1));
@@ -1001,7 +1034,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       // one packed word per entry where the row index fits in 24 bits and
       // the weights in 8 (the large coarse levels), two ints otherwise
       int *dmax = nullptr, hmax = 0;
-      RSB_TRY(hipMalloc((void **)&dmax, sizeof(int)));
+      RSB_TRY(tmp.alloc(dmax, sizeof(int)));
       RSB_TRY(hipMemsetAsync(dmax, 0, sizeof(int), st));
       if (nnz > 0) {
         k_mg_wmax<<<std::min<int64_t>(nblocks(nnz, 256), 2048), 256, 0, st>>>(nnz, C.w, dmax);
@@ -1009,7 +1042,6 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       }
       RSB_TRY(hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, st));
       RSB_TRY(hipStreamSynchronize(st));
-      hipFree(dmax);
       if (mc <= (1 << 24) && hmax < 256) {
         RSB_TRY(hipMalloc((void **)&C.spk, sizeof(uint32_t) * (size_t)std::max<int64_t>(ns, 1)));
       } else {
@@ -1020,10 +1052,7 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
                                                                     C.soff, C.sci, C.sw, C.spk);
       RSB_LAUNCH_CHECK();
       RSB_TRY(hipStreamSynchronize(st));
-      hipFree(sbytes);
     }
-    hipFree(cnt), hipFree(roff), hipFree(key), hipFree(idx), hipFree(wr), hipFree(flag);
-    hipFree(wsum), hipFree(rowcnt), hipFree(flag64), hipFree(pos), hipFree(rc64);
   }
   return 0;
 }
@@ -1055,27 +1084,36 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     rsb_phase_end(g, PH_LANCZOS);
     return 0;
   }
-  // vectors and state
+  // vectors and state: owned by `own`, whose destructor frees them (and the
+  // graph) on every return path; diag0 belongs to the new hierarchy, which
+  // stays with the handle
   const size_t vb = sizeof(double) * (size_t)mpad;
+  mg_scope own(st);
+  rsb_mg_free(g);
+  mg_hier *hp = new mg_hier();
+  g->mg = hp;
+  mg_hier &H = *hp;
+  H.lv.emplace_back();
+  RSB_TRY(hipMalloc((void **)&H.lv[0].diag, vb));
+  double *const diag0 = H.lv[0].diag;
   double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *z0 = nullptr, *p = nullptr,
-         *w = nullptr, *diag0 = nullptr, *Wb = nullptr, *Pb = nullptr, *sc = nullptr,
-         *partial = nullptr, *coef = nullptr;
+         *w = nullptr, *Wb = nullptr, *Pb = nullptr, *sc = nullptr, *partial = nullptr,
+         *coef = nullptr;
   int *stt = nullptr, *active = nullptr;
-  RSB_TRY(hipMalloc((void **)&b, vb));
-  RSB_TRY(hipMalloc((void **)&x, vb));
-  RSB_TRY(hipMalloc((void **)&r, vb));
-  RSB_TRY(hipMalloc((void **)&z, vb));
-  RSB_TRY(hipMalloc((void **)&z0, vb));
-  RSB_TRY(hipMalloc((void **)&p, vb));
-  RSB_TRY(hipMalloc((void **)&w, vb));
-  RSB_TRY(hipMalloc((void **)&diag0, vb));
-  RSB_TRY(hipMalloc((void **)&Wb, vb * (size_t)max_iter));
-  RSB_TRY(hipMalloc((void **)&Pb, vb * (size_t)max_iter));
-  RSB_TRY(hipMalloc((void **)&sc, sizeof(double) * MS_NSC * (size_t)nseg));
-  RSB_TRY(hipMalloc((void **)&stt, sizeof(int) * MI_NI * (size_t)nseg));
-  RSB_TRY(hipMalloc((void **)&partial, sizeof(double) * MG_NP * (size_t)std::max(ntiles, 1)));
-  RSB_TRY(hipMalloc((void **)&coef, sizeof(double) * MG_NP * (size_t)nseg));
-  RSB_TRY(hipMalloc((void **)&active, sizeof(int)));
+  RSB_TRY(own.alloc(b, vb));
+  RSB_TRY(own.alloc(x, vb));
+  RSB_TRY(own.alloc(r, vb));
+  RSB_TRY(own.alloc(z, vb));
+  RSB_TRY(own.alloc(z0, vb));
+  RSB_TRY(own.alloc(p, vb));
+  RSB_TRY(own.alloc(w, vb));
+  RSB_TRY(own.alloc(Wb, vb * (size_t)max_iter));
+  RSB_TRY(own.alloc(Pb, vb * (size_t)max_iter));
+  RSB_TRY(own.alloc(sc, sizeof(double) * MS_NSC * (size_t)nseg));
+  RSB_TRY(own.alloc(stt, sizeof(int) * MI_NI * (size_t)nseg));
+  RSB_TRY(own.alloc(partial, sizeof(double) * MG_NP * (size_t)std::max(ntiles, 1)));
+  RSB_TRY(own.alloc(coef, sizeof(double) * MG_NP * (size_t)nseg));
+  RSB_TRY(own.alloc(active, sizeof(int)));
   RSB_TRY(hipMemsetAsync(sc, 0, sizeof(double) * MS_NSC * (size_t)nseg, st));
   // every eligible segment starts in the inverse iteration
   {
@@ -1089,10 +1127,6 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
             g->d_halo,      g->d_off16, g->d_w16,     g->d_e16};
   k_mg_diag0<<<nblocks(mpad, 256), 256, 0, st>>>(mpad, S, diag0);
   RSB_LAUNCH_CHECK();
-  rsb_mg_free(g);
-  mg_hier *hp = new mg_hier();
-  g->mg = hp;
-  mg_hier &H = *hp;
   MG_TRY(mg_build(g, factor, H, diag0));
   const int nlev = (int)H.lv.size();
   auto fin = [&](int op, int gate, int nslots) {
@@ -1169,18 +1203,19 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     RSB_LAUNCH_CHECK();
     return 0;
   };
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
   {
     const char *ev = getenv("PARRSB_AMD_MG_GRAPH");
     if (!(ev && atoi(ev) == 0)) {
       RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      own.capturing = true;
       int rb = part_b(), ra = part_a();
-      hipError_t ec = hipStreamEndCapture(st, &graph);
+      own.capturing = false;
+      hipError_t ec = hipStreamEndCapture(st, &own.graph);
       if (rb || ra || ec != hipSuccess) RSB_FAIL(2, "MG graph capture failed");
-      RSB_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      RSB_TRY(hipGraphInstantiateWithFlags(&own.exec, own.graph, 0));
     }
   }
+  hipGraphExec_t exec = own.exec;
   for (int pass = 0; pass < max_passes; pass++) {
     // projection (fiedler.c:94-187)
     k_mg_pass0<<<ntiles, RSB_BLOCK, 0, st>>>(a, b, x, r);
@@ -1231,12 +1266,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     if (out->lambda) out->lambda[s] = hsc[(size_t)s * MS_NSC + MS_LAMBDA];
     if (out->resid) out->resid[s] = hsc[(size_t)s * MS_NSC + MS_RES];
   }
-  hipFree(b), hipFree(x), hipFree(r), hipFree(z), hipFree(z0), hipFree(p), hipFree(w);
-  hipFree(Wb), hipFree(Pb), hipFree(sc), hipFree(stt), hipFree(partial), hipFree(coef);
-  hipFree(active);
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
-  // diag0 stays alive with the kept hierarchy (level 0's diagonal)
+  // buffers and graph: freed by `own`; diag0 stays with the kept hierarchy
   rsb_phase_end(g, PH_LANCZOS);
   return 0;
 }
