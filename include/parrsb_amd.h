@@ -57,7 +57,32 @@ typedef struct {
      rsb_max_passes (rsb.c:79-90); 0 where every segment converged */
   double tol_init[PARRSB_AMD_MAX_LEVELS], tol_target[PARRSB_AMD_MAX_LEVELS];
   double tol_final[PARRSB_AMD_MAX_LEVELS];
+  /* allocation-free calls: the most bytes of the fixed device array held at
+     once, and the highest offset touched (freed blocks are reused only when
+     the free part of the array runs short) */
+  double noalloc_bytes, noalloc_high;
 } parrsb_amd_report;
+
+/* Allocation-free partitioning (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code around HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  Same contract as
+ * parrsb_amd_part_mesh_dev, but the call makes no hipMalloc / hipFree: every
+ * device buffer lives in a fixed array of PARRSB_AMD_NOALLOC_BYTES per GPU
+ * (allocated with the library's code object, held until the process ends)
+ * and the call's stream and timing events are created once per GPU, on its
+ * first allocation-free call.  Meshes of more than PARRSB_AMD_NOALLOC_MAX_NEL
+ * elements (PARRSB_AMD_NOALLOC_MAX_NEL_MG with rsb_algo = 1, whose stored
+ * search directions take the most memory per element), or more parts than
+ * PARRSB_AMD_NOALLOC_MAX_NEL, are rejected with 3, and so is tagged mode; at
+ * these limits a call holds at most about a third (Lanczos) or half (MG) of
+ * the array.  The array keeps the last call's contents and is shared
+ * by every allocation-free call on that GPU: calls that overlap on one GPU
+ * (two host threads) race.  Host bookkeeping still uses the C heap. */
+#define PARRSB_AMD_NOALLOC_BYTES (256ull << 20)
+#define PARRSB_AMD_NOALLOC_MAX_NEL 65536
+#define PARRSB_AMD_NOALLOC_MAX_NEL_MG 32768
+int parrsb_amd_part_mesh_dev_noalloc(int *d_part, const long long *d_vtx, const double *d_xyz,
+                                     int nel, int nv, parrsb_options *options);
 
 /* Same contract as parrsb_part_mesh, with every array in GPU memory of the
  * selected device (PARRSB_AMD_DEVICE, default 0): d_part[nel] out,

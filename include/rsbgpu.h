@@ -142,6 +142,16 @@ int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device);
 int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device, int max_groups,
                         int *ngroups, int *counts);
 
+/* Allocation-free calls (noalloc.hip): between begin and end, every device
+ * allocation of this thread's handles is carved from the device's fixed
+ * array and rsbgpu_open borrows the stream and events set up for the device
+ * on its first such call.  begin returns 0, 2 (device set-up failed) or 3
+ * (already inside such a call), with a message in msg; end drains the
+ * borrowed stream and returns the most bytes the call held at once (*high:
+ * the highest offset of the array it touched). */
+int rsbgpu_noalloc_begin(int device, char *msg, int msglen);
+long long rsbgpu_noalloc_end(long long *high);
+
 /* Copy the current order / Fiedler values (per compact position) to the
  * host; used by the parity tests. */
 int rsbgpu_get_order(rsbgpu *g, int *order_host);

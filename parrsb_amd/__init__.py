@@ -60,7 +60,8 @@ class Report(C.Structure):
                 ("warm_start", C.c_int), ("nbrs_min", C.c_int * ML),
                 ("nbrs_max", C.c_int * ML), ("nbrs_mean", C.c_double * ML),
                 ("tol_init", C.c_double * ML), ("tol_target", C.c_double * ML),
-                ("tol_final", C.c_double * ML)]
+                ("tol_final", C.c_double * ML), ("noalloc_bytes", C.c_double),
+                ("noalloc_high", C.c_double)]
 
     def as_dict(self):
         d = {}
@@ -147,6 +148,9 @@ SIGNATURES = {
     "rsbgpu_get_order": (I, [P, P]),
     "rsbgpu_get_fiedler": (I, [P, P]),
     "rsbgpu_set_fiedler": (I, [P, P]),
+    "rsbgpu_noalloc_begin": (I, [I, P, I]),
+    "rsbgpu_noalloc_end": (C.c_longlong, [P]),
+    "parrsb_amd_part_mesh_dev_noalloc": (I, [P, P, P, I, I, C.POINTER(Options)]),
     "rsbgpu_get_level_csr": (I, [P, P, P, P, P, P, P]),
     "rsbgpu_get_graph": (I, [P, P, P, P]),
     "rsbgpu_get_centroids": (I, [P, P]),

@@ -428,13 +428,13 @@ extern "C" int rsbgpu_write_part(rsbgpu *g, int *part, int part_is_device) {
   const int nel = g->nel;
   if (nel == 0) return 0;
   int *dp = part;
-  if (!part_is_device) RSB_TRY(hipMalloc((void **)&dp, sizeof(int) * nel));
+  if (!part_is_device) RSB_TRY(rsb_dev_malloc((void **)&dp, sizeof(int) * nel));
   k_write_part<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_segof, g->d_seg, g->nseg, nel, dp);
   RSB_LAUNCH_CHECK();
   if (!part_is_device) {
     RSB_TRY(hipMemcpyAsync(part, dp, sizeof(int) * nel, hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));
-    hipFree(dp);
+    rsb_dev_free(dp);
   } else {
     RSB_TRY(hipStreamSynchronize(g->stream));
   }
@@ -456,13 +456,13 @@ extern "C" int rsbgpu_write_seq(rsbgpu *g, int *seq, int seq_is_device) {
   const int nel = g->nel;
   if (nel == 0) return 0;
   int *dp = seq;
-  if (!seq_is_device) RSB_TRY(hipMalloc((void **)&dp, sizeof(int) * nel));
+  if (!seq_is_device) RSB_TRY(rsb_dev_malloc((void **)&dp, sizeof(int) * nel));
   k_write_seq<<<nblocks(nel, 256), 256, 0, g->stream>>>(g->d_order, g->d_segof, g->d_seg, nel, dp);
   RSB_LAUNCH_CHECK();
   if (!seq_is_device) {
     RSB_TRY(hipMemcpyAsync(seq, dp, sizeof(int) * nel, hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));
-    hipFree(dp);
+    rsb_dev_free(dp);
   } else {
     RSB_TRY(hipStreamSynchronize(g->stream));
   }
@@ -503,13 +503,13 @@ extern "C" int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device,
   int *tmp = nullptr, *heads = nullptr, *nh = nullptr;
   uint32_t *key = nullptr;
   if (!tag_is_device) {
-    RSB_TRY(hipMalloc((void **)&tmp, sizeof(int) * (size_t)nel));
+    RSB_TRY(rsb_dev_malloc((void **)&tmp, sizeof(int) * (size_t)nel));
     RSB_TRY(hipMemcpyAsync(tmp, tag, sizeof(int) * (size_t)nel, hipMemcpyHostToDevice, st));
     dt = tmp;
   }
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint32_t) * (size_t)nel));
-  RSB_TRY(hipMalloc((void **)&heads, sizeof(int) * (size_t)max_groups));
-  RSB_TRY(hipMalloc((void **)&nh, sizeof(int)));
+  RSB_TRY(rsb_dev_malloc((void **)&key, sizeof(uint32_t) * (size_t)nel));
+  RSB_TRY(rsb_dev_malloc((void **)&heads, sizeof(int) * (size_t)max_groups));
+  RSB_TRY(rsb_dev_malloc((void **)&nh, sizeof(int)));
   k_tag_keys<<<nblocks(nel, 256), 256, 0, st>>>(nel, g->d_order, dt, key);
   RSB_LAUNCH_CHECK();
   int rc = rsb_sort_pairs_u32(g, key, g->d_order, nel, 0, 32);
@@ -527,7 +527,7 @@ extern "C" int rsbgpu_group_by_tag(rsbgpu *g, const int *tag, int tag_is_device,
     std::sort(h.begin(), h.end());
     for (int i = 0; i < n; i++) counts[i] = (i + 1 < n ? h[i + 1] : nel) - h[i];
   }
-  hipFree(tmp), hipFree(key), hipFree(heads), hipFree(nh);
+  rsb_dev_free(tmp), rsb_dev_free(key), rsb_dev_free(heads), rsb_dev_free(nh);
   return 0;
 }
 
@@ -598,20 +598,20 @@ extern "C" int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t 
   if (!g->d_vtx) RSB_FAIL(3, "quality needs vertex ids");
   {
     int *bad = nullptr, hb = 0;
-    RSB_TRY(hipMalloc((void **)&bad, sizeof(int)));
+    RSB_TRY(rsb_dev_malloc((void **)&bad, sizeof(int)));
     RSB_TRY(hipMemsetAsync(bad, 0, sizeof(int), g->stream));
     k_q_check<<<nblocks(nel, 256), 256, 0, g->stream>>>(nel, d_part, nparts, bad);
     RSB_LAUNCH_CHECK();
     RSB_TRY(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));
-    hipFree(bad);
+    rsb_dev_free(bad);
     if (hb) RSB_FAIL(3, "part ids outside [0, %d)", nparts);
   }
   unsigned long long *acc = nullptr;
   int *par = nullptr, *size = nullptr, *ncomp = nullptr;
-  RSB_TRY(hipMalloc((void **)&acc, sizeof(unsigned long long) * 4));
-  RSB_TRY(hipMalloc((void **)&par, sizeof(int) * nel));
-  RSB_TRY(hipMalloc((void **)&size, sizeof(int) * nparts * 2));
+  RSB_TRY(rsb_dev_malloc((void **)&acc, sizeof(unsigned long long) * 4));
+  RSB_TRY(rsb_dev_malloc((void **)&par, sizeof(int) * nel));
+  RSB_TRY(rsb_dev_malloc((void **)&size, sizeof(int) * nparts * 2));
   ncomp = size + nparts;
   RSB_TRY(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 4, g->stream));
   RSB_TRY(hipMemsetAsync(size, 0, sizeof(int) * nparts * 2, g->stream));
@@ -623,8 +623,8 @@ extern "C" int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t 
   int64_t ns = (int64_t)nel * g->nv;
   uint64_t *key = nullptr;
   int *val = nullptr;
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * ns));
-  RSB_TRY(hipMalloc((void **)&val, sizeof(int) * ns));
+  RSB_TRY(rsb_dev_malloc((void **)&key, sizeof(uint64_t) * ns));
+  RSB_TRY(rsb_dev_malloc((void **)&val, sizeof(int) * ns));
   k_q_vkeys<<<nblocks(ns, 256), 256, 0, g->stream>>>(g->d_vtx, d_part, ns, g->nv, key, val);
   RSB_LAUNCH_CHECK();
   int rc = rsb_sort_pairs_u64(g, key, val, ns, 0, 64);
@@ -647,6 +647,6 @@ extern "C" int rsbgpu_quality(rsbgpu *g, const int *d_part, int nparts, int64_t 
   }
   out[0] = (int64_t)ha[0], out[1] = (int64_t)ha[1], out[2] = mx, out[3] = mn, out[4] = tot;
   out[5] = cmx, out[6] = (int64_t)ha[2], out[7] = empty;
-  hipFree(acc), hipFree(par), hipFree(size), hipFree(key), hipFree(val);
+  rsb_dev_free(acc), rsb_dev_free(par), rsb_dev_free(size), rsb_dev_free(key), rsb_dev_free(val);
   return 0;
 }

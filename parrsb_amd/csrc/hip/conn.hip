@@ -321,18 +321,18 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
   if (on_device) {
     dcoord = (double *)coord;
   } else {
-    CON_TRY(hipMalloc((void **)&dcoord, cb));
+    CON_TRY(rsb_dev_malloc((void **)&dcoord, cb));
     CON_TRY(hipMemcpyAsync(dcoord, coord, cb, hipMemcpyHostToDevice, st));
   }
-  CON_TRY(hipMalloc((void **)&x, sizeof(double) * 3 * (size_t)np));
-  CON_TRY(hipMalloc((void **)&dx, sizeof(double) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&lay, sizeof(int) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&val, sizeof(int) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&segp, sizeof(int) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&flag, sizeof(int) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&excl, sizeof(int) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * (size_t)np));
-  CON_TRY(hipMalloc((void **)&err, sizeof(int)));
+  CON_TRY(rsb_dev_malloc((void **)&x, sizeof(double) * 3 * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&dx, sizeof(double) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&lay, sizeof(int) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&val, sizeof(int) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&segp, sizeof(int) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&flag, sizeof(int) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&excl, sizeof(int) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&key, sizeof(uint64_t) * (size_t)np));
+  CON_TRY(rsb_dev_malloc((void **)&err, sizeof(int)));
   {
     const unsigned nb = nblocks(np, 256);
     k_con_points<<<nb, 256, 0, st>>>(dcoord, nel, ndim, nv, x, dx);
@@ -373,7 +373,7 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
     if (on_device) {
       dvtx = vtx_out;
     } else {
-      CON_TRY(hipMalloc((void **)&dvtx, sizeof(long long) * (size_t)np));
+      CON_TRY(rsb_dev_malloc((void **)&dvtx, sizeof(long long) * (size_t)np));
     }
     k_con_out<<<nb, 256, 0, st>>>(np, segp, dvtx);
     CON_TRY(hipGetLastError());
@@ -390,10 +390,10 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
     CON_TRY(hipStreamSynchronize(st));
     if (!herr && ndim == 3) {
       int *els = val;  // np entries, one per corner
-      CON_TRY(hipMalloc((void **)&cnt, sizeof(int) * ((size_t)nseg + 1)));
-      CON_TRY(hipMalloc((void **)&fill, sizeof(int) * ((size_t)nseg + 1)));
-      CON_TRY(hipMalloc((void **)&off, sizeof(int64_t) * ((size_t)nseg + 1)));
-      CON_TRY(hipMalloc((void **)&wide, sizeof(int64_t) * ((size_t)nseg + 1)));
+      CON_TRY(rsb_dev_malloc((void **)&cnt, sizeof(int) * ((size_t)nseg + 1)));
+      CON_TRY(rsb_dev_malloc((void **)&fill, sizeof(int) * ((size_t)nseg + 1)));
+      CON_TRY(rsb_dev_malloc((void **)&off, sizeof(int64_t) * ((size_t)nseg + 1)));
+      CON_TRY(rsb_dev_malloc((void **)&wide, sizeof(int64_t) * ((size_t)nseg + 1)));
       CON_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nseg + 1), st));
       CON_TRY(hipMemsetAsync(fill, 0, sizeof(int) * ((size_t)nseg + 1), st));
       k_con_vcount<<<nb, 256, 0, st>>>(np, segp, cnt);
@@ -411,9 +411,9 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
       // periodic faces (con-periodic.c:258-288): gather the faces' corners,
       // match them on the host, relabel every point on the device
       const int nvf = nv / 2;
-      CON_TRY(hipMalloc((void **)&pdev, sizeof(int) * 2 * (size_t)npinfo));
-      CON_TRY(hipMalloc((void **)&pfx, sizeof(double) * 12 * (size_t)npinfo));
-      CON_TRY(hipMalloc((void **)&pfid, sizeof(int) * 4 * (size_t)npinfo));
+      CON_TRY(rsb_dev_malloc((void **)&pdev, sizeof(int) * 2 * (size_t)npinfo));
+      CON_TRY(rsb_dev_malloc((void **)&pfx, sizeof(double) * 12 * (size_t)npinfo));
+      CON_TRY(rsb_dev_malloc((void **)&pfid, sizeof(int) * 4 * (size_t)npinfo));
       CON_TRY(hipMemcpyAsync(pdev, pel.data(), sizeof(int) * (size_t)npinfo, hipMemcpyHostToDevice, st));
       CON_TRY(hipMemcpyAsync(pdev + npinfo, psf.data(), sizeof(int) * (size_t)npinfo,
                              hipMemcpyHostToDevice, st));
@@ -430,7 +430,7 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
         rc = 1;
       } else if (!gone.empty()) {
         const int ng = (int)gone.size();
-        CON_TRY(hipMalloc((void **)&pgone, sizeof(int) * 2 * (size_t)ng));
+        CON_TRY(rsb_dev_malloc((void **)&pgone, sizeof(int) * 2 * (size_t)ng));
         CON_TRY(hipMemcpyAsync(pgone, gone.data(), sizeof(int) * ng, hipMemcpyHostToDevice, st));
         CON_TRY(hipMemcpyAsync(pgone + ng, root.data(), sizeof(int) * ng, hipMemcpyHostToDevice, st));
         k_con_relabel<<<nb, 256, 0, st>>>(np, ng, pgone, pgone + ng, segp);
@@ -447,12 +447,12 @@ extern "C" int rsbgpu_conn_mesh(int device, long long *vtx_out, const double *co
 done:
 #undef CON_TRY
   if (!on_device) {
-    if (dcoord) hipFree(dcoord);
-    if (dvtx) hipFree(dvtx);
+    if (dcoord) rsb_dev_free(dcoord);
+    if (dvtx) rsb_dev_free(dvtx);
   }
-  hipFree(x), hipFree(dx), hipFree(lay), hipFree(val), hipFree(segp), hipFree(flag),
-      hipFree(excl), hipFree(key), hipFree(err), hipFree(cnt), hipFree(fill), hipFree(off),
-      hipFree(wide), hipFree(pdev), hipFree(pfx), hipFree(pfid), hipFree(pgone);
+  rsb_dev_free(x), rsb_dev_free(dx), rsb_dev_free(lay), rsb_dev_free(val), rsb_dev_free(segp), rsb_dev_free(flag),
+      rsb_dev_free(excl), rsb_dev_free(key), rsb_dev_free(err), rsb_dev_free(cnt), rsb_dev_free(fill), rsb_dev_free(off),
+      rsb_dev_free(wide), rsb_dev_free(pdev), rsb_dev_free(pfx), rsb_dev_free(pfid), rsb_dev_free(pgone);
   if (rc == 2 && msg && msg_len > 0) snprintf(msg, (size_t)msg_len, "device failure in connectivity");
   rsbgpu_close(g);
   return rc;

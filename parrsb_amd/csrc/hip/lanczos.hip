@@ -1127,17 +1127,17 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   size_t vb = (size_t)mpad * sizeof(T);
   if (vb > g->vec_bytes) {
     for (void **q : {&g->d_r, &g->d_p, &g->d_p2, &g->d_w, &g->d_f}) {
-      if (*q) RSB_TRY(hipFree(*q));
+      if (*q) RSB_TRY(rsb_dev_free(*q));
       *q = nullptr;
-      RSB_TRY(hipMalloc(q, vb + 256));
+      RSB_TRY(rsb_dev_malloc(q, vb + 256));
     }
     g->vec_bytes = vb;
   }
   size_t bb = vb * (size_t)miter_max;
   if (bb > g->basis_bytes) {
-    if (g->d_basis) RSB_TRY(hipFree(g->d_basis));
+    if (g->d_basis) RSB_TRY(rsb_dev_free(g->d_basis));
     g->d_basis = nullptr;
-    RSB_TRY(hipMalloc(&g->d_basis, bb + 256));
+    RSB_TRY(rsb_dev_malloc(&g->d_basis, bb + 256));
     g->basis_bytes = bb;
   }
   double *xrow = nullptr;
@@ -1146,23 +1146,23 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   if (nseg > g->lz_nseg_cap || miter_max > g->lz_miter_cap) {
     // iter, k, npass, niter and miter are carved from the status block
     for (void *q : {(void *)L.status, (void *)L.sc, (void *)L.T, (void *)L.y, (void *)L.cnt})
-      if (q) RSB_TRY(hipFree(q));
+      if (q) RSB_TRY(rsb_dev_free(q));
     int cap = std::max(nseg, 1);
-    RSB_TRY(hipMalloc((void **)&L.status, sizeof(int) * cap * 6));
+    RSB_TRY(rsb_dev_malloc((void **)&L.status, sizeof(int) * cap * 6));
     L.iter = L.status + cap, L.k = L.status + 2 * cap, L.npass = L.status + 3 * cap;
     L.niter = L.status + 4 * cap;
-    RSB_TRY(hipMalloc((void **)&L.sc, sizeof(double) * cap * LZ_NSC));
-    RSB_TRY(hipMalloc((void **)&L.T, sizeof(double) * cap * 2 * miter_max));
-    RSB_TRY(hipMalloc((void **)&L.y, sizeof(double) * cap * miter_max));
-    RSB_TRY(hipMalloc((void **)&L.cnt, sizeof(unsigned) * cap));
+    RSB_TRY(rsb_dev_malloc((void **)&L.sc, sizeof(double) * cap * LZ_NSC));
+    RSB_TRY(rsb_dev_malloc((void **)&L.T, sizeof(double) * cap * 2 * miter_max));
+    RSB_TRY(rsb_dev_malloc((void **)&L.y, sizeof(double) * cap * miter_max));
+    RSB_TRY(rsb_dev_malloc((void **)&L.cnt, sizeof(unsigned) * cap));
     g->lz_nseg_cap = cap, g->lz_miter_cap = miter_max;
   }
   if (ntiles > g->lz_tile_cap) {
-    if (L.partial) RSB_TRY(hipFree(L.partial));
-    RSB_TRY(hipMalloc((void **)&L.partial, sizeof(double) * 4 * std::max(ntiles, 1)));
+    if (L.partial) RSB_TRY(rsb_dev_free(L.partial));
+    RSB_TRY(rsb_dev_malloc((void **)&L.partial, sizeof(double) * 4 * std::max(ntiles, 1)));
     g->lz_tile_cap = ntiles;
   }
-  if (!L.running) RSB_TRY(hipMalloc((void **)&L.running, sizeof(int) * 2));
+  if (!L.running) RSB_TRY(rsb_dev_malloc((void **)&L.running, sizeof(int) * 2));
   lz_args a;
   a.act_tiles = nullptr, a.act_segs = nullptr;
   a.seg = g->d_seg, a.tile = g->d_tile, a.seg_tiles = g->d_seg_tiles, a.row_elem = g->d_row_elem;
@@ -1446,10 +1446,10 @@ static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, 
                             int fast) {
   double *dd = nullptr, *de = nullptr, *dy = nullptr, *dl = nullptr;
   if (n < 1 || n > 128) return 3;
-  hipError_t err = hipMalloc((void **)&dd, sizeof(double) * n);
-  if (err == hipSuccess) err = hipMalloc((void **)&de, sizeof(double) * n);
-  if (err == hipSuccess) err = hipMalloc((void **)&dy, sizeof(double) * n);
-  if (err == hipSuccess) err = hipMalloc((void **)&dl, sizeof(double));
+  hipError_t err = rsb_dev_malloc((void **)&dd, sizeof(double) * n);
+  if (err == hipSuccess) err = rsb_dev_malloc((void **)&de, sizeof(double) * n);
+  if (err == hipSuccess) err = rsb_dev_malloc((void **)&dy, sizeof(double) * n);
+  if (err == hipSuccess) err = rsb_dev_malloc((void **)&dl, sizeof(double));
   if (err == hipSuccess) err = hipMemcpy(dd, d, sizeof(double) * n, hipMemcpyHostToDevice);
   if (err == hipSuccess) err = hipMemset(de, 0, sizeof(double) * n);
   if (err == hipSuccess && n > 1)
@@ -1466,9 +1466,9 @@ static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, 
   if (err == hipSuccess) err = hipDeviceSynchronize();
   if (err == hipSuccess) err = hipMemcpy(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(lam, dl, sizeof(double), hipMemcpyDeviceToHost);
-  if (dd) (void)hipFree(dd);
-  if (de) (void)hipFree(de);
-  if (dy) (void)hipFree(dy);
-  if (dl) (void)hipFree(dl);
+  if (dd) (void)rsb_dev_free(dd);
+  if (de) (void)rsb_dev_free(de);
+  if (dy) (void)rsb_dev_free(dy);
+  if (dl) (void)rsb_dev_free(dl);
   return err == hipSuccess ? 0 : 2;
 }

@@ -540,9 +540,9 @@ static int lws_forward(rsbgpu *g, const float *P, lws_ws &w, float level) {
 static int lws_ws_get(rsbgpu *g, bool train, lws_ws &w) {
   size_t need = lws_ws_bytes(g->mpad, g->nseg, train);
   if (need > g->lws_ws_bytes) {
-    if (g->d_lws_ws) RSB_TRY(hipFree(g->d_lws_ws));
+    if (g->d_lws_ws) RSB_TRY(rsb_dev_free(g->d_lws_ws));
     g->d_lws_ws = nullptr;
-    RSB_TRY(hipMalloc(&g->d_lws_ws, need));
+    RSB_TRY(rsb_dev_malloc(&g->d_lws_ws, need));
     g->lws_ws_bytes = need;
   }
   w = lws_carve(g->d_lws_ws, g->mpad, g->nseg, train);
@@ -701,9 +701,9 @@ extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const
   }
   float *dP = nullptr, *dG = nullptr, *dY = nullptr;
   double L = 0;
-  if (!rc && hipMalloc((void **)&dP, sizeof(float) * LWS_NPARAM) != hipSuccess) rc = 2;
-  if (!rc && hipMalloc((void **)&dG, sizeof(float) * LWS_NPARAM) != hipSuccess) rc = 2;
-  if (!rc && hipMalloc((void **)&dY, sizeof(float) * g->mpad) != hipSuccess) rc = 2;
+  if (!rc && rsb_dev_malloc((void **)&dP, sizeof(float) * LWS_NPARAM) != hipSuccess) rc = 2;
+  if (!rc && rsb_dev_malloc((void **)&dG, sizeof(float) * LWS_NPARAM) != hipSuccess) rc = 2;
+  if (!rc && rsb_dev_malloc((void **)&dY, sizeof(float) * g->mpad) != hipSuccess) rc = 2;
   if (!rc && hipMemcpy(dP, params, sizeof(float) * LWS_NPARAM, hipMemcpyHostToDevice) != hipSuccess) rc = 2;
   if (!rc && hipMemset(dG, 0, sizeof(float) * LWS_NPARAM) != hipSuccess) rc = 2;
   if (!rc) rc = rsbgpu_lws_train_step(g, dP, dG, level, &L);
@@ -711,9 +711,9 @@ extern "C" int rsbgpu_lws_test(int M, const double *coord_in, float level, const
   if (!rc && hipMemcpy(y, dY, sizeof(float) * M, hipMemcpyDeviceToHost) != hipSuccess) rc = 2;
   if (!rc && hipMemcpy(grad, dG, sizeof(float) * LWS_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) rc = 2;
   *loss = L;
-  if (dP) hipFree(dP);
-  if (dG) hipFree(dG);
-  if (dY) hipFree(dY);
+  if (dP) rsb_dev_free(dP);
+  if (dG) rsb_dev_free(dG);
+  if (dY) rsb_dev_free(dY);
   rsbgpu_close(g);
   return rc;
 }

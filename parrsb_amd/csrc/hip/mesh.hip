@@ -48,6 +48,14 @@ extern "C" int rsbgpu_open(rsbgpu **out, int device) {
   (void)hipGetLastError();  // start from a clean error state
   rsbgpu *g = new rsbgpu();
   g->device = device;
+  hipEvent_t ev[4];
+  if (rsb_arena_stream(device, &g->stream, ev)) {
+    // allocation-free call: the stream and events set up for this device
+    g->ev0 = ev[0], g->ev1 = ev[1], g->lz_ev0 = ev[2], g->lz_ev1 = ev[3];
+    g->borrowed = true;
+    *out = g;
+    return 0;
+  }
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
       hipEventCreate(&g->lz_ev0) != hipSuccess || hipEventCreate(&g->lz_ev1) != hipSuccess) {
@@ -60,7 +68,7 @@ extern "C" int rsbgpu_open(rsbgpu **out, int device) {
 
 #define FREE(p)                 \
   do {                          \
-    if (p) hipFree((void *)p);  \
+    if (p) rsb_dev_free((void *)p);  \
     p = nullptr;                \
   } while (0)
 
@@ -117,13 +125,15 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_v1);
   FREE(g->d_iscratch);
   for (auto &kv : g->scratch)
-    if (kv.second.first) hipFree(kv.second.first);
+    if (kv.second.first) rsb_dev_free(kv.second.first);
   g->scratch.clear();
-  if (g->ev0) hipEventDestroy(g->ev0);
-  if (g->ev1) hipEventDestroy(g->ev1);
-  if (g->lz_ev0) hipEventDestroy(g->lz_ev0);
-  if (g->lz_ev1) hipEventDestroy(g->lz_ev1);
-  if (g->stream) hipStreamDestroy(g->stream);
+  if (!g->borrowed) {
+    if (g->ev0) hipEventDestroy(g->ev0);
+    if (g->ev1) hipEventDestroy(g->ev1);
+    if (g->lz_ev0) hipEventDestroy(g->lz_ev0);
+    if (g->lz_ev1) hipEventDestroy(g->lz_ev1);
+    if (g->stream) hipStreamDestroy(g->stream);
+  }
   delete g;
 }
 
@@ -271,10 +281,10 @@ static int build_ranks(rsbgpu *g) {
   int n = g->nel;
   uint64_t *key = nullptr;
   int *val = nullptr, *flag = nullptr, *excl = nullptr;
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * (n + 1)));
-  RSB_TRY(hipMalloc((void **)&val, sizeof(int) * (n + 1)));
-  RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * (n + 1)));
-  RSB_TRY(hipMalloc((void **)&excl, sizeof(int) * (n + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&key, sizeof(uint64_t) * (n + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&val, sizeof(int) * (n + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&flag, sizeof(int) * (n + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&excl, sizeof(int) * (n + 1)));
   RSB_TRY(hipMemsetAsync(g->d_rank, 0, sizeof(uint32_t) * 3 * (size_t)n, g->stream));
   uint32_t maxrank = 0;
   for (int d = 0; d < g->ndim; d++) {
@@ -302,7 +312,7 @@ static int build_ranks(rsbgpu *g) {
   int bits = 1;
   while (bits < 32 && (1u << bits) <= maxrank) bits++;
   g->rank_bits = bits;
-  hipFree(key), hipFree(val), hipFree(flag), hipFree(excl);
+  rsb_dev_free(key), rsb_dev_free(val), rsb_dev_free(flag), rsb_dev_free(excl);
   return 0;
 }
 
@@ -314,11 +324,11 @@ static int build_graph(rsbgpu *g) {
   unsigned long long *vmax = nullptr;
   int64_t *rowlen = nullptr;
   int *ovf = nullptr;
-  RSB_TRY(hipMalloc((void **)&key, sizeof(uint64_t) * (ns + 1)));
-  RSB_TRY(hipMalloc((void **)&val, sizeof(int) * (ns + 1)));
-  RSB_TRY(hipMalloc((void **)&flag, sizeof(int) * (ns + 1)));
-  RSB_TRY(hipMalloc((void **)&excl, sizeof(int) * (ns + 1)));
-  RSB_TRY(hipMalloc((void **)&vmax, sizeof(unsigned long long) * 2));
+  RSB_TRY(rsb_dev_malloc((void **)&key, sizeof(uint64_t) * (ns + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&val, sizeof(int) * (ns + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&flag, sizeof(int) * (ns + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&excl, sizeof(int) * (ns + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&vmax, sizeof(unsigned long long) * 2));
   RSB_TRY(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 2, g->stream));
   k_slot_keys<<<std::min<int64_t>(nblocks(ns, RSB_BLOCK), 4096), RSB_BLOCK, 0, g->stream>>>(
       g->d_vtx, ns, key, val, vmax);
@@ -341,19 +351,19 @@ static int build_graph(rsbgpu *g) {
                          g->stream));
   RSB_TRY(hipStreamSynchronize(g->stream));
   int ng = ng_parts[0] + ng_parts[1];
-  RSB_TRY(hipMalloc((void **)&slot_group, sizeof(int) * (ns + 1)));
-  RSB_TRY(hipMalloc((void **)&gstart, sizeof(int) * (ng + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&slot_group, sizeof(int) * (ns + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&gstart, sizeof(int) * (ng + 1)));
   k_group_scatter<<<nblocks(ns, 256), 256, 0, g->stream>>>(flag, excl, val, ns, slot_group,
                                                            gstart);
   RSB_LAUNCH_CHECK();
   int nsi = (int)ns;
   RSB_TRY(hipMemcpyAsync(gstart + ng, &nsi, sizeof(int), hipMemcpyHostToDevice, g->stream));
-  hipFree(key), hipFree(flag), hipFree(excl);
+  rsb_dev_free(key), rsb_dev_free(flag), rsb_dev_free(excl);
   key = nullptr;
   // count, scan, fill
-  RSB_TRY(hipMalloc((void **)&rowlen, sizeof(int64_t) * (n + 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&rowlen, sizeof(int64_t) * (n + 1)));
   RSB_TRY(dev_alloc(g, g->d_growptr, (size_t)n + 1));
-  RSB_TRY(hipMalloc((void **)&ovf, sizeof(int)));
+  RSB_TRY(rsb_dev_malloc((void **)&ovf, sizeof(int)));
   RSB_TRY(hipMemsetAsync(ovf, 0, sizeof(int), g->stream));
   RSB_TRY(hipMemsetAsync(rowlen + n, 0, sizeof(int64_t), g->stream));
   k_dual_rows<false><<<nblocks(n, 128), 128, 0, g->stream>>>(
@@ -379,8 +389,8 @@ static int build_graph(rsbgpu *g) {
   RSB_TRY(hipMemcpyAsync(&ws, vmax, sizeof(ws), hipMemcpyDeviceToHost, g->stream));
   RSB_TRY(hipStreamSynchronize(g->stream));
   g->gwsum = (int64_t)ws;
-  hipFree(val), hipFree(slot_group), hipFree(gstart), hipFree(rowlen), hipFree(ovf),
-      hipFree(vmax);
+  rsb_dev_free(val), rsb_dev_free(slot_group), rsb_dev_free(gstart), rsb_dev_free(rowlen), rsb_dev_free(ovf),
+      rsb_dev_free(vmax);
   if (hovf) RSB_FAIL(3, "two elements share more than 15 vertex ids");
   return 0;
 }
@@ -461,14 +471,14 @@ extern "C" int rsbgpu_get_graph(rsbgpu *g, int64_t *rowptr, int *col, int *wgt) 
                          hipMemcpyDeviceToHost, g->stream));
   if (g->gnnz > 0) {
     int *dc = nullptr, *dw = nullptr;
-    RSB_TRY(hipMalloc((void **)&dc, sizeof(int) * g->gnnz));
-    RSB_TRY(hipMalloc((void **)&dw, sizeof(int) * g->gnnz));
+    RSB_TRY(rsb_dev_malloc((void **)&dc, sizeof(int) * g->gnnz));
+    RSB_TRY(rsb_dev_malloc((void **)&dw, sizeof(int) * g->gnnz));
     k_unpack_adj<<<nblocks(g->gnnz, 256), 256, 0, g->stream>>>(g->d_gadj, g->gnnz, dc, dw);
     RSB_LAUNCH_CHECK();
     RSB_TRY(hipMemcpyAsync(col, dc, sizeof(int) * g->gnnz, hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipMemcpyAsync(wgt, dw, sizeof(int) * g->gnnz, hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));
-    hipFree(dc), hipFree(dw);
+    rsb_dev_free(dc), rsb_dev_free(dw);
   }
   RSB_TRY(hipStreamSynchronize(g->stream));
   return 0;
@@ -510,7 +520,7 @@ extern "C" int rsbgpu_set_fiedler(rsbgpu *g, const double *f) {
 extern "C" int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_out) {
   RSB_TRY(hipSetDevice(g->device));
   int *d = nullptr;
-  RSB_TRY(hipMalloc((void **)&d, sizeof(int) * (size_t)(n > 0 ? n : 1)));
+  RSB_TRY(rsb_dev_malloc((void **)&d, sizeof(int) * (size_t)(n > 0 ? n : 1)));
   if (n > 0) RSB_TRY(hipMemcpy(d, host, sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
   *dev_out = d;
   return 0;
@@ -519,7 +529,7 @@ extern "C" int rsbgpu_upload_ints(rsbgpu *g, const int *host, int n, int **dev_o
 extern "C" int rsbgpu_upload(rsbgpu *g, const void *host, size_t bytes, void **dev_out) {
   RSB_TRY(hipSetDevice(g->device));
   void *d = nullptr;
-  RSB_TRY(hipMalloc(&d, bytes > 0 ? bytes : 1));
+  RSB_TRY(rsb_dev_malloc(&d, bytes > 0 ? bytes : 1));
   if (bytes > 0) RSB_TRY(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
   *dev_out = d;
   return 0;
@@ -532,5 +542,5 @@ extern "C" int rsbgpu_download(rsbgpu *g, const void *dev, size_t bytes, void *h
 }
 
 extern "C" void rsbgpu_free(void *dev) {
-  if (dev) hipFree(dev);
+  if (dev) rsb_dev_free(dev);
 }

@@ -852,9 +852,9 @@ struct mg_hier {
   std::vector<std::vector<int>> h_off, h_n;
   ~mg_hier() {
     for (auto &l : lv) {
-      hipFree(l.off), hipFree(l.n), hipFree(l.rseg), hipFree(l.rp), hipFree(l.ci);
-      hipFree(l.w), hipFree(l.diag), hipFree(l.r0), hipFree(l.sa), hipFree(l.sb), hipFree(l.u);
-      hipFree(l.soff), hipFree(l.rlen), hipFree(l.sci), hipFree(l.sw), hipFree(l.spk);
+      rsb_dev_free(l.off), rsb_dev_free(l.n), rsb_dev_free(l.rseg), rsb_dev_free(l.rp), rsb_dev_free(l.ci);
+      rsb_dev_free(l.w), rsb_dev_free(l.diag), rsb_dev_free(l.r0), rsb_dev_free(l.sa), rsb_dev_free(l.sb), rsb_dev_free(l.u);
+      rsb_dev_free(l.soff), rsb_dev_free(l.rlen), rsb_dev_free(l.sci), rsb_dev_free(l.sw), rsb_dev_free(l.spk);
     }
   }
 };
@@ -894,7 +894,7 @@ struct mg_scope {
   explicit mg_scope(hipStream_t s) : st(s) {}
   template <typename T>
   hipError_t alloc(T *&ptr, size_t bytes) {
-    hipError_t e = hipMalloc((void **)&ptr, bytes);
+    hipError_t e = rsb_dev_malloc((void **)&ptr, bytes);
     if (e == hipSuccess) bufs.push_back((void *)ptr);
     else ptr = nullptr;
     return e;
@@ -908,7 +908,7 @@ struct mg_scope {
     (void)hipStreamSynchronize(st);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void *q : bufs) (void)hipFree(q);
+    for (void *q : bufs) (void)rsb_dev_free(q);
   }
 };
 
@@ -927,15 +927,15 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
   {
     mg_level_dev &L0 = H.lv[0];
     L0.m = g->mpad;
-    RSB_TRY(hipMalloc((void **)&L0.off, sizeof(int) * nseg));
-    RSB_TRY(hipMalloc((void **)&L0.n, sizeof(int) * nseg));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.off, sizeof(int) * nseg));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.n, sizeof(int) * nseg));
     RSB_TRY(hipMemcpyAsync(L0.off, off0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     RSB_TRY(hipMemcpyAsync(L0.n, n0.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     L0.diag = diag0;  // owned by the hierarchy (the caller put it there)
-    RSB_TRY(hipMalloc((void **)&L0.r0, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
-    RSB_TRY(hipMalloc((void **)&L0.sa, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
-    RSB_TRY(hipMalloc((void **)&L0.sb, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
-    RSB_TRY(hipMalloc((void **)&L0.u, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.r0, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.sa, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.sb, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&L0.u, sizeof(double) * (size_t)std::max<int64_t>(g->mpad, 1)));
   }
   mg_sell S{g->d_slice_off, g->d_ell, g->d_rowlen, diag0, nullptr, nullptr, nullptr, nullptr,
             nullptr,          nullptr};
@@ -954,19 +954,19 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     H.lv.emplace_back();
     mg_level_dev &F = H.lv[l], &C = H.lv[l + 1];
     C.m = mc;
-    RSB_TRY(hipMalloc((void **)&C.off, sizeof(int) * nseg));
-    RSB_TRY(hipMalloc((void **)&C.n, sizeof(int) * nseg));
+    RSB_TRY(rsb_dev_malloc((void **)&C.off, sizeof(int) * nseg));
+    RSB_TRY(rsb_dev_malloc((void **)&C.n, sizeof(int) * nseg));
     RSB_TRY(hipMemcpyAsync(C.off, oc.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     RSB_TRY(hipMemcpyAsync(C.n, nc.data(), sizeof(int) * nseg, hipMemcpyHostToDevice, st));
-    RSB_TRY(hipMalloc((void **)&C.rseg, sizeof(int) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.rseg, sizeof(int) * (size_t)mc));
     k_mg_rseg<<<nseg, 256, 0, st>>>(nseg, C.off, C.n, C.rseg);
     RSB_LAUNCH_CHECK();
-    RSB_TRY(hipMalloc((void **)&C.r0, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.sa, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.sb, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.u, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.diag, sizeof(double) * (size_t)mc));
-    RSB_TRY(hipMalloc((void **)&C.rp, sizeof(int64_t) * (size_t)(mc + 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&C.r0, sizeof(double) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.sa, sizeof(double) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.sb, sizeof(double) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.u, sizeof(double) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.diag, sizeof(double) * (size_t)mc));
+    RSB_TRY(rsb_dev_malloc((void **)&C.rp, sizeof(int64_t) * (size_t)(mc + 1)));
     // raw entries
     mg_scope tmp(st);  // this level's temporaries
     int64_t *cnt = nullptr, *roff = nullptr;
@@ -1013,8 +1013,8 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     int64_t nnz = 0;
     RSB_TRY(hipMemcpyAsync(&nnz, C.rp + mc, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     RSB_TRY(hipStreamSynchronize(st));
-    RSB_TRY(hipMalloc((void **)&C.ci, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)));
-    RSB_TRY(hipMalloc((void **)&C.w, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&C.ci, sizeof(int) * (size_t)std::max<int64_t>(nnz, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&C.w, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
     if (nr > 0) k_mg_compact<<<nblocks(nr, 256), 256, 0, st>>>(nr, key, flag, pos, wsum, C.ci, C.w);
     k_mg_rowdiag<<<nblocks(mc, 256), 256, 0, st>>>(mc, C.rp, C.w, C.diag);
     RSB_LAUNCH_CHECK();
@@ -1022,8 +1022,8 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
     {
       const int64_t nsl = (mc + RSB_WAVE - 1) / RSB_WAVE;
       int64_t *sbytes = nullptr;
-      RSB_TRY(hipMalloc((void **)&C.rlen, sizeof(int) * (size_t)(nsl * RSB_WAVE)));
-      RSB_TRY(hipMalloc((void **)&C.soff, sizeof(int64_t) * (size_t)(nsl + 1)));
+      RSB_TRY(rsb_dev_malloc((void **)&C.rlen, sizeof(int) * (size_t)(nsl * RSB_WAVE)));
+      RSB_TRY(rsb_dev_malloc((void **)&C.soff, sizeof(int64_t) * (size_t)(nsl + 1)));
       RSB_TRY(tmp.alloc(sbytes, sizeof(int64_t) * (size_t)(nsl + 1)));
       k_mg_slice_len<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.rlen, sbytes);
       RSB_LAUNCH_CHECK();
@@ -1043,10 +1043,10 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
       RSB_TRY(hipMemcpyAsync(&hmax, dmax, sizeof(int), hipMemcpyDeviceToHost, st));
       RSB_TRY(hipStreamSynchronize(st));
       if (mc <= (1 << 24) && hmax < 256) {
-        RSB_TRY(hipMalloc((void **)&C.spk, sizeof(uint32_t) * (size_t)std::max<int64_t>(ns, 1)));
+        RSB_TRY(rsb_dev_malloc((void **)&C.spk, sizeof(uint32_t) * (size_t)std::max<int64_t>(ns, 1)));
       } else {
-        RSB_TRY(hipMalloc((void **)&C.sci, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
-        RSB_TRY(hipMalloc((void **)&C.sw, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+        RSB_TRY(rsb_dev_malloc((void **)&C.sci, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
+        RSB_TRY(rsb_dev_malloc((void **)&C.sw, sizeof(int) * (size_t)std::max<int64_t>(ns, 1)));
       }
       k_mg_sell_fill<<<nblocks(nsl * RSB_WAVE, 256), 256, 0, st>>>(mc, nsl, C.rp, C.ci, C.w,
                                                                     C.soff, C.sci, C.sw, C.spk);
@@ -1094,7 +1094,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   g->mg = hp;
   mg_hier &H = *hp;
   H.lv.emplace_back();
-  RSB_TRY(hipMalloc((void **)&H.lv[0].diag, vb));
+  RSB_TRY(rsb_dev_malloc((void **)&H.lv[0].diag, vb));
   double *const diag0 = H.lv[0].diag;
   double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *z0 = nullptr, *p = nullptr,
          *w = nullptr, *Wb = nullptr, *Pb = nullptr, *sc = nullptr, *partial = nullptr,

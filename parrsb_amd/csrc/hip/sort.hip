@@ -8,20 +8,20 @@
 
 static int ensure_tmp(rsbgpu *g, size_t bytes) {
   if (bytes <= g->sort_tmp_bytes && g->d_sort_tmp) return 0;
-  if (g->d_sort_tmp) RSB_TRY(hipFree(g->d_sort_tmp));
+  if (g->d_sort_tmp) RSB_TRY(rsb_dev_free(g->d_sort_tmp));
   g->d_sort_tmp = nullptr;
-  RSB_TRY(hipMalloc(&g->d_sort_tmp, bytes + 256));
+  RSB_TRY(rsb_dev_malloc(&g->d_sort_tmp, bytes + 256));
   g->sort_tmp_bytes = bytes + 256;
   return 0;
 }
 
 static int ensure_kv(rsbgpu *g, size_t n) {
   if (n <= g->sort_cap && g->d_k1) return 0;
-  if (g->d_k1) RSB_TRY(hipFree(g->d_k1));
-  if (g->d_v1) RSB_TRY(hipFree(g->d_v1));
+  if (g->d_k1) RSB_TRY(rsb_dev_free(g->d_k1));
+  if (g->d_v1) RSB_TRY(rsb_dev_free(g->d_v1));
   g->d_k1 = nullptr, g->d_v1 = nullptr;
-  RSB_TRY(hipMalloc((void **)&g->d_k1, n * sizeof(uint64_t)));
-  RSB_TRY(hipMalloc((void **)&g->d_v1, n * sizeof(int)));
+  RSB_TRY(rsb_dev_malloc((void **)&g->d_k1, n * sizeof(uint64_t)));
+  RSB_TRY(rsb_dev_malloc((void **)&g->d_v1, n * sizeof(int)));
   g->sort_cap = n;
   return 0;
 }

@@ -12,6 +12,21 @@
 #include "rsbgpu.h"
 #include "common.hpp"
 
+// ---------------------------------------------------------------------------
+// Device memory of the library.  Every device allocation goes through these
+// two functions: hipMalloc / hipFree normally; inside an allocation-free call
+// (noalloc.hip) a bump allocator over a fixed device array, whose frees are
+// no-ops until the call ends.
+hipError_t rsb_dev_malloc(void **p, size_t bytes);
+hipError_t rsb_dev_free(const void *p);
+template <typename T>
+inline hipError_t rsb_dev_malloc(T **p, size_t bytes) {
+  return rsb_dev_malloc((void **)p, bytes);
+}
+// Inside an allocation-free call on `device`: the stream and the four timing
+// events set up once for that device (returns false otherwise).
+bool rsb_arena_stream(int device, hipStream_t *stream, hipEvent_t *ev);
+
 #define RSB_TILE 1024   // Lanczos rows per block (4 rows per thread)
 #define RSB_BLOCK 256
 #define RSB_MAXC 64     // max candidate neighbours gathered per element
@@ -44,6 +59,7 @@ enum {
 struct rsbgpu {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool borrowed = false;  // stream and events belong to the allocation-free set-up
   char err[512] = {0};
 
   // mesh
@@ -159,12 +175,12 @@ template <typename T>
 inline hipError_t dev_ensure(T *&p, size_t &cap_elems, size_t need) {
   if (need <= cap_elems && p) return hipSuccess;
   if (p) {
-    hipError_t e = hipFree(p);
+    hipError_t e = rsb_dev_free(p);
     if (e != hipSuccess) return e;
     p = nullptr;
   }
   size_t n = need ? need : 1;
-  hipError_t e = hipMalloc((void **)&p, n * sizeof(T));
+  hipError_t e = rsb_dev_malloc((void **)&p, n * sizeof(T));
   if (e == hipSuccess) cap_elems = n;
   return e;
 }
@@ -179,11 +195,11 @@ inline hipError_t dev_alloc(rsbgpu *g, T *&p, size_t n) {
   auto it = g->cap.find((const void *)&p);
   if (p && it != g->cap.end() && it->second >= bytes) return hipSuccess;
   if (p) {
-    hipError_t e = hipFree(p);
+    hipError_t e = rsb_dev_free(p);
     if (e != hipSuccess) return e;
     p = nullptr;
   }
-  hipError_t e = hipMalloc((void **)&p, bytes);
+  hipError_t e = rsb_dev_malloc((void **)&p, bytes);
   if (e == hipSuccess) g->cap[(const void *)&p] = bytes;
   return e;
 }
@@ -200,11 +216,11 @@ inline hipError_t rsb_scratch(rsbgpu *g, const void *file, int line, size_t byte
     return hipSuccess;
   }
   if (s.first) {
-    hipError_t e = hipFree(s.first);
+    hipError_t e = rsb_dev_free(s.first);
     s.first = nullptr, s.second = 0;
     if (e != hipSuccess) return e;
   }
-  hipError_t e = hipMalloc(&s.first, bytes ? bytes : 1);
+  hipError_t e = rsb_dev_malloc(&s.first, bytes ? bytes : 1);
   if (e != hipSuccess) {
     s.first = nullptr;
     return e;

@@ -430,6 +430,44 @@ int parrsb_amd_part_mesh_dev(int *d_part, const long long *d_vtx, const double *
   return run(d_part, NULL, d_vtx, d_xyz, NULL, nel, nv, options, options->nparts, 1);
 }
 
+int parrsb_amd_part_mesh_dev_noalloc(int *d_part, const long long *d_vtx, const double *d_xyz,
+                                     int nel, int nv, parrsb_options *options) {
+  update_options(options);
+  memset(&last_report, 0, sizeof(last_report));
+  if (options->tagged == 1) {
+    set_error("tagged partitioning takes host arrays (parrsb_part_mesh)");
+    return 3;
+  }
+  if (nel < 0 || nel > PARRSB_AMD_NOALLOC_MAX_NEL) {
+    set_error("allocation-free partitioning takes at most %d elements (got %d)",
+              PARRSB_AMD_NOALLOC_MAX_NEL, nel);
+    return 3;
+  }
+  if (options->partitioner == 0 && options->rsb_algo == 1 &&
+      nel > PARRSB_AMD_NOALLOC_MAX_NEL_MG) {
+    set_error("allocation-free partitioning with rsb_algo = 1 takes at most %d elements "
+              "(got %d)",
+              PARRSB_AMD_NOALLOC_MAX_NEL_MG, nel);
+    return 3;
+  }
+  if (options->nparts > PARRSB_AMD_NOALLOC_MAX_NEL) {
+    set_error("allocation-free partitioning makes at most %d parts (got %d)",
+              PARRSB_AMD_NOALLOC_MAX_NEL, options->nparts);
+    return 3;
+  }
+  char msg[256] = {0};
+  int rc = rsbgpu_noalloc_begin(device_index(), msg, (int)sizeof(msg));
+  if (rc) {
+    set_error("%s", msg);
+    return rc;
+  }
+  rc = run(d_part, NULL, d_vtx, d_xyz, NULL, nel, nv, options, options->nparts, 1);
+  long long high = 0;
+  last_report.noalloc_bytes = (double)rsbgpu_noalloc_end(&high);
+  last_report.noalloc_high = (double)high;
+  return rc;
+}
+
 int parrsb_conn_mesh(long long *vtx, double *coord, unsigned nel, unsigned nDim,
                      long long *periodicInfo, int nPeriodicFaces, double tol, MPI_Comm comm) {
   last_error[0] = 0;

@@ -154,3 +154,25 @@ def test_fortran_conn_mesh_binding_checks_arguments(monkeypatch):
            C.byref(ints["npf"]), C.byref(tol), C.byref(ints["comm"]), C.byref(ints["err"]))
         assert ints["err"].value == 3
         assert "nDim" in pa.last_error()
+
+
+def test_noalloc_entry_point_rejects_sizes_beyond_its_maximum(monkeypatch):
+    """parrsb_amd_part_mesh_dev_noalloc (no counterpart in the reference)
+    holds every buffer in a fixed device array: meshes above its element
+    limit, more parts than that, and tagged mode return 3 before any device
+    work, with a message."""
+    for k in list(os.environ):
+        if k.startswith("PARRSB_"):
+            monkeypatch.delenv(k)
+    fn = pa.lib().parrsb_amd_part_mesh_dev_noalloc
+    opts = pa.default_options(verbose_level=0, nparts=4)
+    assert fn(None, None, None, 65537, 8, C.byref(opts)) == 3
+    assert "at most 65536 elements" in pa.last_error()
+    opts = pa.default_options(verbose_level=0, nparts=65537)
+    assert fn(None, None, None, 100, 8, C.byref(opts)) == 3
+    assert "at most 65536 parts" in pa.last_error()
+    opts = pa.default_options(verbose_level=0, nparts=4, tagged=1)
+    assert fn(None, None, None, 100, 8, C.byref(opts)) == 3
+    opts = pa.default_options(verbose_level=0, nparts=4, rsb_algo=1)
+    assert fn(None, None, None, 32769, 8, C.byref(opts)) == 3
+    assert "rsb_algo = 1 takes at most 32768 elements" in pa.last_error()
