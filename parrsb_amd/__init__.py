@@ -148,6 +148,7 @@ SIGNATURES = {
     "rsbgpu_get_order": (I, [P, P]),
     "rsbgpu_get_fiedler": (I, [P, P]),
     "rsbgpu_set_fiedler": (I, [P, P]),
+    "rsbgpu_level_form": (I, [P, P, P]),
     "rsbgpu_noalloc_begin": (I, [I, P, I]),
     "rsbgpu_noalloc_end": (C.c_longlong, [P]),
     "parrsb_amd_part_mesh_dev_noalloc": (I, [P, P, P, I, I, C.POINTER(Options)]),
@@ -556,6 +557,12 @@ class Device:
         f = np.zeros(self.nel)
         self._chk(lib().rsbgpu_get_fiedler(self.h, _p(f)))
         return f
+
+    def level_form(self):
+        """(tiles on the global path, 1 if the level uses the vertex form)."""
+        a, b = C.c_int(), C.c_int()
+        self._chk(lib().rsbgpu_level_form(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_fiedler_values(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)

@@ -401,18 +401,18 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
   }
 }
 
+// Stages q = beta (p_old - m) + r (q = r on the first iteration of a pass)
+// for the tile's rows and its nh halo rows into LDS (tile rows first), and r
+// of the tile's rows; rows past the segment's end get q = 0.  SU indices per
+// thread and step, all their loads issued before any value is used (the halo
+// rows are gathers).  Ends with a barrier.
 template <typename T, bool FIRST>
-__device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const tiled_args &ta,
-                                      int nh, T *ql, T *rl, const T *__restrict__ r,
-                                      const T *__restrict__ pold, T *__restrict__ pnew,
-                                      T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
-                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+__device__ inline void stage_window(const tile_t &t, const int *__restrict__ halo, int nh, T *ql,
+                                    T *rl, const T *__restrict__ r, const T *__restrict__ pold,
+                                    T beta, T m) {
   const int nrows = t.row1 - t.row0;
-  const int64_t ho = ta.halo_off[t.ti];
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   const int tot = nrows + nh;
-  // staging: SU indices per thread and step, all their loads issued before
-  // any value is used (the halo rows are gathers)
   constexpr int SU = 4;
   for (int i0 = threadIdx.x; i0 < tot; i0 += SU * blockDim.x) {
     int gr[SU];
@@ -420,7 +420,7 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
 #pragma unroll
     for (int k = 0; k < SU; k++) {
       const int idx = i0 + k * blockDim.x;
-      gr[k] = idx < nrows ? t.row0 + idx : (idx < tot ? ta.halo[ho + idx - nrows] : t.row0);
+      gr[k] = idx < nrows ? t.row0 + idx : (idx < tot ? halo[idx - nrows] : t.row0);
     }
 #pragma unroll
     for (int k = 0; k < SU; k++) {
@@ -440,6 +440,17 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
     }
   }
   __syncthreads();
+}
+
+template <typename T, bool FIRST>
+__device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const tiled_args &ta,
+                                      int nh, T *ql, T *rl, const T *__restrict__ r,
+                                      const T *__restrict__ pold, T *__restrict__ pnew,
+                                      T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
+                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+  const int nrows = t.row1 - t.row0;
+  const int nreal = min(nrows, t.pstart + t.n - t.row0);
+  stage_window<T, FIRST>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta, m);
   // two rows per thread and step (measured 15% faster than one at a time
   // on the 256^3 workload: more loads in flight per lane)
   apply_rows_batched<T, 2>(t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw);
@@ -473,6 +484,142 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
     lz_apply_rows<T, false>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off, ta.slice_w,
                             ta.ell, sq, swq, sw);
   }
+  double v[3] = {sq.value(), swq.value(), sw.value()};
+  publish<3>(a, t, v);
+}
+
+// Vertex form of the tiles (tiles.hip, rsb_build_vtiles): after staging q
+// for the window, S(v) = sum of q over the slots at each of the tile's
+// vertices (one aligned load of the vertex's nv window indices, ascending,
+// padded with the index of an LDS zero), then per row
+// (L q)_e = sum over its slots of (mult(v) q_e - S(v)) -- the reference's
+// gather-scatter form (laplacian.c:159-205), in a fixed order of summation.
+struct vtile_args {
+  const int *vt_n;
+  const int64_t *vt_loff;
+  const uint16_t *list, *rowv;
+  const int *nhalo;
+  const int64_t *halo_off;
+  const int *halo;
+  int nv;
+};
+
+template <typename T, int W>
+__device__ inline void vt_sums(const uint16_t *__restrict__ Ls, int nvert, const T *ql, T *S,
+                               uint8_t *M) {
+  // two vertices per thread and step, both loads issued first
+  for (int v0 = threadIdx.x; v0 < nvert; v0 += 2 * blockDim.x) {
+    const int v1 = v0 + blockDim.x;
+    uint16_t e[2][W];
+    if (W == 8) {
+      const u32x4 a = __builtin_nontemporal_load((const u32x4 *)(Ls + (int64_t)v0 * 8));
+      const u32x4 b = v1 < nvert ? __builtin_nontemporal_load((const u32x4 *)(Ls + (int64_t)v1 * 8))
+                                 : (u32x4){0u, 0u, 0u, 0u};
+      const unsigned wa[4] = {a.x, a.y, a.z, a.w}, wb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        e[0][2 * k] = (uint16_t)(wa[k] & 0xffffu), e[0][2 * k + 1] = (uint16_t)(wa[k] >> 16);
+        e[1][2 * k] = (uint16_t)(wb[k] & 0xffffu), e[1][2 * k + 1] = (uint16_t)(wb[k] >> 16);
+      }
+    } else {
+      const u32x2 a = __builtin_nontemporal_load((const u32x2 *)(Ls + (int64_t)v0 * 4));
+      const u32x2 b = v1 < nvert ? __builtin_nontemporal_load((const u32x2 *)(Ls + (int64_t)v1 * 4))
+                                 : (u32x2){0u, 0u};
+      const unsigned wa[2] = {a.x, a.y}, wb[2] = {b.x, b.y};
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        e[0][2 * k] = (uint16_t)(wa[k] & 0xffffu), e[0][2 * k + 1] = (uint16_t)(wa[k] >> 16);
+        e[1][2 * k] = (uint16_t)(wb[k] & 0xffffu), e[1][2 * k + 1] = (uint16_t)(wb[k] >> 16);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int v = u ? v1 : v0;
+      if (v >= nvert) break;
+      T sum = (T)0;
+      int cnt = 0;
+#pragma unroll
+      for (int k = 0; k < W; k++) {
+        sum += ql[e[u][k]];
+        cnt += e[u][k] != VT_MAXW;
+      }
+      S[v] = sum;
+      M[v] = (uint8_t)cnt;
+    }
+  }
+}
+
+template <typename T, bool FIRST>
+__device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const vtile_args &va,
+                                       T *ql, T *rl, T *S, uint8_t *M, const T *__restrict__ r,
+                                       const T *__restrict__ pold, T *__restrict__ pnew,
+                                       T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
+                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+  const int nrows = t.row1 - t.row0;
+  const int nreal = min(nrows, t.pstart + t.n - t.row0);
+  if (threadIdx.x == 0) ql[VT_MAXW] = (T)0;  // what the lists' padding reads
+  stage_window<T, FIRST>(t, va.halo + va.halo_off[t.ti], va.nhalo[t.ti], ql, rl, r, pold, beta,
+                         m);
+  const int nvert = va.vt_n[t.ti];
+  const uint16_t *Ls = va.list + va.vt_loff[t.ti];
+  if (va.nv == 8) vt_sums<T, 8>(Ls, nvert, ql, S, M);
+  else vt_sums<T, 4>(Ls, nvert, ql, S, M);
+  __syncthreads();
+  // the (up to) four rows of a thread: their index words loaded first
+  constexpr int R = RSB_TILE / RSB_BLOCK;
+  u32x4 pks[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int row = t.row0 + threadIdx.x + k * blockDim.x;
+    pks[k] = row < t.row1 ? __builtin_nontemporal_load((const u32x4 *)(va.rowv + (int64_t)row * 8))
+                          : (u32x4){0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu};
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int row = t.row0 + threadIdx.x + k * blockDim.x;
+    if (row >= t.row1) continue;
+    const int li = row - t.row0;
+    const u32x4 pk = pks[k];
+    const unsigned id[8] = {pk.x & 0xffffu, pk.x >> 16, pk.y & 0xffffu, pk.y >> 16,
+                            pk.z & 0xffffu, pk.z >> 16, pk.w & 0xffffu, pk.w >> 16};
+    const T qi = ql[li];
+    T sum = (T)0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (id[j] != 0xffffu) sum += (T)M[id[j]] * qi - S[id[j]];
+    pnew[row] = qi;
+    w[row] = sum;
+    __builtin_nontemporal_store(rl[li] * rni, &b[row]);
+    if (li < nreal) {
+      sq.add(qi);
+      swq.dot(sum, qi);
+      sw.add(sum);
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__restrict__ r,
+                                                          const T *__restrict__ pold,
+                                                          T *__restrict__ pnew, T *__restrict__ w,
+                                                          T *__restrict__ basis, int64_t mpad,
+                                                          vtile_args va, int nlist) {
+  __shared__ T ql[VT_MAXW + 1];
+  __shared__ T rl[RSB_TILE];
+  __shared__ T S[VT_MAXV];
+  __shared__ uint8_t M[VT_MAXV];
+  tile_t t;
+  if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
+  const int s = t.s, it = a.iter[s];
+  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
+  acc_t<T> sq, swq, sw;
+  T *b = basis + (size_t)it * mpad;
+  if (it == 0)
+    lz_apply_vertex<T, true>(a, t, va, ql, rl, S, M, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                             sw);
+  else
+    lz_apply_vertex<T, false>(a, t, va, ql, rl, S, M, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                              sw);
   double v[3] = {sq.value(), swq.value(), sw.value()};
   publish<3>(a, t, v);
 }
@@ -639,6 +786,94 @@ __device__ void tqli_wave(int n, double *d, double *e, double *z, double *y, dou
   if (lane == 0) *lam_out = lam;
   __syncthreads();
 }
+// The same QL steps for passes longer than 128 iterations: d and e in LDS
+// (every lane computes each update and writes the same value), the
+// eigenvector matrix z (n x n, row-major) in global memory, each lane
+// rotating its own rows.  Operations and their order are those above, so
+// results are bit-identical to it (and to the oracle).
+__device__ void tqli_wave_mem(int n, double *d, double *e, double *z, double *y,
+                              double *lam_out) {
+  const int lane = threadIdx.x;
+  for (int64_t i = lane; i < (int64_t)n * n; i += RSB_WAVE) z[i] = 0.0;
+  __syncthreads();
+  for (int i = lane; i < n; i += RSB_WAVE) z[(int64_t)i * n + i] = 1.0;
+  __syncthreads();
+  int failed = 0;
+  for (int l = 0; l < n && !failed; l++) {
+    int sweeps = 0, m;
+    while (true) {
+      for (m = l; m < n - 1; m++)
+        if (fabs(e[m]) / (fabs(d[m]) + fabs(d[m + 1])) < 1e-12) break;
+      if (m == l) break;
+      if (sweeps++ == 30) {
+        failed = 1;
+        break;
+      }
+      const double dl = d[l], el = e[l];
+      double g = (d[l + 1] - dl) / (2.0 * el);
+      double r = sqrt(g * g + 1.0);
+      g = d[m] - dl + el / (g + sgn_abs(r, g));
+      double s = 1.0, c = 1.0, p = 0.0;
+      for (int i = m - 1; i >= l; i--) {
+        const double ei = e[i];
+        double f = s * ei, b = c * ei;
+        double ei1;
+        if (fabs(f) >= fabs(g)) {
+          c = g / f;
+          r = sqrt(c * c + 1.0);
+          ei1 = f * r;
+          s = 1.0 / r;
+          c = c * s;
+        } else {
+          s = f / g;
+          r = sqrt(s * s + 1.0);
+          ei1 = g * r;
+          c = 1.0 / r;
+          s = s * c;
+        }
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        p = s * r;
+        const double di1 = g + p;
+        g = c * r - b;
+        e[i + 1] = ei1;
+        d[i + 1] = di1;
+        for (int k = lane; k < n; k += RSB_WAVE) {
+          double *zk = z + (int64_t)k * n;
+          const double zk1 = zk[i + 1], zk0 = zk[i];
+          zk[i + 1] = s * zk0 + c * zk1;
+          zk[i] = c * zk0 - s * zk1;
+        }
+      }
+      d[l] = d[l] - p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  __syncthreads();  // rows of z written by other lanes are read below
+  int imin = 0;
+  double emin = fabs(d[0]);
+  for (int i = 1; i < n; i++) {
+    const double v = fabs(d[i]);
+    if (v < emin) emin = v, imin = i;
+  }
+  if (failed) {
+    for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[(int64_t)imin * n + j];
+  } else {
+    __shared__ double scale;
+    if (lane == 0) {
+      double nrm = 0.0;
+      for (int i = 0; i < n; i++) nrm += z[(int64_t)i * n + imin] * z[(int64_t)i * n + imin];
+      if (nrm > 0.0) nrm = sqrt(fabs(nrm));
+      scale = 1.0 / nrm;
+    }
+    __syncthreads();
+    for (int j = lane; j < n; j += RSB_WAVE) y[j] = z[(int64_t)j * n + imin] * scale;
+  }
+  if (lane == 0) *lam_out = d[imin];
+  __syncthreads();
+}
+
 #pragma clang fp contract(fast)
 
 // Fast variant for the FP32 path: the eigenpair of smallest |eigenvalue| of
@@ -808,6 +1043,35 @@ __global__ void __launch_bounds__(RSB_WAVE) k_lz_tqli(lz_args a) {
   }
   __syncthreads();
   tqli_wave(n, d, e, z, a.y + (size_t)s * M, &SC(s, SC_LAMBDA));
+}
+
+// Passes longer than 128 iterations (both precisions): segments s0 +
+// blockIdx.x, each with its n x n eigenvector matrix at zbuf + blockIdx.x *
+// M * M.
+__global__ void __launch_bounds__(RSB_WAVE) k_lz_tqli_g(lz_args a, int s0, double *zbuf) {
+  const int s = s0 + blockIdx.x;
+  if (s >= a.nseg || a.status[s] != LZ_PASSDONE) return;
+  extern __shared__ double lds[];
+  const int n = a.k[s], M = a.miter_max;
+  double *d = lds, *e = d + M;
+  const double *Td = a.T + (size_t)s * 2 * M, *Tu = Td + M;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) {
+    d[i] = Td[i];
+    e[i] = (i < n - 1) ? Tu[i] : 0.0;
+  }
+  __syncthreads();
+  tqli_wave_mem(n, d, e, zbuf + (size_t)blockIdx.x * M * M, a.y + (size_t)s * M,
+                &SC(s, SC_LAMBDA));
+}
+
+__global__ void __launch_bounds__(RSB_WAVE) k_tqli_test_g(int n, const double *dd,
+                                                          const double *ee, double *y,
+                                                          double *lam, double *z) {
+  extern __shared__ double lds[];
+  double *d = lds, *e = d + n;
+  for (int i = threadIdx.x; i < n; i += RSB_WAVE) d[i] = dd[i], e[i] = i < n - 1 ? ee[i] : 0.0;
+  __syncthreads();
+  tqli_wave_mem(n, d, e, z, y, lam);
 }
 
 // Stand-alone tridiagonal solve for the parity tests.
@@ -1122,7 +1386,9 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   int miter_max = 1;
   for (int s = 0; s < nseg; s++)
     if (g->h_p[s] > 1 && g->h_n[s] >= 2) miter_max = std::max(miter_max, std::min(miter_opt, g->h_n[s]));
-  if (miter_max > 128) RSB_FAIL(3, "rsb_max_iter > 128 is not supported");
+  // passes of more than 128 iterations take the tridiagonal solve with its
+  // eigenvector matrix in global memory (k_lz_tqli_g)
+  const bool long_pass = miter_max > 128;
   // buffers
   size_t vb = (size_t)mpad * sizeof(T);
   if (vb > g->vec_bytes) {
@@ -1201,6 +1467,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   ai.act_tiles = act_t, ai.act_segs = act_s;
   tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell, g->d_tnhalo, g->d_thalo_off,
                 g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
+  vtile_args va{g->d_vt_n,  g->d_vt_loff,    g->d_vt_list, g->d_vt_rowv,
+                g->d_tnhalo, g->d_thalo_off, g->d_halo,    g->nv};
 
   RSB_TRY(hipMemsetAsync(L.running, 0, sizeof(int) * 2, st));
   k_lz_init_state<<<nblocks(nseg, 256), 256, 0, st>>>(a, miter_opt);
@@ -1256,8 +1524,12 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
             RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (int q = 0; q < 8; q++) {
               T *po = (q & 1) ? p2 : p, *pn = (q & 1) ? p : p2;
-              k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis, mpad,
-                                                                  ta, nact[0]);
+              if (g->vt_ok)
+                k_lz_apply_v<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis,
+                                                                      mpad, va, nact[0]);
+              else
+                k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis,
+                                                                    mpad, ta, nact[0]);
               k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
               k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
               k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
@@ -1283,8 +1555,12 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         // timing every launch slowed the whole call by ~5%)
         const bool samp = g->prof_spmv && (g->spmv_total++ % LZ_PROF_EVERY) == 0;
         if (samp) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-        k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad, ta,
-                                                            nact[0]);
+        if (g->vt_ok)
+          k_lz_apply_v<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad,
+                                                                va, nact[0]);
+        else
+          k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad,
+                                                              ta, nact[0]);
         if (samp) {
           RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
           nrec++;
@@ -1312,10 +1588,22 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       }
       g->spmv_launches += nrec;
       RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
-      if (sizeof(T) == 4)
+      if (long_pass) {
+        // segments in chunks whose matrices take at most 1 GiB together
+        const size_t zb = sizeof(double) * (size_t)miter_max * miter_max;
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nseg, (1ull << 30) / zb));
+        double *zbuf = nullptr;
+        RSB_TRY(RSB_SCRATCH(zbuf, zb * (size_t)chunk));
+        for (int s0 = 0; s0 < nseg; s0 += chunk) {
+          k_lz_tqli_g<<<std::min(chunk, nseg - s0), RSB_WAVE, sizeof(double) * 2 * miter_max,
+                        st>>>(a, s0, zbuf);
+          RSB_LAUNCH_CHECK();
+        }
+      } else if (sizeof(T) == 4) {
         k_lz_eigmin<<<nseg, RSB_WAVE, sizeof(double) * 4 * miter_max, st>>>(a);
-      else
+      } else {
         k_lz_tqli<<<nseg, RSB_WAVE, tq_lds, st>>>(a);
+      }
       RSB_LAUNCH_CHECK();
       k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
       RSB_LAUNCH_CHECK();
@@ -1444,8 +1732,8 @@ extern "C" int rsbgpu_test_eigmin(const double *d, const double *e, int n, doubl
 
 static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, double *lam,
                             int fast) {
-  double *dd = nullptr, *de = nullptr, *dy = nullptr, *dl = nullptr;
-  if (n < 1 || n > 128) return 3;
+  double *dd = nullptr, *de = nullptr, *dy = nullptr, *dl = nullptr, *dz = nullptr;
+  if (n < 1 || n > 4096 || (fast && n > 128)) return 3;
   hipError_t err = rsb_dev_malloc((void **)&dd, sizeof(double) * n);
   if (err == hipSuccess) err = rsb_dev_malloc((void **)&de, sizeof(double) * n);
   if (err == hipSuccess) err = rsb_dev_malloc((void **)&dy, sizeof(double) * n);
@@ -1457,6 +1745,10 @@ static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, 
   if (err == hipSuccess) {
     if (fast) {
       k_eigmin_test<<<1, RSB_WAVE, sizeof(double) * 4 * n, 0>>>(n, dd, de, dy, dl);
+    } else if (n > 128) {
+      err = rsb_dev_malloc((void **)&dz, sizeof(double) * (size_t)n * n);
+      if (err == hipSuccess)
+        k_tqli_test_g<<<1, RSB_WAVE, sizeof(double) * 2 * n, 0>>>(n, dd, de, dy, dl, dz);
     } else {
       size_t lds = sizeof(double) * (2 * n + n * n);
       k_tqli_test<<<1, RSB_WAVE, lds, 0>>>(n, dd, de, dy, dl);
@@ -1470,5 +1762,6 @@ static int rsb_test_tridiag(const double *d, const double *e, int n, double *y, 
   if (de) (void)rsb_dev_free(de);
   if (dy) (void)rsb_dev_free(dy);
   if (dl) (void)rsb_dev_free(dl);
+  if (dz) (void)rsb_dev_free(dz);
   return err == hipSuccess ? 0 : 2;
 }

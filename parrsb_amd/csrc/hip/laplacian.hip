@@ -175,6 +175,8 @@ extern "C" int rsbgpu_level_laplacian(rsbgpu *g) {
   RSB_LAUNCH_CHECK();
   rc = rsb_build_tiles(g);
   if (rc) return rc;
+  rc = rsb_build_vtiles(g);
+  if (rc) return rc;
   rsb_phase_end(g, PH_LAP);
   return 0;
 }

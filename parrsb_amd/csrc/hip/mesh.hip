@@ -100,6 +100,10 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_off16);
   FREE(g->d_e16);
   FREE(g->d_seg_nnz);
+  FREE(g->d_vt_n);
+  FREE(g->d_vt_loff);
+  FREE(g->d_vt_list);
+  FREE(g->d_vt_rowv);
   for (hipEvent_t ev : g->spmv_ev) hipEventDestroy(ev);
   FREE(g->d_tile);
   FREE(g->d_seg_tiles);

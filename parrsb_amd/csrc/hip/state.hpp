@@ -98,6 +98,13 @@ struct rsbgpu {
   uint16_t *d_e16 = nullptr;
   int64_t nhalo_total = 0, e16_count = 0;
   int nwide = 0;
+  // vertex form of the tiles (tiles.hip, rsb_build_vtiles): per tile, the
+  // distinct vertex ids of its rows, for each the nv window-local indices of
+  // the slots at it (padded); per row its nv local vertex indices
+  int *d_vt_n = nullptr;            // local vertices per tile (-1: none)
+  int64_t *d_vt_loff = nullptr;     // per tile: start of its lists
+  uint16_t *d_vt_list = nullptr, *d_vt_rowv = nullptr;
+  int vt_ok = 0;  // every tile of the level has the vertex form
   int64_t *d_seg_nnz = nullptr;
   std::vector<int64_t> h_seg_nnz;
   // tiles of the level (a tile never straddles a segment)
@@ -234,7 +241,11 @@ inline hipError_t rsb_scratch(rsbgpu *g, const void *file, int line, size_t byte
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-int rsb_build_tiles(rsbgpu *g);  // tiles.hip
+int rsb_build_tiles(rsbgpu *g);   // tiles.hip
+int rsb_build_vtiles(rsbgpu *g);  // tiles.hip
+// limits of the vertex form: window rows (tile + halo) and local vertices
+#define VT_MAXW 2048
+#define VT_MAXV 2048
 void rsb_mg_free(rsbgpu *g);     // mg.hip
 int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows);  // lws.hip
 

@@ -232,3 +232,245 @@ int rsb_build_tiles(rsbgpu *g) {
   RSB_TRY(hipStreamSynchronize(g->stream));
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Vertex form of the tiles: the Laplacian as the reference applies it, by
+// gather-scatter over vertices (laplacian.c:159-205):
+//   (L q)_e = sum over the slots s of e of (mult(v_s) q_e - S(v_s)),
+//   S(v) = sum of q over the slots of the segment's elements at vertex v,
+//   mult(v) = the number of those slots.
+// Every element of the segment that touches a vertex of a tile row is a
+// neighbour of that row, so it lies in the tile's window (rows + halo): per
+// tile, the distinct vertex ids of its rows (at most VT_MAXV) get local
+// indices; each has a list of exactly nv entries, the window indices of the
+// slots at it in ascending order, padded with VT_MAXW (the index of an LDS
+// zero); each row holds the local indices of its nv vertices (0xffff: vertex
+// id 0, which is never shared, as in the reference's gather-scatter).  A
+// tile whose window exceeds VT_MAXW rows, whose rows touch more than VT_MAXV
+// vertices, or with more than nv slots at a vertex (a non-conforming mesh)
+// has no vertex form (vt_n = -1); a level uses the form only if every tile
+// has it.  Per row this reads one 16-byte index word (8-byte for quads) plus
+// one list of nv 16-bit words per vertex of the tile, one aligned load each.
+#define VT_HASH 4096
+#define VT_NONE 0xffffu
+
+__device__ inline unsigned vt_hash(unsigned long long v) {
+  return (unsigned)((v * 0x9E3779B97F4A7C15ull) >> 52) & (VT_HASH - 1);
+}
+
+__device__ inline bool vt_insert(unsigned long long *keys, unsigned long long v) {
+  unsigned h = vt_hash(v);
+  for (int probe = 0; probe < VT_HASH; probe++) {
+    unsigned long long old = atomicCAS(&keys[h], 0ull, v);
+    if (old == 0ull || old == v) return true;
+    h = (h + 1) & (VT_HASH - 1);
+  }
+  return false;
+}
+
+__device__ inline int vt_find(const unsigned long long *keys, unsigned long long v) {
+  unsigned h = vt_hash(v);
+  for (int probe = 0; probe < VT_HASH; probe++) {
+    const unsigned long long k = keys[h];
+    if (k == v) return (int)h;
+    if (k == 0ull) return -1;
+    h = (h + 1) & (VT_HASH - 1);
+  }
+  return -1;
+}
+
+struct vt_in {
+  const int *tile, *nhalo, *halo, *row_elem;
+  const int64_t *halo_off;
+  const long long *vtx;
+  int nv;
+};
+
+// The window row of local index i (tile rows first, then the halo).
+__device__ inline int vt_window_row(const vt_in &in, int t, int row0, int nrows, int i) {
+  return i < nrows ? row0 + i : in.halo[in.halo_off[t] + (i - nrows)];
+}
+
+// Insert the vertex ids of the tile's rows; false if the table overflowed.
+__device__ inline bool vt_collect(const vt_in &in, unsigned long long *keys, int row0, int row1) {
+  __shared__ int fail;
+  for (int i = threadIdx.x; i < VT_HASH; i += blockDim.x) keys[i] = 0ull;
+  if (threadIdx.x == 0) fail = 0;
+  __syncthreads();
+  bool ok = true;
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int e = in.row_elem[row];
+    if (e < 0) continue;
+    for (int j = 0; j < in.nv; j++) {
+      const unsigned long long v = (unsigned long long)in.vtx[(int64_t)e * in.nv + j];
+      if (v && !vt_insert(keys, v)) ok = false;
+    }
+  }
+  if (!ok) atomicOr(&fail, 1);
+  __syncthreads();
+  return fail == 0;
+}
+
+// Local indices in table order (deterministic for a given table); returns
+// the number of vertices.
+__device__ inline int vt_number(const unsigned long long *keys, short *lid) {
+  const int per = VT_HASH / RSB_BLOCK;
+  int c = 0;
+  for (int k = 0; k < per; k++) c += keys[threadIdx.x * per + k] != 0ull;
+  int nvert = 0, base = tb_scan(c, &nvert);
+  for (int k = 0; k < per; k++) {
+    const int slot = threadIdx.x * per + k;
+    if (keys[slot] != 0ull) lid[slot] = (short)base++;
+  }
+  __syncthreads();
+  return nvert;
+}
+
+// Slots of the window at each local vertex, counted into vcnt (LDS); with
+// list != nullptr also written there at list[v * nv + (running count)].
+__device__ inline void vt_slots(const vt_in &in, int t, int row0, int nrows, int nh,
+                                const unsigned long long *keys, const short *lid, int *vcnt,
+                                uint16_t *list) {
+  for (int i = threadIdx.x; i < nrows + nh; i += blockDim.x) {
+    const int e = in.row_elem[vt_window_row(in, t, row0, nrows, i)];
+    if (e < 0) continue;
+    for (int j = 0; j < in.nv; j++) {
+      const unsigned long long v = (unsigned long long)in.vtx[(int64_t)e * in.nv + j];
+      const int slot = v ? vt_find(keys, v) : -1;
+      if (slot < 0) continue;
+      const int lv = lid[slot];
+      const int k = atomicAdd(&vcnt[lv], 1);
+      if (list && k < in.nv) list[lv * in.nv + k] = (uint16_t)i;
+    }
+  }
+  __syncthreads();
+}
+
+// Pass A: local vertices per tile (vt_n = -1: no vertex form).
+__global__ void __launch_bounds__(RSB_BLOCK) k_vt_count(vt_in in, int *__restrict__ vt_n,
+                                                        int64_t *__restrict__ nlist) {
+  __shared__ unsigned long long keys[VT_HASH];
+  __shared__ short lid[VT_HASH];
+  __shared__ int vcnt[VT_MAXV];
+  __shared__ int over;
+  const int t = blockIdx.x, row0 = in.tile[3 * t + 1], row1 = in.tile[3 * t + 2];
+  const int nrows = row1 - row0, nh = in.nhalo[t];
+  if (nh < 0 || nrows + nh > VT_MAXW) {
+    if (threadIdx.x == 0) vt_n[t] = -1, nlist[t] = 0;
+    return;
+  }
+  const bool ok = vt_collect(in, keys, row0, row1);
+  const int nvert = vt_number(keys, lid);
+  if (!ok || nvert > VT_MAXV) {
+    if (threadIdx.x == 0) vt_n[t] = -1, nlist[t] = 0;
+    return;
+  }
+  for (int v = threadIdx.x; v < nvert; v += blockDim.x) vcnt[v] = 0;
+  if (threadIdx.x == 0) over = 0;
+  __syncthreads();
+  vt_slots(in, t, row0, nrows, nh, keys, lid, vcnt, nullptr);
+  for (int v = threadIdx.x; v < nvert; v += blockDim.x)
+    if (vcnt[v] > in.nv) atomicOr(&over, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    vt_n[t] = over ? -1 : nvert;
+    nlist[t] = over ? 0 : (int64_t)nvert * in.nv;
+  }
+}
+
+// Pass B: the lists (window indices ascending, padded with VT_MAXW) and the
+// rows' local vertex indices.
+__global__ void __launch_bounds__(RSB_BLOCK) k_vt_fill(vt_in in, const int *__restrict__ vt_n,
+                                                       const int64_t *__restrict__ vt_loff,
+                                                       uint16_t *__restrict__ list,
+                                                       uint16_t *__restrict__ rowv) {
+  __shared__ unsigned long long keys[VT_HASH];
+  __shared__ short lid[VT_HASH];
+  __shared__ int vcnt[VT_MAXV];
+  const int t = blockIdx.x, row0 = in.tile[3 * t + 1], row1 = in.tile[3 * t + 2];
+  const int nrows = row1 - row0;
+  if (vt_n[t] < 0) return;
+  const int nh = in.nhalo[t];
+  vt_collect(in, keys, row0, row1);
+  const int nvert = vt_number(keys, lid);
+  for (int v = threadIdx.x; v < nvert; v += blockDim.x) vcnt[v] = 0;
+  __syncthreads();
+  uint16_t *Ls = list + vt_loff[t];
+  vt_slots(in, t, row0, nrows, nh, keys, lid, vcnt, Ls);
+  // each list ascending (its order of summation), then padded
+  const int W = in.nv;
+  for (int v = threadIdx.x; v < nvert; v += blockDim.x) {
+    uint16_t *L = Ls + v * W;
+    const int n = vcnt[v];
+    for (int x = 1; x < n; x++) {
+      const uint16_t key = L[x];
+      int y = x - 1;
+      while (y >= 0 && L[y] > key) {
+        L[y + 1] = L[y];
+        y--;
+      }
+      L[y + 1] = key;
+    }
+    for (int x = n; x < W; x++) L[x] = (uint16_t)VT_MAXW;
+  }
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int e = in.row_elem[row];
+    uint16_t ids[8];
+    for (int j = 0; j < 8; j++) {
+      ids[j] = (uint16_t)VT_NONE;
+      if (e >= 0 && j < in.nv) {
+        const unsigned long long v = (unsigned long long)in.vtx[(int64_t)e * in.nv + j];
+        const int slot = v ? vt_find(keys, v) : -1;
+        if (slot >= 0) ids[j] = (uint16_t)lid[slot];
+      }
+    }
+    uint4 pk;
+    pk.x = ids[0] | ((unsigned)ids[1] << 16), pk.y = ids[2] | ((unsigned)ids[3] << 16);
+    pk.z = ids[4] | ((unsigned)ids[5] << 16), pk.w = ids[6] | ((unsigned)ids[7] << 16);
+    *(uint4 *)(rowv + (int64_t)row * 8) = pk;
+  }
+}
+
+int rsb_build_vtiles(rsbgpu *g) {
+  const int nt = g->ntiles;
+  g->vt_ok = 0;
+  // opt-in: PARRSB_AMD_VTILES=1 (at 256^3 it moves fewer bytes per apply but
+  // takes as long as the neighbour-entry form, and its build costs ~3 ms a level)
+  const char *on = getenv("PARRSB_AMD_VTILES");
+  if (nt == 0 || !g->d_vtx || (g->nv != 4 && g->nv != 8) || g->nwide > 0 || !on ||
+      atoi(on) != 1)
+    return 0;
+  RSB_TRY(dev_alloc(g, g->d_vt_n, (size_t)nt + 1));
+  RSB_TRY(dev_alloc(g, g->d_vt_loff, (size_t)nt + 1));
+  int64_t *nl = nullptr;
+  RSB_TRY(RSB_SCRATCH(nl, sizeof(int64_t) * ((size_t)nt + 1)));
+  vt_in in{g->d_tile, g->d_tnhalo, g->d_halo, g->d_row_elem, g->d_thalo_off, g->d_vtx, g->nv};
+  RSB_TRY(hipMemsetAsync(nl, 0, sizeof(int64_t) * ((size_t)nt + 1), g->stream));
+  k_vt_count<<<nt, RSB_BLOCK, 0, g->stream>>>(in, g->d_vt_n, nl);
+  RSB_LAUNCH_CHECK();
+  std::vector<int> vn(nt);
+  RSB_TRY(hipMemcpyAsync(vn.data(), g->d_vt_n, sizeof(int) * nt, hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  for (int v : vn)
+    if (v < 0) return 0;  // some tile has no vertex form: the level keeps the entry form
+  int rc = rsb_exclusive_scan_i64(g, nl, g->d_vt_loff, nt + 1);
+  if (rc) return rc;
+  int64_t tl = 0;
+  RSB_TRY(hipMemcpyAsync(&tl, g->d_vt_loff + nt, sizeof(int64_t), hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  RSB_TRY(dev_alloc(g, g->d_vt_list, (size_t)tl + 8));
+  RSB_TRY(dev_alloc(g, g->d_vt_rowv, (size_t)g->mpad * 8 + 8));
+  k_vt_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(in, g->d_vt_n, g->d_vt_loff, g->d_vt_list,
+                                              g->d_vt_rowv);
+  RSB_LAUNCH_CHECK();
+  g->vt_ok = 1;
+  return 0;
+}
+
+extern "C" int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_form) {
+  *wide_tiles = g->nwide;
+  *vertex_form = g->vt_ok;
+  return 0;
+}

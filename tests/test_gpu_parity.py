@@ -141,8 +141,11 @@ def test_level_laplacian_is_induced_subgraph(oracle):
 
 
 def test_tridiagonal_solver_bit_exact(oracle):
+    """The QL solve against the oracle's, bit for bit: up to 128 with the
+    eigenvector matrix in LDS, above (passes of rsb_max_iter > 128) with it
+    in global memory."""
     rng = np.random.default_rng(0)
-    for n in [1, 2, 3, 7, 20, 50, 64, 100]:
+    for n in [1, 2, 3, 7, 20, 50, 64, 100, 128, 129, 200, 300]:
         d = rng.uniform(0.1, 4, n)
         e = -rng.uniform(0.01, 1.5, max(n - 1, 0))
         if n > 10:
@@ -206,18 +209,27 @@ def _segments_after_rcb(vtx, xyz, nv, p):
     return d
 
 
-@pytest.mark.parametrize("tiles", ["lds", "global"])
-@pytest.mark.parametrize("dims", [(9, 6, 5), (12, 10, 7), (40, 30, 20)])
+@pytest.mark.parametrize("tiles", ["vertex", "lds", "global"])
+@pytest.mark.parametrize("dims", [(9, 6, 5), (12, 10, 7), (40, 30, 20), (70, 52)])
 def test_lanczos_fp64_matches_oracle(oracle, dims, tiles, monkeypatch):
-    """Both forms of the Laplacian apply: LDS tiles with halos, and the
-    global 32-bit path (forced for every tile with PARRSB_AMD_NO_TILES=1)."""
+    """The three forms of the Laplacian apply: the 16-bit neighbour entries
+    of the LDS tiles (the default), the vertex (gather-scatter) form of the
+    LDS tiles (PARRSB_AMD_VTILES=1), and the global 32-bit path
+    (PARRSB_AMD_NO_TILES=1); hex boxes and a quad grid."""
     flush(f"lanczos fp64 {dims} {tiles}")
     if tiles == "global":
         monkeypatch.setenv("PARRSB_AMD_NO_TILES", "1")
-    vtx, xyz, nv = M.box_hex(*dims, seed=4)
+    if tiles == "vertex":
+        monkeypatch.setenv("PARRSB_AMD_VTILES", "1")
+    if len(dims) == 3:
+        vtx, xyz, nv = M.box_hex(*dims, seed=4)
+    else:
+        vtx, xyz, nv = M.quad_grid(*dims)
     with _segments_after_rcb(vtx, xyz, nv, 2) as d:
         order = d.order()
         d.level_laplacian()
+        wide, vform = d.level_form()
+        assert vform == (tiles == "vertex") and (wide > 0) == (tiles == "global")
         out = d.fiedler(1, precision=64)
         f = d.fiedler_values()
     fo, it, lam, rn = oracle.fiedler(vtx, nv, order)
@@ -386,6 +398,33 @@ def test_full_partition_16cube_matches_oracle(oracle):
     r = pa.last_report()
     assert r["ncomp_max"] == [1, 1, 1]
     assert [abs(a - b) <= 1 for a, b in zip(r["niter_max"], rep["niter_max"])] == [True] * 3
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_passes_longer_than_128_iterations(oracle, precision):
+    """rsb_max_iter = 150 (fiedler.c:513-561 takes any value): the segments'
+    tridiagonal matrices exceed the LDS solve and go through the one with
+    its eigenvector matrix in global memory (both precisions).  FP64 gives
+    the oracle's partition quality and iteration counts; FP32 its part sizes
+    and an edge cut within 2%."""
+    flush(f"rsb_max_iter 150, fp{precision}")
+    vtx, xyz, nv = M.quad_grid(300, 3)  # a long strip: the first cut takes 459 iterations
+    pa.set_precision(precision)
+    try:
+        part, opts = pa.part_mesh(vtx, xyz, nv, 4, rsb_max_iter=150)
+    finally:
+        pa.set_precision(64)
+    r = pa.last_report()
+    ref, rep = oracle.part_mesh(vtx, xyz, nv, 4, max_iter=150)
+    assert opts.rsb_max_iter == 150
+    assert max(rep["niter_max"]) > 150  # full passes longer than the LDS solve takes
+    q, qo = pa.quality(part, vtx, nv, 4), oracle.quality(ref, vtx, nv, 4)
+    if precision == 64:
+        assert q == qo
+        assert all(abs(a - b) <= 1 for a, b in zip(r["niter_max"], rep["niter_max"]))
+    else:
+        assert q["max_part"] == qo["max_part"] and q["min_part"] == qo["min_part"]
+        assert abs(q["edge_cut"] - qo["edge_cut"]) <= 0.02 * qo["edge_cut"]
 
 
 @pytest.mark.parametrize("P", [3, 5, 7, 12])
