@@ -242,6 +242,27 @@ def main():
         torch.cuda.synchronize()
         print(f"copied {4 * n} bytes x3", flush=True)
         return
+    if len(sys.argv) > 1 and sys.argv[1] == "--level0":
+        # the first bisection of the 256^3 warped hex box alone: one segment of
+        # every element, RCB order, level Laplacian, then 50 FP32 CG-Lanczos
+        # iterations three times (for kernel profiles of the apply at full size)
+        import parrsb_amd as pa
+        from parrsb_amd import meshgen as M
+        vtx, xyz, nv = M.warped_box_hex(256, seed=5)
+        nel = vtx.shape[0]
+        with pa.Device() as d:
+            d.set_mesh(vtx, xyz, nv)
+            del xyz
+            d.set_segments([0], [nel], [2], [0])
+            d.prepartition(1)
+            d.level_laplacian()
+            print("tiles on the global path, tiles with the vertex form:", d.level_form(),
+                  flush=True)
+            for _ in range(3):
+                t0 = time.perf_counter()
+                d.fiedler(1, precision=32, max_iter=50, max_passes=1)
+                print(f"50 iterations: {time.perf_counter() - t0:.4f} s", flush=True)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-traffic":
         # bench.py --summarize-traffic OUT FETCH.csv WRITE.csv CALFETCH.csv CALWRITE.csv DESC
         a = sys.argv[2:]

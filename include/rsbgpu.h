@@ -157,9 +157,10 @@ long long rsbgpu_noalloc_end(long long *high);
 int rsbgpu_get_order(rsbgpu *g, int *order_host);
 int rsbgpu_get_fiedler(rsbgpu *g, double *fiedler_host);
 /* Form of the current level's Laplacian apply: *wide_tiles = tiles whose
- * halo does not fit LDS (global 32-bit path), *vertex_form = 1 if every tile
- * has the vertex (gather-scatter) form; for the parity tests. */
-int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_form);
+ * halo does not fit LDS (global 32-bit path), *vertex_tiles = tiles with the
+ * vertex (gather-scatter) form, which FP32 Lanczos uses (0 unless
+ * PARRSB_AMD_VTILES=1); for the parity tests. */
+int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_tiles);
 /* Overwrite the Fiedler value of every compact position (host array of nel
  * values): lets the parity tests drive the balance step with chosen values
  * (ties, either sender) through rsbgpu_bisect. */

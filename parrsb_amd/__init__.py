@@ -559,7 +559,7 @@ class Device:
         return f
 
     def level_form(self):
-        """(tiles on the global path, 1 if the level uses the vertex form)."""
+        """(tiles on the global path, tiles with the vertex form)."""
         a, b = C.c_int(), C.c_int()
         self._chk(lib().rsbgpu_level_form(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value

@@ -406,7 +406,7 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
 // of the tile's rows; rows past the segment's end get q = 0.  SU indices per
 // thread and step, all their loads issued before any value is used (the halo
 // rows are gathers).  Ends with a barrier.
-template <typename T, bool FIRST>
+template <typename T, bool FIRST, bool KEEP_R = true>
 __device__ inline void stage_window(const tile_t &t, const int *__restrict__ halo, int nh, T *ql,
                                     T *rl, const T *__restrict__ r, const T *__restrict__ pold,
                                     T beta, T m) {
@@ -433,7 +433,7 @@ __device__ inline void stage_window(const tile_t &t, const int *__restrict__ hal
       if (idx >= tot) break;
       T q = FIRST ? rg[k] : beta * (po[k] - m) + rg[k];
       if (idx < nrows) {
-        rl[idx] = rg[k];
+        if (KEEP_R) rl[idx] = rg[k];
         if (idx >= nreal) q = (T)0;
       }
       ql[idx] = q;
@@ -488,12 +488,16 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
   publish<3>(a, t, v);
 }
 
-// Vertex form of the tiles (tiles.hip, rsb_build_vtiles): after staging q
-// for the window, S(v) = sum of q over the slots at each of the tile's
-// vertices (one aligned load of the vertex's nv window indices, ascending,
-// padded with the index of an LDS zero), then per row
-// (L q)_e = sum over its slots of (mult(v) q_e - S(v)) -- the reference's
-// gather-scatter form (laplacian.c:159-205), in a fixed order of summation.
+// Vertex form of the tiles (tiles.hip, rsb_build_vtiles): the reference's
+// gather-scatter form (laplacian.c:159-205), (L q)_e = sum over the slots of
+// e of sum over the slots at their vertex v of (q_e - q_e'), evaluated as
+// mult(v) (q_e - q_f(v)) - S(v), f(v) the first slot at v and
+// S(v) = sum over the slots at v of (q_e' - q_f(v)): every term a difference
+// of neighbouring values, as in the neighbour-entry form (summing q itself
+// loses the low-frequency part of L q in FP32).  One aligned load per vertex
+// fetches its nv window indices (ascending, padded with VT_MAXW); LDS keeps
+// S(v) and one 16-bit word f(v) | mult(v) << 12 per vertex; fixed order of
+// summation.
 struct vtile_args {
   const int *vt_n;
   const int64_t *vt_loff;
@@ -506,7 +510,7 @@ struct vtile_args {
 
 template <typename T, int W>
 __device__ inline void vt_sums(const uint16_t *__restrict__ Ls, int nvert, const T *ql, T *S,
-                               uint8_t *M) {
+                               uint16_t *F) {
   // two vertices per thread and step, both loads issued first
   for (int v0 = threadIdx.x; v0 < nvert; v0 += 2 * blockDim.x) {
     const int v1 = v0 + blockDim.x;
@@ -536,34 +540,36 @@ __device__ inline void vt_sums(const uint16_t *__restrict__ Ls, int nvert, const
     for (int u = 0; u < 2; u++) {
       const int v = u ? v1 : v0;
       if (v >= nvert) break;
+      const T ref = ql[e[u][0]];  // every list has at least one slot
       T sum = (T)0;
       int cnt = 0;
 #pragma unroll
       for (int k = 0; k < W; k++) {
-        sum += ql[e[u][k]];
-        cnt += e[u][k] != VT_MAXW;
+        const bool real = e[u][k] != VT_MAXW;
+        sum += real ? ql[e[u][k]] - ref : (T)0;
+        cnt += real;
       }
       S[v] = sum;
-      M[v] = (uint8_t)cnt;
+      F[v] = (uint16_t)(e[u][0] | (cnt << 12));
     }
   }
 }
 
 template <typename T, bool FIRST>
 __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const vtile_args &va,
-                                       T *ql, T *rl, T *S, uint8_t *M, const T *__restrict__ r,
+                                       T *ql, T *S, uint16_t *F, const T *__restrict__ r,
                                        const T *__restrict__ pold, T *__restrict__ pnew,
                                        T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
                                        acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   if (threadIdx.x == 0) ql[VT_MAXW] = (T)0;  // what the lists' padding reads
-  stage_window<T, FIRST>(t, va.halo + va.halo_off[t.ti], va.nhalo[t.ti], ql, rl, r, pold, beta,
-                         m);
+  stage_window<T, FIRST, false>(t, va.halo + va.halo_off[t.ti], va.nhalo[t.ti], ql, nullptr, r,
+                                pold, beta, m);
   const int nvert = va.vt_n[t.ti];
   const uint16_t *Ls = va.list + va.vt_loff[t.ti];
-  if (va.nv == 8) vt_sums<T, 8>(Ls, nvert, ql, S, M);
-  else vt_sums<T, 4>(Ls, nvert, ql, S, M);
+  if (va.nv == 8) vt_sums<T, 8>(Ls, nvert, ql, S, F);
+  else vt_sums<T, 4>(Ls, nvert, ql, S, F);
   __syncthreads();
   // the (up to) four rows of a thread: their index words loaded first
   constexpr int R = RSB_TILE / RSB_BLOCK;
@@ -586,10 +592,14 @@ __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const 
     T sum = (T)0;
 #pragma unroll
     for (int j = 0; j < 8; j++)
-      if (id[j] != 0xffffu) sum += (T)M[id[j]] * qi - S[id[j]];
+      if (id[j] != 0xffffu) {
+        const unsigned f = F[id[j]];
+        sum += (T)(f >> 12) * (qi - ql[f & 0xfffu]) - S[id[j]];
+      }
     pnew[row] = qi;
     w[row] = sum;
-    __builtin_nontemporal_store(rl[li] * rni, &b[row]);
+    // r of the tile's rows was read by the staging a moment ago (cache hit)
+    __builtin_nontemporal_store(r[row] * rni, &b[row]);
     if (li < nreal) {
       sq.add(qi);
       swq.dot(sum, qi);
@@ -598,28 +608,41 @@ __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const 
   }
 }
 
+// FP32 only (its LDS: window + per-vertex sums and first-slot / count words
+// = 28 KB, five workgroups per CU); a tile without the vertex form (vt_n < 0) takes the
+// neighbour entries of the same window.  The level has no wide tiles.
 template <typename T>
 __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__restrict__ r,
                                                           const T *__restrict__ pold,
                                                           T *__restrict__ pnew, T *__restrict__ w,
                                                           T *__restrict__ basis, int64_t mpad,
-                                                          vtile_args va, int nlist) {
+                                                          vtile_args va, tiled_args ta,
+                                                          int nlist) {
   __shared__ T ql[VT_MAXW + 1];
-  __shared__ T rl[RSB_TILE];
-  __shared__ T S[VT_MAXV];
-  __shared__ uint8_t M[VT_MAXV];
+  __shared__ T S[VT_MAXV];  // the neighbour-entry fallback keeps r of the tile rows here
+  __shared__ uint16_t F[VT_MAXV];
+  static_assert(VT_MAXV >= RSB_TILE, "the fallback's r of the tile rows lives in S");
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
   const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
   acc_t<T> sq, swq, sw;
   T *b = basis + (size_t)it * mpad;
-  if (it == 0)
-    lz_apply_vertex<T, true>(a, t, va, ql, rl, S, M, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                             sw);
-  else
-    lz_apply_vertex<T, false>(a, t, va, ql, rl, S, M, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                              sw);
+  if (va.vt_n[t.ti] >= 0) {
+    if (it == 0)
+      lz_apply_vertex<T, true>(a, t, va, ql, S, F, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                               sw);
+    else
+      lz_apply_vertex<T, false>(a, t, va, ql, S, F, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                                sw);
+  } else {
+    const int nh = ta.nhalo[t.ti];
+    if (it == 0)
+      lz_apply_tiled<T, true>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw);
+    else
+      lz_apply_tiled<T, false>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq,
+                               sw);
+  }
   double v[3] = {sq.value(), swq.value(), sw.value()};
   publish<3>(a, t, v);
 }
@@ -1469,6 +1492,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
                 g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
   vtile_args va{g->d_vt_n,  g->d_vt_loff,    g->d_vt_list, g->d_vt_rowv,
                 g->d_tnhalo, g->d_thalo_off, g->d_halo,    g->nv};
+  // the vertex form serves the FP32 path (vectors of T = float)
+  const bool vform = g->vt_ok && sizeof(T) == 4;
 
   RSB_TRY(hipMemsetAsync(L.running, 0, sizeof(int) * 2, st));
   k_lz_init_state<<<nblocks(nseg, 256), 256, 0, st>>>(a, miter_opt);
@@ -1524,9 +1549,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
             RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (int q = 0; q < 8; q++) {
               T *po = (q & 1) ? p2 : p, *pn = (q & 1) ? p : p2;
-              if (g->vt_ok)
-                k_lz_apply_v<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis,
-                                                                      mpad, va, nact[0]);
+              if (vform)
+                k_lz_apply_v<float><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(
+                    ai, (const float *)r, (const float *)po, (float *)pn, (float *)w,
+                    (float *)basis, mpad, va, ta, nact[0]);
               else
                 k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis,
                                                                     mpad, ta, nact[0]);
@@ -1555,9 +1581,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         // timing every launch slowed the whole call by ~5%)
         const bool samp = g->prof_spmv && (g->spmv_total++ % LZ_PROF_EVERY) == 0;
         if (samp) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-        if (g->vt_ok)
-          k_lz_apply_v<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad,
-                                                                va, nact[0]);
+        if (vform)
+          k_lz_apply_v<float><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(
+              ai, (const float *)r, (const float *)pold, (float *)pnew, (float *)w,
+              (float *)basis, mpad, va, ta, nact[0]);
         else
           k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad,
                                                               ta, nact[0]);

@@ -104,7 +104,7 @@ struct rsbgpu {
   int *d_vt_n = nullptr;            // local vertices per tile (-1: none)
   int64_t *d_vt_loff = nullptr;     // per tile: start of its lists
   uint16_t *d_vt_list = nullptr, *d_vt_rowv = nullptr;
-  int vt_ok = 0;  // every tile of the level has the vertex form
+  int vt_ok = 0;  // the vertex form was built (tiles with vt_n < 0 use the entries)
   int64_t *d_seg_nnz = nullptr;
   std::vector<int64_t> h_seg_nnz;
   // tiles of the level (a tile never straddles a segment)
@@ -243,8 +243,9 @@ inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs
 
 int rsb_build_tiles(rsbgpu *g);   // tiles.hip
 int rsb_build_vtiles(rsbgpu *g);  // tiles.hip
-// limits of the vertex form: window rows (tile + halo) and local vertices
-#define VT_MAXW 2048
+// limits of the vertex form: window rows (tile + halo, the LDS window of the
+// neighbour-entry form) and local vertices
+#define VT_MAXW 4096
 #define VT_MAXV 2048
 void rsb_mg_free(rsbgpu *g);     // mg.hip
 int rsb_lws_predict(rsbgpu *g, const float *P, float level, float *d_rows);  // lws.hip

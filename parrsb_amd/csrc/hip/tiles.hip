@@ -246,11 +246,12 @@ int rsb_build_tiles(rsbgpu *g) {
 // slots at it in ascending order, padded with VT_MAXW (the index of an LDS
 // zero); each row holds the local indices of its nv vertices (0xffff: vertex
 // id 0, which is never shared, as in the reference's gather-scatter).  A
-// tile whose window exceeds VT_MAXW rows, whose rows touch more than VT_MAXV
-// vertices, or with more than nv slots at a vertex (a non-conforming mesh)
-// has no vertex form (vt_n = -1); a level uses the form only if every tile
-// has it.  Per row this reads one 16-byte index word (8-byte for quads) plus
-// one list of nv 16-bit words per vertex of the tile, one aligned load each.
+// tile whose rows touch more than VT_MAXV vertices, or with more than nv
+// slots at a vertex (a non-conforming mesh), has no vertex form (vt_n = -1)
+// and keeps its neighbour entries.  Per row this reads one 16-byte index
+// word plus one list of nv 16-bit words per vertex of the tile (about 1.4
+// per row on a hex mesh), one aligned load each, against 26-28 16-bit
+// neighbour entries.
 #define VT_HASH 4096
 #define VT_NONE 0xffffu
 
@@ -434,8 +435,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_vt_fill(vt_in in, const int *__re
 int rsb_build_vtiles(rsbgpu *g) {
   const int nt = g->ntiles;
   g->vt_ok = 0;
-  // opt-in: PARRSB_AMD_VTILES=1 (at 256^3 it moves fewer bytes per apply but
-  // takes as long as the neighbour-entry form, and its build costs ~3 ms a level)
+  // opt-in: PARRSB_AMD_VTILES=1 (FP32 Lanczos only)
   const char *on = getenv("PARRSB_AMD_VTILES");
   if (nt == 0 || !g->d_vtx || (g->nv != 4 && g->nv != 8) || g->nwide > 0 || !on ||
       atoi(on) != 1)
@@ -448,12 +448,6 @@ int rsb_build_vtiles(rsbgpu *g) {
   RSB_TRY(hipMemsetAsync(nl, 0, sizeof(int64_t) * ((size_t)nt + 1), g->stream));
   k_vt_count<<<nt, RSB_BLOCK, 0, g->stream>>>(in, g->d_vt_n, nl);
   RSB_LAUNCH_CHECK();
-  std::vector<int> vn(nt);
-  RSB_TRY(hipMemcpyAsync(vn.data(), g->d_vt_n, sizeof(int) * nt, hipMemcpyDeviceToHost,
-                         g->stream));
-  RSB_TRY(hipStreamSynchronize(g->stream));
-  for (int v : vn)
-    if (v < 0) return 0;  // some tile has no vertex form: the level keeps the entry form
   int rc = rsb_exclusive_scan_i64(g, nl, g->d_vt_loff, nt + 1);
   if (rc) return rc;
   int64_t tl = 0;
@@ -469,8 +463,15 @@ int rsb_build_vtiles(rsbgpu *g) {
   return 0;
 }
 
-extern "C" int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_form) {
+extern "C" int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_tiles) {
+  RSB_TRY(hipSetDevice(g->device));
   *wide_tiles = g->nwide;
-  *vertex_form = g->vt_ok;
+  *vertex_tiles = 0;
+  if (!g->vt_ok || g->ntiles == 0) return 0;
+  std::vector<int> vn(g->ntiles);
+  RSB_TRY(hipMemcpyAsync(vn.data(), g->d_vt_n, sizeof(int) * g->ntiles, hipMemcpyDeviceToHost,
+                         g->stream));
+  RSB_TRY(hipStreamSynchronize(g->stream));
+  for (int v : vn) *vertex_tiles += v >= 0;
   return 0;
 }
