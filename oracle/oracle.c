@@ -177,6 +177,62 @@ void oracle_rcb_order(const double *coord, int ndim, int *order, int n, int p) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Smallest eigenpair of a symmetric n x n matrix A (row-major, n <= 64,
+ * overwritten) by cyclic Jacobi rotations (Golub & Van Loan, Matrix
+ * Computations, 4th ed., Alg. 8.5.1 and 8.5.3): sweeps over p < q in order,
+ * a rotation skipped when |a_pq| <= eps^2 (|a_pp| + |a_qq|), until a sweep
+ * rotates nothing (at most 64 sweeps).  v = the column of the smallest
+ * diagonal entry (lowest index on ties), of unit length, sign left as the
+ * rotations give it.  Returns that eigenvalue. */
+double oracle_sym_smallest(double *A, int n, double *v) {
+  double *V = xmalloc(sizeof(double) * (size_t)n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) V[i * n + j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 64; sweep++) {
+    int rotated = 0;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        const double scale = fabs(A[p * n + p]) + fabs(A[q * n + q]);
+        if (fabs(apq) <= DBL_EPSILON * DBL_EPSILON * scale) continue;
+        const double tau = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                    : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        A[p * n + q] = A[q * n + p] = 0.0;
+        for (int k = 0; k < n; k++) {
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+        rotated = 1;
+      }
+    if (!rotated) break;
+  }
+  int imin = 0;
+  for (int i = 1; i < n; i++)
+    if (A[i * n + i] < A[imin * n + imin]) imin = i;
+  double nrm = 0.0;
+  for (int k = 0; k < n; k++) nrm += V[k * n + imin] * V[k * n + imin];
+  nrm = 1.0 / sqrt(nrm);
+  for (int k = 0; k < n; k++) v[k] = V[k * n + imin] * nrm;
+  const double lam = A[imin * n + imin];
+  free(V);
+  return lam;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Dominant eigenvector of a symmetric n x n matrix (n = 2, 3) by cyclic
  * Jacobi rotations (Golub & Van Loan, Matrix Computations, 4th ed.,
  * Alg. 8.5.1 sym.schur2 and 8.5.3 cyclic Jacobi).  The eigenvector of the
@@ -901,12 +957,36 @@ static int mg_project(double *x, int n, const double *b, const gs_lap *L, const 
   return i == miter ? i : i + 1;
 }
 
-/* Inverse iteration (fiedler.c:191-321, rsb_mg_grammian = 0): y = project(z);
- * orthogonalise y; z = y / |y| orthogonalised; stop after a projection that
- * took a single iteration (z was already an eigenvector to tol). */
+/* Inverse iteration (fiedler.c:191-321): y = project(z); orthogonalise y;
+ * z = y / |y| orthogonalised; stop after a projection that took a single
+ * iteration (z was already an eigenvector to tol).  With grammian = 1
+ * (fiedler.c:263-307) the normalised iterates span a basis Z and each pass
+ * replaces z by the Rayleigh-Ritz vector of the Laplacian over Z:
+ * M = Z L Z^T (M[k][j] = Z_k . L Z_j), z = sum_j v_j Z_j orthogonalised, v
+ * the eigenvector of M of smallest eigenvalue.  Where the reference differs:
+ *   - it orthogonalises z against Z once (classical Gram-Schmidt) and always
+ *     adds the result; here twice (Giraud, Langou, Rozloznik, 2005: "twice is
+ *     enough"), and the residual joins Z only if its norm exceeds 1e-10 (z
+ *     has unit length): a z already in span(Z) otherwise adds a normalised
+ *     rounding residual, Z stops being orthonormal and M's smallest
+ *     eigenvalue then belongs to a near-cancelling combination;
+ *   - it finds v by inverting M (LAPACK, BLAS builds) and 100 power steps
+ *     from a srand(time) start; here v is the exact smallest eigenpair of
+ *     (M + M^T) / 2 (cyclic Jacobi), signed so that sum_j v_j (Z_j . z) >= 0
+ *     (the refined vector points the way the iterate does). */
 static int mg_inverse(double *y, double *z, int n, const gs_lap *L, const mgh_t *h, int miter,
-                      int mpass, double tol, int *passes) {
-  int iters = 0, i;
+                      int mpass, double tol, int grammian, int *passes) {
+  int iters = 0, i, nb = 0;
+  double *Z = NULL, *GZ = NULL, *M = NULL, *Ms = NULL, *c = NULL, *v = NULL, *r = NULL;
+  if (grammian) {
+    Z = xmalloc(sizeof(double) * (size_t)mpass * n);
+    GZ = xmalloc(sizeof(double) * (size_t)mpass * n);
+    M = xmalloc(sizeof(double) * (size_t)mpass * mpass);
+    Ms = xmalloc(sizeof(double) * (size_t)mpass * mpass);
+    c = xmalloc(sizeof(double) * (size_t)mpass);
+    v = xmalloc(sizeof(double) * (size_t)mpass);
+    r = xmalloc(sizeof(double) * (size_t)n);
+  }
   for (i = 0; i < mpass; i++) {
     const int ppfi = mg_project(y, n, z, L, h, miter, tol);
     iters += ppfi;
@@ -914,17 +994,59 @@ static int mg_inverse(double *y, double *z, int n, const gs_lap *L, const mgh_t 
     const double nrm = dotp(y, y, n), ni = 1.0 / sqrt(nrm);
     for (int j = 0; j < n; j++) z[j] = y[j] * ni;
     ortho(z, n);
+    if (grammian) {
+      /* r = z minus its projection on Z, twice; c_j = Z_j . z */
+      for (int l = 0; l < n; l++) r[l] = z[l];
+      for (int j = 0; j < nb; j++) c[j] = 0.0;
+      for (int g = 0; g < 2; g++) {
+        for (int j = 0; j < nb; j++) Ms[j] = dotp(Z + (size_t)j * n, r, n);
+        for (int j = 0; j < nb; j++) {
+          c[j] += Ms[j];
+          for (int l = 0; l < n; l++) r[l] = r[l] - Ms[j] * Z[(size_t)j * n + l];
+        }
+      }
+      const double rn = sqrt(dotp(r, r, n));
+      if (rn > 1e-10) {
+        double *Zi = Z + (size_t)nb * n;
+        const double rni = 1.0 / rn;
+        for (int l = 0; l < n; l++) Zi[l] = r[l] * rni;
+        c[nb] = dotp(Zi, z, n);
+        gs_lap_apply(L, Zi, GZ + (size_t)nb * n);
+        /* M grows by its new row and column (older entries: the same
+           vectors as before) */
+        const int N = nb + 1;
+        for (int k = nb - 1; k >= 0; k--)
+          for (int j = nb - 1; j >= 0; j--) M[k * N + j] = M[k * nb + j];
+        for (int j = 0; j < N; j++) M[nb * N + j] = dotp(Zi, GZ + (size_t)j * n, n);
+        for (int k = 0; k < nb; k++) M[k * N + nb] = dotp(Z + (size_t)k * n, GZ + (size_t)nb * n, n);
+        nb = N;
+      }
+      if (nb >= 2) {
+        for (int k = 0; k < nb; k++)
+          for (int j = 0; j < nb; j++) Ms[k * nb + j] = 0.5 * (M[k * nb + j] + M[j * nb + k]);
+        oracle_sym_smallest(Ms, nb, v);
+        double dir = 0.0;
+        for (int j = 0; j < nb; j++) dir += v[j] * c[j];
+        if (dir < 0.0)
+          for (int j = 0; j < nb; j++) v[j] = -v[j];
+        for (int l = 0; l < n; l++) z[l] = 0.0;
+        for (int j = 0; j < nb; j++)
+          for (int l = 0; l < n; l++) z[l] += Z[(size_t)j * n + l] * v[j];
+        ortho(z, n);
+      }
+    }
     if (ppfi == 1) {
       i++;
       break;
     }
   }
   if (passes) *passes = i;
+  free(Z), free(GZ), free(M), free(Ms), free(c), free(v), free(r);
   return iters;
 }
 
 int oracle_fiedler_mg(const long long *vtx, int nv, const int *elems, int n, int max_iter,
-                      int max_passes, double tol, int factor, double *fiedler,
+                      int max_passes, double tol, int factor, int grammian, double *fiedler,
                       int *passes_out) {
   double *z = xmalloc(sizeof(double) * (size_t)n);
   for (int i = 0; i < n; i++) {
@@ -938,7 +1060,8 @@ int oracle_fiedler_mg(const long long *vtx, int nv, const int *elems, int n, int
   gs_lap_init(&L, vtx, nv, elems, n);
   mgh_t h;
   mg_setup(&h, &L, factor);
-  const int it = mg_inverse(fiedler, z, n, &L, &h, max_iter, max_passes, tol, passes_out);
+  const int it =
+      mg_inverse(fiedler, z, n, &L, &h, max_iter, max_passes, tol, grammian, passes_out);
   mg_free(&h);
   gs_lap_free(&L);
   double nrm = 0.0;
@@ -1241,7 +1364,7 @@ int oracle_part_mesh(int *part, const long long *vtx, const double *xyz, int nel
       double lam = 0.0;
       if (n >= 2 && o->rsb_algo == 1) {
         it = oracle_fiedler_mg(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol,
-                               o->mg_factor, fs, NULL);
+                               o->mg_factor, o->mg_grammian, fs, NULL);
       } else if (n >= 2) {
         it = oracle_fiedler(vtx, nv, el, n, o->max_iter, o->max_passes, o->tol, fs, &lam,
                             NULL);

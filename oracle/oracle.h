@@ -40,6 +40,7 @@ typedef struct {
   int nparts;
   int rsb_algo;  /* 0 Lanczos, 1 MG-preconditioned inverse iteration */
   int mg_factor; /* MG coarsening factor (rsb_mg_factor, > 1) */
+  int mg_grammian; /* rsb_mg_grammian: Gram-matrix refinement of the iterates */
 } oracle_opts;
 
 /* Per-cut (level) statistics, as the reference's metric stack records them
@@ -84,6 +85,9 @@ void oracle_rib_axis(const double *coord, int ndim, const int *order, int n, dou
                      double *ev);
 /* Dominant eigenvector of a symmetric n x n (n <= 3) matrix stored 3 x 3. */
 void oracle_sym_dominant(const double *A, int n, double *ev);
+/* Smallest eigenpair of a symmetric n x n matrix (n <= 64, A overwritten) by
+ * cyclic Jacobi; v of unit length; returns the eigenvalue. */
+double oracle_sym_smallest(double *A, int n, double *v);
 
 /* Weighted dual graph of the whole mesh: w_ij = number of shared vertex ids
  * (mat.c:17-61, 203-249).  CSR with columns sorted ascending, no diagonal.
@@ -140,7 +144,7 @@ int oracle_lanczos_pass(const long long *vtx, int nv, const int *elems, int n, i
  * iterations (RSB_FIEDLER_CALC_NITER); passes_out (may be NULL) receives the
  * number of inverse-iteration passes. */
 int oracle_fiedler_mg(const long long *vtx, int nv, const int *elems, int n, int max_iter,
-                      int max_passes, double tol, int factor, double *fiedler,
+                      int max_passes, double tol, int factor, int grammian, double *fiedler,
                       int *passes_out);
 
 /* Pieces of it, for the tests: the MG hierarchy's level sizes and matrices

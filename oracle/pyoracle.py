@@ -38,7 +38,8 @@ def lib(threaded=False):
 class Opts(C.Structure):
     _fields_ = [("find_disconnected_comps", C.c_int), ("rsb_pre", C.c_int),
                 ("max_iter", C.c_int), ("max_passes", C.c_int), ("tol", C.c_double),
-                ("nparts", C.c_int), ("rsb_algo", C.c_int), ("mg_factor", C.c_int)]
+                ("nparts", C.c_int), ("rsb_algo", C.c_int), ("mg_factor", C.c_int),
+                ("mg_grammian", C.c_int)]
 
 
 ML = 32
@@ -96,7 +97,9 @@ def _declare(L):
     L.oracle_lanczos_pass.restype = C.c_int
     L.oracle_components.argtypes = [P, C.c_int, P, C.c_int]
     L.oracle_fiedler_mg.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_double,
-                                    C.c_int, P, P]
+                                    C.c_int, C.c_int, P, P]
+    L.oracle_sym_smallest.argtypes = [P, C.c_int, P]
+    L.oracle_sym_smallest.restype = C.c_double
     L.oracle_fiedler_mg.restype = C.c_int
     L.oracle_mg_levels.argtypes = [P, C.c_int, P, C.c_int, C.c_int, P, P]
     L.oracle_mg_levels.restype = C.c_int
@@ -268,14 +271,23 @@ def balance_step(vtx, nv, elems, fied, nl, sender_lower, scount):
     return el, fv, nl2
 
 
-def fiedler_mg(vtx, nv, elems, max_iter=50, max_passes=50, tol=1e-5, factor=2):
+def sym_smallest(A):
+    """Smallest eigenpair (lambda, v) of a symmetric matrix by cyclic Jacobi."""
+    A = np.array(A, dtype=np.float64, order="C")
+    n = A.shape[0]
+    v = np.zeros(n)
+    lam = lib().oracle_sym_smallest(_p(A), n, _p(v))
+    return lam, v
+
+
+def fiedler_mg(vtx, nv, elems, max_iter=50, max_passes=50, tol=1e-5, factor=2, grammian=0):
     """(fiedler, iterations, passes) of the MG inverse iteration."""
     vtx = np.ascontiguousarray(vtx, dtype=np.int64)
     elems = np.ascontiguousarray(elems, dtype=np.int32)
     f = np.zeros(elems.size)
     npass = C.c_int()
     it = lib().oracle_fiedler_mg(_p(vtx), nv, _p(elems), elems.size, max_iter, max_passes, tol,
-                                 factor, _p(f), C.byref(npass))
+                                 factor, grammian, _p(f), C.byref(npass))
     return f, it, npass.value
 
 
@@ -323,7 +335,8 @@ def mg_project(vtx, nv, elems, b, miter=50, tol=1e-5, factor=2):
 
 
 def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
-              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2, threads=0, verbose=0):
+              max_passes=50, tol=1e-5, rsb_algo=0, mg_factor=2, threads=0, verbose=0,
+              mg_grammian=0):
     """The reference algorithm over virtual ranks.  threads > 0 runs the
     OpenMP build on that many threads (the CPU baseline); its sums depend on
     the thread count, so parity checks keep threads = 0 (serial build)."""
@@ -338,7 +351,8 @@ def part_mesh(vtx, xyz, nv, nparts, rsb_pre=1, find_comps=1, max_iter=50,
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         xp = _p(xyz)
     part = np.full(nel, -1, dtype=np.int32)
-    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts, rsb_algo, mg_factor)
+    o = Opts(find_comps, rsb_pre, max_iter, max_passes, tol, nparts, rsb_algo, mg_factor,
+             mg_grammian)
     rep = Report()
     rc = L.oracle_part_mesh(_p(part), _p(vtx), xp, nel, nv, C.byref(o), C.byref(rep))
     if rc != 0:

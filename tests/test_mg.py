@@ -129,10 +129,48 @@ def test_oracle_fiedler_matches_dense_eigh(oracle):
     assert abs(f @ L @ f - ev[1]) < 1e-8 * ev[1]
 
 
-def test_oracle_partition_with_mg_is_balanced_and_close_to_lanczos(oracle):
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 20, 50])
+def test_oracle_jacobi_smallest_eigenpair_matches_lapack(oracle, n):
+    """The Gram refinement's eigen-solve (cyclic Jacobi) against LAPACK
+    (numpy eigh) on random symmetric matrices, a repeated smallest
+    eigenvalue's leading digits included."""
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + np.diag(rng.uniform(0.0, 1.0, n))
+    lam, v = oracle.sym_smallest(A)
+    ev, V = np.linalg.eigh(A)
+    assert abs(lam - ev[0]) <= 1e-11 * max(1.0, abs(ev).max())
+    assert abs(np.linalg.norm(v) - 1) < 1e-12
+    assert np.linalg.norm(A @ v - lam * v) <= 1e-10 * max(1.0, abs(ev).max())
+
+
+@pytest.mark.parametrize("mesh", ["pipe", "box"])
+def test_oracle_gram_refinement_converges_to_the_fiedler_vector(oracle, mesh):
+    """rsb_mg_grammian = 1 (fiedler.c:263-307): each pass's z is replaced by
+    the Rayleigh-Ritz vector of the Laplacian over the orthonormalised
+    iterates; the result is still v_2 (dense eigh) and, since the refined
+    start is at least as good as the last iterate, it takes no more passes
+    than without the refinement."""
+    if mesh == "pipe":
+        vtx, xyz, nv = M.pipe_hex(3, 8, 12, seed=3, amp=0.1)
+    else:
+        vtx, xyz, nv = M.box_hex(14, 9, 6, jitter=0.1, seed=2)
+    n = vtx.shape[0]
+    el = np.arange(n, dtype=np.int32)
+    L = dense_laplacian(oracle, vtx, nv, el)
+    ev, V = np.linalg.eigh(L)
+    f0, it0, np0 = oracle.fiedler_mg(vtx, nv, el, tol=1e-8)
+    f1, it1, np1 = oracle.fiedler_mg(vtx, nv, el, tol=1e-8, grammian=1)
+    assert abs(abs(f1 @ V[:, 1]) - 1) < 1e-8
+    assert abs(f1 @ L @ f1 - ev[1]) < 1e-8 * ev[1]
+    assert np1 <= np0 and np1 >= 1
+
+
+@pytest.mark.parametrize("grammian", [0, 1])
+def test_oracle_partition_with_mg_is_balanced_and_close_to_lanczos(oracle, grammian):
     vtx, xyz, nv = M.box_hex(12, 10, 8, jitter=0.1, seed=9)
     P = 8
-    a, _ = oracle.part_mesh(vtx, xyz, nv, P, rsb_algo=1, mg_factor=2)
+    a, _ = oracle.part_mesh(vtx, xyz, nv, P, rsb_algo=1, mg_factor=2, mg_grammian=grammian)
     b, _ = oracle.part_mesh(vtx, xyz, nv, P)
     assert np.array_equal(np.bincount(a, minlength=P), np.bincount(b, minlength=P))
     qa, qb = oracle.quality(a, vtx, nv, P), oracle.quality(b, vtx, nv, P)
