@@ -73,7 +73,9 @@ typedef struct {
  * first allocation-free call.  Meshes of more than PARRSB_AMD_NOALLOC_MAX_NEL
  * elements (PARRSB_AMD_NOALLOC_MAX_NEL_MG with rsb_algo = 1, whose stored
  * search directions take the most memory per element), or more parts than
- * PARRSB_AMD_NOALLOC_MAX_NEL, are rejected with 3, and so is tagged mode; at
+ * PARRSB_AMD_NOALLOC_MAX_NEL, are rejected with 3, and so are tagged mode
+ * and rsb_mg_grammian = 1 with rsb_algo = 1 (its Gram matrices take
+ * rsb_max_passes^2 doubles per segment); at
  * these limits a call holds at most about a third (Lanczos) or half (MG) of
  * the array.  The array keeps the last call's contents and is shared
  * by every allocation-free call on that GPU: calls that overlap on one GPU

@@ -97,15 +97,22 @@ int rsbgpu_fiedler(rsbgpu *g, int precision, int max_iter, int max_passes, doubl
                    int init_mode, rsbgpu_fiedler_out *out);
 int rsbgpu_set_start(rsbgpu *g, const float *d_start_per_element);
 /* Fiedler vectors by MG-preconditioned inverse iteration (rsb_algo = 1,
- * fiedler.c:94-321 with rsb_mg_grammian = 0; multigrid.c:184-317), FP64,
- * for every segment with p > 1 and at least 2 elements: each projection is
- * at most max_iter preconditioned CG iterations to tol, at most max_passes
+ * fiedler.c:94-321 and 186-312; multigrid.c:184-317), FP64, for every
+ * segment with p > 1 and at least 2 elements: each projection is at most
+ * max_iter (<= 60) preconditioned CG iterations to tol, at most max_passes
  * passes, stopping after a projection of one iteration.  factor (> 1) is the
- * MG coarsening factor.  out->niter = summed CG iterations, out->npass =
- * passes, out->lambda / out->resid = Rayleigh quotient and relative
- * eigen-residual of the result.  Same result layout as rsbgpu_fiedler. */
+ * MG coarsening factor.  grammian != 0 refines each next start vector by a
+ * Rayleigh-Ritz step over the normalised iterates of the earlier passes
+ * (fiedler.c:250-305; needs max_passes <= 64).  out->niter = summed CG
+ * iterations, out->npass = passes, out->lambda / out->resid = Rayleigh
+ * quotient and relative eigen-residual of the result.  Same result layout
+ * as rsbgpu_fiedler. */
 int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double tol, int factor,
-                      rsbgpu_fiedler_out *out);
+                      int grammian, rsbgpu_fiedler_out *out);
+/* Parity hook: the Ritz step's eigen-solve (cyclic Jacobi) on one symmetric
+ * n x n matrix A (row-major, 1 <= n <= 63): v = unit eigenvector of the
+ * smallest eigenvalue *lam. */
+int rsbgpu_test_sym_smallest(const double *A, int n, double *v, double *lam);
 /* Parity hook: MG level `level` (>= 1) of segment `seg` from the last
  * rsbgpu_fiedler_mg call, in the segment's local numbering: rowptr[n+1],
  * col[nnz], w[nnz] (positive off-diagonal weights), diag[n].  With NULL

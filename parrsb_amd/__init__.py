@@ -140,7 +140,8 @@ SIGNATURES = {
     "rsbgpu_level_laplacian": (I, [P]),
     "rsbgpu_fiedler": (I, [P, I, I, I, C.c_double, I, C.POINTER(FiedlerOut)]),
     "rsbgpu_set_start": (I, [P, P]),
-    "rsbgpu_fiedler_mg": (I, [P, I, I, C.c_double, I, C.POINTER(FiedlerOut)]),
+    "rsbgpu_fiedler_mg": (I, [P, I, I, C.c_double, I, I, C.POINTER(FiedlerOut)]),
+    "rsbgpu_test_sym_smallest": (I, [P, I, P, P]),
     "rsbgpu_mg_level_csr": (I, [P, I, I, C.POINTER(I), C.POINTER(C.c_int64), P, P, P, P]),
     "rsbgpu_fiedler_sort": (I, [P]),
     "rsbgpu_bisect": (I, [P, P, P, P, I, P, P]),
@@ -464,15 +465,16 @@ class Device:
                                        C.byref(fo)))
         return out
 
-    def fiedler_mg(self, nseg, max_iter=50, max_passes=50, tol=1e-5, factor=2):
-        """MG-preconditioned inverse iteration (rsb_algo = 1) on every segment."""
+    def fiedler_mg(self, nseg, max_iter=50, max_passes=50, tol=1e-5, factor=2, grammian=0):
+        """MG-preconditioned inverse iteration (rsb_algo = 1) on every segment;
+        grammian=1 refines each next start vector by a Rayleigh-Ritz step."""
         out = {k: np.zeros(nseg, dtype=np.int32) for k in ("niter", "npass", "fallback")}
         out["lambda"] = np.zeros(nseg)
         out["rnorm"] = np.zeros(3 * nseg)
         out["resid"] = np.zeros(nseg)
         fo = FiedlerOut(_p(out["niter"]), _p(out["npass"]), _p(out["lambda"]),
                         _p(out["rnorm"]), _p(out["resid"]), _p(out["fallback"]))
-        self._chk(lib().rsbgpu_fiedler_mg(self.h, max_iter, max_passes, tol, factor,
+        self._chk(lib().rsbgpu_fiedler_mg(self.h, max_iter, max_passes, tol, factor, grammian,
                                           C.byref(fo)))
         return out
 
@@ -611,6 +613,19 @@ def test_tqli(d, e, fast=False):
     if rc:
         raise ParrsbError(rc, "rsbgpu_test_tqli failed")
     return lam[0], y
+
+
+def test_sym_smallest(A):
+    """The MG Ritz step's Jacobi eigen-solve on the device: (lambda_min, v)
+    of one symmetric n x n matrix, n <= 63."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    v = np.zeros(n)
+    lam = np.zeros(1)
+    rc = lib().rsbgpu_test_sym_smallest(_p(A), n, _p(v), _p(lam))
+    if rc:
+        raise ParrsbError(rc, "rsbgpu_test_sym_smallest failed")
+    return lam[0], v
 
 
 def lws_test(coord, level, rowptr, col, wgt, lam, params):

@@ -1,6 +1,8 @@
 // Fiedler vectors by MG-preconditioned inverse iteration (rsb_algo = 1;
-// fiedler.c:94-321 with rsb_mg_grammian = 0, multigrid.c:184-317, the
-// matrices of mat.c:17-61 and 704-764), for every segment of a level at once.
+// fiedler.c:94-321, multigrid.c:184-317, the matrices of mat.c:17-61 and
+// 704-764), for every segment of a level at once, with the optional
+// Gram-matrix refinement of the start vectors (rsb_mg_grammian = 1,
+// fiedler.c:250-305; see k_mg_gjac).
 //
 // Level 0 of the hierarchy is the level Laplacian already assembled for the
 // Lanczos path (padded rows, SELL-64 entries with 4-bit weights, diagonal =
@@ -16,6 +18,7 @@
 // per-segment kernel (tiles in order), so results do not depend on
 // scheduling.  Segments that finished skip every launch.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <vector>
 
@@ -25,14 +28,17 @@
 
 // per-segment FP64 scalars
 enum { MS_RZ1 = 0, MS_ALPHA, MS_PWINV, MS_BETA, MS_RTOL, MS_RR, MS_MEAN, MS_NI, MS_LAMBDA,
-       MS_RES, MS_NSC };
-// per-segment ints
-enum { MI_INV = 0, MI_PCG, MI_IT, MI_PPFI, MI_ITERS, MI_NPASS, MI_NC, MI_NI };
+       MS_RES, MS_GRNI, MS_NSC };
+// per-segment ints (MI_NB: vectors in the Gram basis, MI_GADD: the pass's
+// vector joins the basis)
+enum { MI_INV = 0, MI_PCG, MI_IT, MI_PPFI, MI_ITERS, MI_NPASS, MI_NC, MI_NB, MI_GADD, MI_NI };
 // what a finishing kernel does with a segment's totals
 enum { FIN_MEAN = 0, FIN_NORM, FIN_PASS0, FIN_RZ1, FIN_ALPHA, FIN_CONV, FIN_BETA,
        FIN_PASSEND, FIN_RAYLEIGH };
-// which segments a launch works on
-enum { GATE_ELIG = 0, GATE_INV, GATE_PCG };
+// which segments a launch works on (GATE_GRAM: continuing segments whose
+// basis grows this pass; GATE_GMIX: continuing segments with >= 2 basis
+// vectors)
+enum { GATE_ELIG = 0, GATE_INV, GATE_PCG, GATE_GRAM, GATE_GMIX };
 
 struct mg_args {
   int nseg;
@@ -57,7 +63,9 @@ __device__ inline bool mg_elig(const mg_args &a, int s) {
 __device__ inline bool mg_gate(const mg_args &a, int s, int gate) {
   if (gate == GATE_ELIG) return mg_elig(a, s);
   if (gate == GATE_INV) return MST(s, MI_INV) != 0;
-  return MST(s, MI_PCG) != 0;
+  if (gate == GATE_PCG) return MST(s, MI_PCG) != 0;
+  if (gate == GATE_GRAM) return MST(s, MI_INV) != 0 && MST(s, MI_GADD) != 0;
+  return MST(s, MI_INV) != 0 && MST(s, MI_NB) >= 2;
 }
 
 // Tile prologue: rows [row0, row1) of segment s, valid while local < n.
@@ -514,6 +522,257 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_mg_rayleigh(mg_args a, mg_sell L,
     acc[0] += f * lf, acc[1] += lf * lf, acc[2] += f * f;
   }
   mg_publish<3>(a, t, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Gram-matrix refinement of the next pass's start vector (rsb_mg_grammian =
+// 1).  The reference (fiedler.c:250-305) keeps a basis Z of the normalised
+// iterates, forms M = Z^T L Z and takes the smallest Ritz vector as the next
+// start; its buffers are laid out so that rows 1.. of Z overlap M (Z = err +
+// lelt, M = Z + lelt, fiedler.c:221-222), so the steps are those its comments
+// state, made stable: the new iterate is orthogonalised against Z twice
+// (classical Gram-Schmidt, twice), it joins Z only if what is left has norm
+// > 1e-10, M grows by the new vector's row and column, and the Ritz vector is
+// the smallest eigenvector of sym(M) by cyclic Jacobi, signed to point along
+// the iterate (sum_j v_j (Z_j . z) >= 0).  Segments whose iteration stops
+// this pass skip the step: it only changes the start of a pass that follows.
+
+struct mg_gram {
+  double *Z, *GZ;  // cap vectors of mpad rows each: the basis and L times it
+  double *M;       // cap x cap per segment: M[k][j] = Z_k . L Z_j
+  double *cg;      // MG_NP per segment: the current Gram-Schmidt sweep's coefficients
+  double *c;       // MG_NP per segment: Z_j . z over both sweeps
+  double *v;       // MG_NP per segment: the Ritz vector's coefficients
+  int cap;         // basis capacity (rsb_max_passes - 1 <= MG_NP - 1)
+};
+
+// slots k0 .. k0 + ns - 1 of the tile's partials <- v, summed over the workgroup
+__device__ inline void mg_publish_at(const mg_args &a, const mg_tile &t, double (&v)[4], int k0,
+                                     int ns) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  block_sum<4>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < ns; k++) a.partial[(size_t)(k0 + k) * a.ntiles + t.ti] = v[k];
+}
+
+// partial slot j = X_j . y for j < MI_NB + extra, X_j = X + j mpad; with
+// xlast given, X_nb is xlast (the newest vector), which is also stored to
+// keep + nb mpad when keep is given
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_gdots(mg_args a, int gate, int64_t mpad,
+                                                       const double *X, const double *xlast,
+                                                       double *keep, int extra, const double *y) {
+#pragma clang fp contract(off)
+  mg_tile t;
+  if (!mg_tile_of(a, gate, t)) return;
+  const int nb = MST(t.s, MI_NB), cnt = nb + extra;
+  if (keep) MG_ROWS(t, row) keep[(size_t)nb * mpad + row] = xlast[row];
+  for (int j0 = 0; j0 < cnt; j0 += 4) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    MG_ROWS(t, row) {
+      const double yv = y[row];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int j = j0 + k;
+        if (j < cnt) {
+          const double xv = (xlast && j == nb) ? xlast[row] : X[(size_t)j * mpad + row];
+          acc[k] += xv * yv;
+        }
+      }
+    }
+    mg_publish_at(a, t, acc, j0, min(4, cnt - j0));
+  }
+}
+
+// dst = src - sum_j cg_j Z_j (j in order); partial dst.dst
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_gsub(mg_args a, mg_gram G, int64_t mpad,
+                                                      const double *src, double *dst) {
+#pragma clang fp contract(off)
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_INV, t)) return;
+  const int nb = MST(t.s, MI_NB);
+  const double *cg = G.cg + (size_t)t.s * MG_NP;
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    double v = src[row];
+    for (int j = 0; j < nb; j++) v = v - cg[j] * G.Z[(size_t)j * mpad + row];
+    dst[row] = v;
+    acc[0] += v * v;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// the new basis vector Z_nb = r / |r| (also into p for L p); partial Z_nb . z
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_gnew(mg_args a, mg_gram G, int64_t mpad,
+                                                      const double *r, const double *z,
+                                                      double *p) {
+#pragma clang fp contract(off)
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_GRAM, t)) return;
+  const int nb = MST(t.s, MI_NB);
+  const double rni = MSC(t.s, MS_GRNI);
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    const double zi = r[row] * rni;
+    p[row] = zi;
+    G.Z[(size_t)nb * mpad + row] = zi;
+    acc[0] += zi * z[row];
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// z = sum_j v_j Z_j (j in order); partial sum z
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_gcomb(mg_args a, mg_gram G, int64_t mpad,
+                                                       double *z) {
+#pragma clang fp contract(off)
+  mg_tile t;
+  if (!mg_tile_of(a, GATE_GMIX, t)) return;
+  const int nb = MST(t.s, MI_NB);
+  const double *v = G.v + (size_t)t.s * MG_NP;
+  double acc[1] = {0.0};
+  MG_ROWS(t, row) {
+    double x = 0.0;
+    for (int j = 0; j < nb; j++) x += G.Z[(size_t)j * mpad + row] * v[j];
+    z[row] = x;
+    acc[0] += x;
+  }
+  mg_publish<1>(a, t, acc);
+}
+
+// what k_mg_gfin does with a segment's totals
+enum { GF_CGS0 = 0, GF_CGS1, GF_NORM, GF_CNEW, GF_MROW, GF_MCOL };
+
+__global__ void __launch_bounds__(RSB_BLOCK) k_mg_gfin(mg_args a, mg_gram G, int op) {
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  const int s = blockIdx.x;
+  if (!mg_gate(a, s, op <= GF_NORM ? GATE_INV : GATE_GRAM)) return;
+  const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
+  const int nb = MST(s, MI_NB);
+  const int cnt = op == GF_MROW ? nb + 1 : (op == GF_NORM || op == GF_CNEW) ? 1 : nb;
+  double *cg = G.cg + (size_t)s * MG_NP, *c = G.c + (size_t)s * MG_NP;
+  double *M = G.M + (size_t)s * G.cap * G.cap;
+  for (int k0 = 0; k0 < cnt; k0 += 4) {
+    double tot[4];
+    const int ns = min(4, cnt - k0);
+    seg_totals(a, first, count, k0, ns, tot, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < ns; k++) {
+        const int j = k0 + k;
+        const double x = tot[k];
+        switch (op) {
+        case GF_CGS0: cg[j] = x, c[j] = x; break;
+        case GF_CGS1: cg[j] = x, c[j] += x; break;
+        case GF_NORM: {
+          const double rn = sqrt(x);
+          MST(s, MI_GADD) = rn > 1e-10;
+          MSC(s, MS_GRNI) = 1.0 / rn;
+          break;
+        }
+        case GF_CNEW: c[nb] = x; break;
+        case GF_MROW: M[(size_t)nb * G.cap + j] = x; break;
+        case GF_MCOL: M[(size_t)j * G.cap + nb] = x; break;
+        }
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && op == GF_MCOL) MST(s, MI_NB) = nb + 1;
+}
+
+// Smallest eigenpair of the symmetric N x N matrix A (row-major in LDS,
+// overwritten; V: N x N LDS workspace) by cyclic Jacobi rotations (Golub &
+// Van Loan, Matrix Computations, 4th ed., Alg. 8.5.1 and 8.5.3), one wave:
+// sweeps over p < q in order, a rotation skipped when |a_pq| <= eps^2
+// (|a_pp| + |a_qq|), until a sweep rotates nothing (at most 64 sweeps).  The
+// threads split the rows of each rotation; every entry takes the same
+// operations in the same order as a serial sweep, so the result is the
+// serial one bit for bit.  v <- the column of the smallest diagonal entry
+// (lowest index on ties), unit length; returns that eigenvalue.
+__device__ double gram_jacobi(double *A, double *V, int N, double *v) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int i = tid; i < N * N; i += nt) V[i] = (i / N == i % N) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int sweep = 0; sweep < 64; sweep++) {
+    bool rotated = false;
+    for (int p = 0; p < N - 1; p++)
+      for (int q = p + 1; q < N; q++) {
+        const double apq = A[p * N + q];
+        if (apq == 0.0) continue;
+        const double app = A[p * N + p], aqq = A[q * N + q];
+        const double scale = fabs(app) + fabs(aqq);
+        if (fabs(apq) <= DBL_EPSILON * DBL_EPSILON * scale) continue;
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
+                                    : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        __syncthreads();  // a_pq, a_pp, a_qq read by every thread
+        for (int k = tid; k < N; k += nt) {
+          const double akp = A[k * N + p], akq = A[k * N + q];
+          A[k * N + p] = c * akp - s * akq;
+          A[k * N + q] = s * akp + c * akq;
+        }
+        __syncthreads();
+        for (int k = tid; k < N; k += nt) {
+          const double apk = A[p * N + k], aqk = A[q * N + k];
+          A[p * N + k] = c * apk - s * aqk;
+          A[q * N + k] = s * apk + c * aqk;
+          const double vkp = V[k * N + p], vkq = V[k * N + q];
+          V[k * N + p] = c * vkp - s * vkq;
+          V[k * N + q] = s * vkp + c * vkq;
+        }
+        __syncthreads();
+        if (tid == 0) A[p * N + q] = A[q * N + p] = 0.0;
+        __syncthreads();
+        rotated = true;
+      }
+    if (!rotated) break;
+  }
+  int imin = 0;
+  for (int i = 1; i < N; i++)
+    if (A[i * N + i] < A[imin * N + imin]) imin = i;
+  double nrm = 0.0;
+  for (int k = 0; k < N; k++) nrm += V[k * N + imin] * V[k * N + imin];
+  nrm = 1.0 / sqrt(nrm);
+  for (int k = tid; k < N; k += nt) v[k] = V[k * N + imin] * nrm;
+  __syncthreads();
+  return A[imin * N + imin];
+}
+
+// Ritz vector of every segment with >= 2 basis vectors: smallest eigenvector
+// of sym(M), signed so that sum_j v_j c_j >= 0.  One wave per segment; LDS
+// (2 N^2 + N) doubles.
+__global__ void __launch_bounds__(RSB_WAVE) k_mg_gjac(mg_args a, mg_gram G) {
+#pragma clang fp contract(off)
+  extern __shared__ double lds[];
+  const int s = blockIdx.x;
+  if (!mg_gate(a, s, GATE_GMIX)) return;
+  const int N = MST(s, MI_NB), cap = G.cap;
+  double *A = lds, *V = lds + N * N, *v = V + N * N;
+  const double *M = G.M + (size_t)s * cap * cap;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
+    const int k = i / N, j = i % N;
+    A[i] = 0.5 * (M[(size_t)k * cap + j] + M[(size_t)j * cap + k]);
+  }
+  __syncthreads();
+  (void)gram_jacobi(A, V, N, v);
+  if (threadIdx.x == 0) {
+    const double *c = G.c + (size_t)s * MG_NP;
+    double dir = 0.0;
+    for (int j = 0; j < N; j++) dir += v[j] * c[j];
+    double *out = G.v + (size_t)s * MG_NP;
+    for (int j = 0; j < N; j++) out[j] = dir < 0.0 ? -v[j] : v[j];
+  }
+}
+
+// parity hook of gram_jacobi on one matrix
+__global__ void __launch_bounds__(RSB_WAVE) k_gjac_test(int n, const double *Ain, double *v,
+                                                        double *lam) {
+  extern __shared__ double lds[];
+  double *A = lds, *V = lds + n * n, *vv = V + n * n;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) A[i] = Ain[i];
+  __syncthreads();
+  const double l = gram_jacobi(A, V, n, vv);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v[k] = vv[k];
+  if (threadIdx.x == 0) *lam = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,10 +1317,12 @@ static int mg_build(rsbgpu *g, int factor, mg_hier &H, double *diag0) {
 }
 
 extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double tol,
-                                 int factor, rsbgpu_fiedler_out *out) {
+                                 int factor, int grammian, rsbgpu_fiedler_out *out) {
   RSB_TRY(hipSetDevice(g->device));
   if (factor < 2) RSB_FAIL(3, "rsb_mg_factor must be > 1");
   if (max_iter < 1 || max_iter > MG_NP - 4) RSB_FAIL(3, "rsb_max_iter must be in 1..%d", MG_NP - 4);
+  if (grammian && max_passes > MG_NP)
+    RSB_FAIL(3, "rsb_max_passes must be <= %d with rsb_mg_grammian = 1", MG_NP);
   const int nseg = g->nseg;
   rsb_phase_begin(g);
   hipStream_t st = g->stream;
@@ -1114,6 +1375,17 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
   RSB_TRY(own.alloc(partial, sizeof(double) * MG_NP * (size_t)std::max(ntiles, 1)));
   RSB_TRY(own.alloc(coef, sizeof(double) * MG_NP * (size_t)nseg));
   RSB_TRY(own.alloc(active, sizeof(int)));
+  // Gram basis: at most one vector per pass that another pass follows
+  mg_gram G{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, max_passes - 1};
+  const bool gram = grammian && G.cap >= 1;
+  if (gram) {
+    RSB_TRY(own.alloc(G.Z, vb * (size_t)G.cap));
+    RSB_TRY(own.alloc(G.GZ, vb * (size_t)G.cap));
+    RSB_TRY(own.alloc(G.M, sizeof(double) * (size_t)G.cap * G.cap * (size_t)nseg));
+    RSB_TRY(own.alloc(G.cg, sizeof(double) * MG_NP * (size_t)nseg));
+    RSB_TRY(own.alloc(G.c, sizeof(double) * MG_NP * (size_t)nseg));
+    RSB_TRY(own.alloc(G.v, sizeof(double) * MG_NP * (size_t)nseg));
+  }
   RSB_TRY(hipMemsetAsync(sc, 0, sizeof(double) * MS_NSC * (size_t)nseg, st));
   // every eligible segment starts in the inverse iteration
   {
@@ -1235,7 +1507,7 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
       }
       MG_TRY(count_active(nact));
     }
-    // end of pass (fiedler.c:261-275): y orthogonalised, z = y / |y|, z orthogonalised
+    // end of pass (fiedler.c:231-247): y orthogonalised, z = y / |y|, z orthogonalised
     k_mg_sum<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
     fin(FIN_MEAN, GATE_INV, 1);
     k_mg_center_dot<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, x);
@@ -1245,6 +1517,32 @@ extern "C" int rsbgpu_fiedler_mg(rsbgpu *g, int max_iter, int max_passes, double
     fin(FIN_PASSEND, GATE_INV, 1);
     k_mg_center<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, b);
     RSB_LAUNCH_CHECK();
+    if (gram) {
+      // z (in b) orthogonalised against the basis twice into r; r / |r|
+      // joins the basis, M grows by its row and column; the Ritz vector
+      // becomes the next start (r, p, w are free between passes)
+      auto gfin = [&](int op) { k_mg_gfin<<<nseg, RSB_BLOCK, 0, st>>>(a, G, op); };
+      k_mg_gdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, mpad, G.Z, nullptr, nullptr, 0, b);
+      gfin(GF_CGS0);
+      k_mg_gsub<<<ntiles, RSB_BLOCK, 0, st>>>(a, G, mpad, b, r);
+      k_mg_gdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_INV, mpad, G.Z, nullptr, nullptr, 0, r);
+      gfin(GF_CGS1);
+      k_mg_gsub<<<ntiles, RSB_BLOCK, 0, st>>>(a, G, mpad, r, r);
+      gfin(GF_NORM);
+      k_mg_gnew<<<ntiles, RSB_BLOCK, 0, st>>>(a, G, mpad, r, b, p);
+      gfin(GF_CNEW);
+      k_mg_apply<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_GRAM, S, p, w);
+      k_mg_gdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_GRAM, mpad, G.GZ, w, G.GZ, 1, p);
+      gfin(GF_MROW);
+      k_mg_gdots<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_GRAM, mpad, G.Z, nullptr, nullptr, 0, w);
+      gfin(GF_MCOL);
+      const size_t lds = sizeof(double) * (2 * (size_t)G.cap * G.cap + G.cap);
+      k_mg_gjac<<<nseg, RSB_WAVE, lds, st>>>(a, G);
+      k_mg_gcomb<<<ntiles, RSB_BLOCK, 0, st>>>(a, G, mpad, b);
+      fin(FIN_MEAN, GATE_GMIX, 1);
+      k_mg_center<<<ntiles, RSB_BLOCK, 0, st>>>(a, GATE_GMIX, b);
+      RSB_LAUNCH_CHECK();
+    }
     int nrun = 0;
     MG_TRY(count_active(nrun));
     if (nrun == 0) break;
@@ -1301,4 +1599,28 @@ extern "C" int rsbgpu_mg_level_csr(rsbgpu *g, int level, int seg, int *n, int64_
   if (rn > 0)
     RSB_TRY(hipMemcpy(diag, L.diag + ro, sizeof(double) * rn, hipMemcpyDeviceToHost));
   return 0;
+}
+
+// Parity hook: the Ritz step's Jacobi eigen-solve on one symmetric n x n
+// matrix (row-major, 1 <= n <= MG_NP - 1): v = unit eigenvector of the
+// smallest eigenvalue *lam.
+extern "C" int rsbgpu_test_sym_smallest(const double *A, int n, double *v, double *lam) {
+  if (n < 1 || n > MG_NP - 1) return 3;
+  double *dA = nullptr, *dv = nullptr, *dl = nullptr;
+  hipError_t err = rsb_dev_malloc((void **)&dA, sizeof(double) * (size_t)n * n);
+  if (err == hipSuccess) err = rsb_dev_malloc((void **)&dv, sizeof(double) * n);
+  if (err == hipSuccess) err = rsb_dev_malloc((void **)&dl, sizeof(double));
+  if (err == hipSuccess)
+    err = hipMemcpy(dA, A, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice);
+  if (err == hipSuccess) {
+    k_gjac_test<<<1, RSB_WAVE, sizeof(double) * (2 * (size_t)n * n + n), 0>>>(n, dA, dv, dl);
+    err = hipGetLastError();
+  }
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  if (err == hipSuccess) err = hipMemcpy(v, dv, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(lam, dl, sizeof(double), hipMemcpyDeviceToHost);
+  if (dA) (void)rsb_dev_free(dA);
+  if (dv) (void)rsb_dev_free(dv);
+  if (dl) (void)rsb_dev_free(dl);
+  return err == hipSuccess ? 0 : 2;
 }

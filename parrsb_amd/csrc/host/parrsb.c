@@ -266,9 +266,15 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, con
     set_error("rsb_algo must be 0 (Lanczos) or 1 (MG inverse iteration)");
     return 3;
   }
-  if (options->partitioner == 0 && options->rsb_algo == 1 &&
-      (options->rsb_mg_factor < 2 || options->rsb_mg_grammian != 0)) {
-    set_error("MG inverse iteration needs rsb_mg_factor > 1 and rsb_mg_grammian = 0");
+  if (options->partitioner == 0 && options->rsb_algo == 1 && options->rsb_mg_factor < 2) {
+    set_error("MG inverse iteration needs rsb_mg_factor > 1");
+    return 3;
+  }
+  /* the refinement is on for rsb_mg_grammian == 1 only (fiedler.c:250) */
+  if (options->partitioner == 0 && options->rsb_algo == 1 && options->rsb_mg_grammian == 1 &&
+      options->rsb_max_passes > 64) {
+    set_error("rsb_mg_grammian = 1 takes at most 64 passes (rsb_max_passes = %d)",
+              options->rsb_max_passes);
     return 3;
   }
   if (vtx == NULL && options->partitioner == 0) {
@@ -340,6 +346,7 @@ static int run(int *part, int *seq, const long long *vtx, const double *xyz, con
       prm.warm = NULL, prm.warm_passes = cfg_warm_passes, prm.warm_max_n = cfg_warm_max_n;
       prm.ngroups = ngroups, prm.group_count = gcount;
       prm.rsb_algo = options->rsb_algo, prm.mg_factor = options->rsb_mg_factor;
+      prm.mg_grammian = options->rsb_mg_grammian == 1;
       void *dw = NULL;
       if (cfg_warm) {
         rc = rsbgpu_upload(g, cfg_warm, sizeof(float) * (size_t)rsbgpu_lws_nparams(), &dw);
@@ -448,6 +455,11 @@ int parrsb_amd_part_mesh_dev_noalloc(int *d_part, const long long *d_vtx, const 
     set_error("allocation-free partitioning with rsb_algo = 1 takes at most %d elements "
               "(got %d)",
               PARRSB_AMD_NOALLOC_MAX_NEL_MG, nel);
+    return 3;
+  }
+  if (options->partitioner == 0 && options->rsb_algo == 1 && options->rsb_mg_grammian == 1) {
+    /* its Gram matrices take rsb_max_passes^2 doubles per segment */
+    set_error("allocation-free partitioning does not take rsb_mg_grammian = 1");
     return 3;
   }
   if (options->nparts > PARRSB_AMD_NOALLOC_MAX_NEL) {

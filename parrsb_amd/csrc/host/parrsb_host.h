@@ -9,6 +9,7 @@ typedef struct {
   int nparts, rsb_pre, max_iter, max_passes, find_comps, precision, verbose, residuals;
   int rsb_algo;  /* 0 Lanczos, 1 MG-preconditioned inverse iteration */
   int mg_factor; /* coarsening factor of the MG hierarchy (rsb_algo = 1) */
+  int mg_grammian; /* 1 = Gram-matrix refinement of the MG start vectors */
   double tol;
   const float *warm;  /* device parameters of the learned start, or NULL */
   int warm_passes, warm_max_n;

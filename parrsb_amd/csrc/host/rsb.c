@@ -147,7 +147,8 @@ int rsb_run(rsbgpu *g, int nel, const rsb_params *prm, parrsb_amd_report *rep) {
     }
     memset(fb, 0, sizeof(int) * (size_t)nseg);
     if (prm->rsb_algo == 1)
-      TRY(rsbgpu_fiedler_mg(g, prm->max_iter, prm->max_passes, prm->tol, prm->mg_factor, &fo));
+      TRY(rsbgpu_fiedler_mg(g, prm->max_iter, prm->max_passes, prm->tol, prm->mg_factor,
+                             prm->mg_grammian, &fo));
     else
       TRY(rsbgpu_fiedler(g, prm->precision, prm->max_iter, prm->max_passes, prm->tol, mode,
                          &fo));

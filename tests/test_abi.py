@@ -111,6 +111,10 @@ def test_input_errors_before_device(monkeypatch):
     assert _call(vtx, xyz, 8, opts) == 3             # no such partitioner
     opts = pa.default_options(verbose_level=0, nparts=2, partitioner=2)
     assert _call(vtx, None, 8, opts) == 3            # RIB needs coordinates
+    opts = pa.default_options(verbose_level=0, nparts=2, rsb_algo=1, rsb_mg_grammian=1,
+                              rsb_max_passes=65)
+    assert _call(vtx, xyz, 8, opts) == 3             # Gram basis of at most 63 vectors
+    assert "at most 64 passes" in pa.last_error()
 
 
 def test_oracle_is_not_linked_by_the_product():
@@ -176,3 +180,6 @@ def test_noalloc_entry_point_rejects_sizes_beyond_its_maximum(monkeypatch):
     opts = pa.default_options(verbose_level=0, nparts=4, rsb_algo=1)
     assert fn(None, None, None, 32769, 8, C.byref(opts)) == 3
     assert "rsb_algo = 1 takes at most 32768 elements" in pa.last_error()
+    opts = pa.default_options(verbose_level=0, nparts=4, rsb_algo=1, rsb_mg_grammian=1)
+    assert fn(None, None, None, 100, 8, C.byref(opts)) == 3
+    assert "rsb_mg_grammian = 1" in pa.last_error()

@@ -261,8 +261,8 @@ def test_device_mg_partition_matches_oracle(oracle):
 @pytest.mark.gpu
 def test_device_mg_option_errors():
     vtx, xyz, nv = M.box_hex(4, 4, 4, seed=1)
-    for kw in (dict(rsb_algo=1, rsb_mg_factor=1), dict(rsb_algo=1, rsb_mg_grammian=1),
-               dict(rsb_algo=2)):
+    for kw in (dict(rsb_algo=1, rsb_mg_factor=1),
+               dict(rsb_algo=1, rsb_mg_grammian=1, rsb_max_passes=65), dict(rsb_algo=2)):
         with pytest.raises(pa.ParrsbError) as e:
             pa.part_mesh(vtx, xyz, nv, 2, **kw)
         assert e.value.rc == 3
@@ -303,3 +303,82 @@ def test_device_mg_lds_tiles_equal_global_form(monkeypatch):
     monkeypatch.delenv("PARRSB_AMD_NO_TILES")
     assert np.array_equal(a, b)
     assert ra["niter_max"] == rb["niter_max"] and ra["lambda_min"] == rb["lambda_min"]
+
+
+# ---------------------------------------------------------------------------
+# Gram-matrix refinement on the device (rsb_mg_grammian = 1)
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 20, 50, 63])
+def test_device_jacobi_equals_oracle_bit_for_bit(oracle, n):
+    """The Ritz step's eigen-solve: one wave splits the rows of each
+    rotation, every entry taking the serial sweep's operations in order
+    (no contraction), so eigenvalue and vector equal the oracle's exactly;
+    the oracle's is pinned against LAPACK."""
+    flush(f"device jacobi n={n}")
+    rng = np.random.default_rng(100 + n)
+    B = rng.standard_normal((n, n))
+    A = B @ B.T + np.diag(rng.uniform(0.0, 1.0, n))
+    lam, v = pa.test_sym_smallest(A)
+    lo, vo = oracle.sym_smallest(A)
+    assert lam == lo
+    assert np.array_equal(v, vo)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh", ["pipe", "box"])
+def test_device_mg_gram_fiedler_matches_oracle(oracle, mesh):
+    flush(f"mg gram fiedler {mesh}")
+    if mesh == "pipe":
+        vtx, xyz, nv = M.pipe_hex(3, 8, 12, seed=3, amp=0.1)
+    else:
+        vtx, xyz, nv = M.box_hex(14, 9, 6, jitter=0.1, seed=2)
+    for tol in (1e-5, 1e-8):
+        with _one_segment(vtx, xyz, nv) as d:
+            order = d.order()
+            out = d.fiedler_mg(1, tol=tol, grammian=1)
+            f = d.fiedler_values()
+        fo, it, npass = oracle.fiedler_mg(vtx, nv, order, tol=tol, grammian=1)
+        assert abs(abs(f @ fo) - 1) < 1e-9, tol
+        assert f @ fo > 0, tol
+        assert out["npass"][0] == npass, tol
+        assert abs(out["niter"][0] - it) <= npass, tol
+
+
+@pytest.mark.gpu
+def test_device_mg_gram_batched_segments_match_oracle(oracle):
+    """Segments grow their bases independently (some stop early, one is too
+    small to bisect): each equals its own oracle solve."""
+    flush("mg gram batched segments")
+    vtx, xyz, nv = M.box_hex(14, 11, 9, seed=6)
+    n = vtx.shape[0]
+    sizes = [300, 1, 700, n - 1001]
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    with pa.Device() as d:
+        d.set_mesh(vtx, xyz, nv)
+        d.set_segments(starts, sizes, [2, 2, 3, 5], [0, 2, 4, 7])
+        d.prepartition(1)
+        order = d.order()
+        d.level_laplacian()
+        out = d.fiedler_mg(4, tol=1e-7, grammian=1)
+        f = d.fiedler_values()
+    for s, (a, sz) in enumerate(zip(starts, sizes)):
+        if sz < 2:
+            assert f[a] == 0.0
+            continue
+        fo, it, npass = oracle.fiedler_mg(vtx, nv, order[a:a + sz], tol=1e-7, grammian=1)
+        assert abs(abs(f[a:a + sz] @ fo) - 1) < 1e-8, s
+        assert out["npass"][s] == npass, s
+
+
+@pytest.mark.gpu
+def test_device_mg_gram_partition_matches_oracle(oracle):
+    flush("mg gram partition 16^3 P=8")
+    vtx, xyz, nv = M.box_hex(16, 16, 16, jitter=0.1, seed=1)
+    P = 8
+    part, _ = pa.part_mesh(vtx, xyz, nv, P, rsb_algo=1, rsb_mg_grammian=1)
+    ref, rep = oracle.part_mesh(vtx, xyz, nv, P, rsb_algo=1, mg_factor=2, mg_grammian=1)
+    assert np.array_equal(np.bincount(part, minlength=P), np.bincount(ref, minlength=P))
+    q, qo = pa.quality(part, vtx, nv, P), oracle.quality(ref, vtx, nv, P)
+    assert abs(q["edge_cut"] - qo["edge_cut"]) <= 0.02 * qo["edge_cut"]
+    assert pa.last_report()["niter_max"][0] > 0
