@@ -322,6 +322,12 @@ def main():
     # the kernel's launch time from one more call with events (not timed)
     prep = profiled_call(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, args.precision, warm, label)
     spmv_gbs, it_gbs, launches = apply_roofline(prep)
+    # what plain streaming kernels reach on this device (the practical ceiling)
+    ceiling = pa.stream_test(1 << 30, 10)
+    log("streaming kernels (GB/s): " + ", ".join(f"{k} {v:.0f}" for k, v in ceiling.items()))
+    traffic = measured_traffic(desc, args.precision)
+    avg_ms = prep["spmv_ms"] / launches
+    traffic_gbs = traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms else None
     out = {
         "metric": METRIC, "value": value, "unit": "Melem/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -338,9 +344,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_lz_apply",
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(desc, args.precision),
+                     "traffic": traffic,
                      "bytes_per_launch": prep["spmv_bytes"] / launches,
-                     "avg_launch_ms": prep["spmv_ms"] / launches,
+                     "avg_launch_ms": avg_ms,
+                     # counted HBM bytes (traffic) over the same mean launch time: the
+                     # real fraction of HBM, beside the model-based one above
+                     "traffic_gbs": traffic_gbs,
+                     "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None,
+                     "stream_ceiling_gbs": ceiling,
+                     "traffic_frac_of_stream_read": (traffic_gbs / ceiling["read"]
+                                                     if traffic_gbs and ceiling["read"] else None),
                      "launches": prep["spmv_launches"],
                      "timing": "HIP events on the solver's stream around every 11th k_lz_apply "
                                "launch of one extra call after the timed steps (mean launch "

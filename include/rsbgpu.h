@@ -58,6 +58,11 @@ int rsbgpu_set_profiling(rsbgpu *g, int spmv_events);
 /* Parity hook: tridiagonal eigen-solve of one (d, e) on the device; y gets
  * the coefficients of the eigenvector of smallest |eigenvalue|. */
 int rsbgpu_test_tqli(const double *d, const double *e, int n, double *y, double *lam);
+/* HBM calibration: best GB/s of plain streaming kernels (16-byte vector
+ * accesses, grids of 1024..16384 workgroups, plain and non-temporal) over
+ * bytes-sized arrays on device `device`, reps timed launches each:
+ * gbs[0] read, gbs[1] write, gbs[2] copy (read + written bytes). */
+int rsbgpu_stream_test(int device, size_t bytes, int reps, double *gbs);
 /* The FP32 path's eigen-solve (multisection + inverse iteration) on the same
  * input: eigenpair of smallest |eigenvalue|, vector signed y[0] >= 0. */
 int rsbgpu_test_eigmin(const double *d, const double *e, int n, double *y, double *lam);

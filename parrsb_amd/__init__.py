@@ -132,6 +132,7 @@ SIGNATURES = {
     "rsbgpu_lanczos_traffic": (I, [P, P]),
     "rsbgpu_set_profiling": (I, [P, I]),
     "rsbgpu_test_tqli": (I, [P, P, I, P, P]),
+    "rsbgpu_stream_test": (I, [I, C.c_size_t, I, P]),
     "rsbgpu_test_eigmin": (I, [P, P, I, P, P]),
     "rsbgpu_set_mesh": (I, [P, P, P, I, I, I]),
     "rsbgpu_graph_stats": (I, [P, P, P]),
@@ -560,6 +561,14 @@ class Device:
         self._chk(lib().rsbgpu_get_fiedler(self.h, _p(f)))
         return f
 
+    def lanczos_traffic(self):
+        """[iteration bytes (model), iteration ms, iterations, apply bytes
+        (model), apply ms (profiling), apply launches (profiling)] over the
+        handle's life."""
+        v = np.zeros(6)
+        self._chk(lib().rsbgpu_lanczos_traffic(self.h, _p(v)))
+        return v
+
     def level_form(self):
         """(tiles on the global path, tiles with the vertex form)."""
         a, b = C.c_int(), C.c_int()
@@ -613,6 +622,16 @@ def test_tqli(d, e, fast=False):
     if rc:
         raise ParrsbError(rc, "rsbgpu_test_tqli failed")
     return lam[0], y
+
+
+def stream_test(nbytes=1 << 30, reps=10, device=0):
+    """Best read / write / copy rates (GB/s) of plain streaming kernels on
+    the device: the practical HBM ceiling."""
+    gbs = np.zeros(3)
+    rc = lib().rsbgpu_stream_test(device, nbytes, reps, _p(gbs))
+    if rc:
+        raise ParrsbError(rc, "rsbgpu_stream_test failed")
+    return {"read": gbs[0], "write": gbs[1], "copy": gbs[2]}
 
 
 def test_sym_smallest(A):

@@ -90,6 +90,8 @@ struct tile_t {
 // eight contiguous parts, one per XCD, so a tile's halo rows are mostly rows
 // its XCD has just read (grid: nlist rounded up to a multiple of 8).
 #define LZ_XCDS 8
+// running tiles up to which an FP32 pass folds its r-update into the apply
+#define LZ_FOLD_TILES 4096
 // profiling samples one SpMV launch in this many (coprime with the 8
 // iterations between list compactions and the 50 of a pass)
 #define LZ_PROF_EVERY 11
@@ -120,7 +122,7 @@ __device__ inline bool tile_of(const lz_args &a, int want_status, tile_t &t, int
 // cross-workgroup handshake inside a kernel, so no device-scope fences.
 template <int NV>
 __device__ inline void publish(const lz_args &a, const tile_t &t, double (&v)[NV]) {
-  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * (NV > 4 ? NV : 4)];
   block_sum<NV>(v, red);
   if (threadIdx.x == 0)
     for (int k = 0; k < NV; k++) a.partial[(size_t)k * a.pstride + t.ti] = v[k];
@@ -200,15 +202,26 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_setup(lz_args a, int phase, co
 // q = beta (p - m) + r (q = r on the first iteration of a pass).  The packed
 // entries of a SELL-64 slice are read 8 at a time so that each thread keeps 8
 // gathers in flight.
-template <typename T, bool FIRST>
-__device__ inline T qval(const T *__restrict__ p, const T *__restrict__ r, uint32_t c, T beta,
-                         T m) {
-  return FIRST ? r[c] : beta * (p[c] - m) + r[c];
+// The residual of iteration it: r itself, or with FOLD (the FP32 path)
+// r_prev - alpha w_prev, the update of the previous iteration, formed where
+// it is read (fiedler.c:486-488).
+// r - alpha w with one rounding (the r-update of both forms)
+__device__ inline float rsub(float r, float alpha, float w) { return __builtin_fmaf(-alpha, w, r); }
+__device__ inline double rsub(double r, double alpha, double w) { return __builtin_fma(-alpha, w, r); }
+template <typename T, bool FOLD>
+__device__ inline T rval(const T *__restrict__ r, const T *__restrict__ wp, uint32_t c, T alpha) {
+  return FOLD ? rsub(r[c], alpha, wp[c]) : r[c];
+}
+template <typename T, bool FIRST, bool FOLD>
+__device__ inline T qval(const T *__restrict__ p, const T *__restrict__ r,
+                         const T *__restrict__ wp, uint32_t c, T beta, T m, T alpha) {
+  return FIRST ? r[c] : beta * (p[c] - m) + rval<T, FOLD>(r, wp, c, alpha);
 }
 
-template <typename T, bool FIRST>
+template <typename T, bool FIRST, bool FOLD>
 __device__ inline T lap_row_q(const uint32_t *__restrict__ e, int wd, T qi,
-                              const T *__restrict__ p, const T *__restrict__ r, T beta, T m) {
+                              const T *__restrict__ p, const T *__restrict__ r,
+                              const T *__restrict__ wp, T beta, T m, T alpha) {
   T sum = (T)0;
   int j = 0;
   for (; j + 8 <= wd; j += 8) {
@@ -217,45 +230,60 @@ __device__ inline T lap_row_q(const uint32_t *__restrict__ e, int wd, T qi,
 #pragma unroll
     for (int u = 0; u < 8; u++) q[u] = __builtin_nontemporal_load(e + (size_t)(j + u) * RSB_WAVE);
 #pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = qval<T, FIRST>(p, r, q[u] & 0x0fffffffu, beta, m);
+    for (int u = 0; u < 8; u++)
+      v[u] = qval<T, FIRST, FOLD>(p, r, wp, q[u] & 0x0fffffffu, beta, m, alpha);
 #pragma unroll
     for (int u = 0; u < 8; u++) sum += (T)(q[u] >> 28) * (qi - v[u]);
   }
   for (; j < wd; j++) {
     uint32_t q = __builtin_nontemporal_load(e + (size_t)j * RSB_WAVE);
-    sum += (T)(q >> 28) * (qi - qval<T, FIRST>(p, r, q & 0x0fffffffu, beta, m));
+    sum += (T)(q >> 28) * (qi - qval<T, FIRST, FOLD>(p, r, wp, q & 0x0fffffffu, beta, m, alpha));
   }
   return sum;
 }
+
+// What the folded apply adds: the residual of iteration it written for the
+// tile's rows, and the sums r.r, r.w and w.w from which the finishing step
+// forms |r - alpha w|^2, the next residual's norm.
+template <typename T>
+struct fold_t {
+  const T *wp;  // w of the previous iteration
+  T *rn;        // r of this iteration (written)
+  T alpha;      // alpha of the previous iteration
+  acc_t<T> rr, rw, ww;
+};
 
 // Fused search-direction update and Laplacian apply (fiedler.c:460-475):
 //   q = beta (p_old - mean_p) + r,  w = L q,  basis[it] = r / |r|
 // and the segment sums of q, w.q and w, from which the last block forms
 // mean_p and pap = w.(q - mean_p), then alpha = rtz1 / pap.
-template <typename T, bool FIRST>
+template <typename T, bool FIRST, bool FOLD>
 __device__ inline void lz_apply_rows(const lz_args &a, const tile_t &t, const T *__restrict__ r,
                                      const T *__restrict__ pold, T *__restrict__ pnew,
                                      T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
                                      const int64_t *__restrict__ slice_off,
                                      const int *__restrict__ slice_w,
                                      const uint32_t *__restrict__ ell, acc_t<T> &sq,
-                                     acc_t<T> &swq, acc_t<T> &sw) {
+                                     acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
+  constexpr bool UPD = FOLD && !FIRST;
   for (int row = t.row0 + threadIdx.x; row < t.row1; row += blockDim.x) {
     const bool real = row - t.pstart < t.n;
-    const T ri = r[row];
+    const T ri = rval<T, UPD>(r, fd.wp, (uint32_t)row, fd.alpha);
     T qi = (T)0, wi = (T)0;
     if (real) {
       qi = FIRST ? ri : beta * (pold[row] - m) + ri;
       const int sl = row >> 6;
-      wi = lap_row_q<T, FIRST>(ell + slice_off[sl] + (row & 63), slice_w[sl], qi, pold, r, beta,
-                               m);
+      wi = lap_row_q<T, FIRST, UPD>(ell + slice_off[sl] + (row & 63), slice_w[sl], qi, pold, r,
+                                    fd.wp, beta, m, fd.alpha);
       sq.add(qi);
       swq.dot(wi, qi);
       sw.add(wi);
+      if (FOLD) fd.rr.dot(ri, ri), fd.rw.dot(ri, wi), fd.ww.dot(wi, wi);
     }
     pnew[row] = qi;
     w[row] = wi;
     b[row] = ri * rni;
+    if (UPD) fd.rn[row] = ri;
   }
 }
 
@@ -285,11 +313,12 @@ __device__ inline void acc_entry(uint32_t e, T qi, const T *ql, T &sum) {
 
 // One row of the tiled apply: its 16-bit entries (groups of 8, then a group
 // of 4) against the LDS window; writes p', w and the basis entry.
-template <typename T>
+template <typename T, bool FOLD>
 __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_args &ta,
                                        const T *ql, const T *rl, T *__restrict__ pnew,
                                        T *__restrict__ w, T *__restrict__ b, T rni, int nreal,
-                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw,
+                                       fold_t<T> &fd) {
   const int li = row - t.row0, sl = row >> 6;
   const uint16_t *sbase = ta.e16 + ta.off16[sl];
   const u32x4 *ep = (const u32x4 *)(sbase + (row & 63) * 8);
@@ -317,11 +346,14 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
   }
   pnew[row] = qi;
   w[row] = sum;
-  __builtin_nontemporal_store(rl[li] * rni, &b[row]);
+  const T ri = rl[li];
+  __builtin_nontemporal_store(ri * rni, &b[row]);
+  if (FOLD && fd.rn) fd.rn[row] = ri;
   if (li < nreal) {
     sq.add(qi);
     swq.dot(sum, qi);
     sw.add(sum);
+    if (FOLD) fd.rr.dot(ri, ri), fd.rw.dot(ri, sum), fd.ww.dot(sum, sum);
   }
 }
 
@@ -329,11 +361,11 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
 // any is summed, so each lane keeps up to 4R 16-byte loads in flight.  Rows
 // whose slice is wider than 32 entries take apply_row_tiled.  Missing groups
 // are zero words (weight 0), which add exact zeros.
-template <typename T, int R>
+template <typename T, int R, bool FOLD>
 __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta, const T *ql,
                                           const T *rl, T *__restrict__ pnew, T *__restrict__ w,
                                           T *__restrict__ b, T rni, int nreal, acc_t<T> &sq,
-                                          acc_t<T> &swq, acc_t<T> &sw) {
+                                          acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
   for (int row0 = t.row0 + threadIdx.x; row0 < t.row1; row0 += R * blockDim.x) {
     int rows[R], wd[R];
     const uint16_t *base[R];
@@ -351,7 +383,8 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
 #pragma unroll
       for (int k = 0; k < R; k++)
         if (rows[k] < t.row1)
-          apply_row_tiled(rows[k], t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw);
+          apply_row_tiled<T, FOLD>(rows[k], t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw,
+                                   fd);
       continue;
     }
     u32x4 v[R][4];
@@ -391,11 +424,14 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
       acc_entry(tl[k].y >> 16, qi, ql, sum);
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
-      __builtin_nontemporal_store(rl[li] * rni, &b[rows[k]]);
+      const T ri = rl[li];
+      __builtin_nontemporal_store(ri * rni, &b[rows[k]]);
+      if (FOLD && fd.rn) fd.rn[rows[k]] = ri;
       if (li < nreal) {
         sq.add(qi);
         swq.dot(sum, qi);
         sw.add(sum);
+        if (FOLD) fd.rr.dot(ri, ri), fd.rw.dot(ri, sum), fd.ww.dot(sum, sum);
       }
     }
   }
@@ -406,17 +442,19 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
 // of the tile's rows; rows past the segment's end get q = 0.  SU indices per
 // thread and step, all their loads issued before any value is used (the halo
 // rows are gathers).  Ends with a barrier.
-template <typename T, bool FIRST, bool KEEP_R = true>
+template <typename T, bool FIRST, bool KEEP_R = true, bool FOLD = false>
 __device__ inline void stage_window(const tile_t &t, const int *__restrict__ halo, int nh, T *ql,
                                     T *rl, const T *__restrict__ r, const T *__restrict__ pold,
-                                    T beta, T m) {
+                                    T beta, T m, const T *__restrict__ wp = nullptr,
+                                    T alpha = (T)0) {
+  constexpr bool UPD = FOLD && !FIRST;
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   const int tot = nrows + nh;
   constexpr int SU = 4;
   for (int i0 = threadIdx.x; i0 < tot; i0 += SU * blockDim.x) {
     int gr[SU];
-    T rg[SU], po[SU];
+    T rg[SU], po[SU], wg[SU];
 #pragma unroll
     for (int k = 0; k < SU; k++) {
       const int idx = i0 + k * blockDim.x;
@@ -426,7 +464,11 @@ __device__ inline void stage_window(const tile_t &t, const int *__restrict__ hal
     for (int k = 0; k < SU; k++) {
       rg[k] = r[gr[k]];
       po[k] = FIRST ? (T)0 : pold[gr[k]];
+      wg[k] = UPD ? wp[gr[k]] : (T)0;
     }
+    if (UPD)
+#pragma unroll
+      for (int k = 0; k < SU; k++) rg[k] = rsub(rg[k], alpha, wg[k]);
 #pragma unroll
     for (int k = 0; k < SU; k++) {
       const int idx = i0 + k * blockDim.x;
@@ -442,50 +484,70 @@ __device__ inline void stage_window(const tile_t &t, const int *__restrict__ hal
   __syncthreads();
 }
 
-template <typename T, bool FIRST>
+template <typename T, bool FIRST, bool FOLD = false>
 __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const tiled_args &ta,
                                       int nh, T *ql, T *rl, const T *__restrict__ r,
                                       const T *__restrict__ pold, T *__restrict__ pnew,
                                       T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
-                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
-  stage_window<T, FIRST>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta, m);
+  stage_window<T, FIRST, true, FOLD>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta,
+                                     m, fd.wp, fd.alpha);
   // two rows per thread and step (measured 15% faster than one at a time
   // on the 256^3 workload: more loads in flight per lane)
-  apply_rows_batched<T, 2>(t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw);
+  apply_rows_batched<T, 2, FOLD>(t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw, fd);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ r,
+// FOLD (FP32): the residual update of the previous iteration is formed in
+// the staging, r = r_prev - alpha w_prev (r_prev, w_prev: the buffers of
+// the other parity; r of this iteration is written to rn), and the launch
+// adds the sums r.r, r.w, w.w (partial slots 3..5) for FIN_APPLYR.  No
+// separate r-update pass and one finishing step per iteration.
+template <typename T, bool FOLD>
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ rprev,
+                                                        const T *__restrict__ r0,
                                                         const T *__restrict__ pold,
                                                         T *__restrict__ pnew, T *__restrict__ w,
                                                         T *__restrict__ basis, int64_t mpad,
-                                                        tiled_args ta, int nlist) {
+                                                        tiled_args ta, int nlist,
+                                                        const T *__restrict__ wprev,
+                                                        T *__restrict__ rn) {
   __shared__ T ql[TB_LOCAL];
   __shared__ T rl[RSB_TILE];
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
+  // the first iteration of a pass reads the residual the set-up wrote (r0)
+  const T *__restrict__ r = it == 0 ? r0 : rprev;
   const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
   acc_t<T> sq, swq, sw;
+  fold_t<T> fd;
+  fd.wp = wprev, fd.rn = it == 0 ? nullptr : rn, fd.alpha = (T)SC(s, SC_ALPHA);
   T *b = basis + (size_t)it * mpad;
   const int nh = ta.nhalo[t.ti];
   if (nh >= 0) {
     if (it == 0)
-      lz_apply_tiled<T, true>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw);
+      lz_apply_tiled<T, true, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq,
+                                    swq, sw, fd);
     else
-      lz_apply_tiled<T, false>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                               sw);
+      lz_apply_tiled<T, false, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq,
+                                     swq, sw, fd);
   } else if (it == 0) {
-    lz_apply_rows<T, true>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off, ta.slice_w,
-                           ta.ell, sq, swq, sw);
+    lz_apply_rows<T, true, FOLD>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off,
+                                 ta.slice_w, ta.ell, sq, swq, sw, fd);
   } else {
-    lz_apply_rows<T, false>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off, ta.slice_w,
-                            ta.ell, sq, swq, sw);
+    lz_apply_rows<T, false, FOLD>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off,
+                                  ta.slice_w, ta.ell, sq, swq, sw, fd);
   }
-  double v[3] = {sq.value(), swq.value(), sw.value()};
-  publish<3>(a, t, v);
+  if (FOLD) {
+    double v[6] = {sq.value(), swq.value(), sw.value(),
+                   fd.rr.value(), fd.rw.value(), fd.ww.value()};
+    publish<6>(a, t, v);
+  } else {
+    double v[3] = {sq.value(), swq.value(), sw.value()};
+    publish<3>(a, t, v);
+  }
 }
 
 // Vertex form of the tiles (tiles.hip, rsb_build_vtiles): the reference's
@@ -637,11 +699,14 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__
                                 sw);
   } else {
     const int nh = ta.nhalo[t.ti];
+    fold_t<T> fd;
+    fd.wp = nullptr, fd.rn = nullptr, fd.alpha = (T)0;
     if (it == 0)
-      lz_apply_tiled<T, true>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw);
+      lz_apply_tiled<T, true>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw,
+                              fd);
     else
       lz_apply_tiled<T, false>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                               sw);
+                               sw, fd);
   }
   double v[3] = {sq.value(), swq.value(), sw.value()};
   publish<3>(a, t, v);
@@ -668,7 +733,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
   for (int k = 0; k < R; k++) {
     const int row = t.row0 + threadIdx.x + k * blockDim.x;
     if (row >= t.row1) continue;
-    const T x = rv[k] - alpha * wv[k];
+    const T x = rsub(rv[k], alpha, wv[k]);
     r[row] = x;
     acc.dot(x, x);
   }
@@ -1225,15 +1290,66 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_resid(lz_args a, int phase,
 // segment; segments not in the phase's state leave at once.
 enum {
   FIN_START0 = 0, FIN_START1, FIN_SETUP0, FIN_SETUP1, FIN_APPLY, FIN_RUPDATE, FIN_RITZ,
-  FIN_FINISH, FIN_RES0, FIN_RES1
+  FIN_FINISH, FIN_RES0, FIN_RES1, FIN_APPLYR
 };
 
 __device__ inline int fin_status(int op) {
   switch (op) {
     case FIN_START0: case FIN_START1: case FIN_SETUP0: case FIN_SETUP1: return LZ_SETUP;
-    case FIN_APPLY: case FIN_RUPDATE: return LZ_RUNNING;
+    case FIN_APPLY: case FIN_RUPDATE: case FIN_APPLYR: return LZ_RUNNING;
     case FIN_RITZ: return LZ_PASSDONE;
     default: return LZ_FINISHED;
+  }
+}
+
+// Totals of partial slots k0 .. k0 + nv - 1 (nv <= 3) of a segment: 8 tiles
+// per thread in flight (large segments have up to 16k tiles), a fixed order
+// of summation (each slot's partials are contiguous).  Thread 0 holds them.
+__device__ inline void fin_sums(const lz_args &a, int first, int count, int k0, int nv,
+                                double (&tot)[3], double *red) {
+  tot[0] = tot[1] = tot[2] = 0.0;
+  for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
+    double v[3][8];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u * blockDim.x;
+        v[k][u] =
+            (k < nv && i < count) ? a.partial[(size_t)(k0 + k) * a.pstride + first + i] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < 8; u++) tot[0] += v[0][u], tot[1] += v[1][u], tot[2] += v[2][u];
+  }
+  block_sum<3>(tot, red);
+}
+
+// The stop test and the next coefficients after the residual update of
+// iteration it, from rtr = r.r of the new residual (fiedler.c:486-509).
+__device__ inline void rupdate_step(const lz_args &a, int s, double rtr) {
+  const int it = a.iter[s], mi = a.miter[s];
+  double rnorm = sqrt(rtr);
+  double rtz1 = SC(s, SC_RTZ1), rtz2 = SC(s, SC_RTZ2), beta = SC(s, SC_BETA);
+  double pap = SC(s, SC_PAP), pap_old = SC(s, SC_PAPOLD);
+  double *Td = a.T + (size_t)s * 2 * a.miter_max, *Tu = Td + a.miter_max;
+  if (it == 0) {
+    Td[0] = pap / rtz1;
+  } else {
+    Td[it] = (beta * beta * pap_old + pap) / rtz1;
+    Tu[it - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
+  }
+  SC(s, SC_RTR) = rtr;
+  SC(s, SC_RNORM) = rnorm;
+  SC(s, SC_RNI) = 1.0 / rnorm;
+  if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
+    a.k[s] = it + 1;
+    a.status[s] = LZ_PASSDONE;
+    atomicSub(&a.running[0], 1);
+  } else {
+    a.iter[s] = it + 1;
+    SC(s, SC_RTZ2) = rtz1;
+    SC(s, SC_RTZ1) = rtr;
+    SC(s, SC_BETA) = rtr / rtz1;
   }
 }
 
@@ -1242,24 +1358,17 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
   const int s = a.act_segs ? a.act_segs[blockIdx.x] : blockIdx.x;
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
-  // 8 tiles per thread in flight (large segments have up to 16k tiles), a
-  // fixed order of summation; only the slots the phase wrote are read (each
-  // slot's partials are contiguous)
-  const int nv = op == FIN_APPLY ? 3 : 1;
-  double tot[3] = {0.0, 0.0, 0.0};
-  for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
-    double v[3][8];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int i = i0 + u * blockDim.x;
-        v[k][u] = (k < nv && i < count) ? a.partial[(size_t)k * a.pstride + first + i] : 0.0;
-      }
-#pragma unroll
-    for (int u = 0; u < 8; u++) tot[0] += v[0][u], tot[1] += v[1][u], tot[2] += v[2][u];
+  // only the slots the phase wrote are read
+  double tot[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  {
+    double t3[3];
+    fin_sums(a, first, count, 0, (op == FIN_APPLY || op == FIN_APPLYR) ? 3 : 1, t3, red);
+    tot[0] = t3[0], tot[1] = t3[1], tot[2] = t3[2];
+    if (op == FIN_APPLYR) {
+      fin_sums(a, first, count, 3, 3, t3, red);
+      tot[3] = t3[0], tot[4] = t3[1], tot[5] = t3[2];
+    }
   }
-  block_sum<3>(tot, red);
   if (threadIdx.x != 0) return;
   const double n = a.seg[a.nseg + s];
   switch (op) {
@@ -1307,31 +1416,24 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
       SC(s, SC_ALPHA) = SC(s, SC_RTZ1) / pap;
       break;
     }
-    case FIN_RUPDATE: {  // fiedler.c:486-509
-      const int it = a.iter[s], mi = a.miter[s];
-      double rtr = tot[0], rnorm = sqrt(rtr);
-      double rtz1 = SC(s, SC_RTZ1), rtz2 = SC(s, SC_RTZ2), beta = SC(s, SC_BETA);
-      double pap = SC(s, SC_PAP), pap_old = SC(s, SC_PAPOLD);
-      double *Td = a.T + (size_t)s * 2 * a.miter_max, *Tu = Td + a.miter_max;
-      if (it == 0) {
-        Td[0] = pap / rtz1;
-      } else {
-        Td[it] = (beta * beta * pap_old + pap) / rtz1;
-        Tu[it - 1] = -beta * pap_old / sqrt(rtz2 * rtz1);
-      }
-      SC(s, SC_RTR) = rtr;
-      SC(s, SC_RNORM) = rnorm;
-      SC(s, SC_RNI) = 1.0 / rnorm;
-      if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
-        a.k[s] = it + 1;
-        a.status[s] = LZ_PASSDONE;
-        atomicSub(&a.running[0], 1);
-      } else {
-        a.iter[s] = it + 1;
-        SC(s, SC_RTZ2) = rtz1;
-        SC(s, SC_RTZ1) = rtr;
-        SC(s, SC_BETA) = rtr / rtz1;
-      }
+    case FIN_RUPDATE: rupdate_step(a, s, tot[0]); break;  // tot: r.r
+    case FIN_APPLYR: {  // FIN_APPLY, then the update's r.r from the sums r.r, r.w, w.w
+      double mq = tot[0] / n;
+      double pap = tot[1] - mq * tot[2];
+      SC(s, SC_MEANP) = mq;
+      SC(s, SC_PAPOLD) = SC(s, SC_PAP);
+      SC(s, SC_PAP) = pap;
+      const double alpha = SC(s, SC_RTZ1) / pap;
+      SC(s, SC_ALPHA) = alpha;
+      // |r - af w|^2 with af = alpha as the next launch rounds it: the FP64
+      // sums of exact FP32 products, from this iteration's stored r and w,
+      // so the value does not drift over the iterations; it differs from
+      // the sum of squares of the rounded new r by at most 2^-23 relative
+      const double af = (double)(float)alpha;
+      double rtr = tot[3] - 2.0 * af * tot[4] + af * af * tot[5];
+      if (!(rtr > 0.0)) rtr = 0.0;
+      SC(s, SC_NFOLD) += 1.0;
+      rupdate_step(a, s, rtr);
       break;
     }
     case FIN_RITZ: {  // mean of the Ritz vector; restart or finish (fiedler.c:529-553)
@@ -1415,7 +1517,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // buffers
   size_t vb = (size_t)mpad * sizeof(T);
   if (vb > g->vec_bytes) {
-    for (void **q : {&g->d_r, &g->d_p, &g->d_p2, &g->d_w, &g->d_f}) {
+    for (void **q : {&g->d_r, &g->d_p, &g->d_p2, &g->d_w, &g->d_f, &g->d_r2, &g->d_w2}) {
       if (*q) RSB_TRY(rsb_dev_free(*q));
       *q = nullptr;
       RSB_TRY(rsb_dev_malloc(q, vb + 256));
@@ -1448,7 +1550,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   }
   if (ntiles > g->lz_tile_cap) {
     if (L.partial) RSB_TRY(rsb_dev_free(L.partial));
-    RSB_TRY(rsb_dev_malloc((void **)&L.partial, sizeof(double) * 4 * std::max(ntiles, 1)));
+    RSB_TRY(rsb_dev_malloc((void **)&L.partial, sizeof(double) * 6 * std::max(ntiles, 1)));
     g->lz_tile_cap = ntiles;
   }
   if (!L.running) RSB_TRY(rsb_dev_malloc((void **)&L.running, sizeof(int) * 2));
@@ -1494,6 +1596,42 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
                 g->d_tnhalo, g->d_thalo_off, g->d_halo,    g->nv};
   // the vertex form serves the FP32 path (vectors of T = float)
   const bool vform = g->vt_ok && sizeof(T) == 4;
+  // FP32 with the neighbour entries: a pass that starts with at most
+  // fold_tiles running tiles folds the r-update into the next apply
+  // (k_lz_apply<float, true>, FIN_APPLYR: two launches per iteration instead
+  // of four; r and w alternate between two buffers by iteration parity).
+  // On more tiles the extra halo gathers of w outweigh the saved pass.
+  // PARRSB_AMD_LZ_FOLD_TILES sets the limit (0: never).
+  const char *fev = getenv("PARRSB_AMD_LZ_FOLD_TILES");
+  const int fold_tiles = (sizeof(T) == 4 && !vform) ? (fev ? atoi(fev) : LZ_FOLD_TILES) : 0;
+  bool fold = false;
+  T *const R[2] = {r, (T *)g->d_r2}, *const W[2] = {w, (T *)g->d_w2};
+  // one iteration (it: the pass iteration, where every running segment is):
+  // the apply launch, then the finishing step(s) and the r-update
+  auto apply_launch = [&](int it) {
+    T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
+    const unsigned grid = xgrid(nact[0]);
+    if (fold)
+      k_lz_apply<T, true><<<grid, RSB_BLOCK, 0, st>>>(ai, R[(it + 1) & 1], R[0], pold, pnew,
+                                                      W[it & 1], basis, mpad, ta, nact[0],
+                                                      W[(it + 1) & 1], R[it & 1]);
+    else if (vform)
+      k_lz_apply_v<float><<<grid, RSB_BLOCK, 0, st>>>(ai, (const float *)r, (const float *)pold,
+                                                      (float *)pnew, (float *)w, (float *)basis,
+                                                      mpad, va, ta, nact[0]);
+    else
+      k_lz_apply<T, false><<<grid, RSB_BLOCK, 0, st>>>(ai, r, r, pold, pnew, w, basis, mpad, ta,
+                                                       nact[0], nullptr, nullptr);
+  };
+  auto after_apply = [&]() {
+    if (fold) {
+      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLYR);
+    } else {
+      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
+      k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
+      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+    }
+  };
 
   RSB_TRY(hipMemsetAsync(L.running, 0, sizeof(int) * 2, st));
   k_lz_init_state<<<nblocks(nseg, 256), 256, 0, st>>>(a, miter_opt);
@@ -1526,6 +1664,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       RSB_LAUNCH_CHECK();
       FIN(FIN_SETUP1);
       if (compact(LZ_RUNNING)) return 2;
+      fold = nact[0] <= fold_tiles;
       RSB_TRY(hipEventRecord(g->lz_ev0, st));
       int it, nrec = 0;
       const char *gv = getenv("PARRSB_AMD_LZ_GRAPH");
@@ -1538,7 +1677,6 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
       }
       for (it = 0; it < miter_max; it++) {
-        T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
         if (nact[0] == 0) break;
         if (use_graph && (it & 7) == 0 && it + 8 < miter_max) {
           // eight iterations as one graph replay (same kernels and grids as
@@ -1547,18 +1685,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
           if (!gexec || gn[0] != nact[0] || gn[1] != nact[1]) {
             hipGraph_t gr = nullptr;
             RSB_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            // (it is a multiple of 8: the chunk's buffer parities are q's)
             for (int q = 0; q < 8; q++) {
-              T *po = (q & 1) ? p2 : p, *pn = (q & 1) ? p : p2;
-              if (vform)
-                k_lz_apply_v<float><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(
-                    ai, (const float *)r, (const float *)po, (float *)pn, (float *)w,
-                    (float *)basis, mpad, va, ta, nact[0]);
-              else
-                k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, po, pn, w, basis,
-                                                                    mpad, ta, nact[0]);
-              k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
-              k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
-              k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+              apply_launch(q);
+              after_apply();
             }
             RSB_TRY(hipStreamEndCapture(st, &gr));
             if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
@@ -1581,20 +1711,12 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         // timing every launch slowed the whole call by ~5%)
         const bool samp = g->prof_spmv && (g->spmv_total++ % LZ_PROF_EVERY) == 0;
         if (samp) RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec], st));
-        if (vform)
-          k_lz_apply_v<float><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(
-              ai, (const float *)r, (const float *)pold, (float *)pnew, (float *)w,
-              (float *)basis, mpad, va, ta, nact[0]);
-        else
-          k_lz_apply<T><<<xgrid(nact[0]), RSB_BLOCK, 0, st>>>(ai, r, pold, pnew, w, basis, mpad,
-                                                              ta, nact[0]);
+        apply_launch(it);
         if (samp) {
           RSB_TRY(hipEventRecord(g->spmv_ev[2 * nrec + 1], st));
           nrec++;
         }
-        k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
-        k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
-        k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+        after_apply();
         RSB_LAUNCH_CHECK();
         g->lz_launches++;
         if ((it & 7) == 7 && it + 1 < miter_max) {
@@ -1691,14 +1813,17 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 9 s n vector
   // streams for the whole iteration; for k_lz_apply alone 4(n+1) + 4 nnz +
   // 5 s n (p_old and r read once, p_new, w and the basis vector written).
-  // Counted iterations, real (unpadded) sizes.
+  // With the folded update the apply is the iteration: 7 s n (r, w and
+  // p_old of the previous iteration read; r, p_new, w and the basis vector
+  // written).  Counted iterations, real (unpadded) sizes.
   for (int s = 0; s < nseg; s++) {
     const double *scs = &sc[(size_t)s * LZ_NSC];
     ni[s] += (int)scs[SC_WARMIT];
     if (g->h_p[s] > 1 && g->h_n[s] >= 2) {
       double n = g->h_n[s], nz = (double)g->h_seg_nnz[s], sz = sizeof(T);
-      g->lz_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 9 * sz * n);
-      g->spmv_bytes += ni[s] * (4 * (n + 1) + 4 * nz + 5 * sz * n);
+      const double nf = scs[SC_NFOLD], nu = ni[s] - nf;
+      g->lz_bytes += nu * (4 * (n + 1) + 4 * nz + 9 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 7 * sz * n);
+      g->spmv_bytes += nu * (4 * (n + 1) + 4 * nz + 5 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 7 * sz * n);
     }
     if (out) {
       if (out->niter) out->niter[s] = ni[s];

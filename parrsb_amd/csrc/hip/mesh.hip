@@ -111,6 +111,8 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_p);
   FREE(g->d_p2);
   FREE(g->d_w);
+  FREE(g->d_r2);
+  FREE(g->d_w2);
   FREE(g->d_f);
   FREE(g->d_basis);
   FREE(g->d_fied);

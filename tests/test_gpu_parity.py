@@ -249,6 +249,9 @@ def test_lanczos_fp32_vertex_form(oracle, name, monkeypatch):
     vtx, xyz, nv = {"hex": lambda: M.box_hex(40, 30, 20, jitter=0.1, seed=4),
                     "warped": lambda: M.warped_box_hex(40, seed=3),
                     "quad": lambda: M.quad_grid(140, 104)}[name]()
+    # both forms with the r-update as its own pass (the vertex form has no
+    # folded variant)
+    monkeypatch.setenv("PARRSB_AMD_LZ_FOLD_TILES", "0")
     res = {}
     for form in ("1", "0"):
         monkeypatch.setenv("PARRSB_AMD_VTILES", form)
@@ -265,6 +268,38 @@ def test_lanczos_fp32_vertex_form(oracle, name, monkeypatch):
     assert abs(ov["lambda"][0] - oe["lambda"][0]) <= 1e-5 * abs(lam)
     assert abs(abs(fv @ fo) - 1) < 1e-3 and abs(abs(fv @ fe) - 1) < 1e-3
     assert abs(ov["niter"][0] - oe["niter"][0]) <= 2
+
+
+@pytest.mark.parametrize("name", ["hex", "warped", "quad"])
+def test_lanczos_fp32_folded_update(oracle, name, monkeypatch):
+    """FP32 passes on few running tiles fold the r-update into the next
+    apply and take the new residual's norm from r.r - 2 alpha r.w +
+    alpha^2 w.w over the stored vectors.  The iterates are the same
+    vectors rounded the same way (one fused multiply-add), so the solve
+    agrees with the separate r-update pass to FP32 rounding: lambda to 1e-5,
+    vectors to 1e-4 in angle, and with the FP64 oracle as the FP32 path does.
+    Rounding moves the iteration at which a slowly falling residual crosses
+    tol: the counts agree within 1% (at least 3)."""
+    flush(f"lanczos fp32 folded update {name}")
+    vtx, xyz, nv = {"hex": lambda: M.box_hex(40, 30, 20, jitter=0.1, seed=4),
+                    "warped": lambda: M.warped_box_hex(40, seed=3),
+                    "quad": lambda: M.quad_grid(140, 104)}[name]()
+    res = {}
+    for tiles in ("100000", "0"):
+        monkeypatch.setenv("PARRSB_AMD_LZ_FOLD_TILES", tiles)
+        with _segments_after_rcb(vtx, xyz, nv, 2) as d:
+            order = d.order()
+            d.level_laplacian()
+            res[tiles] = (d.fiedler(1, precision=32), d.fiedler_values(), d.lanczos_traffic())
+    (of, ff, tf), (ou, fu, tu) = res["100000"], res["0"]
+    fo, it, lam, rn = oracle.fiedler(vtx, nv, order)
+    assert abs(of["lambda"][0] - lam) <= 1e-4 * abs(lam)
+    assert abs(of["lambda"][0] - ou["lambda"][0]) <= 1e-5 * abs(lam)
+    assert abs(abs(ff @ fu) - 1) < 1e-4 and abs(abs(ff @ fo) - 1) < 1e-3
+    assert abs(of["niter"][0] - ou["niter"][0]) <= max(3, 0.01 * ou["niter"][0])
+    # the byte model follows the launches: 7 vector streams per folded
+    # iteration against 9 (apply 5 + update 4)
+    assert tf[0] < tu[0]
 
 
 def test_lanczos_batched_segments_match_oracle(oracle):
