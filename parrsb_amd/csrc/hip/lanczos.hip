@@ -311,8 +311,8 @@ __device__ inline void acc_entry(uint32_t e, T qi, const T *ql, T &sum) {
   sum += (T)(e >> 12) * (qi - ql[e & 0xfffu]);
 }
 
-// One row of the tiled apply: its 16-bit entries (groups of 8, then a group
-// of 4) against the LDS window; writes p', w and the basis entry.
+// One row of the tiled apply: its 16-bit entries (groups of 8, then groups
+// of 4 and 2) against the LDS window; writes p', w and the basis entry.
 template <typename T, bool FOLD>
 __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_args &ta,
                                        const T *ql, const T *rl, T *__restrict__ pnew,
@@ -336,13 +336,19 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
     acc_entry(v.w & 0xffffu, qi, ql, sum);
     acc_entry(v.w >> 16, qi, ql, sum);
   }
-  if (w16 & 4) {  // last group of 4 entries
+  if (w16 & 4) {  // a group of 4 entries
     const u32x2 v =
         __builtin_nontemporal_load((const u32x2 *)(sbase + nb * 512 + (row & 63) * 4));
     acc_entry(v.x & 0xffffu, qi, ql, sum);
     acc_entry(v.x >> 16, qi, ql, sum);
     acc_entry(v.y & 0xffffu, qi, ql, sum);
     acc_entry(v.y >> 16, qi, ql, sum);
+  }
+  if (w16 & 2) {  // a group of 2 entries
+    const unsigned v = __builtin_nontemporal_load(
+        (const unsigned *)(sbase + nb * 512 + (w16 & 4) * 64 + (row & 63) * 2));
+    acc_entry(v & 0xffffu, qi, ql, sum);
+    acc_entry(v >> 16, qi, ql, sum);
   }
   pnew[row] = qi;
   w[row] = sum;
@@ -389,6 +395,7 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
     }
     u32x4 v[R][4];
     u32x2 tl[R];
+    unsigned t2[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
 #pragma unroll
@@ -397,9 +404,12 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
                       ? __builtin_nontemporal_load((const u32x4 *)(base[k] + (rows[k] & 63) * 8) +
                                                    g * RSB_WAVE)
                       : (u32x4){0u, 0u, 0u, 0u};
-      tl[k] = (wd[k] & 4) ? __builtin_nontemporal_load((const u32x2 *)(base[k] + (wd[k] >> 3) * 512 +
-                                                                       (rows[k] & 63) * 4))
+      const uint16_t *tail = base[k] + (wd[k] >> 3) * 512;
+      tl[k] = (wd[k] & 4) ? __builtin_nontemporal_load((const u32x2 *)(tail + (rows[k] & 63) * 4))
                           : (u32x2){0u, 0u};
+      t2[k] = (wd[k] & 2) ? __builtin_nontemporal_load(
+                                (const unsigned *)(tail + (wd[k] & 4) * 64 + (rows[k] & 63) * 2))
+                          : 0u;
     }
 #pragma unroll
     for (int k = 0; k < R; k++) {
@@ -422,6 +432,8 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
       acc_entry(tl[k].x >> 16, qi, ql, sum);
       acc_entry(tl[k].y & 0xffffu, qi, ql, sum);
       acc_entry(tl[k].y >> 16, qi, ql, sum);
+      acc_entry(t2[k] & 0xffffu, qi, ql, sum);
+      acc_entry(t2[k] >> 16, qi, ql, sum);
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
       const T ri = rl[li];

@@ -149,6 +149,12 @@ __device__ inline double tile_offsum(const mg_sell &L, int row, const double *xl
 #pragma unroll
     for (int k = 0; k < 4; k++) s += (double)(e[k] >> 12) * xl[e[k] & 0xfffu];
   }
+  if (w16 & 2) {
+    const unsigned v = *(const unsigned *)(sbase + nb * 512 + (w16 & 4) * 64 + (row & 63) * 2);
+    const unsigned e[2] = {v & 0xffffu, v >> 16};
+#pragma unroll
+    for (int k = 0; k < 2; k++) s += (double)(e[k] >> 12) * xl[e[k] & 0xfffu];
+  }
   return s;
 }
 // Eight entries and the eight values they point at are loaded before any is

@@ -9,10 +9,12 @@
 // fit is marked (nhalo = -1) and its rows use the global 32-bit form.
 //
 // Layout of the 16-bit entries: per 64-row slice, its width w rounded up to
-// a multiple of 4; the first 8 * (w / 8) entries in groups of 8, entry j of
+// an even number; the first 8 * (w / 8) entries in groups of 8, entry j of
 // lane l at (j / 8) * 512 + l * 8 + (j % 8), so one 16-byte load per lane
-// fetches 8 consecutive entries of its row; a last group of 4, if any, at
-// (w / 8) * 512 + l * 4 + (j % 4), one 8-byte load per lane.
+// fetches 8 consecutive entries of its row; then a group of 4, if w % 8 >= 4,
+// at (w / 8) * 512 + l * 4 + (j % 4), one 8-byte load per lane; then a group
+// of 2, if w % 4 == 2, after it (l * 2 + (j % 2)), one 4-byte load per lane.
+// (A hex row has 26 neighbours: 3 groups of 8 and one of 2, no padding.)
 #include <stdlib.h>
 
 #include <vector>
@@ -157,7 +159,12 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
           if (li >= 0) v = (uint16_t)(li | (wt << 12));
         }
       }
-      o[j < full ? (j >> 3) * 512 + lane * 8 + (j & 7) : full * 64 + lane * 4 + (j - full)] = v;
+      // groups of 8 entries (row-interleaved 16-byte words), then a group of
+      // 4 (8-byte words) and a group of 2 (4-byte words) where the width has them
+      const int k = j - full, t4 = (w8 - full) & 4;
+      o[j < full ? (j >> 3) * 512 + lane * 8 + (j & 7)
+                 : (k < t4 ? full * 64 + lane * 4 + k : full * 64 + t4 * 64 + lane * 2 + (k - t4))] =
+          v;
     }
   }
 }
@@ -166,7 +173,7 @@ __global__ void k_w16(int64_t nslices, const int *__restrict__ w, int *__restric
                       int64_t *__restrict__ bytes) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nslices) {
-    int v = (w[i] + 3) / 4 * 4;
+    int v = (w[i] + 1) / 2 * 2;  // even: groups of 8, then of 4 and 2
     w8[i] = v;
     bytes[i] = (int64_t)v * RSB_WAVE;
   }
