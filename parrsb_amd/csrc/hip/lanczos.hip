@@ -10,13 +10,17 @@
 // iteration is two launches over all rows of the level, each followed by
 // its k_lz_fin:
 //   k_lz_apply    p = beta (p_old - mean) + r formed on the fly for each row
-//                 and its neighbours, w = L p (SELL-64), basis[it] = r / |r|,
+//                 and its neighbours, w = L p (SELL-64),
 //                 pap = w . (p - mean(p))
 //   k_lz_rupdate  r = r - alpha w, rtr = r . r, tridiagonal entries, stop test
+// (FP32 passes on few tiles fold the update into the next apply, below).
+// The Lanczos basis vector of iteration it is r_it / |r_it|: the residuals
+// themselves are kept, r_it in basis slot it, and the Ritz step scales them.
 // The mean of p is removed lazily (fiedler.c:20-30 `ortho`): w = L p is
 // evaluated in the difference form sum_j w_ij (p_i - p_j), which the mean
-// does not change, and the stored p keeps its mean, subtracted when used.  Segments that converged or are between passes skip every
-// launch at the first instruction.
+// does not change, and the stored p keeps its mean, subtracted when used.
+// Segments that converged or are between passes skip every launch at the
+// first instruction.
 //
 // Precision: vectors are T (double, or float); every scalar and the
 // tridiagonal eigen-solve are FP64.  With T = float, dot products are
@@ -37,7 +41,7 @@ struct lz_args {
   int nseg;
   int *status, *iter, *k, *npass, *niter, *miter;
   double *sc, *T, *y, *partial;
-  unsigned *cnt;
+  double *rn;  // (miter_max + 1) per segment: 1 / |r_j|
   int *running;
   int mpass;
   int mode;  // start vector: 0 reference, 1 caller per element, 2 learned per row
@@ -254,13 +258,13 @@ struct fold_t {
 };
 
 // Fused search-direction update and Laplacian apply (fiedler.c:460-475):
-//   q = beta (p_old - mean_p) + r,  w = L q,  basis[it] = r / |r|
+//   q = beta (p_old - mean_p) + r,  w = L q  (r: basis slot it)
 // and the segment sums of q, w.q and w, from which the last block forms
 // mean_p and pap = w.(q - mean_p), then alpha = rtz1 / pap.
 template <typename T, bool FIRST, bool FOLD>
 __device__ inline void lz_apply_rows(const lz_args &a, const tile_t &t, const T *__restrict__ r,
                                      const T *__restrict__ pold, T *__restrict__ pnew,
-                                     T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
+                                     T *__restrict__ w, T beta, T m,
                                      const int64_t *__restrict__ slice_off,
                                      const int *__restrict__ slice_w,
                                      const uint32_t *__restrict__ ell, acc_t<T> &sq,
@@ -282,7 +286,6 @@ __device__ inline void lz_apply_rows(const lz_args &a, const tile_t &t, const T 
     }
     pnew[row] = qi;
     w[row] = wi;
-    b[row] = ri * rni;
     if (UPD) fd.rn[row] = ri;
   }
 }
@@ -316,9 +319,8 @@ __device__ inline void acc_entry(uint32_t e, T qi, const T *ql, T &sum) {
 template <typename T, bool FOLD>
 __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_args &ta,
                                        const T *ql, const T *rl, T *__restrict__ pnew,
-                                       T *__restrict__ w, T *__restrict__ b, T rni, int nreal,
-                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw,
-                                       fold_t<T> &fd) {
+                                       T *__restrict__ w, int nreal, acc_t<T> &sq,
+                                       acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
   const int li = row - t.row0, sl = row >> 6;
   const uint16_t *sbase = ta.e16 + ta.off16[sl];
   const u32x4 *ep = (const u32x4 *)(sbase + (row & 63) * 8);
@@ -352,8 +354,7 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
   }
   pnew[row] = qi;
   w[row] = sum;
-  const T ri = rl[li];
-  __builtin_nontemporal_store(ri * rni, &b[row]);
+  const T ri = FOLD ? rl[li] : (T)0;  // the tile's r is kept in LDS only to fold
   if (FOLD && fd.rn) fd.rn[row] = ri;
   if (li < nreal) {
     sq.add(qi);
@@ -370,8 +371,8 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
 template <typename T, int R, bool FOLD>
 __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta, const T *ql,
                                           const T *rl, T *__restrict__ pnew, T *__restrict__ w,
-                                          T *__restrict__ b, T rni, int nreal, acc_t<T> &sq,
-                                          acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
+                                          int nreal, acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw,
+                                          fold_t<T> &fd) {
   for (int row0 = t.row0 + threadIdx.x; row0 < t.row1; row0 += R * blockDim.x) {
     int rows[R], wd[R];
     const uint16_t *base[R];
@@ -389,8 +390,7 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
 #pragma unroll
       for (int k = 0; k < R; k++)
         if (rows[k] < t.row1)
-          apply_row_tiled<T, FOLD>(rows[k], t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw,
-                                   fd);
+          apply_row_tiled<T, FOLD>(rows[k], t, ta, ql, rl, pnew, w, nreal, sq, swq, sw, fd);
       continue;
     }
     u32x4 v[R][4];
@@ -436,8 +436,7 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
       acc_entry(t2[k] >> 16, qi, ql, sum);
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
-      const T ri = rl[li];
-      __builtin_nontemporal_store(ri * rni, &b[rows[k]]);
+      const T ri = FOLD ? rl[li] : (T)0;
       if (FOLD && fd.rn) fd.rn[rows[k]] = ri;
       if (li < nreal) {
         sq.add(qi);
@@ -500,57 +499,60 @@ template <typename T, bool FIRST, bool FOLD = false>
 __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const tiled_args &ta,
                                       int nh, T *ql, T *rl, const T *__restrict__ r,
                                       const T *__restrict__ pold, T *__restrict__ pnew,
-                                      T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
-                                      acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
+                                      T *__restrict__ w, T beta, T m, acc_t<T> &sq,
+                                      acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   stage_window<T, FIRST, true, FOLD>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta,
                                      m, fd.wp, fd.alpha);
   // two rows per thread and step (measured 15% faster than one at a time
   // on the 256^3 workload: more loads in flight per lane)
-  apply_rows_batched<T, 2, FOLD>(t, ta, ql, rl, pnew, w, b, rni, nreal, sq, swq, sw, fd);
+  apply_rows_batched<T, 2, FOLD>(t, ta, ql, rl, pnew, w, nreal, sq, swq, sw, fd);
 }
 
+// The residuals of a pass live in the basis, unscaled: r_it in slot it
+// (the Ritz step scales slot j by 1 / |r_j|), so the apply reads its r
+// there and writes no basis vector.
 // FOLD (FP32): the residual update of the previous iteration is formed in
-// the staging, r = r_prev - alpha w_prev (r_prev, w_prev: the buffers of
-// the other parity; r of this iteration is written to rn), and the launch
-// adds the sums r.r, r.w, w.w (partial slots 3..5) for FIN_APPLYR.  No
-// separate r-update pass and one finishing step per iteration.
+// the staging, r_it = r_{it-1} - alpha w_{it-1} (w_prev: the w buffer of
+// the other parity), written to slot it, and the launch adds the sums r.r,
+// r.w, w.w (partial slots 3..5) for FIN_APPLYR.  No separate r-update pass
+// and one finishing step per iteration.
 template <typename T, bool FOLD>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ rprev,
-                                                        const T *__restrict__ r0,
-                                                        const T *__restrict__ pold,
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__restrict__ pold,
                                                         T *__restrict__ pnew, T *__restrict__ w,
                                                         T *__restrict__ basis, int64_t mpad,
                                                         tiled_args ta, int nlist,
-                                                        const T *__restrict__ wprev,
-                                                        T *__restrict__ rn) {
+                                                        const T *__restrict__ wprev) {
   __shared__ T ql[TB_LOCAL];
+  // the tile's r (read only to fold the update); its 4 KB also hold the
+  // FP32 kernel at seven workgroups per CU: at eight (16.5 KB of LDS) the
+  // level-0 apply of the 256^3 box measured 279 us against 269 us
   __shared__ T rl[RSB_TILE];
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
-  // the first iteration of a pass reads the residual the set-up wrote (r0)
-  const T *__restrict__ r = it == 0 ? r0 : rprev;
-  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
+  T *cur = basis + (size_t)it * mpad;
+  // what the staging reads: r_it, or r_{it-1} to update (FOLD)
+  const T *__restrict__ r = (FOLD && it > 0) ? cur - mpad : cur;
+  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP);
   acc_t<T> sq, swq, sw;
   fold_t<T> fd;
-  fd.wp = wprev, fd.rn = it == 0 ? nullptr : rn, fd.alpha = (T)SC(s, SC_ALPHA);
-  T *b = basis + (size_t)it * mpad;
+  fd.wp = wprev, fd.rn = it == 0 ? nullptr : cur, fd.alpha = (T)SC(s, SC_ALPHA);
   const int nh = ta.nhalo[t.ti];
   if (nh >= 0) {
     if (it == 0)
-      lz_apply_tiled<T, true, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq,
-                                    swq, sw, fd);
+      lz_apply_tiled<T, true, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, beta, m, sq, swq, sw,
+                                    fd);
     else
-      lz_apply_tiled<T, false, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, b, beta, m, rni, sq,
-                                     swq, sw, fd);
+      lz_apply_tiled<T, false, FOLD>(a, t, ta, nh, ql, rl, r, pold, pnew, w, beta, m, sq, swq,
+                                     sw, fd);
   } else if (it == 0) {
-    lz_apply_rows<T, true, FOLD>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off,
-                                 ta.slice_w, ta.ell, sq, swq, sw, fd);
+    lz_apply_rows<T, true, FOLD>(a, t, r, pold, pnew, w, beta, m, ta.slice_off, ta.slice_w,
+                                 ta.ell, sq, swq, sw, fd);
   } else {
-    lz_apply_rows<T, false, FOLD>(a, t, r, pold, pnew, w, b, beta, m, rni, ta.slice_off,
-                                  ta.slice_w, ta.ell, sq, swq, sw, fd);
+    lz_apply_rows<T, false, FOLD>(a, t, r, pold, pnew, w, beta, m, ta.slice_off, ta.slice_w,
+                                  ta.ell, sq, swq, sw, fd);
   }
   if (FOLD) {
     double v[6] = {sq.value(), swq.value(), sw.value(),
@@ -633,8 +635,8 @@ template <typename T, bool FIRST>
 __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const vtile_args &va,
                                        T *ql, T *S, uint16_t *F, const T *__restrict__ r,
                                        const T *__restrict__ pold, T *__restrict__ pnew,
-                                       T *__restrict__ w, T *__restrict__ b, T beta, T m, T rni,
-                                       acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw) {
+                                       T *__restrict__ w, T beta, T m, acc_t<T> &sq,
+                                       acc_t<T> &swq, acc_t<T> &sw) {
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   if (threadIdx.x == 0) ql[VT_MAXW] = (T)0;  // what the lists' padding reads
@@ -672,8 +674,6 @@ __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const 
       }
     pnew[row] = qi;
     w[row] = sum;
-    // r of the tile's rows was read by the staging a moment ago (cache hit)
-    __builtin_nontemporal_store(r[row] * rni, &b[row]);
     if (li < nreal) {
       sq.add(qi);
       swq.dot(sum, qi);
@@ -686,12 +686,11 @@ __device__ inline void lz_apply_vertex(const lz_args &a, const tile_t &t, const 
 // = 28 KB, five workgroups per CU); a tile without the vertex form (vt_n < 0) takes the
 // neighbour entries of the same window.  The level has no wide tiles.
 template <typename T>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__restrict__ r,
-                                                          const T *__restrict__ pold,
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__restrict__ pold,
                                                           T *__restrict__ pnew, T *__restrict__ w,
-                                                          T *__restrict__ basis, int64_t mpad,
-                                                          vtile_args va, tiled_args ta,
-                                                          int nlist) {
+                                                          const T *__restrict__ basis,
+                                                          int64_t mpad, vtile_args va,
+                                                          tiled_args ta, int nlist) {
   __shared__ T ql[VT_MAXW + 1];
   __shared__ T S[VT_MAXV];  // the neighbour-entry fallback keeps r of the tile rows here
   __shared__ uint16_t F[VT_MAXV];
@@ -699,37 +698,36 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
-  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP), rni = (T)SC(s, SC_RNI);
+  const T *__restrict__ r = basis + (size_t)it * mpad;  // r_it (its basis slot)
+  const T beta = (T)SC(s, SC_BETA), m = (T)SC(s, SC_MEANP);
   acc_t<T> sq, swq, sw;
-  T *b = basis + (size_t)it * mpad;
   if (va.vt_n[t.ti] >= 0) {
     if (it == 0)
-      lz_apply_vertex<T, true>(a, t, va, ql, S, F, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                               sw);
+      lz_apply_vertex<T, true>(a, t, va, ql, S, F, r, pold, pnew, w, beta, m, sq, swq, sw);
     else
-      lz_apply_vertex<T, false>(a, t, va, ql, S, F, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                                sw);
+      lz_apply_vertex<T, false>(a, t, va, ql, S, F, r, pold, pnew, w, beta, m, sq, swq, sw);
   } else {
     const int nh = ta.nhalo[t.ti];
     fold_t<T> fd;
     fd.wp = nullptr, fd.rn = nullptr, fd.alpha = (T)0;
     if (it == 0)
-      lz_apply_tiled<T, true>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq, sw,
-                              fd);
+      lz_apply_tiled<T, true>(a, t, ta, nh, ql, S, r, pold, pnew, w, beta, m, sq, swq, sw, fd);
     else
-      lz_apply_tiled<T, false>(a, t, ta, nh, ql, S, r, pold, pnew, w, b, beta, m, rni, sq, swq,
-                               sw, fd);
+      lz_apply_tiled<T, false>(a, t, ta, nh, ql, S, r, pold, pnew, w, beta, m, sq, swq, sw, fd);
   }
   double v[3] = {sq.value(), swq.value(), sw.value()};
   publish<3>(a, t, v);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restrict__ r,
-                                                          const T *__restrict__ w) {
+// r_{it+1} = r_it - alpha w into basis slot it + 1; partial r.r
+__global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restrict__ basis,
+                                                          int64_t mpad, const T *__restrict__ w) {
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t)) return;
   const int s = t.s;
+  const T *__restrict__ r = basis + (size_t)a.iter[s] * mpad;
+  T *__restrict__ rn = basis + (size_t)(a.iter[s] + 1) * mpad;
   const T alpha = (T)SC(s, SC_ALPHA);
   acc_t<T> acc;
   // the (up to) four rows of a thread: all loads first
@@ -746,7 +744,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
     const int row = t.row0 + threadIdx.x + k * blockDim.x;
     if (row >= t.row1) continue;
     const T x = rsub(rv[k], alpha, wv[k]);
-    r[row] = x;
+    rn[row] = x;
     acc.dot(x, x);
   }
   double v[1] = {acc.value()};
@@ -1185,8 +1183,10 @@ __global__ void __launch_bounds__(RSB_WAVE) k_tqli_test(int n, const double *dd,
   tqli_wave(n, d, e, z, y, lam);
 }
 
-// Ritz vector (fiedler.c:547-553): f = sum_j basis_j y_j (FP64 accumulation,
-// j ascending), then its mean (ortho).  Decides restart or finish.
+// Ritz vector (fiedler.c:547-553): f = sum_j (r_j / |r_j|) y_j, the basis
+// slots holding the residuals r_j unscaled (FP64 accumulation of r_j times
+// y_j / |r_j|, j ascending), then its mean (ortho).  Decides restart or
+// finish.
 template <typename T>
 __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__restrict__ basis,
                                                        int64_t mpad, T *__restrict__ f) {
@@ -1194,6 +1194,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
   if (!tile_of(a, LZ_PASSDONE, t)) return;
   const int s = t.s, k = a.k[s];
   const double *y = a.y + (size_t)s * a.miter_max;
+  const double *rn = a.rn + (size_t)s * (a.miter_max + 1);
   double v[1] = {0.0};
   // The (up to) four rows of a thread together and eight basis vectors at a
   // time, every load unconditional (rows past the tile read its last row,
@@ -1218,7 +1219,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
 #pragma unroll
     for (int u = 0; u < U; u++)
       if (j0 + u < k) {
-        const double yj = y[j0 + u];
+        const double yj = y[j0 + u] * rn[j0 + u];
 #pragma unroll
         for (int q = 0; q < R; q++) x[q] += (double)b[u][q] * yj;
       }
@@ -1353,6 +1354,7 @@ __device__ inline void rupdate_step(const lz_args &a, int s, double rtr) {
   SC(s, SC_RTR) = rtr;
   SC(s, SC_RNORM) = rnorm;
   SC(s, SC_RNI) = 1.0 / rnorm;
+  a.rn[(size_t)s * (a.miter_max + 1) + it + 1] = 1.0 / rnorm;
   if (rnorm < SC(s, SC_RTOL) || it + 1 == mi) {
     a.k[s] = it + 1;
     a.status[s] = LZ_PASSDONE;
@@ -1409,6 +1411,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
       SC(s, SC_RNORM0) = rn;
       SC(s, SC_RTOL) = rn * a.tol;
       SC(s, SC_RNI) = 1.0 / rn;
+      a.rn[(size_t)s * (a.miter_max + 1)] = 1.0 / rn;
       SC(s, SC_RTZ1) = rtr;  // iteration 0: rtz2 = 1 (initial rtz1), rtz1 = rtr
       SC(s, SC_RTZ2) = 1.0;
       SC(s, SC_BETA) = 0.0;
@@ -1485,7 +1488,6 @@ __global__ void k_lz_init_state(lz_args a, int miter_opt) {
   a.status[s] = (p > 1 && n >= 2) ? LZ_SETUP : LZ_IDLE;
   a.miter[s] = n < miter_opt ? n : miter_opt;  // fiedler.c:522
   a.npass[s] = 0, a.niter[s] = 0, a.iter[s] = 0, a.k[s] = 0;
-  a.cnt[s] = 0;
   for (int j = 0; j < LZ_NSC; j++) SC(s, j) = 0.0;
 }
 
@@ -1529,14 +1531,15 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // buffers
   size_t vb = (size_t)mpad * sizeof(T);
   if (vb > g->vec_bytes) {
-    for (void **q : {&g->d_r, &g->d_p, &g->d_p2, &g->d_w, &g->d_f, &g->d_r2, &g->d_w2}) {
+    for (void **q : {&g->d_p, &g->d_p2, &g->d_w, &g->d_f, &g->d_w2}) {
       if (*q) RSB_TRY(rsb_dev_free(*q));
       *q = nullptr;
       RSB_TRY(rsb_dev_malloc(q, vb + 256));
     }
     g->vec_bytes = vb;
   }
-  size_t bb = vb * (size_t)miter_max;
+  // the pass's residuals r_0 .. r_miter (the last one written, not used)
+  size_t bb = vb * ((size_t)miter_max + 1);
   if (bb > g->basis_bytes) {
     if (g->d_basis) RSB_TRY(rsb_dev_free(g->d_basis));
     g->d_basis = nullptr;
@@ -1548,7 +1551,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   lz_state &L = g->lz;
   if (nseg > g->lz_nseg_cap || miter_max > g->lz_miter_cap) {
     // iter, k, npass, niter and miter are carved from the status block
-    for (void *q : {(void *)L.status, (void *)L.sc, (void *)L.T, (void *)L.y, (void *)L.cnt})
+    for (void *q : {(void *)L.status, (void *)L.sc, (void *)L.T, (void *)L.y, (void *)L.rn})
       if (q) RSB_TRY(rsb_dev_free(q));
     int cap = std::max(nseg, 1);
     RSB_TRY(rsb_dev_malloc((void **)&L.status, sizeof(int) * cap * 6));
@@ -1557,7 +1560,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     RSB_TRY(rsb_dev_malloc((void **)&L.sc, sizeof(double) * cap * LZ_NSC));
     RSB_TRY(rsb_dev_malloc((void **)&L.T, sizeof(double) * cap * 2 * miter_max));
     RSB_TRY(rsb_dev_malloc((void **)&L.y, sizeof(double) * cap * miter_max));
-    RSB_TRY(rsb_dev_malloc((void **)&L.cnt, sizeof(unsigned) * cap));
+    RSB_TRY(rsb_dev_malloc((void **)&L.rn, sizeof(double) * cap * ((size_t)miter_max + 1)));
     g->lz_nseg_cap = cap, g->lz_miter_cap = miter_max;
   }
   if (ntiles > g->lz_tile_cap) {
@@ -1572,12 +1575,12 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.nseg = nseg;
   a.status = L.status, a.iter = L.iter, a.k = L.k, a.npass = L.npass, a.niter = L.niter;
   a.miter = L.status + 5 * std::max(g->lz_nseg_cap, 1);
-  a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.cnt = L.cnt,
+  a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.rn = L.rn,
   a.running = L.running;
   a.mpass = mpass, a.mode = init_mode, a.tol = tol, a.miter_max = miter_max;
   a.pstride = std::max(g->lz_tile_cap, 1);
   a.max_n = g->lws_max_n;
-  T *r = (T *)g->d_r, *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
+  T *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
     *basis = (T *)g->d_basis;
   hipStream_t st = g->stream;
 #define FIN(op)                                           \
@@ -1611,36 +1614,35 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   // FP32 with the neighbour entries: a pass that starts with at most
   // fold_tiles running tiles folds the r-update into the next apply
   // (k_lz_apply<float, true>, FIN_APPLYR: two launches per iteration instead
-  // of four; r and w alternate between two buffers by iteration parity).
+  // of four; w alternates between two buffers by iteration parity).
   // On more tiles the extra halo gathers of w outweigh the saved pass.
   // PARRSB_AMD_LZ_FOLD_TILES sets the limit (0: never).
   const char *fev = getenv("PARRSB_AMD_LZ_FOLD_TILES");
   const int fold_tiles = (sizeof(T) == 4 && !vform) ? (fev ? atoi(fev) : LZ_FOLD_TILES) : 0;
   bool fold = false;
-  T *const R[2] = {r, (T *)g->d_r2}, *const W[2] = {w, (T *)g->d_w2};
+  T *const W[2] = {w, (T *)g->d_w2};
   // one iteration (it: the pass iteration, where every running segment is):
   // the apply launch, then the finishing step(s) and the r-update
   auto apply_launch = [&](int it) {
     T *pold = (it & 1) ? p2 : p, *pnew = (it & 1) ? p : p2;
     const unsigned grid = xgrid(nact[0]);
     if (fold)
-      k_lz_apply<T, true><<<grid, RSB_BLOCK, 0, st>>>(ai, R[(it + 1) & 1], R[0], pold, pnew,
-                                                      W[it & 1], basis, mpad, ta, nact[0],
-                                                      W[(it + 1) & 1], R[it & 1]);
+      k_lz_apply<T, true><<<grid, RSB_BLOCK, 0, st>>>(ai, pold, pnew, W[it & 1], basis, mpad, ta,
+                                                      nact[0], W[(it + 1) & 1]);
     else if (vform)
-      k_lz_apply_v<float><<<grid, RSB_BLOCK, 0, st>>>(ai, (const float *)r, (const float *)pold,
-                                                      (float *)pnew, (float *)w, (float *)basis,
-                                                      mpad, va, ta, nact[0]);
+      k_lz_apply_v<float><<<grid, RSB_BLOCK, 0, st>>>(ai, (const float *)pold, (float *)pnew,
+                                                      (float *)w, (const float *)basis, mpad, va,
+                                                      ta, nact[0]);
     else
-      k_lz_apply<T, false><<<grid, RSB_BLOCK, 0, st>>>(ai, r, r, pold, pnew, w, basis, mpad, ta,
-                                                       nact[0], nullptr, nullptr);
+      k_lz_apply<T, false><<<grid, RSB_BLOCK, 0, st>>>(ai, pold, pnew, w, basis, mpad, ta,
+                                                       nact[0], nullptr);
   };
   auto after_apply = [&]() {
     if (fold) {
       k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLYR);
     } else {
       k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
-      k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, r, w);
+      k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, basis, mpad, w);
       k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
     }
   };
@@ -1669,10 +1671,11 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   auto passes = [&](int np_max) -> int {
     a.mpass = ai.mpass = np_max;
     for (int pass = 0; pass < np_max; pass++) {
-      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, r, p);
+      // r_0 into basis slot 0
+      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, basis, p);
       RSB_LAUNCH_CHECK();
       FIN(FIN_SETUP0);
-      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, r, p);
+      k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, basis, p);
       RSB_LAUNCH_CHECK();
       FIN(FIN_SETUP1);
       if (compact(LZ_RUNNING)) return 2;
@@ -1822,20 +1825,20 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
                          hipMemcpyDeviceToHost, st));
   RSB_TRY(hipStreamSynchronize(st));
   // Algorithmic bytes, per iteration and segment of n rows and nnz
-  // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 9 s n vector
+  // off-diagonals: 4(n+1) row pointers + 4 nnz packed entries + 7 s n vector
   // streams for the whole iteration; for k_lz_apply alone 4(n+1) + 4 nnz +
-  // 5 s n (p_old and r read once, p_new, w and the basis vector written).
-  // With the folded update the apply is the iteration: 7 s n (r, w and
-  // p_old of the previous iteration read; r, p_new, w and the basis vector
-  // written).  Counted iterations, real (unpadded) sizes.
+  // 4 s n (p_old and r read once, p_new and w written; r stays in its basis
+  // slot).  With the folded update the apply is the iteration: 6 s n (r, w
+  // and p_old of the previous iteration read; r, p_new and w written).
+  // Counted iterations, real (unpadded) sizes.
   for (int s = 0; s < nseg; s++) {
     const double *scs = &sc[(size_t)s * LZ_NSC];
     ni[s] += (int)scs[SC_WARMIT];
     if (g->h_p[s] > 1 && g->h_n[s] >= 2) {
       double n = g->h_n[s], nz = (double)g->h_seg_nnz[s], sz = sizeof(T);
       const double nf = scs[SC_NFOLD], nu = ni[s] - nf;
-      g->lz_bytes += nu * (4 * (n + 1) + 4 * nz + 9 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 7 * sz * n);
-      g->spmv_bytes += nu * (4 * (n + 1) + 4 * nz + 5 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 7 * sz * n);
+      g->lz_bytes += nu * (4 * (n + 1) + 4 * nz + 7 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 6 * sz * n);
+      g->spmv_bytes += nu * (4 * (n + 1) + 4 * nz + 4 * sz * n) + nf * (4 * (n + 1) + 4 * nz + 6 * sz * n);
     }
     if (out) {
       if (out->niter) out->niter[s] = ni[s];

@@ -107,11 +107,9 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   for (hipEvent_t ev : g->spmv_ev) hipEventDestroy(ev);
   FREE(g->d_tile);
   FREE(g->d_seg_tiles);
-  FREE(g->d_r);
   FREE(g->d_p);
   FREE(g->d_p2);
   FREE(g->d_w);
-  FREE(g->d_r2);
   FREE(g->d_w2);
   FREE(g->d_f);
   FREE(g->d_basis);
@@ -123,7 +121,7 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->lz.T);
   FREE(g->lz.y);
   FREE(g->lz.partial);
-  FREE(g->lz.cnt);
+  FREE(g->lz.rn);
   FREE(g->lz.running);
   FREE(g->d_sort_tmp);
   FREE(g->d_k0);

@@ -43,8 +43,8 @@ struct lz_state {
   double *sc;      // LZ_NSC scalars per segment
   double *T;       // 2*miter per segment: diag[miter], upper[miter]
   double *y;       // miter per segment: Ritz coefficients
-  double *partial; // 4 per tile
-  unsigned *cnt;   // last-block counters, one per segment
+  double *partial; // 6 per tile
+  double *rn;      // miter + 1 per segment: 1 / |r_j| of the pass's residuals
   int *running;    // count of RUNNING segments
 };
 enum { LZ_IDLE = 0, LZ_SETUP = 1, LZ_RUNNING = 2, LZ_PASSDONE = 3, LZ_FINISHED = 4 };
@@ -114,8 +114,8 @@ struct rsbgpu {
   std::vector<int> h_tile, h_seg_tiles;
 
   // Lanczos vectors (padded rows)
-  void *d_r = nullptr, *d_p = nullptr, *d_p2 = nullptr, *d_w = nullptr, *d_f = nullptr;
-  void *d_r2 = nullptr, *d_w2 = nullptr;  // the other parity's r and w (FP32 folded update)
+  void *d_p = nullptr, *d_p2 = nullptr, *d_w = nullptr, *d_f = nullptr;
+  void *d_w2 = nullptr;  // the other parity's w (FP32 folded update)
   void *d_basis = nullptr;
   size_t vec_bytes = 0, basis_bytes = 0;
   double *d_fied = nullptr;     // Fiedler value per compact position
