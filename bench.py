@@ -210,15 +210,19 @@ def summarize_traffic(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_c
     fe, wr = per_kernel(fetch_csv), per_kernel(write_csv)
 
     def kern(prefix):
-        kf = [k for k in fe if k.startswith(prefix)][0]
-        kw = [k for k in wr if k.startswith(prefix)][0]
-        f = fe[kf][0] / fe[kf][1] * sf
-        w = wr[kw][0] / wr[kw][1] * sw
-        return {"launches": fe[kf][1], "fetch_bytes_per_launch": f,
+        # every kernel whose name starts with prefix, weighted by its launches
+        kf = [k for k in fe if k.startswith(prefix)]
+        kw = [k for k in wr if k.startswith(prefix)]
+        nf, nw = sum(fe[k][1] for k in kf), sum(wr[k][1] for k in kw)
+        f = sum(fe[k][0] for k in kf) / nf * sf
+        w = sum(wr[k][0] for k in kw) / nw * sw
+        return {"launches": nf, "fetch_bytes_per_launch": f,
                 "write_bytes_per_launch": w, "bytes_per_launch": f + w}
 
-    ap_ = kern("void k_lz_apply<float>")
-    out = {"workload": desc, "kernel": "k_lz_apply<float>", **ap_,
+    # the FP32 apply in both forms (the update folded in, or not)
+    ap_ = kern("void k_lz_apply<float")
+    forms = {k.split("(")[0]: kern(k) for k in fe if k.startswith("void k_lz_apply<float")}
+    out = {"workload": desc, "kernel": "k_lz_apply<float>", **ap_, "forms": forms,
            "scale_fetch": sf, "scale_write": sw,
            "calibration": "FETCH_SIZE and WRITE_SIZE of a 1 GiB device-to-device copy "
                           "(__amd_rocclr_copyBuffer)",

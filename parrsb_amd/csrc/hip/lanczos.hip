@@ -47,6 +47,8 @@ struct lz_args {
   int mode;  // start vector: 0 reference, 1 caller per element, 2 learned per row
   int max_n; // mode 2: segments with more rows take the reference start
   double tol;
+  double stall;  // > 0: a pass whose Ritz value moved by at most this (relative) ends the solve
+  int stall_min; // ... from this pass on
   int miter_max;
   const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
   const int *act_segs;   // if set: blockIdx.x -> segment, for k_lz_fin
@@ -1457,7 +1459,11 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
       int np = a.npass[s] + 1;
       a.npass[s] = np;
       a.niter[s] = (np - 1) * a.miter[s] + k;
-      if (k == a.miter[s] && np < a.mpass) {
+      const double lam = SC(s, SC_LAMBDA), lprev = SC(s, SC_LAMPREV);
+      SC(s, SC_LAMPREV) = lam;
+      const bool stalled = a.stall > 0.0 && np >= max(2, a.stall_min) &&
+                           fabs(lam - lprev) <= a.stall * fabs(lam);
+      if (k == a.miter[s] && np < a.mpass && !stalled) {
         a.status[s] = LZ_SETUP;
         atomicAdd(&a.running[1], 1);
       } else {
@@ -1578,6 +1584,11 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   a.sc = L.sc, a.T = L.T, a.y = L.y, a.partial = L.partial, a.rn = L.rn,
   a.running = L.running;
   a.mpass = mpass, a.mode = init_mode, a.tol = tol, a.miter_max = miter_max;
+  {
+    const char *sv = getenv("PARRSB_AMD_LZ_STALL"), *sm = getenv("PARRSB_AMD_LZ_STALL_MIN");
+    a.stall = sv ? atof(sv) : 0.0;
+    a.stall_min = sm ? atoi(sm) : 2;
+  }
   a.pstride = std::max(g->lz_tile_cap, 1);
   a.max_n = g->lws_max_n;
   T *p = (T *)g->d_p, *p2 = (T *)g->d_p2, *w = (T *)g->d_w, *f = (T *)g->d_f,
