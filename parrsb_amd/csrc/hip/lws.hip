@@ -286,92 +286,6 @@ __global__ void k_lws_agg(int64_t mpad, const int64_t *__restrict__ slice_off,
   }
 }
 
-// The forward aggregation (norm, nothing added) on the level's LDS tiles:
-// one workgroup per tile stages LWS_AGG_CB channels of h for the tile's rows
-// and its halo rows (the window) in LDS, then sums every row's 16-bit
-// entries against it, channel block by channel block (the tile's entries
-// stay in L2 between blocks).  Entries and their order are the 32-bit
-// form's, weight-0 padding skipped, and each sum is formed as in k_lws_agg,
-// so the result equals it bit for bit.  Against k_lws_agg's 27 gathered
-// 128-byte rows per row (through L2), HBM sees h of the window once.  LDS:
-// the level's largest window x 16 bytes.
-#define LWS_AGG_CB 8
-typedef unsigned lws_u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned lws_u32x2 __attribute__((ext_vector_type(2)));
-
-__device__ inline void lws_agg_entry(unsigned e, const uint4 *hl, float (&acc)[8], float &dsum) {
-  const unsigned wt = e >> 12;
-  if (wt == 0) return;
-  const float w = (float)wt;
-  const uint4 v = hl[e & 0xfffu];
-  const unsigned vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    acc[2 * k] += w * bf2f((unsigned short)(vv[k] & 0xffff));
-    acc[2 * k + 1] += w * bf2f((unsigned short)(vv[k] >> 16));
-  }
-  dsum += w;
-}
-
-__global__ void __launch_bounds__(256) k_lws_agg_t(const int *__restrict__ tile,
-                                                   const int *__restrict__ nhalo,
-                                                   const int64_t *__restrict__ halo_off,
-                                                   const int *__restrict__ halo,
-                                                   const int64_t *__restrict__ off16,
-                                                   const int *__restrict__ w16,
-                                                   const uint16_t *__restrict__ e16,
-                                                   const unsigned short *__restrict__ in,
-                                                   unsigned short *__restrict__ out) {
-  extern __shared__ uint4 hl[];  // window rows x LWS_AGG_CB channels
-  const int t = blockIdx.x;
-  const int row0 = tile[3 * t + 1], row1 = tile[3 * t + 2], nrows = row1 - row0;
-  const int nwin = nrows + nhalo[t];
-  const int *__restrict__ hal = halo + halo_off[t];
-  for (int cb = 0; cb < LWS_H; cb += LWS_AGG_CB) {
-    for (int i = threadIdx.x; i < nwin; i += blockDim.x) {
-      const int r = i < nrows ? row0 + i : hal[i - nrows];
-      hl[i] = *(const uint4 *)(in + (size_t)r * LWS_H + cb);
-    }
-    __syncthreads();
-    for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
-      const int sl = row >> 6;
-      const uint16_t *sbase = e16 + off16[sl];
-      const int wd = w16[sl], nb = wd >> 3;
-      float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      float dsum = 0.f;
-      for (int jb = 0; jb < nb; jb++) {
-        const lws_u32x4 v = *((const lws_u32x4 *)(sbase + (row & 63) * 8) + (size_t)jb * RSB_WAVE);
-        const unsigned e[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          lws_agg_entry(e[k] & 0xffffu, hl, acc, dsum);
-          lws_agg_entry(e[k] >> 16, hl, acc, dsum);
-        }
-      }
-      const uint16_t *tail = sbase + nb * 512;
-      if (wd & 4) {
-        const lws_u32x2 v = *(const lws_u32x2 *)(tail + (row & 63) * 4);
-        lws_agg_entry(v.x & 0xffffu, hl, acc, dsum);
-        lws_agg_entry(v.x >> 16, hl, acc, dsum);
-        lws_agg_entry(v.y & 0xffffu, hl, acc, dsum);
-        lws_agg_entry(v.y >> 16, hl, acc, dsum);
-      }
-      if (wd & 2) {
-        const unsigned v = *(const unsigned *)(tail + (wd & 4) * 64 + (row & 63) * 2);
-        lws_agg_entry(v & 0xffffu, hl, acc, dsum);
-        lws_agg_entry(v >> 16, hl, acc, dsum);
-      }
-      const float sc = dsum > 0.f ? 1.f / dsum : 0.f;
-      unsigned o[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        o[k] = (unsigned)f2bf(acc[2 * k] * sc) | ((unsigned)f2bf(acc[2 * k + 1] * sc) << 16);
-      *(uint4 *)(out + (size_t)row * LWS_H + cb) = make_uint4(o[0], o[1], o[2], o[3]);
-    }
-    __syncthreads();  // the window is restaged for the next block
-  }
-}
-
 __global__ void k_lws_dinv(int64_t mpad, const int64_t *__restrict__ slice_off,
                            const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
                            float *__restrict__ dinv) {
@@ -609,21 +523,10 @@ static int lws_forward(rsbgpu *g, const float *P, lws_ws &w, float level) {
   k_lws_w<<<nblocks(LWS_L * 2 * LWS_H * LWS_H, 256), 256, 0, st>>>(P, w.w);
   k_lws_in<<<nblocks(M * LWS_H, 256), 256, 0, st>>>(M, w.x, P, w.h[0]);
   RSB_LAUNCH_CHECK();
-  // the aggregation on the LDS tiles where every tile has them (unless
-  // PARRSB_AMD_LWS_TILED=0), else over the 32-bit global form
-  const char *tv = getenv("PARRSB_AMD_LWS_TILED");
-  const bool tiled = g->nwide == 0 && g->d_e16 && g->tile_maxwin > 0 && g->ntiles > 0 &&
-                     g->tile_maxwin <= 4096 && !(tv && atoi(tv) == 0);
-  const size_t agg_lds = sizeof(uint4) * (size_t)std::max(g->tile_maxwin, 1);
   for (int l = 0; l < LWS_L; l++) {
-    if (tiled)
-      k_lws_agg_t<<<g->ntiles, 256, agg_lds, st>>>(g->d_tile, g->d_tnhalo, g->d_thalo_off,
-                                                   g->d_halo, g->d_off16, g->d_w16, g->d_e16,
-                                                   w.h[l], w.a[l]);
-    else
-      k_lws_agg<<<nblocks(M * 8, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell,
-                                                     w.h[l], LWS_H, 1, 0, w.dinv, nullptr, 0,
-                                                     w.a[l]);
+    k_lws_agg<<<nblocks(M * 8, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell,
+                                                   w.h[l], LWS_H, 1, 0, w.dinv, nullptr, 0,
+                                                   w.a[l]);
     k_gemm_rows<LWS_H, 2 * LWS_H, true, true><<<std::min<unsigned>(nblocks(M, 64), LWS_GEMM_BLOCKS), 256, 0, st>>>(
         (int)M, w.h[l], w.a[l], LWS_H, w.wt + (size_t)l * 2 * LWS_H * LWS_H, P + LWS_BL(l),
         w.h[l + 1], LWS_H);

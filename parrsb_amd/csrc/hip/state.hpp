@@ -98,7 +98,6 @@ struct rsbgpu {
   uint16_t *d_e16 = nullptr;
   int64_t nhalo_total = 0, e16_count = 0;
   int nwide = 0;
-  int tile_maxwin = 0;  // largest window (rows + halo) of a tile with the LDS form
   // vertex form of the tiles (tiles.hip, rsb_build_vtiles): per tile, the
   // distinct vertex ids of its rows, for each the nv window-local indices of
   // the slots at it (padded); per row its nv local vertex indices

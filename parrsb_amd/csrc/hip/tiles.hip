@@ -229,12 +229,7 @@ int rsb_build_tiles(rsbgpu *g) {
   g->nhalo_total = nh;
   g->e16_count = n16;
   g->nwide = 0;
-  g->tile_maxwin = 0;
-  for (int t = 0; t < nt; t++) {
-    g->nwide += nhv[t] < 0;
-    if (nhv[t] >= 0)
-      g->tile_maxwin = std::max(g->tile_maxwin, g->h_tile[3 * t + 2] - g->h_tile[3 * t + 1] + nhv[t]);
-  }
+  for (int v : nhv) g->nwide += v < 0;
   RSB_TRY(dev_alloc(g, g->d_halo, (size_t)nh + 1));
   RSB_TRY(dev_alloc(g, g->d_e16, (size_t)n16 + 8));
   k_tile_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w, g->d_ell,

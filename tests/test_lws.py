@@ -191,38 +191,3 @@ def test_trained_start_gains_on_small_segments():
     q0, q1 = pa.quality(part0, vtx, nv, nparts), pa.quality(part1, vtx, nv, nparts)
     assert q1["edge_cut"] <= 1.02 * q0["edge_cut"]
 
-
-@pytest.mark.gpu
-def test_tiled_aggregation_equals_global_form(monkeypatch):
-    """The forward aggregation on the level's LDS tiles (window of h staged
-    8 channels at a time, 16-bit entries) sums the same entries in the same
-    order as the 32-bit global form: the network's prediction is equal bit
-    for bit, on a level of several segments of several tiles with halos."""
-    import os
-    import parrsb_amd as pa
-    from parrsb_amd import lws_train
-    P = lws_train.load_params(os.path.join(os.path.dirname(pa.__file__), "lws_weights.bin"))
-    vtx, xyz, nv = M.pipe_hex(8, 96, 96, seed=11, amp=0.1)
-    n = vtx.shape[0]
-    sizes = [n // 3, n // 4, n - n // 3 - n // 4]
-    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
-    ys = {}
-    with pa.Device() as d:
-        d.set_mesh(vtx, xyz, nv)
-        d.set_segments(starts, sizes, [2, 3, 2], [0, 2, 5])
-        d.prepartition(1)
-        d.level_laplacian()
-        wide, _ = d.level_form()
-        assert wide == 0
-        dp = d.upload(np.asarray(P, dtype=np.float32))
-        rows = d.upload(np.zeros(n + 4096, dtype=np.float32))
-        try:
-            for form in ("1", "0"):
-                monkeypatch.setenv("PARRSB_AMD_LWS_TILED", form)
-                d.lws_predict(dp, 3.0, rows)
-                ys[form] = d.download(rows, np.zeros(n + 4096, dtype=np.float32))
-        finally:
-            pa.Device.free(dp)
-            pa.Device.free(rows)
-    assert np.isfinite(ys["1"]).all()
-    assert np.array_equal(ys["1"].view(np.uint32), ys["0"].view(np.uint32))
