@@ -223,16 +223,31 @@ __global__ void k_lws_features(const int *__restrict__ row_elem, const int *__re
   o[7] = 0.f;
 }
 
-// h0 = relu(W0 x + b0), FP32 math, BF16 out.
+// h0 = relu(W0 x + b0), FP32 math, BF16 out: one thread per row, its 8
+// inputs and 64 outputs moved as 16-byte words (each output summed over k
+// in order).
 __global__ void k_lws_in(int64_t M, const float *__restrict__ x, const float *__restrict__ P,
                          unsigned short *__restrict__ h) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M * LWS_H) return;
-  const int64_t r = i / LWS_H;
-  const int c = (int)(i % LWS_H);
-  float v = P[LWS_B0 + c];
-  for (int k = 0; k < LWS_IN; k++) v += P[LWS_P0 + c * LWS_IN + k] * x[r * LWS_IN + k];
-  h[i] = f2bf(v > 0.f ? v : 0.f);
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= M) return;
+  static_assert(LWS_IN == 8 && LWS_H % 8 == 0, "16-byte words of inputs and outputs");
+  const float4 x0 = *(const float4 *)(x + r * LWS_IN), x1 = *(const float4 *)(x + r * LWS_IN + 4);
+  const float xv[LWS_IN] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  uint4 *o = (uint4 *)(h + r * LWS_H);
+  for (int c0 = 0; c0 < LWS_H; c0 += 8) {
+    unsigned pk[4];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int c = c0 + q;
+      float v = P[LWS_B0 + c];
+#pragma unroll
+      for (int k = 0; k < LWS_IN; k++) v += P[LWS_P0 + c * LWS_IN + k] * xv[k];
+      const unsigned b = f2bf(v > 0.f ? v : 0.f);
+      if (q & 1) pk[q >> 1] |= b << 16;
+      else pk[q >> 1] = b;
+    }
+    o[c0 / 8] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+  }
 }
 
 // Weighted neighbour aggregation over the level Laplacian (SELL-64 global
@@ -298,13 +313,26 @@ __global__ void k_lws_dinv(int64_t mpad, const int64_t *__restrict__ slice_off,
   dinv[row] = d > 0.f ? 1.f / d : 0.f;
 }
 
-// y = w_o . h + b_o (FP32)
+// y = w_o . h + b_o (FP32): one thread per row, its 64 activations read as
+// 16-byte words, summed over c in order.
 __global__ void k_lws_out(int64_t M, const unsigned short *__restrict__ h,
                           const float *__restrict__ P, float *__restrict__ y) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= M) return;
+  const uint4 *hv = (const uint4 *)(h + r * LWS_H);
+  uint4 w[LWS_H / 8];
+#pragma unroll
+  for (int q = 0; q < LWS_H / 8; q++) w[q] = hv[q];
   float v = P[LWS_BO];
-  for (int c = 0; c < LWS_H; c++) v += P[LWS_WO + c] * bf2f(h[r * LWS_H + c]);
+#pragma unroll
+  for (int q = 0; q < LWS_H / 8; q++) {
+    const unsigned u[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v += P[LWS_WO + 8 * q + 2 * k] * bf2f((unsigned short)(u[k] & 0xffffu));
+      v += P[LWS_WO + 8 * q + 2 * k + 1] * bf2f((unsigned short)(u[k] >> 16));
+    }
+  }
   y[r] = v;
 }
 
@@ -521,7 +549,7 @@ static int lws_forward(rsbgpu *g, const float *P, lws_ws &w, float level) {
   k_lws_dinv<<<nblocks(M, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell, w.dinv);
   k_lws_wt<<<nblocks(LWS_L * 2 * LWS_H * LWS_H, 256), 256, 0, st>>>(P, w.wt);
   k_lws_w<<<nblocks(LWS_L * 2 * LWS_H * LWS_H, 256), 256, 0, st>>>(P, w.w);
-  k_lws_in<<<nblocks(M * LWS_H, 256), 256, 0, st>>>(M, w.x, P, w.h[0]);
+  k_lws_in<<<nblocks(M, 256), 256, 0, st>>>(M, w.x, P, w.h[0]);
   RSB_LAUNCH_CHECK();
   for (int l = 0; l < LWS_L; l++) {
     k_lws_agg<<<nblocks(M * 8, 256), 256, 0, st>>>(M, g->d_slice_off, g->d_slice_w, g->d_ell,
