@@ -173,6 +173,10 @@ int rsbgpu_get_fiedler(rsbgpu *g, double *fiedler_host);
  * vertex (gather-scatter) form, which FP32 Lanczos uses (0 unless
  * PARRSB_AMD_VTILES=1); for the parity tests. */
 int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_tiles);
+/* 1 if the level's tiles also carry the 12-bit form (indices ordered by
+ * weight class; every weight 1, 2 or 4), read by the tiled Lanczos apply
+ * unless PARRSB_AMD_E12=0; else 0. */
+int rsbgpu_level_e12(rsbgpu *g);
 /* Overwrite the Fiedler value of every compact position (host array of nel
  * values): lets the parity tests drive the balance step with chosen values
  * (ties, either sender) through rsbgpu_bisect. */

@@ -133,6 +133,7 @@ SIGNATURES = {
     "rsbgpu_set_profiling": (I, [P, I]),
     "rsbgpu_test_tqli": (I, [P, P, I, P, P]),
     "rsbgpu_stream_test": (I, [I, C.c_size_t, I, P]),
+    "rsbgpu_level_e12": (I, [P]),
     "rsbgpu_test_eigmin": (I, [P, P, I, P, P]),
     "rsbgpu_set_mesh": (I, [P, P, P, I, I, I]),
     "rsbgpu_graph_stats": (I, [P, P, P]),
@@ -568,6 +569,10 @@ class Device:
         v = np.zeros(6)
         self._chk(lib().rsbgpu_lanczos_traffic(self.h, _p(v)))
         return v
+
+    def level_e12(self):
+        """True if the level's tiles carry the 12-bit form."""
+        return lib().rsbgpu_level_e12(self.h) == 1
 
     def level_form(self):
         """(tiles on the global path, tiles with the vertex form)."""

@@ -305,6 +305,11 @@ struct tiled_args {
   const int64_t *off16;
   const int *w16;
   const uint16_t *e16;
+  // the 12-bit form (tiles.hip), or e12 == nullptr
+  const int64_t *off12;
+  const int *w12;
+  const uint32_t *e12;
+  const uint8_t *c12;
 };
 
 #define TB_LOCAL 4096
@@ -450,6 +455,101 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
   }
 }
 
+// 12-bit index k (0..20) of a 32-byte block held in eight words
+template <int K>
+__device__ inline unsigned e12_at(const unsigned (&wd)[8]) {
+  constexpr int bit = 12 * K, i = bit / 32, sh = bit % 32;
+  if constexpr (sh <= 20) return (wd[i] >> sh) & 0xfffu;
+  else return __builtin_amdgcn_alignbit(wd[i + 1], wd[i], sh) & 0xfffu;
+}
+
+// One slot of the 12-bit form: its weight from the row's class counts
+// (slots ordered 4s, 2s, 1s; n42 = n4 + n2).
+template <typename T>
+__device__ inline void acc12(unsigned idx, int s, int n4, int n42, T qi, const T *ql, T &sum) {
+  const T w = s < n4 ? (T)4 : (s < n42 ? (T)2 : (T)1);
+  sum += w * (qi - ql[idx]);
+}
+
+template <typename T, int K0, int N>
+__device__ inline void acc12_words(const unsigned (&wd)[8], int s0, int n4, int n42, T qi,
+                                   const T *ql, T &sum) {
+  if constexpr (N > 0) {
+    acc12(e12_at<K0>(wd), s0 + K0, n4, n42, qi, ql, sum);
+    acc12_words<T, K0 + 1, N - 1>(wd, s0, n4, n42, qi, ql, sum);
+  }
+}
+
+// The tiled rows on the 12-bit form: R rows per thread and step, every
+// word of their blocks and tails loaded before any is summed.  The 64 rows
+// of a wave are one slice, so block and tail counts are uniform in a wave.
+template <typename T, int R, bool FOLD>
+__device__ inline void apply_rows_e12(const tile_t &t, const tiled_args &ta, const T *ql,
+                                      const T *rl, T *__restrict__ pnew, T *__restrict__ w,
+                                      int nreal, acc_t<T> &sq, acc_t<T> &swq, acc_t<T> &sw,
+                                      fold_t<T> &fd) {
+  for (int row0 = t.row0 + threadIdx.x; row0 < t.row1; row0 += R * blockDim.x) {
+    int rows[R], nb[R], tw[R], cc[R];
+    u32x4 bw[R][2][2], tl[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      rows[k] = row0 + k * blockDim.x;
+      const bool has = rows[k] < t.row1;
+      const int sl = (has ? rows[k] : row0) >> 6, lane = rows[k] & 63;
+      const int info = has ? ta.w12[sl] : 0;
+      nb[k] = info & 0xff, tw[k] = info >> 8;
+      const uint32_t *base = ta.e12 + ta.off12[sl];
+      cc[k] = has ? ta.c12[rows[k]] : 0;
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+          bw[k][b][h] = b < nb[k] ? __builtin_nontemporal_load(
+                                        (const u32x4 *)(base + (size_t)(2 * b + h) * 256 + lane * 4))
+                                  : (u32x4){0u, 0u, 0u, 0u};
+      const uint32_t *tail = base + (size_t)nb[k] * 512;
+      if (tw[k] == 4)
+        tl[k] = __builtin_nontemporal_load((const u32x4 *)(tail + lane * 4));
+      else if (tw[k] == 2) {
+        const u32x2 v = __builtin_nontemporal_load((const u32x2 *)(tail + lane * 2));
+        tl[k] = (u32x4){v.x, v.y, 0u, 0u};
+      } else {
+        tl[k] = (u32x4){0u, 0u, 0u, 0u};
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      if (rows[k] >= t.row1) continue;
+      const int li = rows[k] - t.row0;
+      const T qi = ql[li];
+      const int n4 = cc[k] & 15, n42 = n4 + (cc[k] >> 4);
+      T sum = (T)0;
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+        if (b < nb[k]) {
+          const unsigned wd[8] = {bw[k][b][0].x, bw[k][b][0].y, bw[k][b][0].z, bw[k][b][0].w,
+                                  bw[k][b][1].x, bw[k][b][1].y, bw[k][b][1].z, bw[k][b][1].w};
+          acc12_words<T, 0, 21>(wd, 21 * b, n4, n42, qi, ql, sum);
+        }
+      if (tw[k]) {
+        const unsigned wd[8] = {tl[k].x, tl[k].y, tl[k].z, tl[k].w, 0u, 0u, 0u, 0u};
+        if (tw[k] == 4) acc12_words<T, 0, 10>(wd, 21 * nb[k], n4, n42, qi, ql, sum);
+        else acc12_words<T, 0, 5>(wd, 21 * nb[k], n4, n42, qi, ql, sum);
+      }
+      pnew[rows[k]] = qi;
+      w[rows[k]] = sum;
+      const T ri = FOLD ? rl[li] : (T)0;
+      if (FOLD && fd.rn) fd.rn[rows[k]] = ri;
+      if (li < nreal) {
+        sq.add(qi);
+        swq.dot(sum, qi);
+        sw.add(sum);
+        if (FOLD) fd.rr.dot(ri, ri), fd.rw.dot(ri, sum), fd.ww.dot(sum, sum);
+      }
+    }
+  }
+}
+
 // Stages q = beta (p_old - m) + r (q = r on the first iteration of a pass)
 // for the tile's rows and its nh halo rows into LDS (tile rows first), and r
 // of the tile's rows; rows past the segment's end get q = 0.  SU indices per
@@ -509,7 +609,10 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
                                      m, fd.wp, fd.alpha);
   // two rows per thread and step (measured 15% faster than one at a time
   // on the 256^3 workload: more loads in flight per lane)
-  apply_rows_batched<T, 2, FOLD>(t, ta, ql, rl, pnew, w, nreal, sq, swq, sw, fd);
+  if (ta.e12)
+    apply_rows_e12<T, 2, FOLD>(t, ta, ql, rl, pnew, w, nreal, sq, swq, sw, fd);
+  else
+    apply_rows_batched<T, 2, FOLD>(t, ta, ql, rl, pnew, w, nreal, sq, swq, sw, fd);
 }
 
 // The residuals of a pass live in the basis, unscaled: r_it in slot it
@@ -1616,8 +1719,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   lz_args ai = a;  // iteration launches over the running lists
   auto xgrid = [](int n) { return (unsigned)((n + LZ_XCDS - 1) / LZ_XCDS * LZ_XCDS); };
   ai.act_tiles = act_t, ai.act_segs = act_s;
-  tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell, g->d_tnhalo, g->d_thalo_off,
-                g->d_halo,      g->d_off16,   g->d_w16, g->d_e16};
+  tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell,  g->d_tnhalo,
+                g->d_thalo_off, g->d_halo,    g->d_off16, g->d_w16,
+                g->d_e16,       g->d_off12,   g->d_w12,   g->e12_ok ? g->d_e12 : nullptr,
+                g->d_c12};
   vtile_args va{g->d_vt_n,  g->d_vt_loff,    g->d_vt_list, g->d_vt_rowv,
                 g->d_tnhalo, g->d_thalo_off, g->d_halo,    g->nv};
   // the vertex form serves the FP32 path (vectors of T = float)

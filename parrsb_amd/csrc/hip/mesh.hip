@@ -99,6 +99,11 @@ extern "C" void rsbgpu_close(rsbgpu *g) {
   FREE(g->d_thalo_off);
   FREE(g->d_off16);
   FREE(g->d_e16);
+  FREE(g->d_w12);
+  FREE(g->d_e12_bad);
+  FREE(g->d_off12);
+  FREE(g->d_e12);
+  FREE(g->d_c12);
   FREE(g->d_seg_nnz);
   FREE(g->d_vt_n);
   FREE(g->d_vt_loff);

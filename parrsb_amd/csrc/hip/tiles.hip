@@ -15,6 +15,19 @@
 // at (w / 8) * 512 + l * 4 + (j % 4), one 8-byte load per lane; then a group
 // of 2, if w % 4 == 2, after it (l * 2 + (j % 2)), one 4-byte load per lane.
 // (A hex row has 26 neighbours: 3 groups of 8 and one of 2, no padding.)
+//
+// The same entries in 12 bits: on a conforming mesh every weight (shared
+// vertices) is 4 (face), 2 (edge) or 1 (corner), so a row's entries ordered
+// by weight class (4s, 2s, 1s; column order within a class) need only their
+// local index, and one byte per row holds the counts of the 4s and 2s.
+// Per slice of width w: w / 21 blocks of 21 indices in 32 bytes per lane
+// (index k of a block at bit 12 k of its eight words; two 16-byte words per
+// lane, row-interleaved like the 16-bit groups), then a tail of 5 indices
+// in 8 bytes or 10 in 16 bytes (more than 10 left over take another block);
+// unused slots hold the row's own index, which adds zero.  A hex row: one
+// block and a tail of 5, 40 bytes against 52.  A level where some weight is
+// not 1, 2 or 4, a class holds more than 15 entries, or a slice is wider
+// than 42 keeps only the 16-bit form (e12_ok = 0).
 #include <stdlib.h>
 
 #include <vector>
@@ -22,6 +35,8 @@
 #include "state.hpp"
 
 #define TB_HASH 8192  // LDS hash slots per tile (power of two)
+#define E12_BLK 21    // 12-bit indices per 32-byte block
+#define E12_MAXW 42   // widest slice with the 12-bit form
 #define TB_LOCAL 4096 // local index range (12 bits)
 
 __device__ inline unsigned tb_hash(unsigned c) { return (c * 2654435761u) & (TB_HASH - 1); }
@@ -108,13 +123,74 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_count(const int *__restrict_
   if (threadIdx.x == 0) nhalo[t] = (ok && (row1 - row0) + total <= TB_LOCAL) ? total : -1;
 }
 
+// Position of entry j of lane `lane` in its slice's 16-bit entries: groups
+// of 8 (row-interleaved 16-byte words), then a group of 4 (8-byte words)
+// and a group of 2 (4-byte words) where the width w8 has them.
+__device__ inline int e16_pos(int j, int lane, int w8) {
+  const int full = w8 & ~7, k = j - full, t4 = (w8 - full) & 4;
+  return j < full ? (j >> 3) * 512 + lane * 8 + (j & 7)
+                  : (k < t4 ? full * 64 + lane * 4 + k : full * 64 + t4 * 64 + lane * 2 + (k - t4));
+}
+
+// The row's 12-bit form from its 16-bit entries o[] (just written by this
+// thread): indices of the weight-4 entries, then weight 2, then weight 1,
+// each in entry order, then the row's own index; packed into the slice's
+// blocks and tail (info = blocks | tail words << 8) and the class counts.
+__device__ inline void fill_e12(int row, int self, int w, int w8, const uint16_t *o, int lane,
+                                uint32_t *__restrict__ base, int info, uint8_t *__restrict__ c12,
+                                int *__restrict__ bad) {
+  const int nb = info & 0xff, tw = info >> 8;
+  if (nb > E12_MAXW / E12_BLK) return;  // a slice too wide: the level keeps the 16-bit form
+  const int slots = nb * E12_BLK + (tw == 2 ? 5 : tw == 4 ? 10 : 0);
+  uint32_t pw[2 * 8 + 4];
+  for (int i = 0; i < 2 * 8 + 4; i++) pw[i] = 0u;
+  int s = 0, cnt[3] = {0, 0, 0};
+  for (int cls = 0; cls < 3; cls++) {
+    const unsigned want = 4u >> cls;
+    for (int j = 0; j < w8; j++) {
+      const uint16_t e = o[e16_pos(j, lane, w8)];
+      const unsigned wt = e >> 12;
+      if (cls == 0 && wt != 0 && wt != 1 && wt != 2 && wt != 4) atomicOr(bad, 1);
+      if (wt != want || j >= w) continue;
+      if (s < slots) {
+        const int b = s / E12_BLK, k = s - b * E12_BLK;
+        const int word = b * 8 + (12 * k) / 32, sh = (12 * k) % 32;
+        const uint32_t v = e & 0xfffu;
+        pw[word] |= v << sh;
+        if (sh > 20) pw[word + 1] |= v >> (32 - sh);
+      }
+      s++, cnt[cls]++;
+    }
+  }
+  if (s > slots || cnt[0] > 15 || cnt[1] > 15) atomicOr(bad, 1);
+  // unused slots: the row's own local index (q_i - q_i adds zero)
+  for (int t = s; t < slots; t++) {
+    const int b = t / E12_BLK, k = t - b * E12_BLK;
+    const int word = b * 8 + (12 * k) / 32, sh = (12 * k) % 32;
+    pw[word] |= (uint32_t)self << sh;
+    if (sh > 20) pw[word + 1] |= (uint32_t)self >> (32 - sh);
+  }
+  for (int b = 0; b < nb; b++)
+    for (int h = 0; h < 2; h++)
+      *(uint4 *)(base + (size_t)(2 * b + h) * 256 + lane * 4) =
+          make_uint4(pw[8 * b + 4 * h], pw[8 * b + 4 * h + 1], pw[8 * b + 4 * h + 2],
+                     pw[8 * b + 4 * h + 3]);
+  uint32_t *tail = base + (size_t)nb * 512;
+  if (tw == 2) *(uint2 *)(tail + lane * 2) = make_uint2(pw[8 * nb], pw[8 * nb + 1]);
+  if (tw == 4)
+    *(uint4 *)(tail + lane * 4) = make_uint4(pw[8 * nb], pw[8 * nb + 1], pw[8 * nb + 2],
+                                             pw[8 * nb + 3]);
+  c12[row] = (uint8_t)(cnt[0] | (cnt[1] << 4));
+}
+
 // Pass B: halo lists (in slot order) and the 16-bit entries.
 __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     const int *__restrict__ tile, const int64_t *__restrict__ slice_off,
     const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
     const int *__restrict__ nhalo, const int64_t *__restrict__ halo_off,
     const int64_t *__restrict__ off16, const int *__restrict__ w16, int *__restrict__ halo,
-    uint16_t *__restrict__ e16) {
+    uint16_t *__restrict__ e16, const int64_t *__restrict__ off12, const int *__restrict__ w12,
+    uint32_t *__restrict__ e12, uint8_t *__restrict__ c12, int *__restrict__ e12_bad) {
   __shared__ int keys[TB_HASH];
   __shared__ short local[TB_HASH];
   const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
@@ -141,7 +217,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     const int sl = row >> 6, lane = row & 63;
     const uint32_t *e = ell + slice_off[sl] + lane;
     uint16_t *o = e16 + off16[sl];
-    const int w = slice_w[sl], w8 = w16[sl], full = w8 & ~7;
+    const int w = slice_w[sl], w8 = w16[sl];
     const uint16_t self = (uint16_t)(row - row0);
     for (int j = 0; j < w8; j++) {
       uint16_t v = self;  // padding: own row, weight 0
@@ -159,13 +235,9 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
           if (li >= 0) v = (uint16_t)(li | (wt << 12));
         }
       }
-      // groups of 8 entries (row-interleaved 16-byte words), then a group of
-      // 4 (8-byte words) and a group of 2 (4-byte words) where the width has them
-      const int k = j - full, t4 = (w8 - full) & 4;
-      o[j < full ? (j >> 3) * 512 + lane * 8 + (j & 7)
-                 : (k < t4 ? full * 64 + lane * 4 + k : full * 64 + t4 * 64 + lane * 2 + (k - t4))] =
-          v;
+      o[e16_pos(j, lane, w8)] = v;
     }
+    if (e12) fill_e12(row, row - row0, w, w8, o, lane, e12 + off12[sl], w12[sl], c12, e12_bad);
   }
 }
 
@@ -179,6 +251,25 @@ __global__ void k_w16(int64_t nslices, const int *__restrict__ w, int *__restric
   }
   if (i == nslices) bytes[i] = 0;
 }
+
+// Layout of a slice's 12-bit form: blocks and tail words (info), words per
+// slice (all 64 lanes); a slice wider than E12_MAXW marks the level.
+__global__ void k_w12(int64_t nslices, const int *__restrict__ w, int *__restrict__ info,
+                      int64_t *__restrict__ words, int *__restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslices) {
+    const int wd = w[i];
+    int nb = wd / E12_BLK, rem = wd % E12_BLK;
+    if (rem > 10) nb++, rem = 0;
+    const int tw = rem == 0 ? 0 : (rem <= 5 ? 2 : 4);
+    info[i] = nb | (tw << 8);
+    words[i] = (int64_t)(8 * nb + tw) * RSB_WAVE;
+    if (wd > E12_MAXW) atomicOr(bad, 1);
+  }
+  if (i == nslices) words[i] = 0;
+}
+
+__global__ void k_zero_int(int *__restrict__ p) { *p = 0; }
 
 __global__ void k_all_wide(int n, int *__restrict__ nh) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,10 +323,39 @@ int rsb_build_tiles(rsbgpu *g) {
   for (int v : nhv) g->nwide += v < 0;
   RSB_TRY(dev_alloc(g, g->d_halo, (size_t)nh + 1));
   RSB_TRY(dev_alloc(g, g->d_e16, (size_t)n16 + 8));
+  // the 12-bit form (unless PARRSB_AMD_E12=0)
+  const char *ev = getenv("PARRSB_AMD_E12");
+  const bool want12 = !(ev && atoi(ev) == 0);
+  g->e12_ok = 0;
+  if (want12) {
+    RSB_TRY(dev_alloc(g, g->d_w12, (size_t)ns + 1));
+    RSB_TRY(dev_alloc(g, g->d_off12, (size_t)ns + 1));
+    RSB_TRY(dev_alloc(g, g->d_e12_bad, 1));
+    k_zero_int<<<1, 1, 0, g->stream>>>(g->d_e12_bad);
+    k_w12<<<nblocks(ns + 1, 256), 256, 0, g->stream>>>(ns, g->d_slice_w, g->d_w12, tmp,
+                                                       g->d_e12_bad);
+    RSB_LAUNCH_CHECK();
+    rc = rsb_exclusive_scan_i64(g, tmp, g->d_off12, ns + 1);
+    if (rc) return rc;
+    int64_t n12 = 0;
+    RSB_TRY(hipMemcpyAsync(&n12, g->d_off12 + ns, sizeof(int64_t), hipMemcpyDeviceToHost,
+                           g->stream));
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    RSB_TRY(dev_alloc(g, g->d_e12, (size_t)n12 + 8));
+    RSB_TRY(dev_alloc(g, g->d_c12, (size_t)g->mpad + 1));
+  }
   k_tile_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w, g->d_ell,
                                                 g->d_tnhalo, g->d_thalo_off, g->d_off16,
-                                                g->d_w16, g->d_halo, g->d_e16);
+                                                g->d_w16, g->d_halo, g->d_e16, g->d_off12,
+                                                g->d_w12, want12 ? g->d_e12 : nullptr, g->d_c12,
+                                                g->d_e12_bad);
   RSB_LAUNCH_CHECK();
+  if (want12) {
+    int bad = 1;
+    RSB_TRY(hipMemcpyAsync(&bad, g->d_e12_bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    RSB_TRY(hipStreamSynchronize(g->stream));
+    g->e12_ok = bad == 0;
+  }
   RSB_TRY(hipStreamSynchronize(g->stream));
   return 0;
 }
@@ -482,3 +602,7 @@ extern "C" int rsbgpu_level_form(rsbgpu *g, int *wide_tiles, int *vertex_tiles) 
   for (int v : vn) *vertex_tiles += v >= 0;
   return 0;
 }
+
+// 1 if the level's tiles carry the 12-bit form (every weight 1, 2 or 4),
+// else 0
+extern "C" int rsbgpu_level_e12(rsbgpu *g) { return g->e12_ok; }
