@@ -309,7 +309,7 @@ struct tiled_args {
   const int64_t *off12;
   const int *w12;
   const uint32_t *e12;
-  const uint8_t *c12;
+  const int *c12;
 };
 
 #define TB_LOCAL 4096
@@ -463,20 +463,27 @@ __device__ inline unsigned e12_at(const unsigned (&wd)[8]) {
   else return __builtin_amdgcn_alignbit(wd[i + 1], wd[i], sh) & 0xfffu;
 }
 
-// One slot of the 12-bit form: its weight from the row's class counts
-// (slots ordered 4s, 2s, 1s; n42 = n4 + n2).
+// One slot of the 12-bit form: its weight 2^e from the slice's class
+// bounds (slots ordered 4s, 2s, 1s; n42 = n4 + n2; both uniform in the
+// wave, so e is scalar work).  Scaling by 2^e is exact: the sum rounds as
+// sum + w (qi - q) would.
+__device__ inline float ldexp_t(float x, int e) { return __builtin_ldexpf(x, e); }
+__device__ inline double ldexp_t(double x, int e) { return __builtin_ldexp(x, e); }
+// m4 and m42 hold bit s for the slots s < n4 and s < n42 (shifts and masks
+// of wave-uniform words: scalar work).
 template <typename T>
-__device__ inline void acc12(unsigned idx, int s, int n4, int n42, T qi, const T *ql, T &sum) {
-  const T w = s < n4 ? (T)4 : (s < n42 ? (T)2 : (T)1);
-  sum += w * (qi - ql[idx]);
+__device__ inline void acc12(unsigned idx, int s, uint64_t m4, uint64_t m42, T qi, const T *ql,
+                             T &sum) {
+  const int e = (int)((m4 >> s) & 1u) + (int)((m42 >> s) & 1u);
+  sum += ldexp_t(qi - ql[idx], e);
 }
 
 template <typename T, int K0, int N>
-__device__ inline void acc12_words(const unsigned (&wd)[8], int s0, int n4, int n42, T qi,
-                                   const T *ql, T &sum) {
+__device__ inline void acc12_words(const unsigned (&wd)[8], int s0, uint64_t m4, uint64_t m42,
+                                   T qi, const T *ql, T &sum) {
   if constexpr (N > 0) {
-    acc12(e12_at<K0>(wd), s0 + K0, n4, n42, qi, ql, sum);
-    acc12_words<T, K0 + 1, N - 1>(wd, s0, n4, n42, qi, ql, sum);
+    acc12(e12_at<K0>(wd), s0 + K0, m4, m42, qi, ql, sum);
+    acc12_words<T, K0 + 1, N - 1>(wd, s0, m4, m42, qi, ql, sum);
   }
 }
 
@@ -499,7 +506,7 @@ __device__ inline void apply_rows_e12(const tile_t &t, const tiled_args &ta, con
       const int info = has ? ta.w12[sl] : 0;
       nb[k] = info & 0xff, tw[k] = info >> 8;
       const uint32_t *base = ta.e12 + ta.off12[sl];
-      cc[k] = has ? ta.c12[rows[k]] : 0;
+      cc[k] = has ? ta.c12[sl] : 0;
 #pragma unroll
       for (int b = 0; b < 2; b++)
 #pragma unroll
@@ -522,19 +529,22 @@ __device__ inline void apply_rows_e12(const tile_t &t, const tiled_args &ta, con
       if (rows[k] >= t.row1) continue;
       const int li = rows[k] - t.row0;
       const T qi = ql[li];
-      const int n4 = cc[k] & 15, n42 = n4 + (cc[k] >> 4);
+      // the slice's class bounds: every active lane holds the same value
+      const int c = __builtin_amdgcn_readfirstlane(cc[k]);
+      const int n4 = c & 0xff, n42 = n4 + ((c >> 8) & 0xff);  // at most 42 each
+      const uint64_t m4 = (1ull << n4) - 1u, m42 = (1ull << n42) - 1u;
       T sum = (T)0;
 #pragma unroll
       for (int b = 0; b < 2; b++)
         if (b < nb[k]) {
           const unsigned wd[8] = {bw[k][b][0].x, bw[k][b][0].y, bw[k][b][0].z, bw[k][b][0].w,
                                   bw[k][b][1].x, bw[k][b][1].y, bw[k][b][1].z, bw[k][b][1].w};
-          acc12_words<T, 0, 21>(wd, 21 * b, n4, n42, qi, ql, sum);
+          acc12_words<T, 0, 21>(wd, 21 * b, m4, m42, qi, ql, sum);
         }
       if (tw[k]) {
         const unsigned wd[8] = {tl[k].x, tl[k].y, tl[k].z, tl[k].w, 0u, 0u, 0u, 0u};
-        if (tw[k] == 4) acc12_words<T, 0, 10>(wd, 21 * nb[k], n4, n42, qi, ql, sum);
-        else acc12_words<T, 0, 5>(wd, 21 * nb[k], n4, n42, qi, ql, sum);
+        if (tw[k] == 4) acc12_words<T, 0, 10>(wd, 21 * nb[k], m4, m42, qi, ql, sum);
+        else acc12_words<T, 0, 5>(wd, 21 * nb[k], m4, m42, qi, ql, sum);
       }
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
@@ -1719,9 +1729,13 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   lz_args ai = a;  // iteration launches over the running lists
   auto xgrid = [](int n) { return (unsigned)((n + LZ_XCDS - 1) / LZ_XCDS * LZ_XCDS); };
   ai.act_tiles = act_t, ai.act_segs = act_s;
+  // The FP64 apply reads the 16-bit entries: with the 12-bit form it took
+  // 0.253 against 0.233 ms per launch on the 256^3 box (FP32: 0.163 against
+  // 0.177), the class sums in doubles costing more than the bytes they save.
+  const bool use12 = g->e12_ok && (sizeof(T) == 4 || g->e12_f64);
   tiled_args ta{g->d_slice_off, g->d_slice_w, g->d_ell,  g->d_tnhalo,
                 g->d_thalo_off, g->d_halo,    g->d_off16, g->d_w16,
-                g->d_e16,       g->d_off12,   g->d_w12,   g->e12_ok ? g->d_e12 : nullptr,
+                g->d_e16,       g->d_off12,   g->d_w12,   use12 ? g->d_e12 : nullptr,
                 g->d_c12};
   vtile_args va{g->d_vt_n,  g->d_vt_loff,    g->d_vt_list, g->d_vt_rowv,
                 g->d_tnhalo, g->d_thalo_off, g->d_halo,    g->nv};

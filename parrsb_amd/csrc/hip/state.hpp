@@ -98,14 +98,16 @@ struct rsbgpu {
   uint16_t *d_e16 = nullptr;
   int64_t nhalo_total = 0, e16_count = 0;
   // the same entries as 12-bit local indices ordered by weight class
-  // (tiles.hip): per slice its block count and tail words, word offsets,
-  // the packed words, and per row the counts of weight-4 and weight-2
-  // entries; e12_ok = 0 when a weight is not 1, 2 or 4
+  // (tiles.hip): per slice its block count and tail words, its slots per
+  // class (n4 | n2 << 8 | n1 << 16, the same for its 64 rows), word offsets
+  // and the packed words; e12_ok = 0 when a weight is not 1, 2 or 4 or a
+  // slice needs more than 42 slots.  The FP32 solve
+  // reads this form; the FP64 solve reads the 16-bit entries unless e12_f64
   int *d_w12 = nullptr, *d_e12_bad = nullptr;
   int64_t *d_off12 = nullptr;
   uint32_t *d_e12 = nullptr;
-  uint8_t *d_c12 = nullptr;
-  int e12_ok = 0;
+  int *d_c12 = nullptr;
+  int e12_ok = 0, e12_f64 = 0;
   int nwide = 0;
   // vertex form of the tiles (tiles.hip, rsb_build_vtiles): per tile, the
   // distinct vertex ids of its rows, for each the nv window-local indices of

@@ -19,15 +19,17 @@
 // The same entries in 12 bits: on a conforming mesh every weight (shared
 // vertices) is 4 (face), 2 (edge) or 1 (corner), so a row's entries ordered
 // by weight class (4s, 2s, 1s; column order within a class) need only their
-// local index, and one byte per row holds the counts of the 4s and 2s.
-// Per slice of width w: w / 21 blocks of 21 indices in 32 bytes per lane
-// (index k of a block at bit 12 k of its eight words; two 16-byte words per
-// lane, row-interleaved like the 16-bit groups), then a tail of 5 indices
-// in 8 bytes or 10 in 16 bytes (more than 10 left over take another block);
-// unused slots hold the row's own index, which adds zero.  A hex row: one
-// block and a tail of 5, 40 bytes against 52.  A level where some weight is
-// not 1, 2 or 4, a class holds more than 15 entries, or a slice is wider
-// than 42 keeps only the 16-bit form (e12_ok = 0).
+// local index.  Each class has the same number of slots in every row of a
+// slice (the largest count among its 64 rows; per slice n4 | n2 << 8 |
+// n1 << 16), so the weight of a slot is known to the whole wave.  Per slice
+// of S = n4 + n2 + n1 slots: S / 21 blocks of 21 indices in 32 bytes per
+// lane (index k of a block at bit 12 k of its eight words; two 16-byte words
+// per lane, row-interleaved like the 16-bit groups), then a tail of 5
+// indices in 8 bytes or 10 in 16 bytes (more than 10 left over take another
+// block); unused slots hold the row's own index, which adds zero.  A hex
+// row: one block and a tail of 5, 40 bytes against 52.  A level where some
+// weight is not 1, 2 or 4 or a slice needs more than 42 slots keeps only
+// the 16-bit form (e12_ok = 0).
 #include <stdlib.h>
 
 #include <vector>
@@ -132,55 +134,18 @@ __device__ inline int e16_pos(int j, int lane, int w8) {
                   : (k < t4 ? full * 64 + lane * 4 + k : full * 64 + t4 * 64 + lane * 2 + (k - t4));
 }
 
-// The row's 12-bit form from its 16-bit entries o[] (just written by this
-// thread): indices of the weight-4 entries, then weight 2, then weight 1,
-// each in entry order, then the row's own index; packed into the slice's
-// blocks and tail (info = blocks | tail words << 8) and the class counts.
-__device__ inline void fill_e12(int row, int self, int w, int w8, const uint16_t *o, int lane,
-                                uint32_t *__restrict__ base, int info, uint8_t *__restrict__ c12,
-                                int *__restrict__ bad) {
-  const int nb = info & 0xff, tw = info >> 8;
-  if (nb > E12_MAXW / E12_BLK) return;  // a slice too wide: the level keeps the 16-bit form
-  const int slots = nb * E12_BLK + (tw == 2 ? 5 : tw == 4 ? 10 : 0);
-  uint32_t pw[2 * 8 + 4];
-  for (int i = 0; i < 2 * 8 + 4; i++) pw[i] = 0u;
-  int s = 0, cnt[3] = {0, 0, 0};
-  for (int cls = 0; cls < 3; cls++) {
-    const unsigned want = 4u >> cls;
-    for (int j = 0; j < w8; j++) {
-      const uint16_t e = o[e16_pos(j, lane, w8)];
-      const unsigned wt = e >> 12;
-      if (cls == 0 && wt != 0 && wt != 1 && wt != 2 && wt != 4) atomicOr(bad, 1);
-      if (wt != want || j >= w) continue;
-      if (s < slots) {
-        const int b = s / E12_BLK, k = s - b * E12_BLK;
-        const int word = b * 8 + (12 * k) / 32, sh = (12 * k) % 32;
-        const uint32_t v = e & 0xfffu;
-        pw[word] |= v << sh;
-        if (sh > 20) pw[word + 1] |= v >> (32 - sh);
-      }
-      s++, cnt[cls]++;
-    }
+// 21 slots (stride RSB_BLOCK in LDS) into the eight words of a block:
+// index k at bit 12 k.  Unrolled, so the words stay in registers.
+__device__ inline void pack21(const uint16_t *my, uint32_t (&pw)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) pw[i] = 0u;
+#pragma unroll
+  for (int k = 0; k < E12_BLK; k++) {
+    const uint32_t v = my[k * RSB_BLOCK];
+    const int word = (12 * k) / 32, sh = (12 * k) % 32;
+    pw[word] |= v << sh;
+    if (sh > 20) pw[word + 1] |= v >> (32 - sh);
   }
-  if (s > slots || cnt[0] > 15 || cnt[1] > 15) atomicOr(bad, 1);
-  // unused slots: the row's own local index (q_i - q_i adds zero)
-  for (int t = s; t < slots; t++) {
-    const int b = t / E12_BLK, k = t - b * E12_BLK;
-    const int word = b * 8 + (12 * k) / 32, sh = (12 * k) % 32;
-    pw[word] |= (uint32_t)self << sh;
-    if (sh > 20) pw[word + 1] |= (uint32_t)self >> (32 - sh);
-  }
-  for (int b = 0; b < nb; b++)
-    for (int h = 0; h < 2; h++)
-      *(uint4 *)(base + (size_t)(2 * b + h) * 256 + lane * 4) =
-          make_uint4(pw[8 * b + 4 * h], pw[8 * b + 4 * h + 1], pw[8 * b + 4 * h + 2],
-                     pw[8 * b + 4 * h + 3]);
-  uint32_t *tail = base + (size_t)nb * 512;
-  if (tw == 2) *(uint2 *)(tail + lane * 2) = make_uint2(pw[8 * nb], pw[8 * nb + 1]);
-  if (tw == 4)
-    *(uint4 *)(tail + lane * 4) = make_uint4(pw[8 * nb], pw[8 * nb + 1], pw[8 * nb + 2],
-                                             pw[8 * nb + 3]);
-  c12[row] = (uint8_t)(cnt[0] | (cnt[1] << 4));
 }
 
 // Pass B: halo lists (in slot order) and the 16-bit entries.
@@ -189,8 +154,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
     const int *__restrict__ nhalo, const int64_t *__restrict__ halo_off,
     const int64_t *__restrict__ off16, const int *__restrict__ w16, int *__restrict__ halo,
-    uint16_t *__restrict__ e16, const int64_t *__restrict__ off12, const int *__restrict__ w12,
-    uint32_t *__restrict__ e12, uint8_t *__restrict__ c12, int *__restrict__ e12_bad) {
+    uint16_t *__restrict__ e16) {
   __shared__ int keys[TB_HASH];
   __shared__ short local[TB_HASH];
   const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
@@ -237,7 +201,64 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
       }
       o[e16_pos(j, lane, w8)] = v;
     }
-    if (e12) fill_e12(row, row - row0, w, w8, o, lane, e12 + off12[sl], w12[sl], c12, e12_bad);
+  }
+}
+
+// Pass C: the 12-bit form from a tile's 16-bit entries, a thread per row:
+// the row's local indices class by class (4s, 2s, 1s, each class in its
+// region of the slice's size for it, the rest of the region set to the
+// row's own index) into its LDS slots, then packed 21 to a block.
+#define E12_SLOTS (3 * E12_BLK)
+__global__ void __launch_bounds__(RSB_BLOCK) k_tile_e12(
+    const int *__restrict__ tile, const int *__restrict__ slice_w,
+    const int64_t *__restrict__ off16, const int *__restrict__ w16,
+    const uint16_t *__restrict__ e16, const int64_t *__restrict__ off12,
+    const int *__restrict__ w12, const int *__restrict__ c12, uint32_t *__restrict__ e12,
+    int *__restrict__ bad) {
+  __shared__ uint16_t lst[E12_SLOTS * RSB_BLOCK];
+  uint16_t *my = lst + threadIdx.x;  // slot k at my[k * RSB_BLOCK]
+  const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
+  for (int row = row0 + threadIdx.x; row < row1; row += blockDim.x) {
+    const int sl = row >> 6, lane = row & 63;
+    const int info = w12[sl], nb = info & 0xff, tw = info >> 8;
+    if (nb > E12_MAXW / E12_BLK) continue;  // too many slots (k_w12 flagged it)
+    const uint16_t *o = e16 + off16[sl];
+    const int w = slice_w[sl], w8 = w16[sl], cw = c12[sl];
+    const uint16_t self = (uint16_t)(row - row0);
+    int s0 = 0, over = 0;
+#pragma unroll
+    for (int cls = 0; cls < 3; cls++) {
+      const unsigned want = 4u >> cls;
+      const int end = s0 + ((cw >> (8 * cls)) & 0xff);
+      int s = s0;
+      for (int j = 0; j < w; j++) {
+        const uint16_t e = o[e16_pos(j, lane, w8)];
+        if ((unsigned)(e >> 12) != want) continue;
+        if (s < end && s < E12_SLOTS) my[s * RSB_BLOCK] = e & 0xfffu;
+        s++;
+      }
+      over |= s > end;
+      for (; s < end && s < E12_SLOTS; s++) my[s * RSB_BLOCK] = self;
+      s0 = end;
+    }
+    if (over) atomicOr(bad, 1);
+    const int upto = (nb + 1) * E12_BLK;  // the tail packs from a whole group of slots
+    for (int k = s0; k < upto && k < E12_SLOTS; k++) my[k * RSB_BLOCK] = self;
+    uint32_t *base = e12 + off12[sl];
+    for (int b = 0; b < nb; b++) {
+      uint32_t pw[8];
+      pack21(my + b * E12_BLK * RSB_BLOCK, pw);
+      *(uint4 *)(base + (size_t)(2 * b) * 256 + lane * 4) = make_uint4(pw[0], pw[1], pw[2], pw[3]);
+      *(uint4 *)(base + (size_t)(2 * b + 1) * 256 + lane * 4) =
+          make_uint4(pw[4], pw[5], pw[6], pw[7]);
+    }
+    if (tw) {
+      uint32_t pw[8];
+      pack21(my + nb * E12_BLK * RSB_BLOCK, pw);
+      uint32_t *tail = base + (size_t)nb * 512;
+      if (tw == 2) *(uint2 *)(tail + lane * 2) = make_uint2(pw[0], pw[1]);
+      else *(uint4 *)(tail + lane * 4) = make_uint4(pw[0], pw[1], pw[2], pw[3]);
+    }
   }
 }
 
@@ -252,21 +273,50 @@ __global__ void k_w16(int64_t nslices, const int *__restrict__ w, int *__restric
   if (i == nslices) bytes[i] = 0;
 }
 
-// Layout of a slice's 12-bit form: blocks and tail words (info), words per
-// slice (all 64 lanes); a slice wider than E12_MAXW marks the level.
-__global__ void k_w12(int64_t nslices, const int *__restrict__ w, int *__restrict__ info,
-                      int64_t *__restrict__ words, int *__restrict__ bad) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nslices) {
-    const int wd = w[i];
-    int nb = wd / E12_BLK, rem = wd % E12_BLK;
+// Per slice, a wave of 64 threads (one per row): the largest count of
+// weight-4, weight-2 and weight-1 entries over its rows, c12 = n4 | n2 << 8 |
+// n1 << 16 (each class gets that many slots in every row of the slice, so
+// the class bounds are uniform in a wave), and the layout of its 12-bit form
+// for n4 + n2 + n1 slots: blocks and tail words (info = nb | tw << 8), words
+// for all 64 lanes.  A weight other than 0, 1, 2 or 4, or more than
+// E12_MAXW slots, marks the level.
+__global__ void __launch_bounds__(256) k_w12(int64_t nslices, const int64_t *__restrict__ slice_off,
+                                             const int *__restrict__ slice_w,
+                                             const uint32_t *__restrict__ ell,
+                                             int *__restrict__ c12, int *__restrict__ info,
+                                             int64_t *__restrict__ words, int *__restrict__ bad) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t sl = row >> 6;  // uniform in a wave
+  const int lane = (int)(row & 63);
+  if (sl > nslices) return;
+  if (sl == nslices) {
+    if (lane == 0) words[sl] = 0;
+    return;
+  }
+  const uint32_t *e = ell + slice_off[sl] + lane;
+  const int w = slice_w[sl];
+  int c4 = 0, c2 = 0, c1 = 0, odd = 0;
+  for (int j = 0; j < w; j++) {
+    const unsigned wt = e[(size_t)j * RSB_WAVE] >> 28;
+    c4 += wt == 4, c2 += wt == 2, c1 += wt == 1;
+    odd |= wt != 0 && wt != 1 && wt != 2 && wt != 4;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    c4 = max(c4, __shfl_xor(c4, o));
+    c2 = max(c2, __shfl_xor(c2, o));
+    c1 = max(c1, __shfl_xor(c1, o));
+    odd = max(odd, __shfl_xor(odd, o));
+  }
+  if (lane == 0) {
+    const int slots = c4 + c2 + c1;
+    int nb = slots / E12_BLK, rem = slots % E12_BLK;
     if (rem > 10) nb++, rem = 0;
     const int tw = rem == 0 ? 0 : (rem <= 5 ? 2 : 4);
-    info[i] = nb | (tw << 8);
-    words[i] = (int64_t)(8 * nb + tw) * RSB_WAVE;
-    if (wd > E12_MAXW) atomicOr(bad, 1);
+    info[sl] = nb | (tw << 8);
+    words[sl] = (int64_t)(8 * nb + tw) * RSB_WAVE;
+    c12[sl] = c4 | (c2 << 8) | (c1 << 16);
+    if (odd || slots > E12_MAXW) atomicOr(bad, 1);
   }
-  if (i == nslices) words[i] = 0;
 }
 
 __global__ void k_zero_int(int *__restrict__ p) { *p = 0; }
@@ -323,17 +373,20 @@ int rsb_build_tiles(rsbgpu *g) {
   for (int v : nhv) g->nwide += v < 0;
   RSB_TRY(dev_alloc(g, g->d_halo, (size_t)nh + 1));
   RSB_TRY(dev_alloc(g, g->d_e16, (size_t)n16 + 8));
-  // the 12-bit form (unless PARRSB_AMD_E12=0)
+  // the 12-bit form (unless PARRSB_AMD_E12=0; =2 lets the FP64 solve read
+  // it as well)
   const char *ev = getenv("PARRSB_AMD_E12");
   const bool want12 = !(ev && atoi(ev) == 0);
   g->e12_ok = 0;
+  g->e12_f64 = ev && atoi(ev) == 2;
   if (want12) {
     RSB_TRY(dev_alloc(g, g->d_w12, (size_t)ns + 1));
+    RSB_TRY(dev_alloc(g, g->d_c12, (size_t)ns + 1));
     RSB_TRY(dev_alloc(g, g->d_off12, (size_t)ns + 1));
     RSB_TRY(dev_alloc(g, g->d_e12_bad, 1));
     k_zero_int<<<1, 1, 0, g->stream>>>(g->d_e12_bad);
-    k_w12<<<nblocks(ns + 1, 256), 256, 0, g->stream>>>(ns, g->d_slice_w, g->d_w12, tmp,
-                                                       g->d_e12_bad);
+    k_w12<<<nblocks((ns + 1) * RSB_WAVE, 256), 256, 0, g->stream>>>(
+        ns, g->d_slice_off, g->d_slice_w, g->d_ell, g->d_c12, g->d_w12, tmp, g->d_e12_bad);
     RSB_LAUNCH_CHECK();
     rc = rsb_exclusive_scan_i64(g, tmp, g->d_off12, ns + 1);
     if (rc) return rc;
@@ -342,15 +395,16 @@ int rsb_build_tiles(rsbgpu *g) {
                            g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));
     RSB_TRY(dev_alloc(g, g->d_e12, (size_t)n12 + 8));
-    RSB_TRY(dev_alloc(g, g->d_c12, (size_t)g->mpad + 1));
   }
   k_tile_fill<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_off, g->d_slice_w, g->d_ell,
                                                 g->d_tnhalo, g->d_thalo_off, g->d_off16,
-                                                g->d_w16, g->d_halo, g->d_e16, g->d_off12,
-                                                g->d_w12, want12 ? g->d_e12 : nullptr, g->d_c12,
-                                                g->d_e12_bad);
+                                                g->d_w16, g->d_halo, g->d_e16);
   RSB_LAUNCH_CHECK();
   if (want12) {
+    k_tile_e12<<<nt, RSB_BLOCK, 0, g->stream>>>(g->d_tile, g->d_slice_w, g->d_off16, g->d_w16,
+                                                 g->d_e16, g->d_off12, g->d_w12, g->d_c12,
+                                                 g->d_e12, g->d_e12_bad);
+    RSB_LAUNCH_CHECK();
     int bad = 1;
     RSB_TRY(hipMemcpyAsync(&bad, g->d_e12_bad, sizeof(int), hipMemcpyDeviceToHost, g->stream));
     RSB_TRY(hipStreamSynchronize(g->stream));

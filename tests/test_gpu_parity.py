@@ -307,20 +307,24 @@ def test_lanczos_12bit_entries_equal_16bit_form(oracle, name, monkeypatch):
     """The tiles' 12-bit form (indices ordered by weight class, the class
     counts per row) is the same operator as the 16-bit entries summed in
     another order: FP64 lambda agrees to 1e-11, the vectors to 1e-9 and the
-    iteration counts within one; FP32 as the FP32 path does."""
+    iteration counts within one; FP32 as the FP32 path does.  The FP64 solve
+    reads the 12-bit form only with PARRSB_AMD_E12=2."""
     flush(f"lanczos 12-bit entries {name}")
     vtx, xyz, nv = {"hex": lambda: M.box_hex(40, 30, 20, jitter=0.1, seed=4),
                     "warped": lambda: M.warped_box_hex(40, seed=3),
                     "quad": lambda: M.quad_grid(140, 104)}[name]()
     for prec in (64, 32):
         res = {}
-        for form in ("1", "0"):
+        for form in ("2", "1", "0"):
             monkeypatch.setenv("PARRSB_AMD_E12", form)
             with _segments_after_rcb(vtx, xyz, nv, 2) as d:
                 d.level_laplacian()
-                assert d.level_e12() == (form == "1")
+                assert d.level_e12() == (form != "0")
                 res[form] = (d.fiedler(1, precision=prec), d.fiedler_values())
-        (o12, f12), (o16, f16) = res["1"], res["0"]
+        # the default ("1"): FP32 reads the 12-bit form, FP64 the 16-bit one
+        same = res["0"] if prec == 64 else res["2"]
+        assert np.array_equal(res["1"][1], same[1]), prec
+        (o12, f12), (o16, f16) = res["2"], res["0"]
         lam = o16["lambda"][0]
         assert abs(o12["lambda"][0] - lam) <= (1e-11 if prec == 64 else 1e-5) * abs(lam), prec
         assert abs(abs(f12 @ f16) - 1) < (1e-9 if prec == 64 else 1e-4), prec
