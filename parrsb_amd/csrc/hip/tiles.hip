@@ -3,8 +3,9 @@
 // A tile is the block of up to RSB_TILE consecutive rows one workgroup
 // owns.  Its "halo" is the set of rows outside the tile that its rows
 // reference; the apply kernel loads tile + halo values into LDS once and
-// reads every neighbour from LDS.  Entries are re-encoded relative to that
-// LDS window as 16-bit words: local index (12 bits) | weight (4 bits), so a
+// reads every neighbour from LDS (window slot i < nrows: tile row i; then
+// the halo rows in ascending row order).  Entries are re-encoded relative to
+// that LDS window as 16-bit words: local index (12 bits) | weight (4 bits), so a
 // tile plus its halo may hold at most 4096 rows; a tile whose halo does not
 // fit is marked (nhalo = -1) and its rows use the global 32-bit form.
 //
@@ -30,6 +31,7 @@
 // row: one block and a tail of 5, 40 bytes against 52.  A level where some
 // weight is not 1, 2 or 4 or a slice needs more than 42 slots keeps only
 // the 16-bit form (e12_ok = 0).
+#include <limits.h>
 #include <stdlib.h>
 
 #include <vector>
@@ -148,7 +150,25 @@ __device__ inline void pack21(const uint16_t *my, uint32_t (&pw)[8]) {
   }
 }
 
-// Pass B: halo lists (in slot order) and the 16-bit entries.
+// Ascending bitonic sort of a[0, n) in LDS, n a power of two.
+__device__ inline void lds_bitonic(int *a, int n) {
+  for (int k = 2; k <= n; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int x = a[i], y = a[l];
+          if ((x > y) == ((i & k) == 0)) a[i] = y, a[l] = x;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// Pass B: halo lists and the 16-bit entries.  A tile's halo rows are listed
+// in ascending row order (local index nrows + rank), so the apply's staging
+// gathers them at increasing addresses and neighbouring lanes share cache
+// lines.
 __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     const int *__restrict__ tile, const int64_t *__restrict__ slice_off,
     const int *__restrict__ slice_w, const uint32_t *__restrict__ ell,
@@ -157,9 +177,10 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     uint16_t *__restrict__ e16) {
   __shared__ int keys[TB_HASH];
   __shared__ short local[TB_HASH];
+  __shared__ int srt[TB_LOCAL];
   const int t = blockIdx.x, row0 = tile[3 * t + 1], row1 = tile[3 * t + 2];
   const int nrows = row1 - row0;
-  const bool fits = nhalo[t] >= 0;
+  const bool fits = nhalo[t] >= 0;  // then nrows + halo rows <= TB_LOCAL
   if (fits) {
     tb_collect(keys, row0, row1, slice_off, slice_w, ell);
     const int per = TB_HASH / RSB_BLOCK;
@@ -168,12 +189,18 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_tile_fill(
     int total;
     int base = tb_scan(cnt, &total);
     for (int k = 0; k < per; k++) {
-      int slot = threadIdx.x * per + k;
-      if (keys[slot] >= 0) {
-        local[slot] = (short)(nrows + base);
-        halo[halo_off[t] + base] = keys[slot];
-        base++;
-      }
+      const int slot = threadIdx.x * per + k;
+      if (keys[slot] >= 0) srt[base++] = keys[slot];
+    }
+    int n2 = 1;
+    while (n2 < total) n2 <<= 1;
+    for (int i = total + threadIdx.x; i < n2; i += blockDim.x) srt[i] = INT_MAX;
+    __syncthreads();
+    lds_bitonic(srt, n2);
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+      const int c = srt[i];
+      local[tb_find(keys, c)] = (short)(nrows + i);
+      halo[halo_off[t] + i] = c;
     }
     __syncthreads();
   }
