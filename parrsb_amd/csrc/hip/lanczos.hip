@@ -362,7 +362,6 @@ __device__ inline void apply_row_tiled(int row, const tile_t &t, const tiled_arg
   pnew[row] = qi;
   w[row] = sum;
   const T ri = FOLD ? rl[li] : (T)0;  // the tile's r is kept in LDS only to fold
-  if (FOLD && fd.rn) fd.rn[row] = ri;
   if (li < nreal) {
     sq.add(qi);
     swq.dot(sum, qi);
@@ -444,7 +443,6 @@ __device__ inline void apply_rows_batched(const tile_t &t, const tiled_args &ta,
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
       const T ri = FOLD ? rl[li] : (T)0;
-      if (FOLD && fd.rn) fd.rn[rows[k]] = ri;
       if (li < nreal) {
         sq.add(qi);
         swq.dot(sum, qi);
@@ -549,7 +547,6 @@ __device__ inline void apply_rows_e12(const tile_t &t, const tiled_args &ta, con
       pnew[rows[k]] = qi;
       w[rows[k]] = sum;
       const T ri = FOLD ? rl[li] : (T)0;
-      if (FOLD && fd.rn) fd.rn[rows[k]] = ri;
       if (li < nreal) {
         sq.add(qi);
         swq.dot(sum, qi);
@@ -562,20 +559,49 @@ __device__ inline void apply_rows_e12(const tile_t &t, const tiled_args &ta, con
 
 // Stages q = beta (p_old - m) + r (q = r on the first iteration of a pass)
 // for the tile's rows and its nh halo rows into LDS (tile rows first), and r
-// of the tile's rows; rows past the segment's end get q = 0.  SU indices per
-// thread and step, all their loads issued before any value is used (the halo
-// rows are gathers).  Ends with a barrier.
+// of the tile's rows; rows past the segment's end get q = 0.  FP32 without
+// the folded update: four consecutive tile rows per thread in 16-byte
+// accesses (tiles are whole 64-row slices), then the halo rows.  SU indices
+// per thread and step, all their loads issued before any value is used (the
+// halo rows are gathers).  Ends with a barrier.
 template <typename T, bool FIRST, bool KEEP_R = true, bool FOLD = false>
 __device__ inline void stage_window(const tile_t &t, const int *__restrict__ halo, int nh, T *ql,
                                     T *rl, const T *__restrict__ r, const T *__restrict__ pold,
                                     T beta, T m, const T *__restrict__ wp = nullptr,
-                                    T alpha = (T)0) {
+                                    T alpha = (T)0, T *__restrict__ rn = nullptr) {
   constexpr bool UPD = FOLD && !FIRST;
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
   const int tot = nrows + nh;
   constexpr int SU = 4;
-  for (int i0 = threadIdx.x; i0 < tot; i0 += SU * blockDim.x) {
+  int ibeg = threadIdx.x;
+  if constexpr (sizeof(T) == 4) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const int i = 4 * (int)threadIdx.x;
+    if (i < nrows) {
+      f32x4 rv = *(const f32x4 *)(r + t.row0 + i);
+      if (UPD) {
+        const f32x4 wv = *(const f32x4 *)(wp + t.row0 + i);
+        rv.x = rsub(rv.x, alpha, wv.x), rv.y = rsub(rv.y, alpha, wv.y);
+        rv.z = rsub(rv.z, alpha, wv.z), rv.w = rsub(rv.w, alpha, wv.w);
+        if (rn) *(f32x4 *)(rn + t.row0 + i) = rv;  // r_it to its basis slot
+      }
+      f32x4 q = rv;
+      if (!FIRST) {
+        const f32x4 pv = *(const f32x4 *)(pold + t.row0 + i);
+        q.x = beta * (pv.x - m) + rv.x, q.y = beta * (pv.y - m) + rv.y;
+        q.z = beta * (pv.z - m) + rv.z, q.w = beta * (pv.w - m) + rv.w;
+      }
+      if (i + 0 >= nreal) q.x = 0.f;
+      if (i + 1 >= nreal) q.y = 0.f;
+      if (i + 2 >= nreal) q.z = 0.f;
+      if (i + 3 >= nreal) q.w = 0.f;
+      *(f32x4 *)(ql + i) = q;
+      if (KEEP_R) *(f32x4 *)(rl + i) = rv;
+    }
+    ibeg = nrows + threadIdx.x;
+  }
+  for (int i0 = ibeg; i0 < tot; i0 += SU * blockDim.x) {
     int gr[SU];
     T rg[SU], po[SU], wg[SU];
 #pragma unroll
@@ -599,6 +625,7 @@ __device__ inline void stage_window(const tile_t &t, const int *__restrict__ hal
       T q = FIRST ? rg[k] : beta * (po[k] - m) + rg[k];
       if (idx < nrows) {
         if (KEEP_R) rl[idx] = rg[k];
+        if (UPD && rn) rn[gr[k]] = rg[k];  // r_it to its basis slot
         if (idx >= nreal) q = (T)0;
       }
       ql[idx] = q;
@@ -615,8 +642,14 @@ __device__ inline void lz_apply_tiled(const lz_args &a, const tile_t &t, const t
                                       acc_t<T> &swq, acc_t<T> &sw, fold_t<T> &fd) {
   const int nrows = t.row1 - t.row0;
   const int nreal = min(nrows, t.pstart + t.n - t.row0);
-  stage_window<T, FIRST, true, FOLD>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta,
-                                     m, fd.wp, fd.alpha);
+  // the tile's r in LDS: the folded update's row phase reads it; the FP32
+  // kernel writes it anyway, its 4 KB holding the kernel at seven
+  // workgroups per CU (at eight, with rl dropped, the level-0 apply of the
+  // 256^3 box took 279 us against 269); the FP64 kernel drops it and runs
+  // four workgroups per CU instead of three (0.217 -> 0.209 ms per launch)
+  constexpr bool KEEP_R = FOLD || sizeof(T) == 4;
+  stage_window<T, FIRST, KEEP_R, FOLD>(t, ta.halo + ta.halo_off[t.ti], nh, ql, rl, r, pold, beta,
+                                       m, fd.wp, fd.alpha, fd.rn);
   // two rows per thread and step (measured 15% faster than one at a time
   // on the 256^3 workload: more loads in flight per lane)
   if (ta.e12)
@@ -639,11 +672,9 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply(lz_args a, const T *__re
                                                         T *__restrict__ basis, int64_t mpad,
                                                         tiled_args ta, int nlist,
                                                         const T *__restrict__ wprev) {
-  __shared__ T ql[TB_LOCAL];
-  // the tile's r (read only to fold the update); its 4 KB also hold the
-  // FP32 kernel at seven workgroups per CU: at eight (16.5 KB of LDS) the
-  // level-0 apply of the 256^3 box measured 279 us against 269 us
-  __shared__ T rl[RSB_TILE];
+  __shared__ __attribute__((aligned(16))) T ql[TB_LOCAL];
+  // the tile's r (read only to fold the update; see lz_apply_tiled)
+  __shared__ __attribute__((aligned(16))) T rl[RSB_TILE];
   tile_t t;
   if (!tile_of(a, LZ_RUNNING, t, nlist)) return;
   const int s = t.s, it = a.iter[s];
@@ -806,7 +837,7 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_apply_v(lz_args a, const T *__
                                                           const T *__restrict__ basis,
                                                           int64_t mpad, vtile_args va,
                                                           tiled_args ta, int nlist) {
-  __shared__ T ql[VT_MAXW + 1];
+  __shared__ __attribute__((aligned(16))) T ql[VT_MAXW + 1];
   __shared__ T S[VT_MAXV];  // the neighbour-entry fallback keeps r of the tile rows here
   __shared__ uint16_t F[VT_MAXV];
   static_assert(VT_MAXV >= RSB_TILE, "the fallback's r of the tile rows lives in S");
@@ -1330,6 +1361,41 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
   const double *y = a.y + (size_t)s * a.miter_max;
   const double *rn = a.rn + (size_t)s * (a.miter_max + 1);
   double v[1] = {0.0};
+  if constexpr (sizeof(T) == 4) {
+    // FP32: four consecutive rows per thread, one 16-byte load per basis
+    // vector, eight vectors in flight (vectors past k read vector k - 1 and
+    // are skipped); each row sums j ascending, as below
+    static_assert(RSB_TILE == 4 * RSB_BLOCK, "four rows per thread");
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    constexpr int U = 8;
+    const int row = t.row0 + 4 * (int)threadIdx.x;
+    if (row < t.row1) {
+      double x[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int j0 = 0; j0 < k; j0 += U) {
+        f32x4 b[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          b[u] = *(const f32x4 *)(basis + (size_t)min(j0 + u, k - 1) * mpad + row);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (j0 + u < k) {
+            const double yj = y[j0 + u] * rn[j0 + u];
+            x[0] += (double)b[u].x * yj, x[1] += (double)b[u].y * yj;
+            x[2] += (double)b[u].z * yj, x[3] += (double)b[u].w * yj;
+          }
+      }
+      f32x4 o;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float xq = row + q - t.pstart < t.n ? (float)x[q] : 0.f;
+        o[q] = xq;
+        v[0] += (double)xq;
+      }
+      *(f32x4 *)(f + row) = o;
+    }
+    publish<1>(a, t, v);
+    return;
+  }
   // The (up to) four rows of a thread together and eight basis vectors at a
   // time, every load unconditional (rows past the tile read its last row,
   // vectors past k read vector k - 1 and are skipped): 32 loads in flight
