@@ -577,8 +577,7 @@ __device__ inline void stage_window(const tile_t &t, const int *__restrict__ hal
   int ibeg = threadIdx.x;
   if constexpr (sizeof(T) == 4) {
     typedef float f32x4 __attribute__((ext_vector_type(4)));
-    const int i = 4 * (int)threadIdx.x;
-    if (i < nrows) {
+    for (int i = 4 * (int)threadIdx.x; i < nrows; i += 4 * (int)blockDim.x) {
       f32x4 rv = *(const f32x4 *)(r + t.row0 + i);
       if (UPD) {
         const f32x4 wv = *(const f32x4 *)(wp + t.row0 + i);
@@ -880,10 +879,8 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_rupdate(lz_args a, T *__restri
     // FP32: four consecutive rows per thread in one 16-byte access each
     // (tile bounds and vector bases are multiples of 64 rows, so a group
     // lies wholly inside or outside the tile)
-    static_assert(RSB_TILE == 4 * RSB_BLOCK, "four rows per thread");
     typedef float f32x4 __attribute__((ext_vector_type(4)));
-    const int row = t.row0 + 4 * (int)threadIdx.x;
-    if (row < t.row1) {
+    for (int row = t.row0 + 4 * (int)threadIdx.x; row < t.row1; row += 4 * (int)blockDim.x) {
       const f32x4 rv = *(const f32x4 *)(r + row), wv = *(const f32x4 *)(w + row);
       f32x4 x;
       x.x = rsub(rv.x, alpha, wv.x), x.y = rsub(rv.y, alpha, wv.y);
@@ -1365,11 +1362,9 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_ritz(lz_args a, const T *__res
     // FP32: four consecutive rows per thread, one 16-byte load per basis
     // vector, eight vectors in flight (vectors past k read vector k - 1 and
     // are skipped); each row sums j ascending, as below
-    static_assert(RSB_TILE == 4 * RSB_BLOCK, "four rows per thread");
     typedef float f32x4 __attribute__((ext_vector_type(4)));
     constexpr int U = 8;
-    const int row = t.row0 + 4 * (int)threadIdx.x;
-    if (row < t.row1) {
+    for (int row = t.row0 + 4 * (int)threadIdx.x; row < t.row1; row += 4 * (int)blockDim.x) {
       double x[4] = {0.0, 0.0, 0.0, 0.0};
       for (int j0 = 0; j0 < k; j0 += U) {
         f32x4 b[U];
@@ -1518,23 +1513,26 @@ __device__ inline int fin_status(int op) {
 // Totals of partial slots k0 .. k0 + nv - 1 (nv <= 3) of a segment: 8 tiles
 // per thread in flight (large segments have up to 16k tiles), a fixed order
 // of summation (each slot's partials are contiguous).  Thread 0 holds them.
-__device__ inline void fin_sums(const lz_args &a, int first, int count, int k0, int nv,
-                                double (&tot)[3], double *red) {
-  tot[0] = tot[1] = tot[2] = 0.0;
-  for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
-    double v[3][8];
+template <int NV>
+__device__ inline void fin_sums(const lz_args &a, int first, int count, int k0,
+                                double (&tot)[NV], double *red) {
 #pragma unroll
-    for (int k = 0; k < 3; k++)
+  for (int k = 0; k < NV; k++) tot[k] = 0.0;
+  for (int i0 = threadIdx.x; i0 < count; i0 += 8 * blockDim.x) {
+    double v[NV][8];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
 #pragma unroll
       for (int u = 0; u < 8; u++) {
         const int i = i0 + u * blockDim.x;
-        v[k][u] =
-            (k < nv && i < count) ? a.partial[(size_t)(k0 + k) * a.pstride + first + i] : 0.0;
+        v[k][u] = i < count ? a.partial[(size_t)(k0 + k) * a.pstride + first + i] : 0.0;
       }
 #pragma unroll
-    for (int u = 0; u < 8; u++) tot[0] += v[0][u], tot[1] += v[1][u], tot[2] += v[2][u];
+    for (int u = 0; u < 8; u++)
+#pragma unroll
+      for (int k = 0; k < NV; k++) tot[k] += v[k][u];
   }
-  block_sum<3>(tot, red);
+  block_sum<NV>(tot, red);
 }
 
 // The stop test and the next coefficients after the residual update of
@@ -1568,20 +1566,23 @@ __device__ inline void rupdate_step(const lz_args &a, int s, double rtr) {
 }
 
 __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
-  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 4];
+  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 6];
   const int s = a.act_segs ? a.act_segs[blockIdx.x] : blockIdx.x;
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
   // only the slots the phase wrote are read
+  // one pass over the slots the phase wrote (all of them in flight together)
   double tot[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  {
+  if (op == FIN_APPLYR) {
+    fin_sums<6>(a, first, count, 0, tot, red);
+  } else if (op == FIN_APPLY) {
     double t3[3];
-    fin_sums(a, first, count, 0, (op == FIN_APPLY || op == FIN_APPLYR) ? 3 : 1, t3, red);
+    fin_sums<3>(a, first, count, 0, t3, red);
     tot[0] = t3[0], tot[1] = t3[1], tot[2] = t3[2];
-    if (op == FIN_APPLYR) {
-      fin_sums(a, first, count, 3, 3, t3, red);
-      tot[3] = t3[0], tot[4] = t3[1], tot[5] = t3[2];
-    }
+  } else {
+    double t1[1];
+    fin_sums<1>(a, first, count, 0, t1, red);
+    tot[0] = t1[0];
   }
   if (threadIdx.x != 0) return;
   const double n = a.seg[a.nseg + s];
