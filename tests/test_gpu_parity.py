@@ -332,6 +332,35 @@ def test_lanczos_12bit_entries_equal_16bit_form(oracle, name, monkeypatch):
                                                              0.01 * o16["niter"][0]), prec
 
 
+def test_lanczos_12bit_form_falls_back_on_other_weights(oracle, monkeypatch):
+    """A level with a weight other than 1, 2 or 4 (face neighbours of an
+    element with one corner renumbered share three vertices) keeps the 16-bit
+    entries: level_e12() is 0, the FP32 solve equals the one with the 12-bit
+    form switched off bit for bit, and the FP64 solve agrees with the
+    oracle."""
+    flush("lanczos 12-bit fallback")
+    vtx, xyz, nv = M.box_hex(16, 12, 10, jitter=0.1, seed=4)
+    vtx = vtx.copy()
+    picks = np.arange(7, vtx.shape[0], 97)
+    vtx[picks, 0] = vtx.max() + 1 + np.arange(picks.size)
+    rp, col, w = oracle.dual_graph(vtx, nv)
+    assert (w == 3).any()
+    res = {}
+    for form in ("1", "0"):
+        monkeypatch.setenv("PARRSB_AMD_E12", form)
+        with _segments_after_rcb(vtx, xyz, nv, 2) as d:
+            order = d.order()
+            d.level_laplacian()
+            assert d.level_e12() == 0
+            res[form] = (d.fiedler(1, precision=32), d.fiedler_values(),
+                         d.fiedler(1, precision=64), d.fiedler_values())
+    assert np.array_equal(res["1"][1], res["0"][1])
+    out, f = res["1"][2], res["1"][3]
+    fo, it, lam, rn = oracle.fiedler(vtx, nv, order)
+    assert abs(out["lambda"][0] - lam) <= 1e-9 * abs(lam)
+    assert abs(abs(f @ fo) - 1) < 1e-7
+
+
 def test_lanczos_batched_segments_match_oracle(oracle):
     """Four segments of different sizes solved in one batch: each equals the
     oracle's solve of that segment alone."""
