@@ -1907,6 +1907,11 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         g->spmv_ev.resize(2 * miter_max);
         for (size_t q = old; q < g->spmv_ev.size(); q++) RSB_TRY(hipEventCreate(&g->spmv_ev[q]));
       }
+      // the running lists are compacted 8 iterations into the pass and then
+      // at intervals that double (up to 32) while no segment has finished:
+      // each compaction waits on the device, and a finished segment's tiles
+      // only cost launches whose workgroups return at once
+      int next_c = 7, step = 8;
       for (it = 0; it < miter_max; it++) {
         if (nact[0] == 0) break;
         if (use_graph && (it & 7) == 0 && it + 8 < miter_max) {
@@ -1950,10 +1955,13 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         after_apply();
         RSB_LAUNCH_CHECK();
         g->lz_launches++;
-        if ((it & 7) == 7 && it + 1 < miter_max) {
+        if (it == next_c && it + 1 < miter_max) {
           // drop segments that finished their pass from the launch lists
+          const int before = nact[1];
           if (compact(LZ_RUNNING)) return 2;
           if (nact[1] == 0) break;
+          step = nact[1] == before ? std::min(2 * step, 32) : 8;
+          next_c = it + step;
         }
       }
       RSB_TRY(hipEventRecord(g->lz_ev1, st));
