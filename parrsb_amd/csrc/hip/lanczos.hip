@@ -1965,16 +1965,6 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
         }
       }
       RSB_TRY(hipEventRecord(g->lz_ev1, st));
-      RSB_TRY(hipEventSynchronize(g->lz_ev1));
-      if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
-      float ms = 0.f;
-      RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
-      g->lz_ms += ms;
-      for (int q = 0; q < nrec; q++) {
-        RSB_TRY(hipEventElapsedTime(&ms, g->spmv_ev[2 * q], g->spmv_ev[2 * q + 1]));
-        g->spmv_ms += ms;
-      }
-      g->spmv_launches += nrec;
       RSB_TRY(hipMemsetAsync(L.running + 1, 0, sizeof(int), st));
       if (long_pass) {
         // segments in chunks whose matrices take at most 1 GiB together
@@ -1999,6 +1989,16 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       int again = 0;
       RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
       RSB_TRY(hipStreamSynchronize(st));
+      // the pass's events are complete now (one wait per pass end)
+      if (gexec) RSB_TRY(hipGraphExecDestroy(gexec));
+      float ms = 0.f;
+      RSB_TRY(hipEventElapsedTime(&ms, g->lz_ev0, g->lz_ev1));
+      g->lz_ms += ms;
+      for (int q = 0; q < nrec; q++) {
+        RSB_TRY(hipEventElapsedTime(&ms, g->spmv_ev[2 * q], g->spmv_ev[2 * q + 1]));
+        g->spmv_ms += ms;
+      }
+      g->spmv_launches += nrec;
       if (again == 0) break;
     }
     return 0;
