@@ -49,6 +49,9 @@ struct lz_args {
   double tol;
   double stall;  // > 0: a pass whose Ritz value moved by at most this (relative) ends the solve
   int stall_min; // ... from this pass on
+  double plateau; // > 0: a pass whose Ritz residual is at most plateau_max and not below
+                  // this times the one two passes back ends the solve (from the fourth pass)
+  double plateau_max;
   int miter_max;
   const int *act_tiles;  // if set: blockIdx.x -> tile (compacted active list)
   const int *act_segs;   // if set: blockIdx.x -> segment, for k_lz_fin
@@ -1660,8 +1663,23 @@ __global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
       a.niter[s] = (np - 1) * a.miter[s] + k;
       const double lam = SC(s, SC_LAMBDA), lprev = SC(s, SC_LAMPREV);
       SC(s, SC_LAMPREV) = lam;
-      const bool stalled = a.stall > 0.0 && np >= max(2, a.stall_min) &&
-                           fabs(lam - lprev) <= a.stall * fabs(lam);
+      bool stalled = a.stall > 0.0 && np >= max(2, a.stall_min) &&
+                     fabs(lam - lprev) <= a.stall * fabs(lam);
+      if (a.plateau > 0.0 && k >= 2) {
+        // the Ritz pair's residual |L f - lam f| / (lam |f|) from the Lanczos
+        // relation: |t_{k+1,k}| |y_k| / (|y| lam), t_{k+1,k} the next
+        // off-diagonal of T, -beta_k pap_{k-1} / sqrt(rtz_{k-1} rtz_k) with
+        // beta_k = rtz_k / rtz_{k-1} (the state the last update left)
+        const double rk = SC(s, SC_RTR), rk1 = SC(s, SC_RTZ1), pap = SC(s, SC_PAP);
+        const double t = (rk / rk1) * pap / sqrt(rk1 * rk);
+        const double *y = a.y + (size_t)s * a.miter_max;
+        double yy = 0.0;
+        for (int j = 0; j < k; j++) yy += y[j] * y[j];
+        const double rho = fabs(t * y[k - 1]) / (sqrt(yy) * fabs(lam));
+        stalled |= np >= 4 && rho <= a.plateau_max && rho >= a.plateau * SC(s, SC_RHO2);
+        SC(s, SC_RHO2) = SC(s, SC_RHO1);
+        SC(s, SC_RHO1) = rho;
+      }
       if (k == a.miter[s] && np < a.mpass && !stalled) {
         a.status[s] = LZ_SETUP;
         atomicAdd(&a.running[1], 1);
@@ -1787,6 +1805,9 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
     const char *sv = getenv("PARRSB_AMD_LZ_STALL"), *sm = getenv("PARRSB_AMD_LZ_STALL_MIN");
     a.stall = sv ? atof(sv) : 0.0;
     a.stall_min = sm ? atoi(sm) : 2;
+    const char *pv = getenv("PARRSB_AMD_LZ_PLATEAU"), *pm = getenv("PARRSB_AMD_LZ_PLATEAU_MAX");
+    a.plateau = pv ? atof(pv) : 0.0;
+    a.plateau_max = pm ? atof(pm) : 1e-2;
   }
   a.pstride = std::max(g->lz_tile_cap, 1);
   a.max_n = g->lws_max_n;

@@ -52,7 +52,7 @@ enum { LZ_IDLE = 0, LZ_SETUP = 1, LZ_RUNNING = 2, LZ_PASSDONE = 3, LZ_FINISHED =
 enum {
   SC_RTR = 0, SC_RTZ1, SC_RTZ2, SC_PAP, SC_PAPOLD, SC_BETA, SC_ALPHA, SC_MEANP,
   SC_RNI, SC_RNORM, SC_RTOL, SC_RNORM0, SC_MEANF, SC_LAMBDA, SC_NORMF, SC_TMP, SC_RES,
-  SC_M0, SC_N0, SC_WARMIT, SC_FALLBACK, SC_NFOLD, SC_LAMPREV,
+  SC_M0, SC_N0, SC_WARMIT, SC_FALLBACK, SC_NFOLD, SC_LAMPREV, SC_RHO1, SC_RHO2,
   LZ_NSC
 };
 

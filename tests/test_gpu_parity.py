@@ -593,6 +593,31 @@ def test_fp32_partition_quality(oracle):
     assert abs(q["edge_cut"] - qo["edge_cut"]) <= 0.02 * qo["edge_cut"]
 
 
+def test_lanczos_plateau_stop(monkeypatch):
+    """The opt-in plateau stop (PARRSB_AMD_LZ_PLATEAU, off by default) on the
+    64^3 warped box into 4 parts, whose second level has a nearly repeated
+    lambda_2 (the reference rule runs all 50 passes there): it ends those
+    passes early, only where the Ritz residual from the Lanczos relation is at
+    most the ceiling, so the device's explicit residual of the returned vector
+    stays under it (this pins the estimate), and the parts stay balanced and
+    connected.  (Where lambda_2 is nearly repeated the stopped vector is
+    another mixture of the near-equal modes than the one 50 passes reach, so
+    the cut can differ; the bench measures the effect on whole workloads.)"""
+    flush("lanczos plateau stop")
+    vtx, xyz, nv = M.warped_box_hex(64, seed=5)
+    reps = {}
+    for ratio in ("0", "0.9"):
+        monkeypatch.setenv("PARRSB_AMD_LZ_PLATEAU", ratio)
+        monkeypatch.setenv("PARRSB_AMD_LZ_PLATEAU_MAX", "1e-2")
+        part, _ = pa.part_mesh(vtx, xyz, nv, 4)
+        reps[ratio] = (pa.last_report(), pa.quality(part, vtx, nv, 4))
+    (r0, q0), (r1, q1) = reps["0"], reps["0.9"]
+    assert r0["npass_max"][1] == 50
+    assert 4 <= r1["npass_max"][1] < 50
+    assert r1["resid_max"][1] <= 1e-2 * (1 + 1e-6)
+    assert q1["max_part"] - q1["min_part"] <= 1 and q1["max_components"] == 1
+
+
 def test_deterministic():
     vtx, xyz, nv = M.box_hex(20, 18, 14, jitter=0.1, seed=8)
     a, _ = pa.part_mesh(vtx, xyz, nv, 24)
