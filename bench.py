@@ -5,8 +5,10 @@ bisection level, part ids) on a mesh already resident in HBM.  Default
 workload: 256^3 warped hex box (16,777,216 elements) into 65,536 parts with
 batched FP32 Lanczos and the learned warm start (the committed network
 parameters, parrsb_amd/lws_weights.bin); beside it, on the same mesh, the
-batched FP32 path from the reference start and the plain FP64 GPU path
-(reference arithmetic and start), and the CPU oracle on a bounded sample.
+batched FP32 path from the reference start, the plain FP64 GPU path
+(reference arithmetic and start), the learned+batched path with the opt-in
+plateau stop of the restarts (reported beside the value, never as it), and
+the CPU oracle on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config warped256]
                     [--warm-start PATH|none] [--mg] [--compare-steps 3]
@@ -25,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Melem/s to 65536 parts: learned+batched vs plain GPU RSB, edge cut within 2%"
 HBM_PEAK_GBS = 8000.0
+# the opt-in plateau stop of the Lanczos restarts, measured as an extra leg
+PLATEAU_ENV = {"PARRSB_AMD_LZ_PLATEAU": "0.9", "PARRSB_AMD_LZ_PLATEAU_MAX": "3e-2"}
 
 CONFIGS = {
     # name: (generator, kwargs, nparts, description)
@@ -420,6 +424,29 @@ def main():
                                           "avg_launch_ms": p64["spmv_ms"] / l64}}
         out["speedup_vs_plain_fp64"] = (nel / dt64 / 1e6 and value / (nel / dt64 / 1e6))
         out["edge_cut_vs_plain_fp64"] = q["edge_cut"] / max(q64["edge_cut"], 1)
+        # the opt-in plateau stop of the restarts (off by default; the library
+        # reads the switches at every solve): the same learned+batched solver,
+        # reported beside `value`, never as it
+        saved = {k: os.environ.get(k) for k in PLATEAU_ENV}
+        os.environ.update(PLATEAU_ENV)
+        try:
+            dtp, repsp = run_steps(pa, torch, d_part, d_vtx, d_xyz, nel, nv, P, 32, cs, cw, warm,
+                                   "fp32 plateau stop")
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        dtp /= cs
+        qp = pa.quality_dev(d_part.data_ptr(), d_vtx.data_ptr(), nel, nv, P)
+        out["plateau_stop"] = {
+            "value": nel / dtp / 1e6, "ms": dtp * 1e3, "quality": qp, "steps": cs, "warmup": cw,
+            "switches": dict(PLATEAU_ENV),
+            "speedup_vs_plain_fp64": (nel / dtp / 1e6) / (nel / dt64 / 1e6),
+            "edge_cut_vs_plain_fp64": qp["edge_cut"] / max(q64["edge_cut"], 1),
+            "components_vs_plain_fp64": qp["components"] / max(q64["components"], 1),
+            "npass_max": repsp[-1]["npass_max"]}
     if not args.no_compare:
         # parrsb_part_mesh on host arrays (the library copies vtx and xyz in and
         # part out over PCIe): reported beside `value`, never as it
