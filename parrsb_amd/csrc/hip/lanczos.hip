@@ -10,8 +10,9 @@
 // iteration is two launches over all rows of the level, each followed by
 // its k_lz_fin:
 //   k_lz_apply    p = beta (p_old - mean) + r formed on the fly for each row
-//                 and its neighbours, w = L p (SELL-64),
-//                 pap = w . (p - mean(p))
+//                 and its neighbours (staged in LDS per tile), w = L p from the
+//                 tile's 12- or 16-bit entries (the SELL-64 global form where a
+//                 tile's halo does not fit), pap = w . (p - mean(p))
 //   k_lz_rupdate  r = r - alpha w, rtr = r . r, tridiagonal entries, stop test
 // (FP32 passes on few tiles fold the update into the next apply, below).
 // The Lanczos basis vector of iteration it is r_it / |r_it|: the residuals
