@@ -1908,6 +1908,10 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   const size_t tq_lds = sizeof(double) * (2 * miter_max + miter_max * miter_max);
   auto passes = [&](int np_max) -> int {
     a.mpass = ai.mpass = np_max;
+    // the running lists of a pass: compacted at the end of the pass before
+    // (the segments that restart are LZ_SETUP there and RUNNING once set up,
+    // which saves a device wait per pass), at its start for the first
+    bool lists_ok = false;
     for (int pass = 0; pass < np_max; pass++) {
       // r_0 into basis slot 0
       k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 0, f, basis, p);
@@ -1916,7 +1920,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       k_lz_setup<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, 1, f, basis, p);
       RSB_LAUNCH_CHECK();
       FIN(FIN_SETUP1);
-      if (compact(LZ_RUNNING)) return 2;
+      if (!lists_ok && compact(LZ_RUNNING)) return 2;
       fold = nact[0] <= fold_tiles;
       RSB_TRY(hipEventRecord(g->lz_ev0, st));
       int it, nrec = 0;
@@ -2008,6 +2012,8 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       k_lz_ritz<T><<<ntiles, RSB_BLOCK, 0, st>>>(a, basis, mpad, f);
       RSB_LAUNCH_CHECK();
       FIN(FIN_RITZ);
+      if (compact(LZ_SETUP)) return 2;  // waits on the device
+      lists_ok = true;
       int again = 0;
       RSB_TRY(hipMemcpyAsync(&again, L.running + 1, sizeof(int), hipMemcpyDeviceToHost, st));
       RSB_TRY(hipStreamSynchronize(st));
