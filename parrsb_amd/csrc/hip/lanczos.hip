@@ -102,6 +102,8 @@ struct tile_t {
 #define LZ_XCDS 8
 // running tiles up to which an FP32 pass folds its r-update into the apply
 #define LZ_FOLD_TILES 4096
+// segments of more tiles than this take 512-thread finishing workgroups
+#define LZ_FIN_WIDE 1024
 // profiling samples one SpMV launch in this many (coprime with the 8
 // iterations between list compactions and the 50 of a pass)
 #define LZ_PROF_EVERY 11
@@ -1569,8 +1571,13 @@ __device__ inline void rupdate_step(const lz_args &a, int s, double rtr) {
   }
 }
 
-__global__ void __launch_bounds__(RSB_BLOCK) k_lz_fin(lz_args a, int op) {
-  __shared__ double red[(RSB_BLOCK / RSB_WAVE) * 6];
+// NT threads per segment: 256, or 512 for the per-iteration sums of levels
+// whose segments hold more than LZ_FIN_WIDE tiles (more partials in flight;
+// on thousands of small segments the wider workgroup costs more than it
+// saves)
+template <int NT>
+__global__ void __launch_bounds__(NT) k_lz_fin(lz_args a, int op) {
+  __shared__ double red[(NT / RSB_WAVE) * 6];
   const int s = a.act_segs ? a.act_segs[blockIdx.x] : blockIdx.x;
   if (a.status[s] != fin_status(op)) return;
   const int first = a.seg_tiles[2 * s], count = a.seg_tiles[2 * s + 1];
@@ -1817,7 +1824,7 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
   hipStream_t st = g->stream;
 #define FIN(op)                                           \
   do {                                                    \
-    k_lz_fin<<<nseg, RSB_BLOCK, 0, st>>>(a, (op));        \
+    k_lz_fin<RSB_BLOCK><<<nseg, RSB_BLOCK, 0, st>>>(a, (op)); \
     RSB_LAUNCH_CHECK();                                   \
   } while (0)
   // compacted lists of running tiles / segments for the iteration launches
@@ -1875,13 +1882,21 @@ static int run_lanczos(rsbgpu *g, int miter_opt, int mpass, double tol, int init
       k_lz_apply<T, false><<<grid, RSB_BLOCK, 0, st>>>(ai, pold, pnew, w, basis, mpad, ta,
                                                        nact[0], nullptr);
   };
+  // the iteration sums: wide workgroups where the segments are big
+  int max_seg_tiles = 0;
+  for (int q = 0; q < nseg; q++) max_seg_tiles = std::max(max_seg_tiles, g->h_seg_tiles[2 * q + 1]);
+  const bool wide_fin = max_seg_tiles > LZ_FIN_WIDE;
+  auto fin_launch = [&](int op) {
+    if (wide_fin) k_lz_fin<2 * RSB_BLOCK><<<nact[1], 2 * RSB_BLOCK, 0, st>>>(ai, op);
+    else k_lz_fin<RSB_BLOCK><<<nact[1], RSB_BLOCK, 0, st>>>(ai, op);
+  };
   auto after_apply = [&]() {
     if (fold) {
-      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLYR);
+      fin_launch(FIN_APPLYR);
     } else {
-      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_APPLY);
+      fin_launch(FIN_APPLY);
       k_lz_rupdate<T><<<nact[0], RSB_BLOCK, 0, st>>>(ai, basis, mpad, w);
-      k_lz_fin<<<nact[1], RSB_BLOCK, 0, st>>>(ai, FIN_RUPDATE);
+      fin_launch(FIN_RUPDATE);
     }
   };
 
